@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark: OC iterations/s of the GPU density-method loop.
+
+Workload (default): structured hex8 120x60x60 (432,000 elements, 1,350,723
+DOF), x=0 face clamped, 8 unit point loads at seeded random nodes of the
+x=120 face solved as one 8-column block, SIMP p=3, E0=1, Emin=1e-9, nu=0.3,
+volume fraction 0.2 (initial density 0.2), Gaussian neighbour filter r=2.0,
+Heaviside beta=1 (eta=0.5), OC move limit 0.2, eta=0.5.  The state solve is
+the mixed-precision batched PCG (float32 inner iterations, float64
+refinement to relative residual 1e-8); the float64 PCG runs beside it for the
+mixed-vs-fp64 ratio and the compliance error.
+
+A step is one OC iteration (filter, projection, 8-load solve, energies,
+sensitivities, filter adjoint, device-side lambda bisection).  W warm-up
+iterations run untimed, then K iterations are timed between device
+synchronisations.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "scikit-topt_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "hex8-120x60x60-8load": dict(grid=(120, 60, 60), n_loads=8, vol=0.2, r=2.0),
+    "hex8-40x20x20-8load": dict(grid=(40, 20, 20), n_loads=8, vol=0.2, r=2.0),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_optimizer(wl, precision, rtol, iters, dst):
+    import numpy as np  # noqa: F401
+    from sktopt_amd import core
+    from sktopt_amd.mesh import toy_problem
+    from sktopt_amd.tools import SchedulerConfig as SC
+    nx, ny, nz = wl["grid"]
+    task = toy_problem.multiload(nx, ny, nz, n_loads=wl["n_loads"], seed=0)
+    cfg = core.OC_Config(
+        dst_path=dst, max_iters=iters, p=SC.constant(target_value=3.0),
+        vol_frac=SC.constant(target_value=wl["vol"]), beta=SC.constant(target_value=1.0),
+        filter_type="spacial", filter_radius=SC.constant(target_value=wl["r"]),
+        move_limit=SC.constant(target_value=0.2), E_min_coeff=1e-9, precision=precision,
+        solver_rtol=rtol)
+    opt = core.OC_Optimizer(cfg, task)
+    opt.parameterize()
+    return opt
+
+
+def timed_run(opt, warmup, steps, timing=False):
+    import torch
+    from sktopt_amd import _lib
+    import ctypes as C
+    if warmup:
+        opt.optimize_steps(warmup)
+    torch.cuda.synchronize()
+    if timing:
+        _lib.call("skt_solver_set_timing", opt.fem._solver, 1)
+        ms = (C.c_double * 2)()
+        n = (C.c_int64 * 2)()
+        _lib.call("skt_solver_get_timing", opt.fem._solver, ms, n, 1)
+    t0 = time.perf_counter()
+    opt.optimize_steps(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    apply = None
+    if timing:
+        ms = (C.c_double * 2)()
+        n = (C.c_int64 * 2)()
+        _lib.call("skt_solver_get_timing", opt.fem._solver, ms, n, 1)
+        _lib.call("skt_solver_set_timing", opt.fem._solver, 0)
+        apply = dict(f64_ms=ms[0], f64_n=n[0], f32_ms=ms[1], f32_n=n[1])
+    return t1 - t0, apply
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="hex8-120x60x60-8load", choices=sorted(WORKLOADS))
+    ap.add_argument("--fp64-steps", type=int, default=None,
+                    help="timed float64 iterations for the ratio (default: --steps)")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU split is a design note only).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import sktopt_amd  # noqa: F401
+    from sktopt_amd import _lib
+    _lib.load()
+    if not torch.cuda.is_available():
+        print("no GPU", file=sys.stderr)
+        sys.exit(1)
+    wl = WORKLOADS[args.workload]
+    K, W = args.steps, args.warmup
+    iters = W + K + 2
+    t_setup = time.perf_counter()
+    opt = make_optimizer(wl, "mixed", 1e-8, iters, "/tmp/skt_bench_mixed")
+    log(f"setup {time.perf_counter() - t_setup:.1f}s  ndof={opt.fem.ndof} "
+        f"n_el={opt.fem.n_el} loads={opt.fem.n_tasks}")
+    dt, apply = timed_run(opt, W, K, timing=True)
+    value = K / dt
+    comp_mixed = opt.recorder.as_object().compliance
+    solves = opt.fem.solve_history[-K:]
+    log(f"mixed: {value:.3f} OC it/s  ms/step={1e3 * dt / K:.1f}  solves={solves}")
+    # roofline of the dominant kernel: the float32 operator apply
+    ndof, n_el, k = opt.fem.ndof, opt.fem.n_el, opt.fem.n_tasks
+    bytes_apply = k * ndof * (4 + 4) + n_el * 4
+    avg_ms = apply["f32_ms"] / max(apply["f32_n"], 1)
+    achieved = bytes_apply / (avg_ms * 1e-3) / 1e9
+    fp64 = None
+    err = None
+    if not args.no_fp64:
+        del opt
+        torch.cuda.empty_cache()
+        K64 = args.fp64_steps or K
+        opt64 = make_optimizer(wl, "fp64", 1e-8, W + K64 + 2, "/tmp/skt_bench_fp64")
+        dt64, apply64 = timed_run(opt64, W, K64, timing=True)
+        comp64 = opt64.recorder.as_object().compliance
+        n = min(len(comp64), len(comp_mixed))
+        err = max(abs(a - b) / abs(b) for a, b in zip(comp_mixed[:n], comp64[:n]))
+        fp64 = dict(value=K64 / dt64, ms_per_step=1e3 * dt64 / K64,
+                    apply_avg_us=1e3 * apply64["f64_ms"] / max(apply64["f64_n"], 1))
+        log(f"fp64: {fp64}  max compliance rel err mixed vs fp64 = {err:.2e}")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import cpu_baseline
+        cpu = cpu_baseline.measure(wl, solves)
+    line = {
+        "metric": "OC iters/s, 1.36M-DOF hex8, GPU FP32-mixed vs FP64 at compliance err≤1e-5",
+        "value": value, "unit": "OC iterations/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": 1e3 * dt / K, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-operator/f64-reductions",
+        "data": "synthetic (seeded random point loads)",
+        "config": {"workload": args.workload, "ndof": ndof, "n_elements": n_el,
+                   "load_cases": k, "solver": "batched Jacobi-PCG, fp32 inner + fp64 refinement",
+                   "rtol": 1e-8},
+        "roofline": {"bound": "hbm", "kernel": "k_hex_apply<float,3>",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_apply, "avg_launch_us": 1e3 * avg_ms,
+                     "launches": apply["f32_n"]},
+        "fp64": fp64, "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
+        "compliance_rel_err_vs_fp64": err,
+        "cpu_baseline": cpu,
+        "host": platform.node(),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

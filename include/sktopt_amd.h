@@ -1,0 +1,276 @@
+/*
+ * sktopt_amd -- C ABI of the MI355X (gfx950) density-method hot path.
+ *
+ * Every entry point takes device pointers (owned by the caller, never freed
+ * by the library), plain sizes and a hipStream_t, enqueues its work on that
+ * stream and returns an int status (SKT_OK on success).  skt_last_error()
+ * returns a message for the last failing call on the calling host thread.
+ * Handles (skt_operator, skt_filter, skt_solver) own their internal
+ * workspaces and release them in their destroy call.
+ *
+ * Reference interfaces replaced (paths relative to scikit-topt/sktopt/):
+ * see the comment above each declaration.  The reference is pure Python;
+ * the binding a maintainer would add is a ctypes stub.
+ *
+ * Array conventions
+ *   element arrays      (n_el,)                 float64 unless stated
+ *   block vectors       (ndof, ncols) row-major, i.e. the ncols load cases of
+ *                       one DOF are contiguous; ndof = ncomp * n_nodes and
+ *                       dof = ncomp * node + component.
+ *   dtype codes         SKT_F64 = 0, SKT_F32 = 1
+ */
+#ifndef SKTOPT_AMD_H
+#define SKTOPT_AMD_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SKT_OK 0
+#define SKT_EINVAL 1      /* bad argument (maps to ValueError)                 */
+#define SKT_EHIP 2        /* HIP runtime error (maps to RuntimeError)          */
+#define SKT_ENOCONV 3     /* iterative solve did not reach its tolerance        */
+#define SKT_ENOTIMPL 4    /* unsupported combination (NotImplementedError)      */
+
+#define SKT_F64 0
+#define SKT_F32 1
+
+#define SKT_SIMP 0
+#define SKT_RAMP 1
+
+#define SKT_ELASTIC 0
+#define SKT_HEAT 1
+
+typedef struct skt_operator skt_operator;
+typedef struct skt_filter skt_filter;
+typedef struct skt_solver skt_solver;
+
+const char* skt_last_error(void);
+int skt_abi_version(void);
+
+/* ------------------------------------------------------------------------
+ * Design-variable transforms (float64, elementwise)
+ * ---------------------------------------------------------------------- */
+
+/* core/projection.py:27-77  heaviside_projection_inplace(x, beta, eta, out) */
+int skt_heaviside_project(int64_t n, const double* x, double beta, double eta,
+                          double* out, hipStream_t stream);
+
+/* core/projection.py:80-118  heaviside_projection_derivative_inplace */
+int skt_heaviside_derivative(int64_t n, const double* x, double beta,
+                             double eta, double* out, hipStream_t stream);
+
+/* fea/composer.py:19-39  simp_interpolation / ramp_interpolation.
+ * out has dtype out_dtype (SKT_F64 or SKT_F32). */
+int skt_interpolate(int kind, int64_t n, const double* rho, double E0,
+                    double Emin, double p, void* out, int out_dtype,
+                    hipStream_t stream);
+
+/* core/derivatives.py:42-52,65-68  dC_drho_simp / dC_drho_ramp */
+int skt_dC_drho(int kind, int64_t n, const double* rho, const double* energy,
+                int64_t energy_stride, double E0, double Emin, double p,
+                double* out, hipStream_t stream);
+
+/* core/optimizers/common_density.py:1073-1101 fused over load cases:
+ * g[e] = scale * sum_k dC_kind(rho_proj[e], energy[e,k]) * H'(rho_filt[e]).
+ * energy is (n_el, ncols) row-major; scale = 1/n_tasks folds the mean
+ * (the filter adjoint that follows is linear). */
+int skt_sensitivity_chain(int kind, int64_t n, int ncols,
+                          const double* rho_proj, const double* energy,
+                          const double* rho_filt, double beta, double eta,
+                          double E0, double Emin, double p, double scale,
+                          double* g, hipStream_t stream);
+
+/* Elementwise helpers of the update rules (logmoc.py:186-225):
+ * op 0: out = a*x + b*y (y may be NULL: out = a*x); op 1: out = a*x*y. */
+int skt_ewise(int op, int64_t n, double a, const double* x, double b,
+              const double* y, double* out, hipStream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Reductions (float64; results are written to device memory)
+ * ---------------------------------------------------------------------- */
+#define SKT_RED_SUM 0        /* sum x                                     */
+#define SKT_RED_DOT 1        /* sum x*y                                   */
+#define SKT_RED_MAXABS 2     /* max |x|                                   */
+#define SKT_RED_MAXABSDIFF 3 /* max |x - y|                               */
+
+int skt_reduce(int op, int64_t n, const double* x, const double* y,
+               double* out, hipStream_t stream);
+
+/* Column-wise reductions of a (n, ncols) row-major block:
+ * op SKT_RED_SUM -> out[c] = sum_i x[i,c];
+ * op SKT_RED_DOT -> out[c] = sum_i x[i,c]*y[i,c];
+ * op SKT_RED_MAXABS -> out[c] = max_i |x[i,c]|. */
+int skt_reduce_cols(int op, int64_t n, int ncols, const double* x,
+                    const double* y, double* out, hipStream_t stream);
+
+/* np.percentile(np.abs(x), q) with numpy's default 'linear' method
+ * (core/optimizers/oc.py:188-189, logmoc.py:167-174,220).  Exact radix
+ * select on the float64 bit patterns. */
+int skt_percentile_abs(int64_t n, const double* x, double q, double* out,
+                       hipStream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Matrix-free SIMP/RAMP-weighted operators
+ *   K(E) = sum_e E_e K_e^0, Dirichlet rows replaced by identity
+ *   (skfem.enforce semantics as used at fea/solver_elastic.py:205-211).
+ * ---------------------------------------------------------------------- */
+
+/* Structured nx*ny*nz box grid of hx*hy*hz hexahedra (hex8), nodes numbered
+ * x-fastest.  physics SKT_ELASTIC (3 dof/node, Poisson ratio nu) or SKT_HEAT
+ * (1 dof/node).  quad_order: Gauss points per direction for K_e^0 (>= 1).
+ * dof_fixed: (ndof,) uint8 device array, 1 = Dirichlet DOF (copied).
+ * Replaces composer.assemble_stiffness_matrix (fea/composer.py:53-101) and
+ * assemble_conduction_matrix (fea/composer.py:104-145) on hex meshes. */
+int skt_hex8_create(int physics, int64_t nx, int64_t ny, int64_t nz,
+                    double hx, double hy, double hz, double nu,
+                    int quad_order, const uint8_t* dof_fixed,
+                    skt_operator** out);
+
+/* Unstructured linear tetrahedra.  coords: (3, n_nodes) float64 device
+ * array (x row, y row, z row); conn: (4, n_el) int32 device array.
+ * Same operator semantics as above on tet meshes. */
+int skt_tet4_create(int physics, int64_t n_nodes, int64_t n_el,
+                    const double* coords, const int32_t* conn, double nu,
+                    const uint8_t* dof_fixed, skt_operator** out);
+
+int skt_operator_destroy(skt_operator* op);
+
+/* ndof, n_el, ncomp (3 or 1) of the operator. */
+int skt_operator_info(const skt_operator* op, int64_t* ndof, int64_t* n_el,
+                      int* ncomp);
+
+/* Copy the float64 unit-modulus element matrix of a hex8 operator
+ * ((ncomp*8)^2 values, host memory). */
+int skt_operator_element_matrix(const skt_operator* op, double* host_out);
+
+/* Y = K_e(E) X for ncols columns.  E, X, Y all of dtype. */
+int skt_operator_apply(skt_operator* op, int dtype, int ncols, const void* E,
+                       const void* X, void* Y, hipStream_t stream);
+
+/* diag[dof] of K_e(E) (1 on Dirichlet DOFs), dtype. */
+int skt_operator_diag(skt_operator* op, int dtype, const void* E, void* diag,
+                      hipStream_t stream);
+
+/* energy[e, c] = 1/2 E_e u_e^T K_e^0 u_e for each column of U (float64 out).
+ * fea/solver_elastic.py:470-537 (strain_energy_skfem_multi) and
+ * fea/solver_heat.py:256-303 (heat_energy_skfem_multi). */
+int skt_operator_energy(skt_operator* op, int dtype, int ncols, const void* E,
+                        const void* U, double* energy, hipStream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Batched Jacobi-preconditioned CG on an operator
+ *   Replaces solve_u / solve_multi_load (fea/solver_elastic.py:61-143,
+ *   240-395) and solve_heat_system_multi (fea/solver_heat.py:136-195):
+ *   all ncols right-hand sides share one operator apply per iteration.
+ * mode SKT_F64: float64 CG to ||r|| <= rtol ||F|| per column.
+ * mode SKT_F32: iterative refinement -- float64 residual F - K U recomputed
+ *   with the float64 operator, float32 inner PCG with float64 dot products
+ *   and norms, U accumulated in float64.
+ * ---------------------------------------------------------------------- */
+int skt_solver_create(skt_operator* op, int max_cols, skt_solver** out);
+int skt_solver_destroy(skt_solver* s);
+
+/* E64: (n_el,) float64 moduli; E32: float32 copy (required in SKT_F32 mode,
+ * may be NULL in SKT_F64 mode).  F: (ndof, ncols) float64 enforced
+ * right-hand side -- F[D] holds the prescribed Dirichlet values (0 for a
+ * clamp), as skfem.enforce builds it.  U: (ndof, ncols) float64, the initial
+ * guess on entry (warm start; U[D] is overwritten with F[D]) and the
+ * solution on exit.
+ * iters[0] = total inner iterations, iters[1] = outer refinement steps;
+ * relres[c] = final ||F - K U|| / ||F|| per column (host arrays). */
+int skt_solver_solve(skt_solver* s, int mode, int ncols, const double* E64,
+                     const float* E32, const double* F, double* U,
+                     double rtol, int maxiter, double inner_rtol,
+                     int64_t* iters, double* relres, hipStream_t stream);
+
+/* Measurement hooks (no reference counterpart; used by the benchmark):
+ * with timing enabled, every operator apply inside the CG loop is bracketed
+ * by hipEvents on the solve stream.  get_timing returns the accumulated
+ * milliseconds and launch counts per dtype ([0] float64, [1] float32). */
+int skt_solver_set_timing(skt_solver* s, int enable);
+int skt_solver_get_timing(skt_solver* s, double* apply_ms, int64_t* n_apply,
+                          int reset);
+
+/* ------------------------------------------------------------------------
+ * Gaussian density filter on element centres (filters/spacial.py)
+ * ---------------------------------------------------------------------- */
+
+/* Build the row-normalised neighbour-list (CSR) filter over design
+ * elements: neighbours j of design element i are design elements with
+ * dx^2+dy^2+dz^2 <= support^2 (cKDTree.query_ball_point rule), weights
+ * exp(-0.5 (d/sigma)^2), columns in the compressed design numbering and
+ * ascending order, rows scaled by 1/rowsum (spacial.py:51-103).
+ * centers: (3, n_el) float64 device array; design: (n_el,) uint8. */
+int skt_filter_create(int64_t n_el, const double* centers,
+                      const uint8_t* design, double sigma, double support,
+                      skt_filter** out);
+int skt_filter_destroy(skt_filter* f);
+int skt_filter_info(const skt_filter* f, int64_t* n_design, int64_t* nnz);
+
+/* Copy the CSR to device buffers: indptr (n_design+1) int64,
+ * indices (nnz) int32, values (nnz) float64, design_ids (n_design) int32. */
+int skt_filter_export(const skt_filter* f, int64_t* indptr, int32_t* indices,
+                      double* values, int32_t* design_ids, hipStream_t stream);
+
+/* spacial.py:135-157: y = x; y[design] = W x[design]  (n_el arrays). */
+int skt_filter_forward(skt_filter* f, const double* x, double* y,
+                       hipStream_t stream);
+
+/* spacial.py:159-166: g = 0; g[design] = W^T v[design]. */
+int skt_filter_gradient(skt_filter* f, const double* v, double* g,
+                        hipStream_t stream);
+
+/* ------------------------------------------------------------------------
+ * OC update: lambda bisection on the physical volume, all on the device
+ *   core/optimizers/oc.py:16-95 (bisection_with_physical_volume), driven as
+ *   in oc.py:199-228.  Every step: candidate update, filter gather,
+ *   Heaviside projection and float64 volume sum; the bracket update and
+ *   exit test run on the device (no host round trip).
+ * rho_full (n_el): current densities (non-design entries are carried over);
+ * rho_e, dC, vol: (n_design,) in design order; rho_out (n_design,): the
+ * candidate at the final lambda.  result (device, 4 doubles):
+ * {lmid, vol_error, n_evals, l2 - l1}.
+ * ---------------------------------------------------------------------- */
+int skt_oc_bisect(skt_filter* f, const double* rho_full, const double* rho_e,
+                  const double* dC, const double* vol, double vol_sum,
+                  double vol_frac, double move_limit, double eta,
+                  double rho_min, double rho_max, double rate_min,
+                  double rate_max, double beta, double beta_eta, double l1,
+                  double l2, double tolerance, double vol_tol, int max_iter,
+                  double* rho_out, double* result, hipStream_t stream);
+
+/* core/optimizers/oc.py:187-196: s = max(*scale, floor); running_scale =
+ * first ? s : keep*running_scale + (1-keep)*s; out = dC / running_scale.
+ * scale and running_scale are device scalars (no host round trip). */
+int skt_oc_normalize(int64_t n, const double* dC, const double* scale,
+                     double floor_, double keep, int first, double* running_scale,
+                     double* out, hipStream_t stream);
+
+/* core/optimizers/oc.py:230-240: KKT residual max |dC_raw + lmid*rs*dV| over
+ * rho in (rho_min+1e-6, rho_max-1e-6); lmid = bisect_result[0],
+ * rs = running_scale[0]; 0 when no element qualifies.  out: device scalar. */
+int skt_oc_kkt(int64_t n, const double* dC_raw, const double* rho, const double* dV,
+               const double* bisect_result, const double* running_scale,
+               double rho_min, double rho_max, double* out, hipStream_t stream);
+
+/* logmoc.py:36-66 lagrangian_log_update, in place on rho (n,). */
+int skt_logmoc_update(int64_t n, double* rho, const double* dL, double eta,
+                      double move_limit, double rho_min, double rho_max,
+                      double clip, hipStream_t stream);
+
+/* Gather / scatter between full (n_el) and design-ordered (n_design) arrays. */
+int skt_filter_gather_design(skt_filter* f, const double* full, double* design,
+                             hipStream_t stream);
+int skt_filter_scatter_design(skt_filter* f, const double* design, double* full,
+                              hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SKTOPT_AMD_H */

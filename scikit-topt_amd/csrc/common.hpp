@@ -1,0 +1,111 @@
+// Shared helpers for the gfx950 kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/sktopt_amd.h"
+
+namespace skt {
+
+// Last error message of the calling host thread.
+void set_error(const std::string& msg);
+const char* get_error();
+
+#define SKT_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t err_ = (call);                                                  \
+    if (err_ != hipSuccess) {                                                  \
+      ::skt::set_error(std::string(#call) + ": " + hipGetErrorString(err_));   \
+      return SKT_EHIP;                                                         \
+    }                                                                          \
+  } while (0)
+
+#define SKT_CHECK(cond, msg)                                                   \
+  do {                                                                         \
+    if (!(cond)) {                                                             \
+      ::skt::set_error(msg);                                                   \
+      return SKT_EINVAL;                                                       \
+    }                                                                          \
+  } while (0)
+
+#define SKT_LAUNCH_CHECK()                                                     \
+  do {                                                                         \
+    hipError_t err_ = hipGetLastError();                                       \
+    if (err_ != hipSuccess) {                                                  \
+      ::skt::set_error(std::string("kernel launch: ") + hipGetErrorString(err_)); \
+      return SKT_EHIP;                                                         \
+    }                                                                          \
+  } while (0)
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+// Wave-level sum (fixed butterfly order, deterministic).
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+template <typename T>
+__device__ inline T wave_max(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    T o = __shfl_xor(v, off, kWave);
+    v = v > o ? v : o;
+  }
+  return v;
+}
+
+// Deterministic block sum of one value per thread (blockDim.x == kBlock).
+template <typename T>
+__device__ inline T block_sum(T v, T* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T s = T(0);
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) s += sh[i];
+  }
+  return s;  // valid in thread 0 only
+}
+
+template <typename T>
+__device__ inline T block_max(T v, T* sh) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T s = sh[0];
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) s = s > sh[i] ? s : sh[i];
+  }
+  return s;  // valid in thread 0 only
+}
+
+// Final fixed-order reduction of per-block partials into out[0] (one block).
+template <typename T>
+__global__ void reduce_partials_sum(const T* __restrict__ part, int n, T* __restrict__ out) {
+  __shared__ T sh[kBlock / kWave];
+  T v = T(0);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+  T s = block_sum(v, sh);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+}  // namespace skt

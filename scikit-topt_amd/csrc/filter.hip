@@ -1,0 +1,621 @@
+// Gaussian density filter as a precomputed neighbour-list (CSR) gather, and
+// the device-side OC lambda bisection that runs the filter once per step.
+//
+// Setup (skt_filter_create) bins element centres on a uniform grid, finds
+// neighbours with the inclusion rule dx^2+dy^2+dz^2 <= support^2 evaluated
+// in IEEE float64 without contraction, sorts every row by column, and stores
+//   row_ptr (int64), gcol (int32 full element index), W and W^T values.
+// The neighbour relation is symmetric, so W^T has W's pattern; its values
+// are placed by a binary search of the mirrored entry.
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+struct skt_filter {
+  int64_t n_el = 0;
+  int64_t n_design = 0;
+  int64_t nnz = 0;
+  int32_t* design_ids = nullptr;   // (n_design)
+  int32_t* colmap = nullptr;       // (n_el) compressed index or -1
+  int64_t* row_ptr = nullptr;      // (n_design + 1)
+  int32_t* gcol = nullptr;         // (nnz) full element index of the column
+  double* w = nullptr;             // (nnz) row-normalised weights
+  double* wt = nullptr;            // (nnz) transposed weights, same pattern
+  // bisection workspace
+  double* cand_full = nullptr;     // (n_el)
+  double* part = nullptr;          // (kPartMax)
+  double* state = nullptr;         // bisection state
+};
+
+namespace skt {
+
+constexpr int kPartMax = 4096;
+constexpr int kSeg = 8;  // lanes per CSR row in the gathers
+
+__device__ __forceinline__ double dist2(const double* __restrict__ c, int64_t n,
+                                        int64_t i, int64_t j) {
+#pragma clang fp contract(off)
+  double dx = c[j] - c[i];
+  double dy = c[n + j] - c[n + i];
+  double dz = c[2 * n + j] - c[2 * n + i];
+  double s = dx * dx;
+  s = s + dy * dy;
+  s = s + dz * dz;
+  return s;
+}
+
+__global__ void k_cell_of(int64_t n, const double* __restrict__ c, double x0,
+                          double y0, double z0, double inv, int64_t gx,
+                          int64_t gy, int64_t gz, int64_t* __restrict__ cell,
+                          int32_t* __restrict__ ids) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t a = (int64_t)floor((c[i] - x0) * inv);
+    int64_t b = (int64_t)floor((c[n + i] - y0) * inv);
+    int64_t d = (int64_t)floor((c[2 * n + i] - z0) * inv);
+    a = a < 0 ? 0 : (a >= gx ? gx - 1 : a);
+    b = b < 0 ? 0 : (b >= gy ? gy - 1 : b);
+    d = d < 0 ? 0 : (d >= gz ? gz - 1 : d);
+    cell[i] = a + gx * (b + gy * d);
+    ids[i] = (int32_t)i;
+  }
+}
+
+__global__ void k_cell_bounds(int64_t n, const int64_t* __restrict__ cell_sorted,
+                              int64_t* __restrict__ start, int64_t* __restrict__ end) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t c = cell_sorted[i];
+    if (i == 0 || cell_sorted[i - 1] != c) start[c] = i;
+    if (i == n - 1 || cell_sorted[i + 1] != c) end[c] = i + 1;
+  }
+}
+
+__global__ void k_design_flags(int64_t n, const uint8_t* __restrict__ design,
+                               int32_t* __restrict__ flag) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    flag[i] = design[i] ? 1 : 0;
+}
+
+__global__ void k_design_ids(int64_t n, const uint8_t* __restrict__ design,
+                             const int32_t* __restrict__ pos,
+                             int32_t* __restrict__ ids, int32_t* __restrict__ colmap) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (design[i]) {
+      ids[pos[i]] = (int32_t)i;
+      colmap[i] = pos[i];
+    } else {
+      colmap[i] = -1;
+    }
+  }
+}
+
+// count (fill == false) or fill (fill == true) the neighbours of each row
+template <bool kFill>
+__global__ void k_neighbours(int64_t n_el, int64_t n_design,
+                             const double* __restrict__ c,
+                             const int32_t* __restrict__ design_ids,
+                             const int32_t* __restrict__ colmap,
+                             const int64_t* __restrict__ cell,
+                             const int32_t* __restrict__ sorted_ids,
+                             const int64_t* __restrict__ cstart,
+                             const int64_t* __restrict__ cend, int64_t gx,
+                             int64_t gy, int64_t gz, double r2,
+                             int64_t* __restrict__ counts,
+                             const int64_t* __restrict__ row_ptr,
+                             int32_t* __restrict__ gcol) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = design_ids[r];
+    const int64_t ci = cell[i];
+    const int64_t a = ci % gx, b = (ci / gx) % gy, d = ci / (gx * gy);
+    int64_t cnt = 0;
+    int64_t out = kFill ? row_ptr[r] : 0;
+    for (int64_t dd = d - 1; dd <= d + 1; ++dd) {
+      if (dd < 0 || dd >= gz) continue;
+      for (int64_t bb = b - 1; bb <= b + 1; ++bb) {
+        if (bb < 0 || bb >= gy) continue;
+        for (int64_t aa = a - 1; aa <= a + 1; ++aa) {
+          if (aa < 0 || aa >= gx) continue;
+          const int64_t cc = aa + gx * (bb + gy * dd);
+          const int64_t s0 = cstart[cc], s1 = cend[cc];
+          for (int64_t q = s0; q < s1; ++q) {
+            const int64_t j = sorted_ids[q];
+            if (colmap[j] < 0) continue;
+            if (dist2(c, n_el, i, j) <= r2) {
+              if (kFill) gcol[out++] = (int32_t)j;
+              ++cnt;
+            }
+          }
+        }
+      }
+    }
+    if (!kFill) counts[r] = cnt;
+  }
+}
+
+// Row-wise insertion sort of gcol (rows are short), then Gaussian weights and
+// row normalisation in ascending column order.
+__global__ void k_sort_weights(int64_t n_el, int64_t n_design,
+                               const double* __restrict__ c,
+                               const int32_t* __restrict__ design_ids,
+                               const int64_t* __restrict__ row_ptr,
+                               int32_t* __restrict__ gcol, double sigma,
+                               double* __restrict__ w) {
+#pragma clang fp contract(off)
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s0 = row_ptr[r], s1 = row_ptr[r + 1];
+    for (int64_t q = s0 + 1; q < s1; ++q) {
+      int32_t v = gcol[q];
+      int64_t p = q - 1;
+      while (p >= s0 && gcol[p] > v) {
+        gcol[p + 1] = gcol[p];
+        --p;
+      }
+      gcol[p + 1] = v;
+    }
+    const int64_t i = design_ids[r];
+    double sum = 0.0;
+    for (int64_t q = s0; q < s1; ++q) {
+      double d = sqrt(dist2(c, n_el, i, gcol[q]));
+      double t = d / sigma;
+      double wq = exp(-0.5 * (t * t));
+      w[q] = wq;
+      sum = sum + wq;
+    }
+    if (sum == 0.0) sum = 1.0;
+    const double inv = 1.0 / sum;
+    for (int64_t q = s0; q < s1; ++q) w[q] = inv * w[q];
+  }
+}
+
+__global__ void k_transpose_values(int64_t n_design, const int32_t* __restrict__ design_ids,
+                                   const int32_t* __restrict__ colmap,
+                                   const int64_t* __restrict__ row_ptr,
+                                   const int32_t* __restrict__ gcol,
+                                   const double* __restrict__ w,
+                                   double* __restrict__ wt) {
+  // For entry (r -> j) at q, W^T[j, r] = W[r, j]; W^T row j has W row j's
+  // pattern, so locate column design_ids[r] in row colmap[j].
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t gi = design_ids[r];
+    for (int64_t q = row_ptr[r]; q < row_ptr[r + 1]; ++q) {
+      const int64_t rj = colmap[gcol[q]];
+      int64_t lo = row_ptr[rj], hi = row_ptr[rj + 1] - 1;
+      while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (gcol[mid] < gi) lo = mid + 1; else hi = mid;
+      }
+      wt[lo] = w[q];
+    }
+  }
+}
+
+// y[e] = x[e] for non-design e; y[design row] = sum_q val[q] x[gcol[q]].
+// kSeg lanes share one row; fixed butterfly reduction inside the segment.
+__global__ void k_csr_apply(int64_t n_el, const int32_t* __restrict__ colmap,
+                            const int64_t* __restrict__ row_ptr,
+                            const int32_t* __restrict__ gcol,
+                            const double* __restrict__ val,
+                            const double* __restrict__ x,
+                            double* __restrict__ y, int zero_nondesign) {
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int sub = threadIdx.x & (kSeg - 1);
+  for (int64_t e = tid / kSeg; e < n_el; e += ((int64_t)gridDim.x * blockDim.x) / kSeg) {
+    const int32_t r = colmap[e];
+    double acc = 0.0;
+    if (r >= 0) {
+      const int64_t s0 = row_ptr[r], s1 = row_ptr[r + 1];
+      for (int64_t q = s0 + sub; q < s1; q += kSeg) acc += val[q] * x[gcol[q]];
+    }
+#pragma unroll
+    for (int off = kSeg / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kSeg);
+    if (sub == 0) y[e] = (r >= 0) ? acc : (zero_nondesign ? 0.0 : x[e]);
+  }
+}
+
+__global__ void k_gather_design(int64_t n_design, const int32_t* __restrict__ ids,
+                                const double* __restrict__ full,
+                                double* __restrict__ des) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
+       r += (int64_t)gridDim.x * blockDim.x)
+    des[r] = full[ids[r]];
+}
+
+__global__ void k_scatter_design(int64_t n_design, const int32_t* __restrict__ ids,
+                                 const double* __restrict__ des,
+                                 double* __restrict__ full) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
+       r += (int64_t)gridDim.x * blockDim.x)
+    full[ids[r]] = des[r];
+}
+
+// ------------------------------------------------------------------ bisection
+// state: [0] l1 [1] l2 [2] lmid [3] vol_error [4] iter [5] done [6] n_evals
+
+__global__ void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+__global__ void k_oc_init(double* st, double l1, double l2) {
+  st[0] = l1;
+  st[1] = l2;
+  st[2] = 0.5 * (l1 + l2);
+  st[3] = 0.0;
+  st[4] = 0.0;
+  st[5] = 0.0;
+  st[6] = 0.0;
+}
+
+__device__ __forceinline__ double clip_np(double x, double lo, double hi) {
+  double y = x < lo ? lo : x;
+  return y > hi ? hi : y;
+}
+
+__global__ void k_oc_candidate(int64_t n_design, const int32_t* __restrict__ ids,
+                               const double* __restrict__ rho_e,
+                               const double* __restrict__ dC,
+                               const double* __restrict__ st, double eps,
+                               double eta, double rate_min, double rate_max,
+                               double move_limit, double rho_min, double rho_max,
+                               double* __restrict__ cand_full,
+                               double* __restrict__ rho_out) {
+  if (st[5] != 0.0) return;
+  const double lmid = st[2];
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    double s = -dC[r];
+    s = s / (lmid + eps);
+    s = pow(s, eta);
+    s = clip_np(s, rate_min, rate_max);
+    const double re = rho_e[r];
+    double lo = re - move_limit;
+    lo = (lo < rho_min) ? rho_min : lo;     // np.maximum
+    double hi = re + move_limit;
+    hi = (hi > rho_max) ? rho_max : hi;     // np.minimum
+    double c = clip_np(re * s, lo, hi);
+    rho_out[r] = c;
+    cand_full[ids[r]] = c;
+  }
+}
+
+// Filter gather on the candidate, projection, volume partial sums.
+__global__ void k_oc_volume(int64_t n_design, const int32_t* __restrict__ ids,
+                            const int64_t* __restrict__ row_ptr,
+                            const int32_t* __restrict__ gcol,
+                            const double* __restrict__ w,
+                            const double* __restrict__ cand_full,
+                            const double* __restrict__ vol, double beta,
+                            double beta_eta, double tbe, double denom,
+                            const double* __restrict__ st,
+                            double* __restrict__ part) {
+  __shared__ double sh[kBlock / kWave];
+  if (st[5] != 0.0) return;
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int sub = threadIdx.x & (kSeg - 1);
+  double vsum = 0.0;
+  for (int64_t r = tid / kSeg; r < n_design; r += ((int64_t)gridDim.x * blockDim.x) / kSeg) {
+    const int64_t s0 = row_ptr[r], s1 = row_ptr[r + 1];
+    double acc = 0.0;
+    for (int64_t q = s0 + sub; q < s1; q += kSeg) acc += w[q] * cand_full[gcol[q]];
+#pragma unroll
+    for (int off = kSeg / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kSeg);
+    if (sub == 0) {
+      double t = (acc - beta_eta) * beta;
+      t = tanh(t);
+      double proj = (tbe + t) / denom;
+      vsum += proj * vol[r];
+    }
+  }
+  double b = block_sum(vsum, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
+__global__ void k_oc_bracket(int nb, const double* __restrict__ part,
+                             double vol_sum, double vol_frac, double vol_tol,
+                             double tolerance, int max_iter,
+                             double* __restrict__ st) {
+  __shared__ double sh[kBlock / kWave];
+  if (st[5] != 0.0) return;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) v += part[i];
+  double tot = block_sum(v, sh);
+  if (threadIdx.x == 0) {
+    const double vol_error = tot / vol_sum - vol_frac;
+    st[3] = vol_error;
+    st[6] += 1.0;
+    double l1 = st[0], l2 = st[1];
+    const double it = st[4];
+    if (fabs(vol_error) < vol_tol || it >= (double)max_iter ||
+        fabs(l2 - l1) <= tolerance) {
+      st[5] = 1.0;
+    } else {
+      if (vol_error > 0) l1 = st[2]; else l2 = st[2];
+      st[0] = l1;
+      st[1] = l2;
+      st[4] = it + 1.0;
+      st[2] = 0.5 * (l1 + l2);
+    }
+  }
+}
+
+__global__ void k_oc_result(const double* __restrict__ st, double* __restrict__ res) {
+  res[0] = st[2];
+  res[1] = st[3];
+  res[2] = st[6];
+  res[3] = st[1] - st[0];
+}
+
+}  // namespace skt
+
+using namespace skt;
+
+template <typename F>
+static int cub_call(F f, hipStream_t stream) {
+  size_t bytes = 0;
+  SKT_HIP(f(nullptr, bytes));
+  void* tmp = nullptr;
+  SKT_HIP(hipMalloc(&tmp, bytes ? bytes : 16));
+  hipError_t e = f(tmp, bytes);
+  hipError_t e2 = hipStreamSynchronize(stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    set_error(std::string("hipcub: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    return SKT_EHIP;
+  }
+  return SKT_OK;
+}
+
+extern "C" {
+
+int skt_filter_create(int64_t n_el, const double* centers,
+                      const uint8_t* design, double sigma, double support,
+                      skt_filter** out) {
+  SKT_CHECK(out && centers && design, "null pointer");
+  SKT_CHECK(n_el >= 1 && n_el < (int64_t)INT32_MAX, "n_el out of range");
+  SKT_CHECK(sigma > 0.0 && support > 0.0, "sigma and support must be positive");
+  *out = nullptr;
+  hipStream_t stream = nullptr;
+  skt_filter* f = new skt_filter();
+  f->n_el = n_el;
+  int st = SKT_OK;
+  // --- bounding box on the host (setup only)
+  std::vector<double> hc(3 * n_el);
+  SKT_HIP(hipMemcpy(hc.data(), centers, sizeof(double) * 3 * n_el, hipMemcpyDeviceToHost));
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = hi[d] = hc[d * n_el];
+    for (int64_t i = 0; i < n_el; ++i) {
+      double v = hc[d * n_el + i];
+      lo[d] = v < lo[d] ? v : lo[d];
+      hi[d] = v > hi[d] ? v : hi[d];
+    }
+  }
+  // cells at least the support wide, capped so the cell table stays small
+  double cs = support * 1.000001;
+  int64_t g[3];
+  for (;;) {
+    for (int d = 0; d < 3; ++d) g[d] = (int64_t)std::floor((hi[d] - lo[d]) / cs) + 1;
+    if (g[0] * g[1] * g[2] <= 4 * n_el + 64) break;
+    cs *= 1.5;
+  }
+  const int64_t ncell = g[0] * g[1] * g[2];
+  int64_t *cell = nullptr, *cell_sorted = nullptr, *cstart = nullptr, *cend = nullptr;
+  int32_t *ids = nullptr, *ids_sorted = nullptr, *flag = nullptr, *pos = nullptr;
+  int64_t* counts = nullptr;
+  SKT_HIP(hipMalloc(&cell, sizeof(int64_t) * n_el));
+  SKT_HIP(hipMalloc(&cell_sorted, sizeof(int64_t) * n_el));
+  SKT_HIP(hipMalloc(&ids, sizeof(int32_t) * n_el));
+  SKT_HIP(hipMalloc(&ids_sorted, sizeof(int32_t) * n_el));
+  SKT_HIP(hipMalloc(&cstart, sizeof(int64_t) * ncell));
+  SKT_HIP(hipMalloc(&cend, sizeof(int64_t) * ncell));
+  SKT_HIP(hipMemset(cstart, 0, sizeof(int64_t) * ncell));
+  SKT_HIP(hipMemset(cend, 0, sizeof(int64_t) * ncell));
+  SKT_HIP(hipMalloc(&flag, sizeof(int32_t) * n_el));
+  SKT_HIP(hipMalloc(&pos, sizeof(int32_t) * n_el));
+  const unsigned gb = grid_for(n_el);
+  hipLaunchKernelGGL(k_cell_of, dim3(gb), dim3(kBlock), 0, stream, n_el, centers,
+                     lo[0], lo[1], lo[2], 1.0 / cs, g[0], g[1], g[2], cell, ids);
+  SKT_LAUNCH_CHECK();
+  st = cub_call([&](void* t, size_t& b) {
+    return hipcub::DeviceRadixSort::SortPairs(t, b, cell, cell_sorted, ids, ids_sorted,
+                                              (int)n_el, 0, 64, stream);
+  }, stream);
+  if (st) return st;
+  hipLaunchKernelGGL(k_cell_bounds, dim3(gb), dim3(kBlock), 0, stream, n_el,
+                     cell_sorted, cstart, cend);
+  hipLaunchKernelGGL(k_design_flags, dim3(gb), dim3(kBlock), 0, stream, n_el, design, flag);
+  SKT_LAUNCH_CHECK();
+  st = cub_call([&](void* t, size_t& b) {
+    return hipcub::DeviceScan::ExclusiveSum(t, b, flag, pos, (int)n_el, stream);
+  }, stream);
+  if (st) return st;
+  int32_t last_pos = 0, last_flag = 0;
+  SKT_HIP(hipMemcpy(&last_pos, pos + n_el - 1, sizeof(int32_t), hipMemcpyDeviceToHost));
+  SKT_HIP(hipMemcpy(&last_flag, flag + n_el - 1, sizeof(int32_t), hipMemcpyDeviceToHost));
+  const int64_t nd = (int64_t)last_pos + last_flag;
+  SKT_CHECK(nd >= 1, "design_elements is empty");
+  f->n_design = nd;
+  SKT_HIP(hipMalloc(&f->design_ids, sizeof(int32_t) * nd));
+  SKT_HIP(hipMalloc(&f->colmap, sizeof(int32_t) * n_el));
+  hipLaunchKernelGGL(k_design_ids, dim3(gb), dim3(kBlock), 0, stream, n_el, design,
+                     pos, f->design_ids, f->colmap);
+  SKT_LAUNCH_CHECK();
+  SKT_HIP(hipMalloc(&counts, sizeof(int64_t) * (nd + 1)));
+  SKT_HIP(hipMalloc(&f->row_ptr, sizeof(int64_t) * (nd + 1)));
+  SKT_HIP(hipMemset(counts, 0, sizeof(int64_t) * (nd + 1)));
+  const double r2 = support * support;
+  const unsigned gd = grid_for(nd);
+  hipLaunchKernelGGL(k_neighbours<false>, dim3(gd), dim3(kBlock), 0, stream, n_el, nd,
+                     centers, f->design_ids, f->colmap, cell, ids_sorted, cstart, cend,
+                     g[0], g[1], g[2], r2, counts, (const int64_t*)nullptr, (int32_t*)nullptr);
+  SKT_LAUNCH_CHECK();
+  st = cub_call([&](void* t, size_t& b) {
+    return hipcub::DeviceScan::ExclusiveSum(t, b, counts, f->row_ptr, (int)(nd + 1), stream);
+  }, stream);
+  if (st) return st;
+  int64_t nnz = 0;
+  SKT_HIP(hipMemcpy(&nnz, f->row_ptr + nd, sizeof(int64_t), hipMemcpyDeviceToHost));
+  SKT_CHECK(nnz >= 1, "No neighbor relations found; check support_radius or design_mask.");
+  SKT_CHECK(nnz < (int64_t)1 << 40, "filter too large");
+  f->nnz = nnz;
+  SKT_HIP(hipMalloc(&f->gcol, sizeof(int32_t) * nnz));
+  SKT_HIP(hipMalloc(&f->w, sizeof(double) * nnz));
+  SKT_HIP(hipMalloc(&f->wt, sizeof(double) * nnz));
+  hipLaunchKernelGGL(k_neighbours<true>, dim3(gd), dim3(kBlock), 0, stream, n_el, nd,
+                     centers, f->design_ids, f->colmap, cell, ids_sorted, cstart, cend,
+                     g[0], g[1], g[2], r2, (int64_t*)nullptr, f->row_ptr, f->gcol);
+  hipLaunchKernelGGL(k_sort_weights, dim3(gd), dim3(kBlock), 0, stream, n_el, nd,
+                     centers, f->design_ids, f->row_ptr, f->gcol, sigma, f->w);
+  hipLaunchKernelGGL(k_transpose_values, dim3(gd), dim3(kBlock), 0, stream, nd,
+                     f->design_ids, f->colmap, f->row_ptr, f->gcol, f->w, f->wt);
+  SKT_LAUNCH_CHECK();
+  SKT_HIP(hipMalloc(&f->cand_full, sizeof(double) * n_el));
+  SKT_HIP(hipMalloc(&f->part, sizeof(double) * kPartMax));
+  SKT_HIP(hipMalloc(&f->state, sizeof(double) * 16));
+  SKT_HIP(hipStreamSynchronize(stream));
+  (void)hipFree(cell);
+  (void)hipFree(cell_sorted);
+  (void)hipFree(ids);
+  (void)hipFree(ids_sorted);
+  (void)hipFree(cstart);
+  (void)hipFree(cend);
+  (void)hipFree(flag);
+  (void)hipFree(pos);
+  (void)hipFree(counts);
+  *out = f;
+  return SKT_OK;
+}
+
+int skt_filter_destroy(skt_filter* f) {
+  if (!f) return SKT_OK;
+  (void)hipFree(f->design_ids);
+  (void)hipFree(f->colmap);
+  (void)hipFree(f->row_ptr);
+  (void)hipFree(f->gcol);
+  (void)hipFree(f->w);
+  (void)hipFree(f->wt);
+  (void)hipFree(f->cand_full);
+  (void)hipFree(f->part);
+  (void)hipFree(f->state);
+  delete f;
+  return SKT_OK;
+}
+
+int skt_filter_info(const skt_filter* f, int64_t* n_design, int64_t* nnz) {
+  SKT_CHECK(f, "null filter");
+  if (n_design) *n_design = f->n_design;
+  if (nnz) *nnz = f->nnz;
+  return SKT_OK;
+}
+
+__global__ void k_export_cols(int64_t nnz, const int32_t* __restrict__ colmap,
+                              const int32_t* __restrict__ gcol, int32_t* __restrict__ out) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nnz;
+       q += (int64_t)gridDim.x * blockDim.x)
+    out[q] = colmap[gcol[q]];
+}
+
+int skt_filter_export(const skt_filter* f, int64_t* indptr, int32_t* indices,
+                      double* values, int32_t* design_ids, hipStream_t stream) {
+  SKT_CHECK(f, "null filter");
+  if (indptr)
+    SKT_HIP(hipMemcpyAsync(indptr, f->row_ptr, sizeof(int64_t) * (f->n_design + 1),
+                           hipMemcpyDeviceToDevice, stream));
+  if (values)
+    SKT_HIP(hipMemcpyAsync(values, f->w, sizeof(double) * f->nnz,
+                           hipMemcpyDeviceToDevice, stream));
+  if (design_ids)
+    SKT_HIP(hipMemcpyAsync(design_ids, f->design_ids, sizeof(int32_t) * f->n_design,
+                           hipMemcpyDeviceToDevice, stream));
+  if (indices) {
+    hipLaunchKernelGGL(k_export_cols, dim3(grid_for(f->nnz)), dim3(kBlock), 0, stream,
+                       f->nnz, f->colmap, f->gcol, indices);
+    SKT_LAUNCH_CHECK();
+  }
+  return SKT_OK;
+}
+
+int skt_filter_forward(skt_filter* f, const double* x, double* y, hipStream_t stream) {
+  SKT_CHECK(f && x && y, "null pointer");
+  SKT_CHECK(x != y, "filter forward cannot run in place");
+  hipLaunchKernelGGL(k_csr_apply, dim3(grid_for(f->n_el * kSeg)), dim3(kBlock), 0, stream,
+                     f->n_el, f->colmap, f->row_ptr, f->gcol, f->w, x, y, 0);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_filter_gradient(skt_filter* f, const double* v, double* g, hipStream_t stream) {
+  SKT_CHECK(f && v && g, "null pointer");
+  SKT_CHECK(v != g, "filter gradient cannot run in place");
+  hipLaunchKernelGGL(k_csr_apply, dim3(grid_for(f->n_el * kSeg)), dim3(kBlock), 0, stream,
+                     f->n_el, f->colmap, f->row_ptr, f->gcol, f->wt, v, g, 1);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_filter_gather_design(skt_filter* f, const double* full, double* design,
+                             hipStream_t stream) {
+  SKT_CHECK(f && full && design, "null pointer");
+  hipLaunchKernelGGL(k_gather_design, dim3(grid_for(f->n_design)), dim3(kBlock), 0, stream,
+                     f->n_design, f->design_ids, full, design);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_filter_scatter_design(skt_filter* f, const double* design, double* full,
+                              hipStream_t stream) {
+  SKT_CHECK(f && full && design, "null pointer");
+  hipLaunchKernelGGL(k_scatter_design, dim3(grid_for(f->n_design)), dim3(kBlock), 0, stream,
+                     f->n_design, f->design_ids, design, full);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_oc_bisect(skt_filter* f, const double* rho_full, const double* rho_e,
+                  const double* dC, const double* vol, double vol_sum,
+                  double vol_frac, double move_limit, double eta,
+                  double rho_min, double rho_max, double rate_min,
+                  double rate_max, double beta, double beta_eta, double l1,
+                  double l2, double tolerance, double vol_tol, int max_iter,
+                  double* rho_out, double* result, hipStream_t stream) {
+  SKT_CHECK(f && rho_full && rho_e && dC && vol && rho_out && result, "null pointer");
+  SKT_CHECK(l1 < l2, "lambda_lower must be < lambda_upper");
+  SKT_CHECK(tolerance > 0.0 && max_iter >= 0, "bad bisection limits");
+  // number of evaluations is bounded by the bracket halving to `tolerance`
+  const double w0 = l2 - l1;
+  int steps = (int)std::ceil(std::log2(w0 / tolerance)) + 3;
+  if (steps < 1) steps = 1;
+  if (steps > max_iter + 1) steps = max_iter + 1;
+  const double tbe = std::tanh(beta * beta_eta);
+  const double denom = std::tanh(beta * beta_eta) + std::tanh(beta * (1.0 - beta_eta)) + 1e-12;
+  const int64_t nd = f->n_design;
+  unsigned nb = grid_for(nd * kSeg, kBlock, kPartMax);
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(f->n_el)), dim3(kBlock), 0, stream, f->n_el,
+                     rho_full, f->cand_full);
+  hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(1), 0, stream, f->state, l1, l2);
+  for (int s = 0; s < steps; ++s) {
+    hipLaunchKernelGGL(k_oc_candidate, dim3(grid_for(nd)), dim3(kBlock), 0, stream, nd,
+                       f->design_ids, rho_e, dC, f->state, 1e-12, eta, rate_min, rate_max,
+                       move_limit, rho_min, rho_max, f->cand_full, rho_out);
+    hipLaunchKernelGGL(k_oc_volume, dim3(nb), dim3(kBlock), 0, stream, nd, f->design_ids,
+                       f->row_ptr, f->gcol, f->w, f->cand_full, vol, beta, beta_eta, tbe,
+                       denom, f->state, f->part);
+    hipLaunchKernelGGL(k_oc_bracket, dim3(1), dim3(kBlock), 0, stream, (int)nb, f->part,
+                       vol_sum, vol_frac, vol_tol, tolerance, max_iter, f->state);
+  }
+  hipLaunchKernelGGL(k_oc_result, dim3(1), dim3(1), 0, stream, f->state, result);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+"""Matrix-free GPU FEA with the reference's call surface.
+
+``FEM_SimpLinearElasticity`` mirrors ``fea/solver_elastic.py:540-676`` and
+``FEM_SimpLinearHeatConduction`` mirrors ``fea/solver_heat.py:628-980``
+(``objective='compliance'``, no Robin terms):
+
+* ``objectives_multi_load(rho, p, u_dofs, timer=None, force_scale=1.0)``
+  solves ``K(E(rho)) u_k = f_k`` for every load case, writes ``u_dofs``
+  (ndof, n_tasks) in place and returns the per-case objective;
+* ``energy_multi_load(rho, p, u_dofs)`` returns element energies
+  ``1/2 E_e u_e^T K_e^0 u_e`` (n_el, n_tasks);
+* the heat class also has ``compliance_sensitivity_multi_load``.
+
+Arguments may be float64 device tensors (the device-resident path) or numpy
+arrays (copied to the GPU and back, ``u_dofs`` updated in place) so the
+reference's own driver can call it unchanged.
+
+The solve is a batched Jacobi-preconditioned CG (all load cases share each
+operator apply): ``precision="mixed"`` runs float32 inner iterations inside
+a float64 iterative refinement, ``precision="fp64"`` runs float64 CG.  The
+reference's solver names are accepted; all of them select this solve.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import logging
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import _dev, _lib
+from .composer import interpolation_kind, simp_interpolation
+
+logger = logging.getLogger(__name__)
+
+_SOLVER_NAMES = ("auto", "spsolve", "cg_jacobi", "cg_pyamg", "petsc", "petsc_spdirect", "pcg")
+
+
+@dataclass(frozen=True)
+class LinearSolverConfig:
+    """``fea/solver_elastic.py:34-40`` plus the GPU solve's own knobs."""
+    solver: str = "auto"
+    rtol: float = 1e-8
+    maxiter: int | None = None
+    petsc_options: object = None
+    allow_fallback_to_spsolve: bool = False
+    precision: str = "mixed"          # "mixed" (fp32 inner + fp64 refinement) or "fp64"
+    inner_rtol: float = 1e-3          # relative tolerance of each fp32 inner solve
+    warm_start: bool = True           # start from the previous u_dofs
+    on_noconv: str = "raise"          # "raise" or "warn"
+
+
+def normalize_linear_solver_config(solver="auto", rtol=1e-8, maxiter=None,
+                                   petsc_options=None, allow_fallback_to_spsolve=False,
+                                   **kw) -> LinearSolverConfig:
+    if isinstance(solver, LinearSolverConfig):
+        return solver
+    if solver not in _SOLVER_NAMES:
+        raise ValueError(f"Unknown solver: {solver}")
+    return LinearSolverConfig(solver=solver, rtol=rtol, maxiter=maxiter,
+                              petsc_options=petsc_options,
+                              allow_fallback_to_spsolve=allow_fallback_to_spsolve, **kw)
+
+
+class _GpuFEM:
+    physics = _lib.ELASTIC
+
+    def __init__(self, task, E_min_coeff, density_interpolation=simp_interpolation,
+                 solver_config=None, solver_option="spsolve", quad_order=2):
+        self.task = task
+        self.kind = interpolation_kind(density_interpolation)
+        self.density_interpolation = density_interpolation
+        cfg = solver_config if solver_config is not None else \
+            normalize_linear_solver_config(solver_option)
+        if cfg.solver not in _SOLVER_NAMES:
+            raise ValueError(f"Unknown solver: {cfg.solver}")
+        if cfg.precision not in ("mixed", "fp64"):
+            raise ValueError("precision must be 'mixed' or 'fp64'")
+        self.solver_config = cfg
+        self.solver_option = cfg.solver
+        dev = _dev.device()
+        self.device = dev
+        mesh = task.mesh
+        fixed = torch.as_tensor(task.dof_fixed_mask(), device=dev)
+        h = C.c_void_p()
+        nu = float(getattr(task, "nu", 0.0))
+        if mesh.kind == "hex8":
+            nx, ny, nz = mesh.grid
+            hx, hy, hz = mesh.h
+            _lib.call("skt_hex8_create", self.physics, nx, ny, nz, hx, hy, hz, nu,
+                      int(quad_order), _dev.ptr(fixed), C.byref(h))
+        else:
+            coords = _dev.to_dev(mesh.p)
+            conn = _dev.to_dev(mesh.t, torch.int32)
+            _lib.call("skt_tet4_create", self.physics, mesh.nvertices, mesh.nelements,
+                      _dev.ptr(coords), _dev.ptr(conn), nu, _dev.ptr(fixed), C.byref(h))
+        self._op = h
+        ndof = C.c_int64()
+        nel = C.c_int64()
+        ncomp = C.c_int()
+        _lib.call("skt_operator_info", h, C.byref(ndof), C.byref(nel), C.byref(ncomp))
+        self.ndof, self.n_el, self.ncomp = ndof.value, nel.value, ncomp.value
+        self.n_tasks = task.n_tasks
+        s = C.c_void_p()
+        _lib.call("skt_solver_create", h, self.n_tasks, C.byref(s))
+        self._solver = s
+        self.E64 = torch.empty(self.n_el, dtype=torch.float64, device=dev)
+        self.E32 = torch.empty(self.n_el, dtype=torch.float32, device=dev)
+        self.F0 = self._build_rhs().contiguous()        # enforced RHS, (ndof, k)
+        self.F = self.F0.clone()
+        self._force_scale = 1.0
+        self.last_solve = {}
+        self.solve_history = []
+
+    def __del__(self):
+        try:
+            if getattr(self, "_solver", None):
+                _lib.load().skt_solver_destroy(self._solver)
+            if getattr(self, "_op", None):
+                _lib.load().skt_operator_destroy(self._op)
+        except Exception:
+            pass
+
+    # --- helpers -------------------------------------------------------
+    def _moduli(self, rho, p):
+        emin, emax = self.E_min, self.E_max
+        r = rho
+        _lib.call("skt_interpolate", self.kind, self.n_el, _dev.ptr(r), emax, emin, float(p),
+                  _dev.ptr(self.E64), _lib.F64, _dev.stream())
+        if self.solver_config.precision == "mixed":
+            _lib.call("skt_interpolate", self.kind, self.n_el, _dev.ptr(r), emax, emin,
+                      float(p), _dev.ptr(self.E32), _lib.F32, _dev.stream())
+
+    def _maxiter(self):
+        m = self.solver_config.maxiter
+        return int(m) if m is not None else 200000
+
+    def solve(self, U: torch.Tensor):
+        """Solve K(E64) U = F in place (U is the warm start)."""
+        cfg = self.solver_config
+        if not cfg.warm_start:
+            U.zero_()
+        mode = _lib.F32 if cfg.precision == "mixed" else _lib.F64
+        iters = (C.c_int64 * 2)()
+        relres = (C.c_double * self.n_tasks)()
+        st = _lib.load().skt_solver_solve(
+            self._solver, mode, self.n_tasks, _dev.ptr(self.E64),
+            _dev.ptr(self.E32) if mode == _lib.F32 else None, _dev.ptr(self.F), _dev.ptr(U),
+            float(cfg.rtol), self._maxiter(), float(cfg.inner_rtol), iters, relres,
+            _dev.stream())
+        self.last_solve = dict(iters=int(iters[0]), outer=int(iters[1]),
+                               relres=list(relres), converged=(st == _lib.SKT_OK))
+        self.solve_history.append(self.last_solve)
+        if st == _lib.SKT_ENOCONV and cfg.on_noconv == "warn":
+            logger.warning("state solve did not converge: %s", _lib.last_error())
+        else:
+            _lib.check(st, "skt_solver_solve")
+
+    def _set_force_scale(self, force_scale):
+        if force_scale != self._force_scale:
+            torch.mul(self.F0, float(force_scale), out=self.F)
+            self._force_scale = force_scale
+
+    def energy(self, rho, p, U):
+        """(n_el, k) float64 element energies on the device."""
+        self._moduli(rho, p)
+        out = torch.empty((self.n_el, self.n_tasks), dtype=torch.float64, device=self.device)
+        _lib.call("skt_operator_energy", self._op, _lib.F64, self.n_tasks, _dev.ptr(self.E64),
+                  _dev.ptr(U), _dev.ptr(out), _dev.stream())
+        return out
+
+    def col_reduce(self, op, X, Y=None):
+        out = torch.empty(self.n_tasks, dtype=torch.float64, device=self.device)
+        _lib.call("skt_reduce_cols", op, X.shape[0], self.n_tasks, _dev.ptr(X),
+                  _dev.ptr(Y) if Y is not None else None, _dev.ptr(out), _dev.stream())
+        return out
+
+    def apply(self, X, E=None, dtype=torch.float64):
+        """Y = K_e(E) X on the device (E defaults to the last moduli)."""
+        Y = torch.empty_like(X)
+        code = _lib.F64 if dtype == torch.float64 else _lib.F32
+        if E is None:
+            E = self.E64 if code == _lib.F64 else self.E32
+        ncols = 1 if X.dim() == 1 else X.shape[1]
+        _lib.call("skt_operator_apply", self._op, code, ncols, _dev.ptr(E), _dev.ptr(X),
+                  _dev.ptr(Y), _dev.stream())
+        return Y
+
+    # --- host/device dispatch -----------------------------------------
+    def _objectives_dev(self, rho, p, U, force_scale):
+        raise NotImplementedError
+
+    def objectives_multi_load(self, rho, p, u_dofs, timer=None, force_scale=1.0):
+        if _dev.is_host(rho) or _dev.is_host(u_dofs):
+            r = _dev.to_dev(rho)
+            U = _dev.to_dev(u_dofs).reshape(self.ndof, self.n_tasks).contiguous()
+            out = self._objectives_dev(r, p, U, force_scale)
+            u_dofs[...] = U.cpu().numpy().reshape(np.shape(u_dofs))
+            return out.cpu().numpy()
+        return self._objectives_dev(rho, p, u_dofs, force_scale)
+
+    def energy_multi_load(self, rho, p, u_dofs):
+        if _dev.is_host(rho) or _dev.is_host(u_dofs):
+            r = _dev.to_dev(rho)
+            U = _dev.to_dev(u_dofs).reshape(self.ndof, self.n_tasks).contiguous()
+            return self.energy(r, p, U).cpu().numpy()
+        return self.energy(rho, p, u_dofs)
+
+
+class FEM_SimpLinearElasticity(_GpuFEM):
+    """GPU counterpart of ``fea/solver_elastic.py:540-676``."""
+    physics = _lib.ELASTIC
+
+    def __init__(self, task, E_min_coeff, density_interpolation=simp_interpolation,
+                 solver_config=None, solver_option="spsolve", petsc_options=None,
+                 quad_order=2):
+        self.E_max = task.E * 1.0
+        self.E_min = task.E * E_min_coeff
+        super().__init__(task, E_min_coeff, density_interpolation, solver_config,
+                         solver_option, quad_order)
+
+    def _build_rhs(self):
+        F = np.column_stack([np.asarray(f, dtype=np.float64) for f in self.task.neumann_linear])
+        F[self.task.dirichlet_dofs, :] = 0.0          # enforce: b_D = 0
+        return _dev.to_dev(F)
+
+    def _objectives_dev(self, rho, p, U, force_scale=1.0):
+        """compliance_k = F_e[:, k] . u_k (``solver_elastic.py:236,465``)."""
+        self._set_force_scale(force_scale)
+        self._moduli(rho, p)
+        self.solve(U)
+        return self.col_reduce(_lib.RED_DOT, self.F, U)
+
+
+class FEM_SimpLinearHeatConduction(_GpuFEM):
+    """GPU counterpart of ``fea/solver_heat.py:628-980`` for
+    ``objective='compliance'`` without Robin terms."""
+    physics = _lib.HEAT
+
+    def __init__(self, task, E_min_coeff, density_interpolation=simp_interpolation,
+                 solver_config=None, solver_option="spsolve", q=4, quad_order=2):
+        if getattr(task, "robin_coefficient", None) is not None:
+            raise NotImplementedError("Robin boundaries are not built on the GPU path")
+        if getattr(task, "objective", "compliance") != "compliance":
+            raise NotImplementedError("only objective='compliance' is built")
+        self.k_max = task.k * 1.0
+        self.k_min = task.k * E_min_coeff
+        self.E_max, self.E_min = self.k_max, self.k_min
+        self.q = q
+        super().__init__(task, E_min_coeff, density_interpolation, solver_config,
+                         solver_option, quad_order)
+
+    def _build_rhs(self):
+        cols = []
+        for val in self.task.dirichlet_values:
+            f = np.array(self.task.emit, dtype=np.float64)
+            f[self.task.dirichlet_dofs] = val            # enforce: b_D = x_D
+            cols.append(f)
+        return _dev.to_dev(np.column_stack(cols))
+
+    def _objectives_dev(self, rho, p, U, force_scale=1.0):
+        """``J_k = T_k^T K T_k`` with the unenforced K (``solver_heat.py:785-789``),
+        i.e. twice the sum of the element energies."""
+        self._moduli(rho, p)
+        self.solve(U)
+        en = torch.empty((self.n_el, self.n_tasks), dtype=torch.float64, device=self.device)
+        _lib.call("skt_operator_energy", self._op, _lib.F64, self.n_tasks, _dev.ptr(self.E64),
+                  _dev.ptr(U), _dev.ptr(en), _dev.stream())
+        return 2.0 * self.col_reduce(_lib.RED_SUM, en)
+
+    def compliance_sensitivity_multi_load(self, rho, p, u_dofs):
+        """Always the SIMP sensitivity of the energies
+        (``solver_heat.py:928-950``)."""
+        host = _dev.is_host(rho) or _dev.is_host(u_dofs)
+        r = _dev.to_dev(rho)
+        U = _dev.to_dev(u_dofs).reshape(self.ndof, self.n_tasks).contiguous()
+        en = self.energy(r, p, U)
+        out = torch.empty_like(en)
+        for k in range(self.n_tasks):
+            col = en[:, k].contiguous()
+            tmp = torch.empty_like(col)
+            _lib.call("skt_dC_drho", _lib.SIMP, self.n_el, _dev.ptr(r), _dev.ptr(col), 1,
+                      self.k_max, self.k_min, float(p), _dev.ptr(tmp), _dev.stream())
+            out[:, k] = tmp
+        return out.cpu().numpy() if host else out

@@ -1,0 +1,125 @@
+"""End-to-end parity: the GPU OC / LogMOC drivers against the FP64 oracle
+loop on the same small problems, plus full-vs-stepwise determinism."""
+import numpy as np
+import pytest
+
+from oracle import driver as odrv
+from oracle import problems as oprob
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_cfg(method="OC", vol=0.3, r=1.5, beta=1.0, emin=1e-9, iters=50, **kw):
+    S = odrv.Sched
+    return odrv.Config(method=method, max_iters=iters, p=S("Constant", 3.0, 3.0),
+                       vol_frac=S("Constant", vol, vol), beta=S("Constant", beta, beta),
+                       filter_radius=S("Constant", r, r), move_limit=S("Constant", 0.2, 0.2),
+                       E_min_coeff=emin, solver="spsolve", **kw)
+
+
+def _gpu_opt(task_o, method="OC", vol=0.3, r=1.5, beta=1.0, emin=1e-9, iters=50,
+             precision="fp64", rtol=1e-12, tmp="/tmp/skt", **kw):
+    from sktopt_amd import core, mesh as pmesh
+    from sktopt_amd.tools import SchedulerConfig as SC
+    pm = task_o.mesh
+    m = pmesh.structured_hex(*pm.grid, *pm.h) if pm.kind == "hex8" else pmesh.MeshTet(pm.p, pm.t)
+    if task_o.physics == "elastic":
+        t = pmesh.LinearElasticity(m, task_o.dirichlet_nodes, task_o.dirichlet_dofs,
+                                   task_o.forces, task_o.neumann_nodes, None, task_o.E0,
+                                   task_o.nu)
+    else:
+        t = pmesh.LinearHeatConduction(m, task_o.dirichlet_nodes, task_o.dirichlet_values,
+                                       task_o.forces[0], None, task_o.E0)
+    common = dict(dst_path=tmp, max_iters=iters, p=SC.constant(target_value=3.0),
+                  vol_frac=SC.constant(target_value=vol), beta=SC.constant(target_value=beta),
+                  filter_type="spacial", filter_radius=SC.constant(target_value=r),
+                  move_limit=SC.constant(target_value=0.2), E_min_coeff=emin,
+                  precision=precision, solver_rtol=rtol, **kw)
+    if method == "OC":
+        cfg = core.OC_Config(**common)
+        opt = core.OC_Optimizer(cfg, t)
+    else:
+        cfg = core.LogMOC_Config(**common)
+        opt = core.LogMOC_Optimizer(cfg, t)
+    opt.parameterize()
+    return opt
+
+
+@pytest.mark.parametrize("precision,rtol,ctol,dtol", [("fp64", 1e-13, 1e-9, 1e-8),
+                                                     ("mixed", 1e-10, 1e-7, 1e-5)])
+def test_oc_trajectory_matches_oracle(gpu, precision, rtol, ctol, dtol, tmp_path):
+    task = oprob.cantilever(16, 8, 8)
+    loop = odrv.DensityLoop(_oracle_cfg(), task)
+    opt = _gpu_opt(oprob.cantilever(16, 8, 8), precision=precision, rtol=rtol,
+                   tmp=str(tmp_path))
+    n = 6
+    for it in range(1, n + 1):
+        rec = loop.step(it)
+        opt.optimize_steps(1)
+        c = opt.recorder.latest("compliance")
+        assert abs(c - rec["compliance"]) <= ctol * abs(rec["compliance"]), (it, c, rec)
+        assert opt.recorder.latest("bisection_evals") == rec["n_evals"]
+        np.testing.assert_allclose(opt.recorder.latest("lmid"), rec["lmid"], rtol=1e-6)
+        rho_g = opt.rho_host()
+        assert np.abs(rho_g - loop.rho).max() <= dtol, it
+        np.testing.assert_allclose(opt.recorder.latest("kkt_residual"), rec["kkt_residual"],
+                                   rtol=1e-5)
+        np.testing.assert_allclose(opt.recorder.latest("rho_change_max"), rec["rho_change_max"],
+                                   atol=1e-7)
+
+
+def test_multiload_trajectory_matches_oracle(gpu, tmp_path):
+    task = oprob.multiload(12, 6, 6, n_loads=8, seed=0)
+    loop = odrv.DensityLoop(_oracle_cfg(vol=0.2, r=2.0), task)
+    opt = _gpu_opt(oprob.multiload(12, 6, 6, n_loads=8, seed=0), vol=0.2, r=2.0,
+                   precision="fp64", rtol=1e-13, tmp=str(tmp_path))
+    for it in range(1, 4):
+        rec = loop.step(it)
+        opt.optimize_steps(1)
+        np.testing.assert_allclose(opt.recorder.latest("compliance_each"),
+                                   rec["compliance_each"], rtol=1e-9)
+        assert np.abs(opt.rho_host() - loop.rho).max() <= 1e-8
+
+
+def test_logmoc_heat_trajectory_matches_oracle(gpu, tmp_path):
+    task = oprob.heat_patch(10, patch=4, source=1e-2)
+    cfg = _oracle_cfg(method="LogMOC", vol=0.3, r=2.0, emin=1e-3, eta=odrv.Sched("Constant", 0.6, 0.6),
+                      lambda_lower=-1e7, lambda_upper=1e7)
+    loop = odrv.DensityLoop(cfg, task)
+    opt = _gpu_opt(oprob.heat_patch(10, patch=4, source=1e-2), method="LogMOC", vol=0.3, r=2.0,
+                   emin=1e-3, precision="fp64", rtol=1e-13, tmp=str(tmp_path))
+    for it in range(1, 5):
+        rec = loop.step(it)
+        opt.optimize_steps(1)
+        c = opt.recorder.latest("compliance")
+        assert abs(c - rec["compliance"]) <= 1e-9 * abs(rec["compliance"])
+        np.testing.assert_allclose(opt.recorder.latest("lambda_v"), rec["lambda_v"], rtol=1e-7,
+                                   atol=1e-12)
+        assert np.abs(opt.rho_host() - loop.rho).max() <= 1e-8
+
+
+def test_tet_cantilever_matches_oracle(gpu, tmp_path):
+    task = oprob.cantilever(8, 4, 4, kind="tet4", jitter=0.2, seed=3)
+    loop = odrv.DensityLoop(_oracle_cfg(), task)
+    opt = _gpu_opt(oprob.cantilever(8, 4, 4, kind="tet4", jitter=0.2, seed=3),
+                   precision="fp64", rtol=1e-13, tmp=str(tmp_path))
+    for it in range(1, 4):
+        rec = loop.step(it)
+        opt.optimize_steps(1)
+        assert abs(opt.recorder.latest("compliance") - rec["compliance"]) <= \
+            1e-9 * rec["compliance"]
+        assert np.abs(opt.rho_host() - loop.rho).max() <= 1e-8
+
+
+def test_full_vs_stepwise_bit_identical(gpu, tmp_path):
+    """tests/test_optimize_steps_equivalence.py:49-52 of the reference."""
+    a = _gpu_opt(oprob.cantilever(10, 5, 5), iters=5, precision="mixed", rtol=1e-8,
+                 tmp=str(tmp_path / "a"))
+    a.optimize()
+    b = _gpu_opt(oprob.cantilever(10, 5, 5), iters=5, precision="mixed", rtol=1e-8,
+                 tmp=str(tmp_path / "b"))
+    for _ in range(5):
+        b.optimize_steps(1)
+    np.testing.assert_array_equal(a.rho_host(), b.rho_host())
+    np.testing.assert_array_equal(np.array(a.recorder.as_object().compliance),
+                                  np.array(b.recorder.as_object().compliance))
