@@ -123,3 +123,19 @@ def test_full_vs_stepwise_bit_identical(gpu, tmp_path):
     np.testing.assert_array_equal(a.rho_host(), b.rho_host())
     np.testing.assert_array_equal(np.array(a.recorder.as_object().compliance),
                                   np.array(b.recorder.as_object().compliance))
+
+
+@pytest.mark.parametrize("precision,rtol,ctol,dtol", [("fp64", 1e-12, 1e-9, 1e-7),
+                                                     ("mixed", 1e-9, 1e-6, 1e-4)])
+def test_c0_matches_golden_trajectory(gpu, precision, rtol, ctol, dtol, tmp_path):
+    """c0 (40x20x20 hex8, 54,243 DOF) against the committed FP64 oracle run."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "c0_oc_3iter.npz"))
+    opt = _gpu_opt(oprob.cantilever(40, 20, 20), precision=precision, rtol=rtol,
+                   tmp=str(tmp_path))
+    opt.optimize_steps(3)
+    c = np.array(opt.recorder.as_object().compliance)
+    np.testing.assert_allclose(c, g["compliance"], rtol=ctol)
+    assert list(opt.recorder.as_object().bisection_evals) == list(g["n_evals"])
+    rho = opt.rho_host()[g["design_elements"]]
+    assert np.abs(rho - g["rho_final"]).max() <= dtol

@@ -1,0 +1,83 @@
+"""Oracle driver: full-vs-stepwise bit identity (the reference's own
+determinism test, tests/test_optimize_steps_equivalence.py:49-52), the
+initial-density rule, and the committed c0 golden trajectory."""
+import os
+
+import numpy as np
+
+from oracle import driver as od
+from oracle import problems as op
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "c0_oc_3iter.npz")
+
+
+def _cfg(iters=5, method="OC"):
+    S = od.Sched
+    return od.Config(method=method, max_iters=iters, p=S("Constant", 3.0, 3.0),
+                     vol_frac=S("Constant", 0.3, 0.3), beta=S("Constant", 1.0, 1.0),
+                     filter_radius=S("Constant", 1.5, 1.5), move_limit=S("Constant", 0.2, 0.2),
+                     E_min_coeff=1e-9)
+
+
+def test_full_vs_stepwise_bit_identical():
+    a = od.DensityLoop(_cfg(), op.cantilever(8, 4, 4))
+    a.optimize()
+    b = od.DensityLoop(_cfg(), op.cantilever(8, 4, 4))
+    for _ in range(5):
+        b.optimize_steps(1)
+    np.testing.assert_array_equal(a.rho, b.rho)
+    np.testing.assert_array_equal([h["compliance"] for h in a.history],
+                                  [h["compliance"] for h in b.history])
+
+
+def test_initial_density_rule():
+    task = op.cantilever(8, 4, 4)
+    loop = od.DensityLoop(_cfg(), task)
+    rho = loop.rho
+    assert np.all(rho[task.dirichlet_neumann_elements] == 1.0)
+    assert np.all(rho[task.design_elements] == 0.3)
+    # design excludes elements touching the clamp and the load nodes
+    assert not np.isin(task.dirichlet_elements, task.design_elements).any()
+    assert not np.isin(task.neumann_elements, task.design_elements).any()
+
+
+def test_compliance_decreases_and_volume_holds():
+    loop = od.DensityLoop(_cfg(iters=6), op.cantilever(12, 6, 6))
+    loop.optimize()
+    c = [h["compliance"] for h in loop.history]
+    assert all(b < a for a, b in zip(c, c[1:]))
+    assert all(abs(h["vol_error"]) < 1e-4 or h["n_evals"] == 41 for h in loop.history)
+
+
+def test_logmoc_runs_and_tracks_volume():
+    task = op.heat_patch(8, patch=3)
+    S = od.Sched
+    cfg = od.Config(method="LogMOC", max_iters=8, p=S("Constant", 3.0, 3.0),
+                    vol_frac=S("Constant", 0.3, 0.3), beta=S("Constant", 1.0, 1.0),
+                    filter_radius=S("Constant", 2.0, 2.0), move_limit=S("Constant", 0.2, 0.2),
+                    eta=S("Constant", 0.6, 0.6), E_min_coeff=1e-3, lambda_lower=-1e7,
+                    lambda_upper=1e7)
+    loop = od.DensityLoop(cfg, task)
+    loop.optimize()
+    assert np.isfinite([h["compliance"] for h in loop.history]).all()
+    assert loop.rho.min() >= 0.01 and loop.rho.max() <= 1.0
+    # EMA dual ascent (logmoc.py:186-200): lambda_v = 0.9 lambda + 0.1 * 5 vol_error
+    for a, b in zip(loop.history, loop.history[1:]):
+        np.testing.assert_allclose(b["lambda_v"], 0.9 * a["lambda_v"] + 0.1 * 5.0 * b["vol_error"],
+                                   rtol=1e-12)
+
+
+def test_golden_c0_trajectory_reproduces():
+    """The committed FP64 trajectory (tests/golden/make_golden.py) is what the
+    oracle computes today (regression pin for the GPU parity tests)."""
+    g = np.load(GOLDEN)
+    import sys
+    sys.path.insert(0, os.path.dirname(GOLDEN))
+    from make_golden import c0_config
+    task = op.cantilever(40, 20, 20)
+    loop = od.DensityLoop(c0_config(), task)
+    for it in range(1, 4):
+        rec = loop.step(it)
+        np.testing.assert_allclose(rec["compliance"], g["compliance"][it - 1], rtol=1e-10)
+        assert rec["n_evals"] == g["n_evals"][it - 1]
+    np.testing.assert_allclose(loop.rho[task.design_elements], g["rho_final"], atol=1e-10)
