@@ -1,0 +1,355 @@
+// Structured hex8 operator apply in the modal (Walsh-Hadamard) basis of the
+// trilinear element, LDS-tiled and streamed along z.
+//
+// For an axis-aligned box element the element matrix becomes very sparse in
+// the basis of the 8 corner sign patterns (1, x, y, z, xy, yz, xz, xyz):
+// per element and column y_e = E_e * H (Kmod (H^T u_e)), with H the +-1
+// butterfly (24 add/sub per component each way) and Kmod a 21x21 matrix with
+// 33 distinct coefficients (elasticity) or a 7-entry diagonal (conduction),
+// instead of a dense 24x24 (8x8) product.
+//
+// Tiling: a workgroup owns a 15x15 patch of nodes in x-y and a chunk of node
+// planes in z.  It walks the element layers bottom-up: each step loads one
+// new node plane (17x17 with halo) into LDS, 256 threads compute the 16x16
+// elements of the layer that touch the patch, park the 8 corner results of
+// each element in LDS, and every owned node sums its 4 contributions from
+// the layer below and 4 from the layer above in a fixed order (no atomics,
+// bit-reproducible).  Completed planes are written out coalesced.
+#include "operator.hpp"
+
+namespace skt {
+
+constexpr int kTE = 16;          // elements per tile edge (one ghost element)
+constexpr int kTN = kTE - 1;     // owned nodes per tile edge
+constexpr int kPW = kTE + 1;     // node plane width incl. halo
+constexpr int kPN = kPW * kPW;
+
+// LDS slot of halo-plane node (lx, ly), lx, ly in [0, 17): columns 0..15 at
+// lx + 16 ly, column 16 packed after them.  A half-wavefront reads two
+// element rows (16 consecutive lx each, rows ly and ly + 1), which land in
+// disjoint bank ranges with a 16-word row pitch.
+__device__ __forceinline__ int node_slot(int lx, int ly) {
+  return lx < 16 ? lx + 16 * ly : 16 * kPW + ly;
+}
+
+// Value of the next lane (i + 1) within each row of 16 lanes (DPP
+// row_shl:1); lane 15 of a row receives 0.  Element lanes are laid out 16 per
+// x-row, so this fetches the right-hand neighbour element's result.
+__device__ __forceinline__ float from_right(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x101, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double from_right(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x101, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x101, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// 3-stage Walsh-Hadamard butterfly on 8 values indexed by corner bits
+// (bit 0: x, bit 1: y, bit 2: z): out[S] = sum_i (-1)^popcount(i & S) v[i].
+// The matrix is symmetric and squares to 8 I, so the same butterfly maps
+// modal forces back to corners.
+template <typename T>
+__device__ __forceinline__ void wht8(T v[8]) {
+#pragma unroll
+  for (int b = 1; b < 8; b <<= 1) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (i & b) continue;
+      const T p = v[i] + v[i | b];
+      const T q = v[i] - v[i | b];
+      v[i] = p;
+      v[i | b] = q;
+    }
+  }
+}
+
+// f = Kmod m for elasticity, modes indexed by bits (1=x 2=y 4=z 3=xy 6=yz
+// 5=xz 7=xyz), m[c][S] component c.  Coefficient layout (see host packer):
+// lin[6] (x0,y1,z2 sym: 00 11 22 01 02 12), sxy[3] {(x,1),(y,0)}: aa bb ab,
+// sxz[3] {(x,2),(z,0)}, syz[3] {(y,2),(z,1)}, b1[3] {(xy,0),(yz,2)},
+// b2[3] {(xy,1),(xz,2)}, b3[3] {(yz,1),(xz,0)},
+// b4[6] {(xy,2),(yz,0),(xz,1)} sym: 00 11 22 01 02 12, t[3] (xyz,c).
+template <typename T>
+__device__ __forceinline__ void modal_elastic(const T* __restrict__ c, const T m[3][8],
+                                              T f[3][8]) {
+  {
+    const T a = m[0][1], b = m[1][2], d = m[2][4];
+    f[0][1] = c[0] * a + c[3] * b + c[4] * d;
+    f[1][2] = c[3] * a + c[1] * b + c[5] * d;
+    f[2][4] = c[4] * a + c[5] * b + c[2] * d;
+  }
+#define SKT_PAIR(o, ra, sa, rb, sb)                    \
+  {                                                    \
+    const T a = m[ra][sa], b = m[rb][sb];              \
+    f[ra][sa] = c[o] * a + c[o + 2] * b;               \
+    f[rb][sb] = c[o + 2] * a + c[o + 1] * b;           \
+  }
+  SKT_PAIR(6, 1, 1, 0, 2)    // sxy
+  SKT_PAIR(9, 2, 1, 0, 4)    // sxz
+  SKT_PAIR(12, 2, 2, 1, 4)   // syz
+  SKT_PAIR(15, 0, 3, 2, 6)   // b1
+  SKT_PAIR(18, 1, 3, 2, 5)   // b2
+  SKT_PAIR(21, 1, 6, 0, 5)   // b3
+#undef SKT_PAIR
+  {
+    const T a = m[2][3], b = m[0][6], d = m[1][5];
+    f[2][3] = c[24] * a + c[27] * b + c[28] * d;
+    f[0][6] = c[27] * a + c[25] * b + c[29] * d;
+    f[1][5] = c[28] * a + c[29] * b + c[26] * d;
+  }
+  f[0][7] = c[30] * m[0][7];
+  f[1][7] = c[31] * m[1][7];
+  f[2][7] = c[32] * m[2][7];
+  f[0][0] = f[1][0] = f[2][0] = T(0);
+}
+
+template <typename T>
+__device__ __forceinline__ void modal_heat(const T* __restrict__ c, const T m[1][8], T f[1][8]) {
+  f[0][0] = T(0);
+#pragma unroll
+  for (int s = 1; s < 8; ++s) f[0][s] = c[s - 1] * m[0][s];
+}
+
+template <typename T>
+struct ModalArgs {
+  T c[40];
+};
+
+// LDS row stride for a structure-of-arrays buffer of n entries holding
+// `combos` interleaved (component, column) rows: odd and ~32/combos mod 32
+// so the combos written by neighbouring lanes land in different banks.
+constexpr int soa_stride(int n, int combos) {
+  const int want = ((32 / combos) | 1) % 32;
+  int s = n;
+  while (s % 32 != want) ++s;
+  return s;
+}
+
+template <typename T, int NC, int KC>
+__global__ void __launch_bounds__(256)
+k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int k, int lz, int tiles_x,
+                  int tiles_y, int n_zchunks, ModalArgs<T> coef,
+                  const T* __restrict__ E, const T* __restrict__ X, T* __restrict__ Y) {
+  // LDS, structure-of-arrays so that consecutive lanes touch consecutive
+  // words: two rotating node planes [NC*KC][kPS]; per element lane the
+  // x-paired corner sums [side][cy][NC][KC][256] (side 0: node plane below the
+  // element, 1: above; cy: node row); the finished output plane [NC*KC][kOS].
+  constexpr int kPS = soa_stride(kPN, NC * KC);
+  constexpr int kOS = soa_stride(kTN * kTN, NC * KC);
+  constexpr int kPlane = kPS * NC * KC;
+  constexpr int kItems = kPN * NC * KC;
+  constexpr int kPre = (kItems + 255) / 256;
+  constexpr int kOut = kTN * kTN * NC * KC;
+  constexpr int kWr = (kOut + 255) / 256;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* planes = reinterpret_cast<T*>(smem_raw);
+  T* pairs = planes + 2 * kPlane;
+  T* oplane = pairs + 4 * NC * KC * 256;
+  const int64_t nnx = nx + 1, nny = ny + 1, nnz = nz + 1;
+  const int64_t plane_stride = nnx * nny * NC * k;
+  int b = blockIdx.x;
+  const int tx = b % tiles_x; b /= tiles_x;
+  const int ty = b % tiles_y; b /= tiles_y;
+  const int zc = b % n_zchunks; b /= n_zchunks;
+  const int c0 = b * KC;
+  const int64_t x0 = (int64_t)tx * kTN, y0 = (int64_t)ty * kTN;
+  const int64_t z0 = (int64_t)zc * lz;
+  const int64_t z1 = (z0 + lz < nnz) ? z0 + lz : nnz;
+  const int tid = threadIdx.x;
+  const int ex_l = tid % kTE, ey_l = tid / kTE;
+  const int64_t ex = x0 - 1 + ex_l, ey = y0 - 1 + ey_l;
+  const bool el_in = ex >= 0 && ex < nx && ey >= 0 && ey < ny;
+  const int64_t e_base = el_in ? ex + nx * ey : 0;
+  const bool is_node = tid < kTN * kTN;
+  const int nx_l = tid % kTN, ny_l = tid / kTN;
+  const int e_lo = nx_l + kTE * ny_l;  // element lane holding row ny_l's pair
+
+  int f_off[kPre], f_lds[kPre];
+#pragma unroll
+  for (int r = 0; r < kPre; ++r) {
+    const int i = tid + 256 * r;
+    f_off[r] = -1;
+    f_lds[r] = -1;
+    if (i < kItems) {
+      const int col = i % KC, comp = (i / KC) % NC, nl = i / (KC * NC);
+      const int64_t gx = x0 - 1 + nl % kPW, gy = y0 - 1 + nl / kPW;
+      f_lds[r] = (comp * KC + col) * kPS + node_slot(nl % kPW, nl / kPW);
+      if (gx >= 0 && gx < nnx && gy >= 0 && gy < nny)
+        f_off[r] = (int)(((gy * nnx + gx) * NC + comp) * k + c0 + col);
+    }
+  }
+  int w_off[kWr], w_lds[kWr];
+#pragma unroll
+  for (int r = 0; r < kWr; ++r) {
+    const int i = tid + 256 * r;
+    w_off[r] = -1;
+    w_lds[r] = 0;
+    if (i < kOut) {
+      const int col = i % KC, comp = (i / KC) % NC, nl = i / (KC * NC);
+      const int64_t gx = x0 + nl % kTN, gy = y0 + nl / kTN;
+      w_lds[r] = (comp * KC + col) * kOS + nl;
+      if (gx < nnx && gy < nny) w_off[r] = (int)(((gy * nnx + gx) * NC + comp) * k + c0 + col);
+    }
+  }
+
+  T pre[kPre];
+  auto fetch = [&](int64_t gz) {
+    const bool zin = gz >= 0 && gz < nnz;
+    const T* Xp = X + (zin ? gz : 0) * plane_stride;
+#pragma unroll
+    for (int r = 0; r < kPre; ++r) pre[r] = (zin && f_off[r] >= 0) ? Xp[f_off[r]] : T(0);
+  };
+  auto stash = [&](T* dst) {
+#pragma unroll
+    for (int r = 0; r < kPre; ++r)
+      if (f_lds[r] >= 0) dst[f_lds[r]] = pre[r];
+  };
+
+  T partial[KC][NC];
+#pragma unroll
+  for (int c = 0; c < KC; ++c)
+#pragma unroll
+    for (int d = 0; d < NC; ++d) partial[c][d] = T(0);
+
+  int bot = 0, top = 1;
+  fetch(z0 - 1);
+  stash(planes + bot * kPlane);
+  fetch(z0);
+  stash(planes + top * kPlane);
+  for (int64_t ez = z0 - 1; ez < z1; ++ez) {
+    fetch(ez + 2);  // in flight while this layer is computed
+    const T Ee = (el_in && ez >= 0 && ez < nz) ? E[e_base + nx * ny * ez] : T(0);
+    __syncthreads();
+    const T* pb = planes + bot * kPlane;
+    const T* pt = planes + top * kPlane;
+#pragma unroll
+    for (int col = 0; col < KC; ++col) {
+      T m[NC][8];
+#pragma unroll
+      for (int idx = 0; idx < 8; ++idx) {
+        const int cx = idx & 1, cy = (idx >> 1) & 1, cz = idx >> 2;
+        const int nl = node_slot(ex_l + cx, ey_l + cy);
+        const T* p = cz ? pt : pb;
+#pragma unroll
+        for (int d = 0; d < NC; ++d) m[d][idx] = p[(d * KC + col) * kPS + nl] * Ee;
+      }
+#pragma unroll
+      for (int d = 0; d < NC; ++d) wht8(m[d]);
+      T f[NC][8];
+      if constexpr (NC == 3) modal_elastic(coef.c, m, f);
+      else modal_heat(coef.c, m, f);
+#pragma unroll
+      for (int d = 0; d < NC; ++d) wht8(f[d]);
+      // node (ex+1, ey+cy) of plane side cz gets this element's corner cx=1
+      // plus the right-hand element's corner cx=0
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {  // q = cy | cz << 1
+        const int cy = q & 1, cz = q >> 1;
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          const T v = f[d][1 | (cy << 1) | (cz << 2)] + from_right(f[d][(cy << 1) | (cz << 2)]);
+          pairs[(((cz * 2 + cy) * NC + d) * KC + col) * 256 + tid] = v;
+        }
+      }
+    }
+    __syncthreads();
+    if (is_node) {
+#pragma unroll
+      for (int col = 0; col < KC; ++col)
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          const T below = pairs[(((0 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
+                          pairs[(((0 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE];
+          const T above = pairs[(((1 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
+                          pairs[(((1 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE];
+          oplane[(d * KC + col) * kOS + tid] = partial[col][d] + below;
+          partial[col][d] = above;
+        }
+    }
+    __syncthreads();
+    if (ez >= z0) {
+      T* Yp = Y + ez * plane_stride;
+#pragma unroll
+      for (int r = 0; r < kWr; ++r)
+        if (w_off[r] >= 0) Yp[w_off[r]] = oplane[w_lds[r]];
+    }
+    // the bottom plane receives plane ez + 2 and becomes the new top
+    stash(planes + bot * kPlane);
+    const int t = bot;
+    bot = top;
+    top = t;
+  }
+}
+
+// Dirichlet rows of the enforced operator: y[d, :] = x[d, :].
+template <typename T>
+__global__ void k_fix_rows(int64_t n_fixed, const int32_t* __restrict__ dofs, int k,
+                           const T* __restrict__ X, T* __restrict__ Y) {
+  const int64_t total = n_fixed * k;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = (int64_t)dofs[t / k] * k + t % k;
+    Y[i] = X[i];
+  }
+}
+
+// Launch helper used by op_apply for structured grids whose element matrix
+// has the modal pattern.
+template <typename T, int NC, int KC>
+static int launch_modal(const skt_operator* op, int k, const T* coef, const T* E, const T* X,
+                        T* Y, hipStream_t s) {
+  SKT_CHECK((op->nx + 1) * (op->ny + 1) * NC * (int64_t)k < INT32_MAX,
+            "node plane too large for 32-bit in-plane offsets");
+  ModalArgs<T> a;
+  for (int i = 0; i < 40; ++i) a.c[i] = coef[i];
+  const int64_t nnx = op->nx + 1, nny = op->ny + 1, nnz = op->nz + 1;
+  const int tiles_x = (int)((nnx + kTN - 1) / kTN);
+  const int tiles_y = (int)((nny + kTN - 1) / kTN);
+  const int cblocks = k / KC;
+  // z chunk length: aim for >= 4 workgroups per CU (256 CUs), but keep the
+  // one-layer warm-up of each chunk below ~1/6 of its work
+  int64_t lz = nnz;
+  const int64_t per = (int64_t)tiles_x * tiles_y * cblocks;
+  while (lz > 6 && per * ((nnz + lz - 1) / lz) < 1024) lz = (lz + 1) / 2;
+  const int nzc = (int)((nnz + lz - 1) / lz);
+  const int64_t nblocks = per * nzc;
+  const size_t shmem = sizeof(T) * ((size_t)2 * soa_stride(kPN, NC * KC) * NC * KC +
+                                    (size_t)4 * NC * KC * 256 +
+                                    (size_t)soa_stride(kTN * kTN, NC * KC) * NC * KC);
+  static bool attr_set = false;
+  if (!attr_set) {
+    SKT_HIP(hipFuncSetAttribute((const void*)k_hex_apply_modal<T, NC, KC>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_hex_apply_modal<T, NC, KC>), dim3((unsigned)nblocks), dim3(256), shmem, s,
+                     op->nx, op->ny, op->nz, k, (int)lz, tiles_x, tiles_y, nzc, a, E, X, Y);
+  if (op->n_fixed > 0)
+    hipLaunchKernelGGL(k_fix_rows<T>, dim3(grid_for(op->n_fixed * k)), dim3(256), 0, s,
+                       op->n_fixed, op->fixed_list, k, X, Y);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+template <typename T>
+int hex_apply_modal(const skt_operator* op, int k, const T* coef, const T* E, const T* X, T* Y,
+                    hipStream_t s) {
+  // two columns per workgroup: more columns cut LDS occupancy below three
+  // workgroups per CU, fewer multiply the re-reads of each cache line
+  if (op->ncomp == 3) {
+    if (k % 2 == 0) return launch_modal<T, 3, 2>(op, k, coef, E, X, Y, s);
+    return launch_modal<T, 3, 1>(op, k, coef, E, X, Y, s);
+  }
+  if (k % 4 == 0) return launch_modal<T, 1, 4>(op, k, coef, E, X, Y, s);
+  if (k % 2 == 0) return launch_modal<T, 1, 2>(op, k, coef, E, X, Y, s);
+  return launch_modal<T, 1, 1>(op, k, coef, E, X, Y, s);
+}
+
+template int hex_apply_modal<float>(const skt_operator*, int, const float*, const float*,
+                                    const float*, float*, hipStream_t);
+template int hex_apply_modal<double>(const skt_operator*, int, const double*, const double*,
+                                     const double*, double*, hipStream_t);
+
+}  // namespace skt

@@ -17,6 +17,10 @@ struct skt_operator {
   double* K0d = nullptr;   // (8 ncomp)^2 float64, unit modulus
   float* K0f = nullptr;    // float32 copy
   double K0h[24 * 24];     // host copy
+  // modal (Walsh-Hadamard) coefficients of K_e^0 when it has the box pattern
+  int modal_ok = 0;
+  double modal_d[40];
+  float modal_f[40];
   // tet4
   int32_t* conn = nullptr;     // (4, n_el)
   double* g64 = nullptr;       // (12, n_el) barycentric gradients * sqrt-free
@@ -29,9 +33,14 @@ struct skt_operator {
   double lam0 = 0.0, mu0 = 0.0;  // unit-modulus Lame parameters
   // Dirichlet DOFs
   uint8_t* fixed = nullptr;    // (ndof)
+  int32_t* fixed_list = nullptr;  // indices of the fixed DOFs
+  int64_t n_fixed = 0;
 };
 
 namespace skt {
+template <typename T>
+int hex_apply_modal(const skt_operator* op, int k, const T* coef, const T* E, const T* X, T* Y,
+                    hipStream_t s);
 int op_apply(skt_operator* op, int dtype, int ncols, const void* E,
              const void* X, void* Y, hipStream_t stream);
 int op_diag(skt_operator* op, int dtype, const void* E, void* diag,

@@ -107,6 +107,62 @@ static void hex_K0(int physics, double nu, const double h[3], int order, double*
     }
 }
 
+// Modal coefficients: Kmod = W K0 W / 64 with W[i][S] = (-1)^popcount(i & S)
+// the symmetric Walsh-Hadamard matrix over corner bits (bit 0 x, bit 1 y,
+// bit 2 z; W W = 8 I), modal DOF 3*S + c.
+// Returns 1 and fills coef when every entry outside the box-element pattern
+// vanishes (relative 1e-12), else 0 (the dense kernel is used).
+static int hex_modal_coefficients(int physics, const double* K0, double* coef) {
+  const int nc = physics == SKT_ELASTIC ? 3 : 1;
+  const int nd = 8 * nc;
+  int perm[8];  // perm[bits] = corner id
+  for (int a = 0; a < 8; ++a) perm[kCx[a] | (kCy[a] << 1) | (kCz[a] << 2)] = a;
+  auto h = [](int idx, int S) {  // symmetric Walsh-Hadamard sign
+    return (__builtin_popcount(idx & S) & 1) ? -1.0 : 1.0;
+  };
+  std::vector<double> Km(nd * nd, 0.0);
+  for (int S = 0; S < 8; ++S)
+    for (int c = 0; c < nc; ++c)
+      for (int S2 = 0; S2 < 8; ++S2)
+        for (int c2 = 0; c2 < nc; ++c2) {
+          double acc = 0.0;
+          for (int i = 0; i < 8; ++i)
+            for (int j = 0; j < 8; ++j) {
+              const double kk = K0[(nc * perm[i] + c) * nd + nc * perm[j] + c2];
+              acc += h(i, S) * kk * h(j, S2);
+            }
+          Km[(nc * S + c) * nd + nc * S2 + c2] = acc / 64.0;
+        }
+  double mx = 0.0;
+  for (double v : Km) mx = std::fabs(v) > mx ? std::fabs(v) : mx;
+  std::vector<char> used(nd * nd, 0);
+  auto take = [&](int i, int j) {
+    used[i * nd + j] = used[j * nd + i] = 1;
+    return Km[i * nd + j];
+  };
+  if (nc == 1) {
+    for (int s = 1; s < 8; ++s) coef[s - 1] = take(s, s);
+  } else {
+    const int lin[3] = {3, 7, 14};
+    coef[0] = take(lin[0], lin[0]); coef[1] = take(lin[1], lin[1]); coef[2] = take(lin[2], lin[2]);
+    coef[3] = take(lin[0], lin[1]); coef[4] = take(lin[0], lin[2]); coef[5] = take(lin[1], lin[2]);
+    const int pairs[6][2] = {{4, 6}, {5, 12}, {8, 13}, {9, 20}, {10, 17}, {19, 15}};
+    for (int q = 0; q < 6; ++q) {
+      const int i1 = pairs[q][0], i2 = pairs[q][1];
+      coef[6 + 3 * q] = take(i1, i1);
+      coef[7 + 3 * q] = take(i2, i2);
+      coef[8 + 3 * q] = take(i1, i2);
+    }
+    const int b4[3] = {11, 18, 16};
+    coef[24] = take(b4[0], b4[0]); coef[25] = take(b4[1], b4[1]); coef[26] = take(b4[2], b4[2]);
+    coef[27] = take(b4[0], b4[1]); coef[28] = take(b4[0], b4[2]); coef[29] = take(b4[1], b4[2]);
+    coef[30] = take(21, 21); coef[31] = take(22, 22); coef[32] = take(23, 23);
+  }
+  for (int i = 0; i < nd * nd; ++i)
+    if (!used[i] && std::fabs(Km[i]) > 1e-12 * mx) return 0;
+  return 1;
+}
+
 // ------------------------------------------------------------ hex8 kernels
 
 // Node-centric gather: one thread per (node, column) sums the contributions
@@ -513,6 +569,13 @@ int op_apply(skt_operator* op, int dtype, int ncols, const void* E, const void* 
   SKT_CHECK(ncols >= 1 && ncols <= 64, "ncols must be in [1, 64]");
   SKT_CHECK(X != Y, "operator apply cannot run in place");
   SKT_CHECK(dtype == SKT_F64 || dtype == SKT_F32, "dtype must be SKT_F64 or SKT_F32");
+  if (op->mesh == SKT_MESH_HEX8 && op->modal_ok) {
+    if (dtype == SKT_F64)
+      return hex_apply_modal<double>(op, ncols, op->modal_d, (const double*)E, (const double*)X,
+                                     (double*)Y, s);
+    return hex_apply_modal<float>(op, ncols, op->modal_f, (const float*)E, (const float*)X,
+                                  (float*)Y, s);
+  }
   if (op->mesh == SKT_MESH_HEX8) {
     if (dtype == SKT_F64)
       return hex_apply_t<double>(op, ncols, op->K0d, (const double*)E, (const double*)X,
@@ -599,6 +662,7 @@ int skt_hex8_create(int physics, int64_t nx, int64_t ny, int64_t nz, double hx,
   SKT_CHECK(quad_order >= 1 && quad_order <= 4, "quad_order must be in [1, 4]");
   SKT_CHECK(physics == SKT_HEAT || (nu > -1.0 && nu < 0.5), "nu must be in (-1, 0.5)");
   SKT_CHECK(dof_fixed, "dof_fixed is required");
+  SKT_CHECK((nx + 1) * (ny + 1) * (nz + 1) * 3 < (int64_t)INT32_MAX, "grid too large");
   skt_operator* op = new skt_operator();
   op->mesh = SKT_MESH_HEX8;
   op->physics = physics;
@@ -610,6 +674,8 @@ int skt_hex8_create(int physics, int64_t nx, int64_t ny, int64_t nz, double hx,
   op->ndof = op->n_nodes * op->ncomp;
   const int nd = 8 * op->ncomp;
   hex_K0(physics, nu, op->h, quad_order, op->K0h);
+  op->modal_ok = hex_modal_coefficients(physics, op->K0h, op->modal_d);
+  for (int i = 0; i < 40; ++i) op->modal_f[i] = (float)op->modal_d[i];
   std::vector<float> kf(nd * nd);
   for (int i = 0; i < nd * nd; ++i) kf[i] = (float)op->K0h[i];
   SKT_HIP(hipMalloc(&op->K0d, sizeof(double) * nd * nd));
@@ -618,6 +684,19 @@ int skt_hex8_create(int physics, int64_t nx, int64_t ny, int64_t nz, double hx,
   SKT_HIP(hipMemcpy(op->K0f, kf.data(), sizeof(float) * nd * nd, hipMemcpyHostToDevice));
   SKT_HIP(hipMalloc(&op->fixed, op->ndof));
   SKT_HIP(hipMemcpy(op->fixed, dof_fixed, op->ndof, hipMemcpyDeviceToDevice));
+  {
+    std::vector<uint8_t> hf(op->ndof);
+    SKT_HIP(hipMemcpy(hf.data(), dof_fixed, op->ndof, hipMemcpyDeviceToHost));
+    std::vector<int32_t> lst;
+    for (int64_t i = 0; i < op->ndof; ++i)
+      if (hf[i]) lst.push_back((int32_t)i);
+    op->n_fixed = (int64_t)lst.size();
+    if (!lst.empty()) {
+      SKT_HIP(hipMalloc(&op->fixed_list, sizeof(int32_t) * lst.size()));
+      SKT_HIP(hipMemcpy(op->fixed_list, lst.data(), sizeof(int32_t) * lst.size(),
+                        hipMemcpyHostToDevice));
+    }
+  }
   *out = op;
   return SKT_OK;
 }
@@ -704,6 +783,7 @@ int skt_operator_destroy(skt_operator* op) {
   (void)hipFree(op->nel);
   (void)hipFree(op->nloc);
   (void)hipFree(op->fixed);
+  (void)hipFree(op->fixed_list);
   delete op;
   return SKT_OK;
 }

@@ -289,3 +289,37 @@ def test_oc_bisection_matches_oracle(gpu, it_seed):
     np.testing.assert_allclose(r[0], ref[0], rtol=1e-12)
     np.testing.assert_allclose(r[1], ref[1], rtol=1e-8, atol=1e-12)
     np.testing.assert_allclose(out.cpu().numpy(), ref[2], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("grid,physics,ks", [((31, 17, 9), "elastic", (1, 2, 3, 4, 8)),
+                                              ((16, 16, 16), "elastic", (8,)),
+                                              ((33, 20, 7), "heat", (1, 2, 4, 5))])
+def test_hex8_tiled_apply_matches_assembled(gpu, grid, physics, ks):
+    """Multi-tile / multi-z-chunk grids (tiles of 15x15 nodes) for the modal
+    kernel, every column blocking, float64 and float32."""
+    import torch
+    from sktopt_amd import _lib
+    task = oprob.cantilever(*grid) if physics == "elastic" else oprob.heat_patch(grid[0], 3)
+    if physics == "heat":
+        from oracle import mesh as om2
+        task.mesh = om2.structured_hex(*grid)
+        task.dirichlet_dofs = np.arange(0, (grid[0] + 1) * (grid[1] + 1), 7)
+        task.dirichlet_nodes = task.dirichlet_dofs
+        task.forces = [np.ones(task.mesh.nnodes)]
+    fem = _gpu_fem(task, physics)
+    op = task.element_operator()
+    n_el = task.mesh.nelements
+    rng = np.random.default_rng(5)
+    E = rng.uniform(1e-9, 1.0, n_el)
+    K = op.assemble(E)
+    Ke, _ = ofem.enforce(K, np.zeros(K.shape[0]), task.dirichlet_dofs)
+    for k in ks:
+        X = rng.standard_normal((K.shape[0], k))
+        ref = Ke @ X
+        Y = torch.empty((K.shape[0], k), dtype=torch.float64, device="cuda")
+        _lib.call("skt_operator_apply", fem._op, _lib.F64, k, _p(_t(E)), _p(_t(X)), _p(Y), _s())
+        assert np.abs(Y.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max(), k
+        Y32 = torch.empty((K.shape[0], k), dtype=torch.float32, device="cuda")
+        _lib.call("skt_operator_apply", fem._op, _lib.F32, k, _p(_t(E, torch.float32)),
+                  _p(_t(X, torch.float32)), _p(Y32), _s())
+        assert np.abs(Y32.cpu().numpy() - ref).max() <= 2e-5 * np.abs(ref).max(), k
