@@ -58,6 +58,45 @@ def make_optimizer(wl, precision, rtol, iters, dst):
     return opt
 
 
+APPLY_KERNEL = "skt::k_hex_apply_modal<float, 3, 2>"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hex8_modal_apply_f32_120x60x60_8col_pmc.txt")
+COPY_DOUBLES = 1 << 25            # 256 MiB read + 256 MiB written per copy
+
+
+def hbm_copy(reps=5):
+    """Plain device copy of a known size through the library's element-wise
+    kernel: the HBM counter calibration for the pmc runs and a sanity
+    bandwidth figure.  Runs outside the timed region."""
+    import torch
+    from sktopt_amd import _lib
+    x = torch.ones(COPY_DOUBLES, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    s = torch.cuda.current_stream().cuda_stream
+    for _ in range(2):
+        _lib.call("skt_ewise", 0, x.numel(), 1.0, x.data_ptr(), 0.0, None, y.data_ptr(), s)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        _lib.call("skt_ewise", 0, x.numel(), 1.0, x.data_ptr(), 0.0, None, y.data_ptr(), s)
+    e1.record()
+    torch.cuda.synchronize()
+    us = 1e3 * e0.elapsed_time(e1) / reps
+    return dict(bytes=2 * 8 * COPY_DOUBLES, us=us, gbs=2 * 8 * COPY_DOUBLES / us / 1e3)
+
+
+def traffic_from_profile():
+    """HBM bytes per apply launch from the committed pmc summary (FETCH_SIZE
+    and WRITE_SIZE passes, scaled by the copy calibration), if present."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def timed_run(opt, warmup, steps, timing=False):
     import torch
     from sktopt_amd import _lib
@@ -108,6 +147,8 @@ def main():
         sys.exit(1)
     wl = WORKLOADS[args.workload]
     K, W = args.steps, args.warmup
+    copy = hbm_copy()
+    log(f"hbm copy: {copy['gbs']:.0f} GB/s ({copy['us']:.1f} us for {copy['bytes']} B)")
     iters = W + K + 2
     t_setup = time.perf_counter()
     opt = make_optimizer(wl, "mixed", 1e-8, iters, "/tmp/skt_bench_mixed")
@@ -150,11 +191,11 @@ def main():
         "config": {"workload": args.workload, "ndof": ndof, "n_elements": n_el,
                    "load_cases": k, "solver": "batched Jacobi-PCG, fp32 inner + fp64 refinement",
                    "rtol": 1e-8},
-        "roofline": {"bound": "hbm", "kernel": "k_hex_apply<float,3>",
+        "roofline": {"bound": "hbm", "kernel": APPLY_KERNEL,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
                      "bytes_per_launch": bytes_apply, "avg_launch_us": 1e3 * avg_ms,
-                     "launches": apply["f32_n"]},
+                     "launches": apply["f32_n"], "hbm_copy_gbs": copy["gbs"]},
         "fp64": fp64, "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
         "compliance_rel_err_vs_fp64": err,
         "cpu_baseline": cpu,

@@ -168,9 +168,11 @@ int skt_operator_energy(skt_operator* op, int dtype, int ncols, const void* E,
  *   240-395) and solve_heat_system_multi (fea/solver_heat.py:136-195):
  *   all ncols right-hand sides share one operator apply per iteration.
  * mode SKT_F64: float64 CG to ||r|| <= rtol ||F|| per column.
- * mode SKT_F32: iterative refinement -- float64 residual F - K U recomputed
- *   with the float64 operator, float32 inner PCG with float64 dot products
- *   and norms, U accumulated in float64.
+ * mode SKT_F32: float32 PCG with float64 dot products and norms and
+ *   reliable updates: once a column's recurrence residual has fallen by
+ *   inner_rtol since the last update, the float32 correction is added to U
+ *   in float64 and the true residual F - K U (float64 operator) replaces the
+ *   recurrence residual; convergence is declared on true residuals only.
  * ---------------------------------------------------------------------- */
 int skt_solver_create(skt_operator* op, int max_cols, skt_solver** out);
 int skt_solver_destroy(skt_solver* s);
@@ -181,7 +183,7 @@ int skt_solver_destroy(skt_solver* s);
  * clamp), as skfem.enforce builds it.  U: (ndof, ncols) float64, the initial
  * guess on entry (warm start; U[D] is overwritten with F[D]) and the
  * solution on exit.
- * iters[0] = total inner iterations, iters[1] = outer refinement steps;
+ * iters[0] = PCG iterations, iters[1] = reliable updates (0 in SKT_F64);
  * relres[c] = final ||F - K U|| / ||F|| per column (host arrays). */
 int skt_solver_solve(skt_solver* s, int mode, int ncols, const double* E64,
                      const float* E32, const double* F, double* U,

@@ -5,9 +5,13 @@
 // The host only polls the per-column convergence flags every kCheck steps.
 //
 // mode SKT_F64: float64 vectors and operator.
-// mode SKT_F32: iterative refinement -- float64 residual F - K U with the
-// float64 operator, float32 inner PCG on the scaled residual (float64 dots),
-// U += s * d accumulated in float64.
+// mode SKT_F32: one float32 PCG (float64 dot products) with reliable
+// updates: when a column's recurrence residual has dropped by delta^2 in r.r
+// since the last update (or claims convergence), the float32 correction is
+// folded into the float64 solution, the true residual F - K U is recomputed
+// with the float64 operator and replaces the recurrence residual; the search
+// direction is kept, so the Krylov space is not restarted.  Convergence is
+// only ever declared on a true float64 residual.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -37,7 +41,15 @@ constexpr int kParts = 512;
 constexpr int kSc = 16;
 constexpr int kCheck = 32;
 // scalar slots
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_CONV, S_TOL2, S_RZN, S_SCALE, S_BAD };
+enum {
+  S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_CONV, S_TOL2, S_RZN, S_SCALE, S_BAD,
+  S_RREF,    // r.r at the last reliable update (0: no updates, float64 mode)
+  S_REQ,     // 1: the column asks for a reliable update
+  S_DELTA2,  // reliable-update threshold on r.r / r.r at the last update
+  S_NRST,    // float32 breakdowns restarted so far (mixed mode)
+  S_RST      // 1: restart this column's search direction at the update
+};
+constexpr int kMaxRestarts = 8;
 
 // Column-parallel block layout: thread -> (row slice, column c = tid % cp).
 struct ColMap {
@@ -103,7 +115,7 @@ __global__ void k_resid(int64_t n, int ncols, int cp, const double* __restrict__
   col_block_reduce(sh, bb, cp, ncols, part + (int64_t)ncols * gridDim.x, gridDim.x);
 }
 
-// p = dinv r; partial r.z
+// p = dinv r (skipped when p is null); partial r.z
 template <typename T>
 __global__ void k_pinit(int64_t n, int ncols, int cp, const T* __restrict__ r,
                         const T* __restrict__ dinv, T* __restrict__ p,
@@ -117,7 +129,7 @@ __global__ void k_pinit(int64_t n, int ncols, int cp, const T* __restrict__ r,
          i += (int64_t)gridDim.x * rpb) {
       const int64_t t = i * ncols + c;
       const T z = dinv[i] * r[t];
-      p[t] = z;
+      if (p) p[t] = z;
       rz += (double)r[t] * (double)z;
     }
   }
@@ -127,7 +139,7 @@ __global__ void k_pinit(int64_t n, int ncols, int cp, const T* __restrict__ r,
 // scalars after initialisation: rr, bb from part[0..], rz from part2
 __global__ void k_init_final(int nb, int ncols, const double* __restrict__ part_rb,
                              const double* __restrict__ part_rz, double rtol2,
-                             int use_rr_as_bb, double* __restrict__ sc) {
+                             double delta2, double* __restrict__ sc) {
   __shared__ double sh[kBlock / kWave];
   for (int c = 0; c < ncols; ++c) {
     double a = 0, b = 0, z = 0;
@@ -144,13 +156,18 @@ __global__ void k_init_final(int nb, int ncols, const double* __restrict__ part_
     if (threadIdx.x == 0) {
       double* s = sc + c * kSc;
       s[S_RR] = rr;
-      s[S_BB] = use_rr_as_bb ? rr : bb;
+      s[S_BB] = bb;
       s[S_RZ] = rz;
-      s[S_TOL2] = rtol2 * s[S_BB];
+      s[S_TOL2] = rtol2 * bb;
       s[S_CONV] = (rr <= s[S_TOL2]) ? 1.0 : 0.0;
       s[S_ALPHA] = 0.0;
       s[S_BETA] = 0.0;
       s[S_BAD] = 0.0;
+      s[S_RREF] = delta2 > 0.0 ? rr : 0.0;
+      s[S_REQ] = 0.0;
+      s[S_DELTA2] = delta2;
+      s[S_NRST] = 0.0;
+      if (rr != rr) { s[S_CONV] = 1.0; s[S_BAD] = 1.0; }
     }
     __syncthreads();
   }
@@ -244,6 +261,7 @@ __global__ void k_beta(int nb, int ncols, const double* __restrict__ part,
         s[S_BETA] = rz / s[S_RZ];
         s[S_RZ] = rz;
         if (rr <= s[S_TOL2]) s[S_CONV] = 1.0;
+        if (rr < s[S_DELTA2] * s[S_RREF]) s[S_REQ] = 1.0;
         if (rr != rr) { s[S_CONV] = 1.0; s[S_BAD] = 1.0; }
       }
     }
@@ -266,37 +284,72 @@ __global__ void k_pupdate(int64_t n, int ncols, const double* __restrict__ sc,
   }
 }
 
-// r32 = r64 / scale_c (scale = sqrt(r.r)), d32 = 0
-__global__ void k_scale_resid(int64_t n, int ncols, const double* __restrict__ r64,
-                              const double* __restrict__ scale, float* __restrict__ r32,
-                              float* __restrict__ d32) {
-  const int64_t total = n * ncols;
+// reliable update, step 1: U += d32 (float64 accumulation)
+__global__ void k_accumulate(int64_t total, const float* __restrict__ d32,
+                             double* __restrict__ U) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x)
+    U[t] += (double)d32[t];
+}
+
+// reliable update, step 2: r32 = r64 (the true residual), d32 = 0
+__global__ void k_to32(int64_t total, const double* __restrict__ r64, float* __restrict__ r32,
+                       float* __restrict__ d32) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(t % ncols);
-    const double s = scale[c * kSc + S_SCALE];
-    r32[t] = s > 0.0 ? (float)(r64[t] / s) : 0.0f;
+    r32[t] = (float)r64[t];
     d32[t] = 0.0f;
   }
 }
 
-__global__ void k_set_scale(int ncols, const double* __restrict__ outer, double rtol2,
-                            double* __restrict__ inner) {
-  if (threadIdx.x != 0) return;
-  for (int c = 0; c < ncols; ++c) {
-    const double rr = outer[c * kSc + S_RR], bb = outer[c * kSc + S_BB];
-    inner[c * kSc + S_SCALE] = (rr > rtol2 * bb && rr > 0.0) ? sqrt(rr) : 0.0;
-  }
-}
-
-__global__ void k_accumulate(int64_t n, int ncols, const double* __restrict__ scale,
-                             const float* __restrict__ d32, double* __restrict__ U) {
+// reliable update, step 4 (only for columns flagged by step 3): restart the
+// search direction, p = dinv r
+template <typename T>
+__global__ void k_restart_dir(int64_t n, int ncols, const double* __restrict__ sc,
+                              const T* __restrict__ r, const T* __restrict__ dinv,
+                              T* __restrict__ p) {
   const int64_t total = n * ncols;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int c = (int)(t % ncols);
-    const double s = scale[c * kSc + S_SCALE];
-    if (s > 0.0) U[t] += s * (double)d32[t];
+    if (sc[c * kSc + S_RST] != 0.0) p[t] = dinv[t / ncols] * r[t];
+  }
+}
+
+// reliable update, step 3: r.r (float64 residual) and r.z of its float32
+// copy replace the recurrence values; convergence is decided here only.
+// The search direction is kept unless the true residual has drifted well
+// above the recurrence one (the float32 iteration has lost accuracy, and
+// the old direction no longer fits the new residual) or the float32
+// recurrence broke down; a breakdown is retried up to kMaxRestarts times.
+__global__ void k_replace_final(int nb, int ncols, const double* __restrict__ part_rr,
+                                const double* __restrict__ part_rz, double* __restrict__ sc) {
+  __shared__ double sh[kBlock / kWave];
+  for (int c = 0; c < ncols; ++c) {
+    double a = 0, z = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      a += part_rr[(int64_t)c * nb + i];
+      z += part_rz[(int64_t)c * nb + i];
+    }
+    const double rr = block_sum(a, sh);
+    __syncthreads();
+    const double rz = block_sum(z, sh);
+    if (threadIdx.x == 0) {
+      double* s = sc + c * kSc;
+      s[S_RST] = (rr > 4.0 * s[S_RR]) ? 1.0 : 0.0;
+      s[S_RR] = rr;
+      s[S_RZ] = rz;
+      s[S_RREF] = rr;
+      s[S_REQ] = 0.0;
+      if (s[S_BAD] != 0.0 && s[S_NRST] < kMaxRestarts) {
+        s[S_BAD] = 0.0;
+        s[S_NRST] += 1.0;
+        s[S_RST] = 1.0;
+      }
+      if (s[S_BAD] == 0.0) s[S_CONV] = (rr <= s[S_TOL2]) ? 1.0 : 0.0;
+      if (rr != rr) { s[S_CONV] = 1.0; s[S_BAD] = 1.0; }
+    }
+    __syncthreads();
   }
 }
 
@@ -307,57 +360,62 @@ static int colmap_for(int ncols, int* cp) {
   return kBlock / c;
 }
 
-// One PCG run on (x, r, p, q) of type T with preconditioner dinv.  The
-// scalars sc must be initialised (k_init_final).  Returns iterations run.
+// Copy the per-column scalars to the host and wait for them.
+static int poll(skt_solver* s, int ncols, hipStream_t st) {
+  SKT_HIP(hipMemcpyAsync(s->host, s->sc, sizeof(double) * kSc * ncols, hipMemcpyDeviceToHost, st));
+  SKT_HIP(hipStreamSynchronize(st));
+  return SKT_OK;
+}
+
+static bool all_flag(const skt_solver* s, int ncols, int slot) {
+  for (int c = 0; c < ncols; ++c)
+    if (s->host[c * kSc + slot] == 0.0) return false;
+  return true;
+}
+
+static bool any_flag(const skt_solver* s, int ncols, int slot) {
+  for (int c = 0; c < ncols; ++c)
+    if (s->host[c * kSc + slot] != 0.0) return true;
+  return false;
+}
+
+// Enqueue `count` PCG iterations on (x, r, p, q) of type T with the Jacobi
+// preconditioner dinv; the scalars sc must be initialised.  Columns that
+// have stopped (S_CONV) are left untouched by every kernel.
 template <typename T>
-static int pcg_loop(skt_solver* s, int dtype, int ncols, const void* E, T* x, T* r, T* p,
-                    T* q, const T* dinv, int maxiter, int64_t* iters_out, int* all_conv,
-                    hipStream_t st) {
+static int pcg_iterations(skt_solver* s, int dtype, int ncols, const void* E, T* x, T* r, T* p,
+                          T* q, const T* dinv, int64_t count, hipStream_t st) {
   const int64_t n = s->n;
   int cp;
   const int rpb = colmap_for(ncols, &cp);
   const unsigned nb = grid_for(n, rpb, kParts);
   const unsigned ge = grid_for(n * ncols);
-  int64_t it = 0;
-  *all_conv = 0;
-  // initial convergence check
-  SKT_HIP(hipMemcpyAsync(s->host, s->sc, sizeof(double) * kSc * ncols, hipMemcpyDeviceToHost, st));
-  SKT_HIP(hipStreamSynchronize(st));
-  auto done = [&]() {
-    for (int c = 0; c < ncols; ++c)
-      if (s->host[c * kSc + S_CONV] == 0.0) return false;
-    return true;
-  };
-  if (done()) { *all_conv = 1; *iters_out = 0; return SKT_OK; }
-  while (it < maxiter) {
-    const int64_t chunk = std::min<int64_t>(kCheck, maxiter - it);
-    for (int64_t k = 0; k < chunk; ++k) {
-      if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * k], st));
-      int stt = op_apply(s->op, dtype, ncols, E, p, q, st);
-      if (stt) return stt;
-      if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * k + 1], st));
-      hipLaunchKernelGGL(k_dot_pq<T>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, p, q, s->part);
-      hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part, s->sc);
-      hipLaunchKernelGGL(k_update<T>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, s->sc, p, q,
-                         dinv, x, r, s->part);
-      hipLaunchKernelGGL(k_beta, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part, s->sc);
-      hipLaunchKernelGGL(k_pupdate<T>, dim3(ge), dim3(kBlock), 0, st, n, ncols, s->sc, r, dinv, p);
-    }
-    SKT_LAUNCH_CHECK();
-    it += chunk;
-    SKT_HIP(hipMemcpyAsync(s->host, s->sc, sizeof(double) * kSc * ncols, hipMemcpyDeviceToHost, st));
-    SKT_HIP(hipStreamSynchronize(st));
-    if (s->timing) {
-      for (int64_t k = 0; k < chunk; ++k) {
-        float ms = 0.f;
-        SKT_HIP(hipEventElapsedTime(&ms, s->ev[2 * k], s->ev[2 * k + 1]));
-        s->apply_ms[dtype] += ms;
-        s->n_apply[dtype] += 1;
-      }
-    }
-    if (done()) { *all_conv = 1; break; }
+  for (int64_t k = 0; k < count; ++k) {
+    if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * k], st));
+    int stt = op_apply(s->op, dtype, ncols, E, p, q, st);
+    if (stt) return stt;
+    if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * k + 1], st));
+    hipLaunchKernelGGL(k_dot_pq<T>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, p, q, s->part);
+    hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part, s->sc);
+    hipLaunchKernelGGL(k_update<T>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, s->sc, p, q,
+                       dinv, x, r, s->part);
+    hipLaunchKernelGGL(k_beta, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part, s->sc);
+    hipLaunchKernelGGL(k_pupdate<T>, dim3(ge), dim3(kBlock), 0, st, n, ncols, s->sc, r, dinv, p);
   }
-  *iters_out = it;
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+// Fold the apply timings of the last chunk into the totals (events of the
+// chunk have completed once the poll has synchronised the stream).
+static int collect_timing(skt_solver* s, int dtype, int64_t count) {
+  if (!s->timing) return SKT_OK;
+  for (int64_t k = 0; k < count; ++k) {
+    float ms = 0.f;
+    SKT_HIP(hipEventElapsedTime(&ms, s->ev[2 * k], s->ev[2 * k + 1]));
+    s->apply_ms[dtype] += ms;
+    s->n_apply[dtype] += 1;
+  }
   return SKT_OK;
 }
 
@@ -432,6 +490,8 @@ int skt_solver_solve(skt_solver* s, int mode, int ncols, const double* E64,
   SKT_CHECK(mode == SKT_F64 || mode == SKT_F32, "mode must be SKT_F64 or SKT_F32");
   SKT_CHECK(mode == SKT_F64 || E32, "float32 moduli are required in SKT_F32 mode");
   SKT_CHECK(rtol > 0.0 && maxiter >= 1, "rtol must be > 0 and maxiter >= 1");
+  SKT_CHECK(mode == SKT_F64 || (inner_rtol > 0.0 && inner_rtol < 1.0),
+            "inner_rtol (reliable-update threshold) must be in (0, 1)");
   const int64_t n = s->n;
   int cp;
   const int rpb = colmap_for(ncols, &cp);
@@ -443,94 +503,86 @@ int skt_solver_solve(skt_solver* s, int mode, int ncols, const double* E64,
   iters[1] = 0;
   hipLaunchKernelGGL(k_set_fixed, dim3(ge), dim3(kBlock), 0, st, n, ncols, s->op->fixed, F,
                      U);
-  int stt;
+  const int dt = mode == SKT_F64 ? SKT_F64 : SKT_F32;
+  const void* E = mode == SKT_F64 ? (const void*)E64 : (const void*)E32;
+  // preconditioner
+  int stt = op_diag(s->op, dt, E, mode == SKT_F64 ? (void*)s->q64 : (void*)s->q32, st);
+  if (stt) return stt;
+  if (mode == SKT_F64)
+    hipLaunchKernelGGL(k_dinv<double>, dim3(gn), dim3(kBlock), 0, st, n, s->q64, s->op->fixed,
+                       s->dinv64);
+  else
+    hipLaunchKernelGGL(k_dinv<float>, dim3(gn), dim3(kBlock), 0, st, n, s->q32, s->op->fixed,
+                       s->dinv32);
+  // true residual of the initial guess
+  stt = op_apply(s->op, SKT_F64, ncols, E64, U, s->q64, st);
+  if (stt) return stt;
+  hipLaunchKernelGGL(k_resid<double>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, F, s->q64,
+                     s->r64, s->part);
   if (mode == SKT_F64) {
-    stt = op_diag(s->op, SKT_F64, E64, s->q64, st);
-    if (stt) return stt;
-    hipLaunchKernelGGL(k_dinv<double>, dim3(gn), dim3(kBlock), 0, st, n, s->q64,
-                       s->op->fixed, s->dinv64);
-    stt = op_apply(s->op, SKT_F64, ncols, E64, U, s->q64, st);
-    if (stt) return stt;
-    hipLaunchKernelGGL(k_resid<double>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, F,
-                       s->q64, s->r64, s->part);
     hipLaunchKernelGGL(k_pinit<double>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, s->r64,
                        s->dinv64, s->p64, area1);
-    hipLaunchKernelGGL(k_init_final, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part,
-                       area1, rtol * rtol, 0, s->sc);
-    SKT_LAUNCH_CHECK();
-    int conv = 0;
-    stt = pcg_loop<double>(s, SKT_F64, ncols, E64, U, s->r64, s->p64, s->q64, s->dinv64,
-                           maxiter, &iters[0], &conv, st);
-    if (stt) return stt;
-    iters[1] = 1;
-    bool bad = false;
-    for (int c = 0; c < ncols; ++c) {
-      const double* h = s->host + c * kSc;
-      relres[c] = h[S_BB] > 0 ? std::sqrt(h[S_RR] / h[S_BB]) : 0.0;
-      bad = bad || h[S_BAD] != 0.0;
-    }
-    if (bad) { set_error("CG breakdown (non-positive p.Kp or NaN)"); return SKT_ENOCONV; }
-    if (!conv) { set_error("CG did not converge within maxiter"); return SKT_ENOCONV; }
-    return SKT_OK;
+  } else {
+    hipLaunchKernelGGL(k_to32, dim3(ge), dim3(kBlock), 0, st, n * ncols, s->r64, s->r32, s->d32);
+    hipLaunchKernelGGL(k_pinit<float>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, s->r32,
+                       s->dinv32, s->p32, area1);
   }
-  // ---- mixed precision: float64 refinement around a float32 PCG
-  stt = op_diag(s->op, SKT_F32, E32, s->q32, st);
-  if (stt) return stt;
-  hipLaunchKernelGGL(k_dinv<float>, dim3(gn), dim3(kBlock), 0, st, n, s->q32, s->op->fixed,
-                     s->dinv32);
-  double* sc_outer = s->sc + kSc * s->max_cols;
-  const int max_outer = 60;
-  int64_t inner_total = 0;
-  bool converged = false;
-  for (int outer = 0; outer < max_outer; ++outer) {
+  const double delta2 = mode == SKT_F64 ? 0.0 : inner_rtol * inner_rtol;
+  hipLaunchKernelGGL(k_init_final, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part,
+                     area1, rtol * rtol, delta2, s->sc);
+  SKT_LAUNCH_CHECK();
+  if ((stt = poll(s, ncols, st))) return stt;
+  int64_t it = 0, updates = 0;
+  bool conv = all_flag(s, ncols, S_CONV);
+  while (!conv && it < maxiter) {
+    const int64_t chunk = std::min<int64_t>(kCheck, maxiter - it);
+    if (mode == SKT_F64)
+      stt = pcg_iterations<double>(s, SKT_F64, ncols, E64, U, s->r64, s->p64, s->q64,
+                                   s->dinv64, chunk, st);
+    else
+      stt = pcg_iterations<float>(s, SKT_F32, ncols, E32, s->d32, s->r32, s->p32, s->q32,
+                                  s->dinv32, chunk, st);
+    if (stt) return stt;
+    it += chunk;
+    if ((stt = poll(s, ncols, st))) return stt;
+    if ((stt = collect_timing(s, dt, chunk))) return stt;
+    const bool bad = any_flag(s, ncols, S_BAD);
+    const bool all_stop = all_flag(s, ncols, S_CONV);
+    if (mode == SKT_F64) {
+      if (bad) break;
+      conv = all_stop;
+      continue;
+    }
+    if (!(bad || all_stop || any_flag(s, ncols, S_REQ) || it >= maxiter)) continue;
+    // reliable update of every column
+    hipLaunchKernelGGL(k_accumulate, dim3(ge), dim3(kBlock), 0, st, n * ncols, s->d32, U);
     stt = op_apply(s->op, SKT_F64, ncols, E64, U, s->q64, st);
     if (stt) return stt;
     hipLaunchKernelGGL(k_resid<double>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, F,
                        s->q64, s->r64, s->part);
-    // use the r.r partials also as the "r.z" slot (unused here)
-    hipLaunchKernelGGL(k_init_final, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part,
-                       s->part, rtol * rtol, 0, sc_outer);
-    SKT_LAUNCH_CHECK();
-    SKT_HIP(hipMemcpyAsync(s->host + kSc * s->max_cols, sc_outer, sizeof(double) * kSc * ncols,
-                           hipMemcpyDeviceToHost, st));
-    SKT_HIP(hipStreamSynchronize(st));
-    iters[1] = outer;
-    bool all = true;
-    for (int c = 0; c < ncols; ++c) {
-      const double* h = s->host + kSc * s->max_cols + c * kSc;
-      relres[c] = h[S_BB] > 0 ? std::sqrt(h[S_RR] / h[S_BB]) : 0.0;
-      if (!(h[S_RR] <= h[S_TOL2])) all = false;
-      if (h[S_RR] != h[S_RR]) { set_error("NaN residual in refinement"); return SKT_ENOCONV; }
-    }
-    if (all) { converged = true; break; }
-    if (inner_total >= maxiter) break;
-    hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, st, ncols, sc_outer, rtol * rtol,
-                       s->sc);
-    hipLaunchKernelGGL(k_scale_resid, dim3(ge), dim3(kBlock), 0, st, n, ncols, s->r64, s->sc,
-                       s->r32, s->d32);
-    // inner init: bb = r.r of the scaled residual; p = dinv r
+    hipLaunchKernelGGL(k_to32, dim3(ge), dim3(kBlock), 0, st, n * ncols, s->r64, s->r32, s->d32);
     hipLaunchKernelGGL(k_pinit<float>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, s->r32,
-                       s->dinv32, s->p32, area1);
-    hipLaunchKernelGGL(k_dot_pq<float>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, s->r32,
-                       s->r32, s->part);
-    hipLaunchKernelGGL(k_init_final, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part,
-                       area1, inner_rtol * inner_rtol, 1, s->sc);
-    // k_init_final overwrote S_SCALE? no: it leaves slot S_SCALE untouched
+                       s->dinv32, (float*)nullptr, area1);
+    hipLaunchKernelGGL(k_replace_final, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part,
+                       area1, s->sc);
+    hipLaunchKernelGGL(k_restart_dir<float>, dim3(ge), dim3(kBlock), 0, st, n, ncols, s->sc,
+                       s->r32, s->dinv32, s->p32);
     SKT_LAUNCH_CHECK();
-    int conv = 0;
-    int64_t it = 0;
-    stt = pcg_loop<float>(s, SKT_F32, ncols, E32, s->d32, s->r32, s->p32, s->q32, s->dinv32,
-                          (int)std::max<int64_t>(1, maxiter - inner_total), &it, &conv, st);
-    if (stt) return stt;
-    inner_total += it;
-    hipLaunchKernelGGL(k_accumulate, dim3(ge), dim3(kBlock), 0, st, n, ncols, s->sc, s->d32, U);
-    SKT_LAUNCH_CHECK();
+    ++updates;
+    if ((stt = poll(s, ncols, st))) return stt;
+    if (any_flag(s, ncols, S_BAD)) break;
+    conv = all_flag(s, ncols, S_CONV);
   }
-  iters[0] = inner_total;
-  if (!converged) {
-    set_error("mixed-precision refinement did not reach rtol");
-    return SKT_ENOCONV;
+  iters[0] = it;
+  iters[1] = updates;
+  bool bad = false;
+  for (int c = 0; c < ncols; ++c) {
+    const double* h = s->host + c * kSc;
+    relres[c] = h[S_BB] > 0 ? std::sqrt(h[S_RR] / h[S_BB]) : 0.0;
+    bad = bad || h[S_BAD] != 0.0;
   }
+  if (bad) { set_error("CG breakdown (non-positive p.Kp or NaN)"); return SKT_ENOCONV; }
+  if (!conv) { set_error("CG did not reach rtol within maxiter"); return SKT_ENOCONV; }
   return SKT_OK;
 }
 

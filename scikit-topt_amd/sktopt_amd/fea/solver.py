@@ -45,8 +45,8 @@ class LinearSolverConfig:
     maxiter: int | None = None
     petsc_options: object = None
     allow_fallback_to_spsolve: bool = False
-    precision: str = "mixed"          # "mixed" (fp32 inner + fp64 refinement) or "fp64"
-    inner_rtol: float = 1e-3          # relative tolerance of each fp32 inner solve
+    precision: str = "mixed"          # "mixed" (fp32 PCG, fp64 reliable updates) or "fp64"
+    inner_rtol: float = 1e-1          # fp32 residual drop that triggers a reliable update
     warm_start: bool = True           # start from the previous u_dofs
     on_noconv: str = "raise"          # "raise" or "warn"
 
@@ -149,7 +149,7 @@ class _GpuFEM:
             _dev.ptr(self.E32) if mode == _lib.F32 else None, _dev.ptr(self.F), _dev.ptr(U),
             float(cfg.rtol), self._maxiter(), float(cfg.inner_rtol), iters, relres,
             _dev.stream())
-        self.last_solve = dict(iters=int(iters[0]), outer=int(iters[1]),
+        self.last_solve = dict(iters=int(iters[0]), updates=int(iters[1]),
                                relres=list(relres), converged=(st == _lib.SKT_OK))
         self.solve_history.append(self.last_solve)
         if st == _lib.SKT_ENOCONV and cfg.on_noconv == "warn":
