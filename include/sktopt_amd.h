@@ -163,7 +163,7 @@ int skt_operator_energy(skt_operator* op, int dtype, int ncols, const void* E,
                         const void* U, double* energy, hipStream_t stream);
 
 /* ------------------------------------------------------------------------
- * Batched Jacobi-preconditioned CG on an operator
+ * Batched preconditioned CG on an operator
  *   Replaces solve_u / solve_multi_load (fea/solver_elastic.py:61-143,
  *   240-395) and solve_heat_system_multi (fea/solver_heat.py:136-195):
  *   all ncols right-hand sides share one operator apply per iteration.
@@ -189,6 +189,24 @@ int skt_solver_solve(skt_solver* s, int mode, int ncols, const double* E64,
                      const float* E32, const double* F, double* U,
                      double rtol, int maxiter, double inner_rtol,
                      int64_t* iters, double* relres, hipStream_t stream);
+
+/* Preconditioner selection (the reference's iterative options are scipy
+ * cg with Jacobi or pyamg, fea/solver_elastic.py:84-101):
+ *   SKT_PC_JACOBI  diagonal;
+ *   SKT_PC_MG      geometric multigrid V-cycle on structured hex8 grids:
+ *                  Galerkin coarse operators of trilinear interpolation with
+ *                  the Dirichlet DOFs removed, Chebyshev-Jacobi smoothing of
+ *                  smooth_degree steps, dense inverse on the coarsest level
+ *                  (at most max_coarse DOFs);
+ *   SKT_PC_AUTO    multigrid where the grid allows it, else Jacobi (default).
+ * skt_solver_info reports the preconditioner the last solve used and its
+ * number of levels (1 for Jacobi). */
+#define SKT_PC_JACOBI 0
+#define SKT_PC_MG 1
+#define SKT_PC_AUTO 2
+int skt_solver_configure(skt_solver* s, int precond, int smooth_degree,
+                         int max_coarse);
+int skt_solver_info(const skt_solver* s, int* precond_used, int* levels);
 
 /* Measurement hooks (no reference counterpart; used by the benchmark):
  * with timing enabled, every operator apply inside the CG loop is bracketed

@@ -14,7 +14,15 @@
 // elements of the layer that touch the patch, park the 8 corner results of
 // each element in LDS, and every owned node sums its 4 contributions from
 // the layer below and 4 from the layer above in a fixed order (no atomics,
-// bit-reproducible).  Completed planes are written out coalesced.
+// bit-reproducible).  Completed planes are written out coalesced; Dirichlet
+// rows (identity) are applied on the way out.
+//
+// Vector layout: value (dof, column c) lives at
+//   (c / lkc) * gstride + (c % lkc) + dof * dstride,
+// which covers the public (ndof, k) row-major block (lkc = k, dstride = k)
+// and the solver's column-blocked [k/kc][ndof][kc] layout (lkc = kc,
+// gstride = ndof*kc, dstride = kc).  A workgroup handles KC columns; with
+// the blocked layout its loads and stores are contiguous.
 #include "operator.hpp"
 
 namespace skt {
@@ -129,9 +137,10 @@ constexpr int soa_stride(int n, int combos) {
 
 template <typename T, int NC, int KC>
 __global__ void __launch_bounds__(256)
-k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int k, int lz, int tiles_x,
-                  int tiles_y, int n_zchunks, ModalArgs<T> coef,
-                  const T* __restrict__ E, const T* __restrict__ X, T* __restrict__ Y) {
+k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, int dstride,
+                  int lz, int tiles_x, int tiles_y, int n_zchunks, ModalArgs<T> coef,
+                  const T* __restrict__ E, const uint8_t* __restrict__ fixed,
+                  const T* __restrict__ X, T* __restrict__ Y) {
   // LDS, structure-of-arrays so that consecutive lanes touch consecutive
   // words: two rotating node planes [NC*KC][kPS]; per element lane the
   // x-paired corner sums [side][cy][NC][KC][256] (side 0: node plane below the
@@ -148,12 +157,15 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int k, int lz, int tiles_x
   T* pairs = planes + 2 * kPlane;
   T* oplane = pairs + 4 * NC * KC * 256;
   const int64_t nnx = nx + 1, nny = ny + 1, nnz = nz + 1;
-  const int64_t plane_stride = nnx * nny * NC * k;
+  const int64_t plane_stride = nnx * nny * NC * (int64_t)dstride;
   int b = blockIdx.x;
   const int tx = b % tiles_x; b /= tiles_x;
   const int ty = b % tiles_y; b /= tiles_y;
   const int zc = b % n_zchunks; b /= n_zchunks;
   const int c0 = b * KC;
+  const int64_t cbase = (int64_t)(c0 / lkc) * gstride + (c0 % lkc);
+  X += cbase;
+  Y += cbase;
   const int64_t x0 = (int64_t)tx * kTN, y0 = (int64_t)ty * kTN;
   const int64_t z0 = (int64_t)zc * lz;
   const int64_t z1 = (z0 + lz < nnz) ? z0 + lz : nnz;
@@ -165,6 +177,9 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int k, int lz, int tiles_x
   const bool is_node = tid < kTN * kTN;
   const int nx_l = tid % kTN, ny_l = tid / kTN;
   const int e_lo = nx_l + kTE * ny_l;  // element lane holding row ny_l's pair
+  const int own_slot = node_slot(nx_l + 1, ny_l + 1);  // owned node in the halo plane
+  const bool own_in = is_node && x0 + nx_l < nnx && y0 + ny_l < nny;
+  const int64_t own_node = (y0 + ny_l) * nnx + (x0 + nx_l);  // in-plane index
 
   int f_off[kPre], f_lds[kPre];
 #pragma unroll
@@ -177,7 +192,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int k, int lz, int tiles_x
       const int64_t gx = x0 - 1 + nl % kPW, gy = y0 - 1 + nl / kPW;
       f_lds[r] = (comp * KC + col) * kPS + node_slot(nl % kPW, nl / kPW);
       if (gx >= 0 && gx < nnx && gy >= 0 && gy < nny)
-        f_off[r] = (int)(((gy * nnx + gx) * NC + comp) * k + c0 + col);
+        f_off[r] = (int)(((gy * nnx + gx) * NC + comp) * dstride + col);
     }
   }
   int w_off[kWr], w_lds[kWr];
@@ -190,7 +205,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int k, int lz, int tiles_x
       const int col = i % KC, comp = (i / KC) % NC, nl = i / (KC * NC);
       const int64_t gx = x0 + nl % kTN, gy = y0 + nl / kTN;
       w_lds[r] = (comp * KC + col) * kOS + nl;
-      if (gx < nnx && gy < nny) w_off[r] = (int)(((gy * nnx + gx) * NC + comp) * k + c0 + col);
+      if (gx < nnx && gy < nny) w_off[r] = (int)(((gy * nnx + gx) * NC + comp) * dstride + col);
     }
   }
 
@@ -256,6 +271,12 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int k, int lz, int tiles_x
     }
     __syncthreads();
     if (is_node) {
+      // Dirichlet rows of the plane being completed (node plane ez, held in
+      // the bottom LDS plane): y = x
+      bool fix[NC];
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+        fix[d] = fixed && own_in && ez >= z0 && fixed[(ez * nnx * nny + own_node) * NC + d];
 #pragma unroll
       for (int col = 0; col < KC; ++col)
 #pragma unroll
@@ -264,7 +285,8 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int k, int lz, int tiles_x
                           pairs[(((0 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE];
           const T above = pairs[(((1 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
                           pairs[(((1 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE];
-          oplane[(d * KC + col) * kOS + tid] = partial[col][d] + below;
+          oplane[(d * KC + col) * kOS + tid] =
+              fix[d] ? pb[(d * KC + col) * kPS + own_slot] : partial[col][d] + below;
           partial[col][d] = above;
         }
     }
@@ -283,25 +305,14 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int k, int lz, int tiles_x
   }
 }
 
-// Dirichlet rows of the enforced operator: y[d, :] = x[d, :].
-template <typename T>
-__global__ void k_fix_rows(int64_t n_fixed, const int32_t* __restrict__ dofs, int k,
-                           const T* __restrict__ X, T* __restrict__ Y) {
-  const int64_t total = n_fixed * k;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = (int64_t)dofs[t / k] * k + t % k;
-    Y[i] = X[i];
-  }
-}
-
-// Launch helper used by op_apply for structured grids whose element matrix
-// has the modal pattern.
+// Launch helper for structured grids whose element matrix has the modal
+// pattern.  k columns in the layout described above.
 template <typename T, int NC, int KC>
-static int launch_modal(const skt_operator* op, int k, const T* coef, const T* E, const T* X,
-                        T* Y, hipStream_t s) {
-  SKT_CHECK((op->nx + 1) * (op->ny + 1) * NC * (int64_t)k < INT32_MAX,
+static int launch_modal(const skt_operator* op, int k, int lkc, int64_t gstride, int dstride,
+                        const T* coef, const T* E, const T* X, T* Y, hipStream_t s) {
+  SKT_CHECK((op->nx + 1) * (op->ny + 1) * NC * (int64_t)dstride < INT32_MAX,
             "node plane too large for 32-bit in-plane offsets");
+  SKT_CHECK(k % KC == 0 && lkc % KC == 0, "column blocking does not divide the layout");
   ModalArgs<T> a;
   for (int i = 0; i < 40; ++i) a.c[i] = coef[i];
   const int64_t nnx = op->nx + 1, nny = op->ny + 1, nnz = op->nz + 1;
@@ -325,31 +336,33 @@ static int launch_modal(const skt_operator* op, int k, const T* coef, const T* E
     attr_set = true;
   }
   hipLaunchKernelGGL((k_hex_apply_modal<T, NC, KC>), dim3((unsigned)nblocks), dim3(256), shmem, s,
-                     op->nx, op->ny, op->nz, k, (int)lz, tiles_x, tiles_y, nzc, a, E, X, Y);
-  if (op->n_fixed > 0)
-    hipLaunchKernelGGL(k_fix_rows<T>, dim3(grid_for(op->n_fixed * k)), dim3(256), 0, s,
-                       op->n_fixed, op->fixed_list, k, X, Y);
+                     op->nx, op->ny, op->nz, lkc, gstride, dstride, (int)lz, tiles_x, tiles_y,
+                     nzc, a, E, op->n_fixed > 0 ? op->fixed : nullptr, X, Y);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }
 
 template <typename T>
-int hex_apply_modal(const skt_operator* op, int k, const T* coef, const T* E, const T* X, T* Y,
-                    hipStream_t s) {
-  // two columns per workgroup: more columns cut LDS occupancy below three
-  // workgroups per CU, fewer multiply the re-reads of each cache line
+int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
+                    T* Y, hipStream_t s) {
+  // kc == k: public (ndof, k) layout; otherwise column-blocked with group kc
+  const int lkc = kc;
+  const int64_t gstride = (kc == k) ? 0 : op->ndof * kc;
+  const int dstride = (kc == k) ? k : kc;
+  // two columns per workgroup for elasticity: more columns cut LDS occupancy
+  // below three workgroups per CU
   if (op->ncomp == 3) {
-    if (k % 2 == 0) return launch_modal<T, 3, 2>(op, k, coef, E, X, Y, s);
-    return launch_modal<T, 3, 1>(op, k, coef, E, X, Y, s);
+    if (kc % 2 == 0) return launch_modal<T, 3, 2>(op, k, lkc, gstride, dstride, coef, E, X, Y, s);
+    return launch_modal<T, 3, 1>(op, k, lkc, gstride, dstride, coef, E, X, Y, s);
   }
-  if (k % 4 == 0) return launch_modal<T, 1, 4>(op, k, coef, E, X, Y, s);
-  if (k % 2 == 0) return launch_modal<T, 1, 2>(op, k, coef, E, X, Y, s);
-  return launch_modal<T, 1, 1>(op, k, coef, E, X, Y, s);
+  if (kc % 4 == 0) return launch_modal<T, 1, 4>(op, k, lkc, gstride, dstride, coef, E, X, Y, s);
+  if (kc % 2 == 0) return launch_modal<T, 1, 2>(op, k, lkc, gstride, dstride, coef, E, X, Y, s);
+  return launch_modal<T, 1, 1>(op, k, lkc, gstride, dstride, coef, E, X, Y, s);
 }
 
-template int hex_apply_modal<float>(const skt_operator*, int, const float*, const float*,
+template int hex_apply_modal<float>(const skt_operator*, int, int, const float*, const float*,
                                     const float*, float*, hipStream_t);
-template int hex_apply_modal<double>(const skt_operator*, int, const double*, const double*,
+template int hex_apply_modal<double>(const skt_operator*, int, int, const double*, const double*,
                                      const double*, double*, hipStream_t);
 
 }  // namespace skt

@@ -1,0 +1,912 @@
+// Geometric multigrid preconditioner for structured hex8 operators: level
+// planning, Galerkin coarsening, stencil assembly, coarsest dense inverse,
+// smoother bounds and the V-cycle.  Every reduction runs in a fixed order,
+// so a V-cycle is bit-reproducible.
+//
+// Element-matrix indexing inside this file: corner bits (bit 0 x, bit 1 y,
+// bit 2 z), local DOF = NC * bits + component.  Vectors use the solver's
+// column-blocked layout: value (dof, c) at ((c / kc) * ndof + dof) * kc + c % kc.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mg.hpp"
+
+namespace skt {
+
+// skfem corner order of the operator's K_e^0 (see operators.hip)
+static int corner_id_h(int cx, int cy, int cz) {
+  return cz * 4 + (cy ? (cx ? 2 : 3) : (cx ? 1 : 0));
+}
+
+static int axis_factor(int64_t n) {
+  for (int f : {2, 3, 5})
+    if (n % f == 0 && n / f >= 1) return f;
+  return 1;
+}
+
+bool mg_plan(const skt_operator* op, const MGConfig& cfg, std::vector<MGLevel>* out) {
+  out->clear();
+  if (op->mesh != SKT_MESH_HEX8) return false;
+  const int NC = op->ncomp;
+  MGLevel l0;
+  l0.nx = op->nx; l0.ny = op->ny; l0.nz = op->nz;
+  l0.nn = (l0.nx + 1) * (l0.ny + 1) * (l0.nz + 1);
+  l0.ndof = l0.nn * NC;
+  out->push_back(l0);
+  while (out->back().ndof > cfg.max_coarse) {
+    const MGLevel& f = out->back();
+    const int fac[3] = {axis_factor(f.nx), axis_factor(f.ny), axis_factor(f.nz)};
+    if (fac[0] == 1 && fac[1] == 1 && fac[2] == 1) return false;
+    MGLevel c;
+    c.nx = f.nx / fac[0]; c.ny = f.ny / fac[1]; c.nz = f.nz / fac[2];
+    c.f[0] = fac[0]; c.f[1] = fac[1]; c.f[2] = fac[2];
+    c.nn = (c.nx + 1) * (c.ny + 1) * (c.nz + 1);
+    c.ndof = c.nn * NC;
+    out->push_back(c);
+  }
+  return out->size() >= 2;
+}
+
+// ------------------------------------------------------------- Galerkin
+
+// Trilinear weight of coarse corner Q (bits) at fine corner q of child
+// (a, b, c) inside a coarse element divided by (fx, fy, fz).
+__device__ __forceinline__ double tri_weight(int q, int Q, int a, int b, int c, int fx, int fy,
+                                             int fz) {
+  const double sx = (double)(a + (q & 1)) / fx;
+  const double sy = (double)(b + ((q >> 1) & 1)) / fy;
+  const double sz = (double)(c + ((q >> 2) & 1)) / fz;
+  const double wx = (Q & 1) ? sx : 1.0 - sx;
+  const double wy = (Q & 2) ? sy : 1.0 - sy;
+  const double wz = (Q & 4) ? sz : 1.0 - sz;
+  return wx * wy * wz;
+}
+
+// One workgroup of 64 threads per coarse element:
+//   Kc = sum_children P_j^T K_j P_j,
+// K_j = E_j Z_j K0 Z_j on the finest level (Z_j removes Dirichlet DOFs), or
+// the finer level's Galerkin element matrix.
+template <int NC>
+__global__ void __launch_bounds__(64)
+k_galerkin(int64_t ncx, int64_t ncy, int64_t ncz, int fx, int fy, int fz,
+           const double* __restrict__ Kf, const double* __restrict__ K0b,
+           const double* __restrict__ E, const uint8_t* __restrict__ fixed,
+           double* __restrict__ Kc) {
+  constexpr int ND = 8 * NC;
+  constexpr int NN = ND * ND;
+  constexpr int PER = (NN + 63) / 64;
+  __shared__ double sK[NN];
+  __shared__ double sT[NN];
+  __shared__ double sP[64];
+  const int64_t ce = blockIdx.x;
+  const int64_t CX = ce % ncx, CY = (ce / ncx) % ncy, CZ = ce / (ncx * ncy);
+  const int64_t nfx = ncx * fx, nfy = ncy * fy;
+  const int64_t nfnx = nfx + 1, nfny = nfy + 1;
+  const int tid = threadIdx.x;
+  double acc[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) acc[i] = 0.0;
+  for (int c = 0; c < fz; ++c)
+    for (int b = 0; b < fy; ++b)
+      for (int a = 0; a < fx; ++a) {
+        const int64_t ex = CX * fx + a, ey = CY * fy + b, ez = CZ * fz + c;
+        const int64_t fe = ex + nfx * (ey + nfy * ez);
+        // child matrix
+        if (Kf) {
+          for (int i = tid; i < NN; i += 64) sK[i] = Kf[fe * NN + i];
+        } else {
+          const double Ee = E[fe];
+          for (int i = tid; i < NN; i += 64) {
+            const int r = i / ND, s = i % ND;
+            double v = Ee * K0b[i];
+            if (fixed) {
+              const int qr = r / NC, qs = s / NC;
+              const int64_t nr = (ex + (qr & 1)) + nfnx * ((ey + ((qr >> 1) & 1)) +
+                                                           nfny * (ez + (qr >> 2)));
+              const int64_t ns = (ex + (qs & 1)) + nfnx * ((ey + ((qs >> 1) & 1)) +
+                                                           nfny * (ez + (qs >> 2)));
+              if (fixed[nr * NC + r % NC] || fixed[ns * NC + s % NC]) v = 0.0;
+            }
+            sK[i] = v;
+          }
+        }
+        sP[tid] = tri_weight(tid >> 3, tid & 7, a, b, c, fx, fy, fz);  // sP[q*8+Q]
+        __syncthreads();
+        // T[(q,d)][(R,e)] = sum_r K[(q,d)][(r,e)] P[r][R]
+        for (int i = tid; i < NN; i += 64) {
+          const int row = i / ND, col = i % ND;
+          const int R = col / NC, e = col % NC;
+          double t = 0.0;
+#pragma unroll
+          for (int r = 0; r < 8; ++r) t += sK[row * ND + r * NC + e] * sP[r * 8 + R];
+          sT[i] = t;
+        }
+        __syncthreads();
+        // Kc[(Q,d)][(R,e)] += sum_q P[q][Q] T[(q,d)][(R,e)]
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+          const int i = tid + 64 * k;
+          if (i < NN) {
+            const int row = i / ND, col = i % ND;
+            const int Q = row / NC, d = row % NC;
+            double t = 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) t += sP[q * 8 + Q] * sT[(q * NC + d) * ND + col];
+            acc[k] += t;
+          }
+        }
+        __syncthreads();
+      }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = tid + 64 * k;
+    if (i < NN) Kc[ce * NN + i] = acc[k];
+  }
+}
+
+// Sum of the element-matrix entries coupling (node n, comp d) with (node
+// n + o, comp e) over the elements that hold both, in a fixed order.
+template <int NC>
+__device__ __forceinline__ double couple(const double* __restrict__ Kel, int64_t nx, int64_t ny,
+                                         int64_t nz, int64_t i, int64_t j, int64_t k, int ox,
+                                         int oy, int oz, int d, int e) {
+  constexpr int ND = 8 * NC;
+  double v = 0.0;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int sx = s & 1, sy = (s >> 1) & 1, sz = s >> 2;
+    const int64_t ex = i - 1 + sx, ey = j - 1 + sy, ez = k - 1 + sz;
+    if (ex < 0 || ex >= nx || ey < 0 || ey >= ny || ez < 0 || ez >= nz) continue;
+    const int ax = 1 - sx, ay = 1 - sy, az = 1 - sz;
+    const int bx = ax + ox, by = ay + oy, bz = az + oz;
+    if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+    const int A = ax | (ay << 1) | (az << 2), B = bx | (by << 1) | (bz << 2);
+    v += Kel[(ex + nx * (ey + ny * ez)) * (ND * ND) + (A * NC + d) * ND + B * NC + e];
+  }
+  return v;
+}
+
+// stencil[((o * NC + d) * NC + e) * nn + node], o = (ox+1) + 3 (oy+1) + 9 (oz+1)
+template <typename T, int NC>
+__global__ void k_stencil_asm(int64_t nx, int64_t ny, int64_t nz, const double* __restrict__ Kel,
+                              T* __restrict__ st) {
+  const int64_t nnx = nx + 1, nny = ny + 1;
+  const int64_t nn = nnx * nny * (nz + 1);
+  const int64_t total = nn * 27 * NC * NC;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t node = t % nn;
+    const int rest = (int)(t / nn);
+    const int e = rest % NC, d = (rest / NC) % NC, o = rest / (NC * NC);
+    const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
+    const int64_t i = node % nnx, j = (node / nnx) % nny, k = node / (nnx * nny);
+    st[t] = (T)couple<NC>(Kel, nx, ny, nz, i, j, k, ox, oy, oz, d, e);
+  }
+}
+
+// Jacobi inverse of a stencil level; rows with no stiffness (DOFs that no
+// free fine DOF depends on) get 0 and stay 0 through the cycle.
+template <typename T, int NC>
+__global__ void k_stencil_dinv(int64_t nn, const T* __restrict__ st, T* __restrict__ dinv) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nn * NC;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t node = t / NC;
+    const int d = (int)(t % NC);
+    const T v = st[((13 * NC + d) * NC + d) * nn + node];
+    dinv[t] = v > T(0) ? T(1) / v : T(0);
+  }
+}
+
+// Dense coarsest matrix (fp64, n x n row-major); zero rows become identity.
+template <int NC>
+__global__ void k_dense_asm(int64_t nx, int64_t ny, int64_t nz, const double* __restrict__ Kel,
+                            double* __restrict__ A) {
+  const int64_t nnx = nx + 1, nny = ny + 1;
+  const int64_t n = nnx * nny * (nz + 1) * NC;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / n, c = t % n;
+    const int64_t nr = r / NC, ncn = c / NC;
+    const int d = (int)(r % NC), e = (int)(c % NC);
+    const int64_t i = nr % nnx, j = (nr / nnx) % nny, k = nr / (nnx * nny);
+    const int64_t i2 = ncn % nnx, j2 = (ncn / nnx) % nny, k2 = ncn / (nnx * nny);
+    const int64_t ox = i2 - i, oy = j2 - j, oz = k2 - k;
+    double v = 0.0;
+    if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1)
+      v = couple<NC>(Kel, nx, ny, nz, i, j, k, (int)ox, (int)oy, (int)oz, d, e);
+    A[t] = v;
+  }
+}
+
+__global__ void k_dense_fix_diag(int64_t n, double* __restrict__ A) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x)
+    if (!(A[r * n + r] > 0.0)) A[r * n + r] = 1.0;
+}
+
+// ------------------------------------------- blocked Gauss-Jordan inverse
+// In-place inversion of an SPD matrix (no pivoting needed) by pivot blocks B
+// of kGJ rows: with R the remaining indices,
+//   A_BB <- A_BB^-1, A_BR <- A_BB^-1 A_BR, A_RR <- A_RR - A_RB A_BR(new),
+//   A_RB <- -A_RB A_BB^-1.
+constexpr int kGJ = 32;
+
+// invert the kGJ x kGJ pivot block in LDS (plain Gauss-Jordan), write it to
+// Binv (kGJ x kGJ) and back into A
+__global__ void __launch_bounds__(256) k_gj_block_inv(int64_t n, int64_t b0, int nb,
+                                                       double* __restrict__ A,
+                                                       double* __restrict__ Binv) {
+  __shared__ double M[kGJ][kGJ + 1];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kGJ * kGJ; i += 256) {
+    const int r = i / kGJ, c = i % kGJ;
+    M[r][c] = (r < nb && c < nb) ? A[(b0 + r) * n + b0 + c] : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int k = 0; k < kGJ; ++k) {
+    const double p = M[k][k];
+    double rowk = 0.0, colk = 0.0;
+    const int r = tid / kGJ, c = tid % kGJ;  // 8 rows x 32 cols per pass
+    __syncthreads();
+    // new values computed from the old pivot row / column
+    double nv[4];
+#pragma unroll
+    for (int pass = 0; pass < 4; ++pass) {
+      const int rr = r + 8 * pass;
+      rowk = M[k][c];
+      colk = M[rr][k];
+      double v;
+      if (rr == k && c == k) v = 1.0 / p;
+      else if (rr == k) v = rowk / p;
+      else if (c == k) v = -colk / p;
+      else v = M[rr][c] - colk * rowk / p;
+      nv[pass] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int pass = 0; pass < 4; ++pass) M[r + 8 * pass][c] = nv[pass];
+    __syncthreads();
+  }
+  for (int i = tid; i < kGJ * kGJ; i += 256) {
+    const int r = i / kGJ, c = i % kGJ;
+    Binv[i] = M[r][c];
+    if (r < nb && c < nb) A[(b0 + r) * n + b0 + c] = M[r][c];
+  }
+}
+
+// A_BR <- Binv A_BR (rows b0..b0+nb, columns outside the block)
+__global__ void k_gj_rows(int64_t n, int64_t b0, int nb, const double* __restrict__ Binv,
+                          double* __restrict__ A) {
+  const int64_t total = n;  // one thread per column
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < total;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    if (c >= b0 && c < b0 + nb) continue;
+    double col[kGJ];
+    for (int r = 0; r < nb; ++r) col[r] = A[(b0 + r) * n + c];
+    for (int r = 0; r < nb; ++r) {
+      double v = 0.0;
+      for (int q = 0; q < nb; ++q) v += Binv[r * kGJ + q] * col[q];
+      A[(b0 + r) * n + c] = v;
+    }
+  }
+}
+
+// A_RR <- A_RR - A_RB A_BR for rows and columns outside the block
+__global__ void k_gj_update(int64_t n, int64_t b0, int nb, double* __restrict__ A) {
+  const int64_t total = n * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / n, c = t % n;
+    if ((r >= b0 && r < b0 + nb) || (c >= b0 && c < b0 + nb)) continue;
+    double v = A[t];
+    for (int q = 0; q < nb; ++q) v -= A[r * n + b0 + q] * A[(b0 + q) * n + c];
+    A[t] = v;
+  }
+}
+
+// A_RB <- -A_RB Binv
+__global__ void k_gj_cols(int64_t n, int64_t b0, int nb, const double* __restrict__ Binv,
+                          double* __restrict__ A) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    if (r >= b0 && r < b0 + nb) continue;
+    double row[kGJ];
+    for (int q = 0; q < nb; ++q) row[q] = A[r * n + b0 + q];
+    for (int c = 0; c < nb; ++c) {
+      double v = 0.0;
+      for (int q = 0; q < nb; ++q) v += row[q] * Binv[q * kGJ + c];
+      A[r * n + b0 + c] = -v;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_cast(int64_t n, const double* __restrict__ a, T* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (T)a[i];
+}
+
+// ------------------------------------------------------------ operators
+
+// y = S x on a stencil level, KC columns per thread (one column group)
+template <typename T, int NC, int KC>
+__global__ void k_stencil_apply(int64_t nx, int64_t ny, int64_t nz, int groups,
+                                const T* __restrict__ st, const T* __restrict__ X,
+                                T* __restrict__ Y) {
+  const int64_t nnx = nx + 1, nny = ny + 1, nnz = nz + 1;
+  const int64_t nn = nnx * nny * nnz;
+  const int64_t total = nn * groups;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t node = t % nn;
+    const int64_t g = t / nn;
+    const int64_t i = node % nnx, j = (node / nnx) % nny, k = node / (nnx * nny);
+    const T* Xg = X + g * nn * NC * KC;
+    T acc[NC][KC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int c = 0; c < KC; ++c) acc[d][c] = T(0);
+#pragma unroll
+    for (int o = 0; o < 27; ++o) {
+      const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
+      const int64_t i2 = i + ox, j2 = j + oy, k2 = k + oz;
+      if (i2 < 0 || i2 >= nnx || j2 < 0 || j2 >= nny || k2 < 0 || k2 >= nnz) continue;
+      const int64_t m = i2 + nnx * (j2 + nny * k2);
+      T xv[NC][KC];
+#pragma unroll
+      for (int e = 0; e < NC; ++e)
+#pragma unroll
+        for (int c = 0; c < KC; ++c) xv[e][c] = Xg[(m * NC + e) * KC + c];
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+          const T a = st[((o * NC + d) * NC + e) * nn + node];
+#pragma unroll
+          for (int c = 0; c < KC; ++c) acc[d][c] += a * xv[e][c];
+        }
+    }
+    T* Yg = Y + g * nn * NC * KC;
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int c = 0; c < KC; ++c) Yg[(node * NC + d) * KC + c] = acc[d][c];
+  }
+}
+
+// Chebyshev step on a level (blocked vectors, total = ndof * k):
+//   d <- c1 d + c2 dinv (b - w);  x <- x + d   (w null: the iterate is 0)
+// coef holds (c1, c2) for this step.
+template <typename T>
+__global__ void k_cheb(int64_t total, int64_t ndof, int kc, const double* __restrict__ coef,
+                       const T* __restrict__ b, const T* __restrict__ w,
+                       const T* __restrict__ dinv, T* __restrict__ d, T* __restrict__ x,
+                       int x_is_zero) {
+  const T c1 = (T)coef[0], c2 = (T)coef[1];
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t dof = (t / kc) % ndof;
+    const T r = w ? b[t] - w[t] : b[t];
+    const T dn = ((x_is_zero || c1 == T(0)) ? T(0) : c1 * d[t]) + c2 * dinv[dof] * r;
+    d[t] = dn;
+    x[t] = x_is_zero ? dn : x[t] + dn;
+  }
+}
+
+// b_c = P^T Z (b - w) for one coarse node and column group
+template <typename T, int NC, int KC>
+__global__ void k_restrict(int64_t fnx, int64_t fny, int64_t fnz, int fx, int fy, int fz,
+                           int groups, const T* __restrict__ b, const T* __restrict__ w,
+                           const uint8_t* __restrict__ fixed, T* __restrict__ bc) {
+  const int64_t fnnx = fnx + 1, fnny = fny + 1, fnnz = fnz + 1;
+  const int64_t cnnx = fnx / fx + 1, cnny = fny / fy + 1, cnnz = fnz / fz + 1;
+  const int64_t cnn = cnnx * cnny * cnnz, fnn = fnnx * fnny * fnnz;
+  const int64_t total = cnn * groups;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t cnode = t % cnn, g = t / cnn;
+    const int64_t I = cnode % cnnx, J = (cnode / cnnx) % cnny, K = cnode / (cnnx * cnny);
+    const T* bg = b + g * fnn * NC * KC;
+    const T* wg = w ? w + g * fnn * NC * KC : nullptr;
+    T acc[NC][KC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int c = 0; c < KC; ++c) acc[d][c] = T(0);
+    for (int dz = -(fz - 1); dz <= fz - 1; ++dz) {
+      const int64_t k = K * fz + dz;
+      if (k < 0 || k >= fnnz) continue;
+      const T wz = T(1) - (T)(dz < 0 ? -dz : dz) / (T)fz;
+      for (int dy = -(fy - 1); dy <= fy - 1; ++dy) {
+        const int64_t j = J * fy + dy;
+        if (j < 0 || j >= fnny) continue;
+        const T wy = T(1) - (T)(dy < 0 ? -dy : dy) / (T)fy;
+        for (int dx = -(fx - 1); dx <= fx - 1; ++dx) {
+          const int64_t i = I * fx + dx;
+          if (i < 0 || i >= fnnx) continue;
+          const T wgt = (T(1) - (T)(dx < 0 ? -dx : dx) / (T)fx) * wy * wz;
+          const int64_t fn = i + fnnx * (j + fnny * k);
+#pragma unroll
+          for (int d = 0; d < NC; ++d) {
+            if (fixed && fixed[fn * NC + d]) continue;
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+              const int64_t idx = (fn * NC + d) * KC + c;
+              const T r = wg ? bg[idx] - wg[idx] : bg[idx];
+              acc[d][c] += wgt * r;
+            }
+          }
+        }
+      }
+    }
+    T* out = bc + g * cnn * NC * KC;
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int c = 0; c < KC; ++c) out[(cnode * NC + d) * KC + c] = acc[d][c];
+  }
+}
+
+// x_f += Z P x_c for one fine node and column group
+template <typename T, int NC, int KC>
+__global__ void k_prolong_add(int64_t fnx, int64_t fny, int64_t fnz, int fx, int fy, int fz,
+                              int groups, const T* __restrict__ xc,
+                              const uint8_t* __restrict__ fixed, T* __restrict__ xf) {
+  const int64_t fnnx = fnx + 1, fnny = fny + 1, fnnz = fnz + 1;
+  const int64_t cnnx = fnx / fx + 1, cnny = fny / fy + 1, cnnz = fnz / fz + 1;
+  const int64_t cnn = cnnx * cnny * cnnz, fnn = fnnx * fnny * fnnz;
+  const int64_t total = fnn * groups;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t fnode = t % fnn, g = t / fnn;
+    const int64_t i = fnode % fnnx, j = (fnode / fnnx) % fnny, k = fnode / (fnnx * fnny);
+    const int64_t I = i / fx, J = j / fy, K = k / fz;
+    const int ax = (int)(i % fx), ay = (int)(j % fy), az = (int)(k % fz);
+    const T sx = (T)ax / (T)fx, sy = (T)ay / (T)fy, sz = (T)az / (T)fz;
+    const T* xg = xc + g * cnn * NC * KC;
+    T acc[NC][KC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int c = 0; c < KC; ++c) acc[d][c] = T(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
+      if ((qx && ax == 0) || (qy && ay == 0) || (qz && az == 0)) continue;
+      const T wgt = (qx ? sx : T(1) - sx) * (qy ? sy : T(1) - sy) * (qz ? sz : T(1) - sz);
+      const int64_t cn = (I + qx) + cnnx * ((J + qy) + cnny * (K + qz));
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+#pragma unroll
+        for (int c = 0; c < KC; ++c) acc[d][c] += wgt * xg[(cn * NC + d) * KC + c];
+    }
+    T* xfg = xf + g * fnn * NC * KC;
+#pragma unroll
+    for (int d = 0; d < NC; ++d) {
+      if (fixed && fixed[fnode * NC + d]) continue;
+#pragma unroll
+      for (int c = 0; c < KC; ++c) xfg[(fnode * NC + d) * KC + c] += acc[d][c];
+    }
+  }
+}
+
+// x = Ainv b on the coarsest level: one wavefront per (row, column group),
+// lanes stride the row, fixed-order wave reduction
+template <typename T, int KC>
+__global__ void __launch_bounds__(256) k_dense_apply(int64_t n, int groups,
+                                                     const T* __restrict__ Ainv,
+                                                     const T* __restrict__ b,
+                                                     T* __restrict__ x) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  if (wid >= n * groups) return;
+  const int64_t row = wid % n, g = wid / n;
+  const T* bg = b + g * n * KC;
+  double acc[KC];
+#pragma unroll
+  for (int c = 0; c < KC; ++c) acc[c] = 0.0;
+  for (int64_t j = lane; j < n; j += kWave) {
+    const double a = (double)Ainv[row * n + j];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) acc[c] += a * (double)bg[j * KC + c];
+  }
+#pragma unroll
+  for (int c = 0; c < KC; ++c) {
+    const double v = wave_sum(acc[c]);
+    if (lane == 0) x[(g * n + row) * KC + c] = (T)v;
+  }
+}
+
+// ------------------------------------------------- smoother bounds (power)
+
+__device__ __forceinline__ double hash_unit(int64_t i) {
+  uint64_t z = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+}
+
+template <typename T>
+__global__ void k_pow_init(int64_t n, const T* __restrict__ dinv, T* __restrict__ v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = dinv[i] != T(0) ? (T)hash_unit(i) : T(0);
+}
+
+// y <- dinv y (if scale_by_dinv), partial sums of y^2 per block
+template <typename T>
+__global__ void k_pow_norm(int64_t n, const T* __restrict__ dinv, T* __restrict__ y,
+                           int scale_by_dinv, double* __restrict__ part) {
+  __shared__ double sh[kBlock / kWave];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    T v = y[i];
+    if (scale_by_dinv) { v *= dinv[i]; y[i] = v; }
+    acc += (double)v * (double)v;
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// norm = sqrt(sum part); lam[level] = norm (input had unit norm); inv stored
+__global__ void k_pow_finish(int nb, const double* __restrict__ part, double* __restrict__ lam_out,
+                             double* __restrict__ inv_out) {
+  __shared__ double sh[kBlock / kWave];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) a += part[i];
+  const double s = block_sum(a, sh);
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt(s);
+    if (lam_out) *lam_out = nrm;
+    *inv_out = nrm > 0.0 ? 1.0 / nrm : 0.0;
+  }
+}
+
+template <typename T>
+__global__ void k_pow_scale(int64_t n, const double* __restrict__ inv, const T* __restrict__ y,
+                            T* __restrict__ v) {
+  const T s = (T)*inv;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = y[i] * s;
+}
+
+// Chebyshev (c1, c2) per step for the interval [beta/ratio, beta],
+// beta = 1.1 * lambda estimate
+__global__ void k_cheb_coef(int nlev, int degree, double ratio, const double* __restrict__ lam,
+                            double* __restrict__ coef) {
+  const int l = threadIdx.x;
+  if (l >= nlev) return;
+  const double bmax = 1.1 * lam[l];
+  const double amin = bmax / ratio;
+  const double theta = 0.5 * (bmax + amin), delta = 0.5 * (bmax - amin);
+  const double sigma = theta / delta;
+  double* c = coef + (size_t)l * 2 * degree;
+  double rho_old = 1.0 / sigma;
+  c[0] = 0.0;
+  c[1] = 1.0 / theta;
+  for (int s = 1; s < degree; ++s) {
+    const double rho = 1.0 / (2.0 * sigma - rho_old);
+    c[2 * s] = rho * rho_old;
+    c[2 * s + 1] = 2.0 * rho / delta;
+    rho_old = rho;
+  }
+}
+
+// ------------------------------------------------------------ host side
+
+static size_t tsize(int dtype) { return dtype == SKT_F64 ? sizeof(double) : sizeof(float); }
+
+void mg_free(MGData* mg) {
+  for (MGLevel& l : mg->lv) {
+    (void)hipFree(l.Kel); (void)hipFree(l.st); (void)hipFree(l.dinv);
+    (void)hipFree(l.b); (void)hipFree(l.x); (void)hipFree(l.w); (void)hipFree(l.d);
+    l.Kel = nullptr; l.st = l.dinv = l.b = l.x = l.w = l.d = nullptr;
+  }
+  (void)hipFree(mg->Ad); (void)hipFree(mg->Ad2); (void)hipFree(mg->Ainv);
+  (void)hipFree(mg->lam); (void)hipFree(mg->coef); (void)hipFree(mg->part); (void)hipFree(mg->pv);
+  (void)hipFree(mg->K0b);
+  mg->Ad = mg->Ad2 = nullptr; mg->Ainv = nullptr; mg->lam = mg->coef = mg->part = nullptr;
+  mg->K0b = nullptr;
+  mg->pv = nullptr;
+  mg->lv.clear();
+  mg->dtype = -1;
+}
+
+int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype, int max_cols) {
+  if (mg->dtype == dtype && mg->max_cols == max_cols) return SKT_OK;
+  mg_free(mg);
+  std::vector<MGLevel> lv;
+  if (!mg_plan(op, cfg, &lv)) {
+    set_error("grid cannot be coarsened for multigrid");
+    return SKT_EINVAL;
+  }
+  const int NC = op->ncomp, ND = 8 * NC;
+  const size_t ts = tsize(dtype);
+  const int L = (int)lv.size() - 1;
+  for (int l = 0; l <= L; ++l) {
+    MGLevel& v = lv[l];
+    const size_t nv = (size_t)v.ndof * max_cols;
+    if (l == 0) {
+      SKT_HIP(hipMalloc(&v.d, ts * nv));
+      continue;
+    }
+    SKT_HIP(hipMalloc(&v.Kel, sizeof(double) * (size_t)(v.nx * v.ny * v.nz) * ND * ND));
+    SKT_HIP(hipMalloc(&v.b, ts * nv));
+    SKT_HIP(hipMalloc(&v.x, ts * nv));
+    if (l < L) {
+      SKT_HIP(hipMalloc(&v.st, ts * (size_t)v.nn * 27 * NC * NC));
+      SKT_HIP(hipMalloc(&v.dinv, ts * v.ndof));
+      SKT_HIP(hipMalloc(&v.w, ts * nv));
+      SKT_HIP(hipMalloc(&v.d, ts * nv));
+    }
+  }
+  const int64_t nc = lv[L].ndof;
+  SKT_HIP(hipMalloc(&mg->Ad, sizeof(double) * nc * nc));
+  SKT_HIP(hipMalloc(&mg->Ad2, sizeof(double) * kGJ * kGJ));
+  SKT_HIP(hipMalloc(&mg->Ainv, ts * nc * nc));
+  SKT_HIP(hipMalloc(&mg->lam, sizeof(double) * (L + 1) * 2));
+  SKT_HIP(hipMalloc(&mg->coef, sizeof(double) * (L + 1) * 2 * std::max(cfg.degree, 1)));
+  SKT_HIP(hipMalloc(&mg->part, sizeof(double) * 1024));
+  SKT_HIP(hipMalloc(&mg->pv, ts * 2 * lv[0].ndof));
+  // unit element matrix in corner-bit order
+  std::vector<double> K0b((size_t)ND * ND);
+  for (int A = 0; A < 8; ++A)
+    for (int B = 0; B < 8; ++B) {
+      const int a = corner_id_h(A & 1, (A >> 1) & 1, A >> 2);
+      const int b = corner_id_h(B & 1, (B >> 1) & 1, B >> 2);
+      for (int d = 0; d < NC; ++d)
+        for (int e = 0; e < NC; ++e)
+          K0b[(A * NC + d) * ND + B * NC + e] = op->K0h[(a * NC + d) * ND + b * NC + e];
+    }
+  SKT_HIP(hipMalloc(&mg->K0b, sizeof(double) * K0b.size()));
+  SKT_HIP(hipMemcpy(mg->K0b, K0b.data(), sizeof(double) * K0b.size(), hipMemcpyHostToDevice));
+  mg->lv = lv;
+  mg->dtype = dtype;
+  mg->max_cols = max_cols;
+  mg->degree = std::max(cfg.degree, 1);
+  mg->ratio = cfg.ratio;
+  mg->power_iters = cfg.power_iters;
+  return SKT_OK;
+}
+
+// apply of level l (k columns, group kc) in T
+template <typename T>
+static int level_apply(MGData* mg, const skt_operator* op, const void* E, int l, int k, int kc,
+                       const T* x, T* y, hipStream_t s) {
+  if (l == 0) return op_apply(const_cast<skt_operator*>(op), mg->dtype, k, kc, E, x, y, s);
+  const MGLevel& v = mg->lv[l];
+  const int groups = k / kc;
+  const unsigned g = grid_for(v.nn * groups);
+  const T* st = (const T*)v.st;
+#define SKT_SA(NC_, KC_)                                                                   \
+  hipLaunchKernelGGL((k_stencil_apply<T, NC_, KC_>), dim3(g), dim3(kBlock), 0, s, v.nx, v.ny, \
+                     v.nz, groups, st, x, y)
+  if (op->ncomp == 3) {
+    if (kc == 1) SKT_SA(3, 1);
+    else if (kc == 2) SKT_SA(3, 2);
+    else { set_error("unsupported column block"); return SKT_EINVAL; }
+  } else {
+    if (kc == 1) SKT_SA(1, 1);
+    else if (kc == 2) SKT_SA(1, 2);
+    else if (kc == 4) SKT_SA(1, 4);
+    else { set_error("unsupported column block"); return SKT_EINVAL; }
+  }
+#undef SKT_SA
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+template <typename T>
+static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const void* E,
+                   const void* dinv0, hipStream_t s) {
+  const int NC = op->ncomp, ND = 8 * NC;
+  const int L = (int)mg->lv.size() - 1;
+  const double* dK0 = mg->K0b;
+  // Galerkin hierarchy
+  for (int l = 1; l <= L; ++l) {
+    const MGLevel& c = mg->lv[l];
+    const int64_t nce = c.nx * c.ny * c.nz;
+    const double* Kf = l == 1 ? nullptr : mg->lv[l - 1].Kel;
+    const uint8_t* fx = (l == 1 && op->n_fixed > 0) ? op->fixed : nullptr;
+    if (NC == 3)
+      hipLaunchKernelGGL((k_galerkin<3>), dim3((unsigned)nce), dim3(64), 0, s, c.nx, c.ny, c.nz,
+                         c.f[0], c.f[1], c.f[2], Kf, dK0, E64, fx, c.Kel);
+    else
+      hipLaunchKernelGGL((k_galerkin<1>), dim3((unsigned)nce), dim3(64), 0, s, c.nx, c.ny, c.nz,
+                         c.f[0], c.f[1], c.f[2], Kf, dK0, E64, fx, c.Kel);
+    SKT_LAUNCH_CHECK();
+  }
+  // stencils and Jacobi inverses of the intermediate levels
+  for (int l = 1; l < L; ++l) {
+    const MGLevel& v = mg->lv[l];
+    const int64_t tot = v.nn * 27 * NC * NC;
+    if (NC == 3) {
+      hipLaunchKernelGGL((k_stencil_asm<T, 3>), dim3(grid_for(tot)), dim3(kBlock), 0, s, v.nx,
+                         v.ny, v.nz, v.Kel, (T*)v.st);
+      hipLaunchKernelGGL((k_stencil_dinv<T, 3>), dim3(grid_for(v.ndof)), dim3(kBlock), 0, s,
+                         v.nn, (const T*)v.st, (T*)v.dinv);
+    } else {
+      hipLaunchKernelGGL((k_stencil_asm<T, 1>), dim3(grid_for(tot)), dim3(kBlock), 0, s, v.nx,
+                         v.ny, v.nz, v.Kel, (T*)v.st);
+      hipLaunchKernelGGL((k_stencil_dinv<T, 1>), dim3(grid_for(v.ndof)), dim3(kBlock), 0, s,
+                         v.nn, (const T*)v.st, (T*)v.dinv);
+    }
+    SKT_LAUNCH_CHECK();
+  }
+  // coarsest: dense matrix, blocked Gauss-Jordan inverse, cast to T
+  {
+    const MGLevel& c = mg->lv[L];
+    const int64_t n = c.ndof;
+    if (NC == 3)
+      hipLaunchKernelGGL((k_dense_asm<3>), dim3(grid_for(n * n)), dim3(kBlock), 0, s, c.nx, c.ny,
+                         c.nz, c.Kel, mg->Ad);
+    else
+      hipLaunchKernelGGL((k_dense_asm<1>), dim3(grid_for(n * n)), dim3(kBlock), 0, s, c.nx, c.ny,
+                         c.nz, c.Kel, mg->Ad);
+    hipLaunchKernelGGL(k_dense_fix_diag, dim3(grid_for(n)), dim3(kBlock), 0, s, n, mg->Ad);
+    for (int64_t b0 = 0; b0 < n; b0 += kGJ) {
+      const int nb = (int)std::min<int64_t>(kGJ, n - b0);
+      hipLaunchKernelGGL(k_gj_block_inv, dim3(1), dim3(256), 0, s, n, b0, nb, mg->Ad, mg->Ad2);
+      hipLaunchKernelGGL(k_gj_rows, dim3(grid_for(n)), dim3(kBlock), 0, s, n, b0, nb, mg->Ad2,
+                         mg->Ad);
+      hipLaunchKernelGGL(k_gj_update, dim3(grid_for(n * n)), dim3(kBlock), 0, s, n, b0, nb,
+                         mg->Ad);
+      hipLaunchKernelGGL(k_gj_cols, dim3(grid_for(n)), dim3(kBlock), 0, s, n, b0, nb, mg->Ad2,
+                         mg->Ad);
+    }
+    hipLaunchKernelGGL(k_cast<T>, dim3(grid_for(n * n)), dim3(kBlock), 0, s, n * n, mg->Ad,
+                       (T*)mg->Ainv);
+    SKT_LAUNCH_CHECK();
+  }
+  // smoother bounds: power iteration on D^-1 A (one column) per level
+  double* inv = mg->lam + (L + 1);  // scratch scalar slot
+  for (int l = 0; l < L; ++l) {
+    const MGLevel& v = mg->lv[l];
+    const int64_t n = v.ndof;
+    T* pv0 = (T*)mg->pv;
+    T* pv1 = pv0 + mg->lv[0].ndof;
+    const T* dinv = l == 0 ? (const T*)dinv0 : (const T*)v.dinv;
+    const unsigned nb = std::min<unsigned>(grid_for(n), 1024u);
+    hipLaunchKernelGGL(k_pow_init<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, dinv, pv0);
+    hipLaunchKernelGGL(k_pow_norm<T>, dim3(nb), dim3(kBlock), 0, s, n, dinv, pv0, 0, mg->part);
+    hipLaunchKernelGGL(k_pow_finish, dim3(1), dim3(kBlock), 0, s, (int)nb, mg->part,
+                       (double*)nullptr, inv);
+    hipLaunchKernelGGL(k_pow_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, inv, pv0, pv0);
+    for (int it = 0; it < mg->power_iters; ++it) {
+      int stt = level_apply<T>(mg, op, E, l, 1, 1, pv0, pv1, s);
+      if (stt) return stt;
+      hipLaunchKernelGGL(k_pow_norm<T>, dim3(nb), dim3(kBlock), 0, s, n, dinv, pv1, 1, mg->part);
+      hipLaunchKernelGGL(k_pow_finish, dim3(1), dim3(kBlock), 0, s, (int)nb, mg->part,
+                         mg->lam + l, inv);
+      hipLaunchKernelGGL(k_pow_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, inv, pv1,
+                         pv0);
+    }
+    SKT_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_cheb_coef, dim3(1), dim3(64), 0, s, L, mg->degree, mg->ratio, mg->lam,
+                     mg->coef);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int mg_setup(MGData* mg, const skt_operator* op, const double* E64, const void* E,
+             const void* dinv0, hipStream_t s) {
+  SKT_CHECK(mg->dtype >= 0 && !mg->lv.empty(), "multigrid not allocated");
+  if (mg->dtype == SKT_F64) return setup_t<double>(mg, op, E64, E, dinv0, s);
+  return setup_t<float>(mg, op, E64, E, dinv0, s);
+}
+
+template <typename T, int NC, int KC>
+static void launch_restrict(const MGLevel& f, const MGLevel& c, int groups, const T* b,
+                            const T* w, const uint8_t* fixed, T* bc, hipStream_t s) {
+  hipLaunchKernelGGL((k_restrict<T, NC, KC>), dim3(grid_for(c.nn * groups)), dim3(kBlock), 0, s,
+                     f.nx, f.ny, f.nz, c.f[0], c.f[1], c.f[2], groups, b, w, fixed, bc);
+}
+
+template <typename T, int NC, int KC>
+static void launch_prolong(const MGLevel& f, const MGLevel& c, int groups, const T* xc,
+                           const uint8_t* fixed, T* xf, hipStream_t s) {
+  hipLaunchKernelGGL((k_prolong_add<T, NC, KC>), dim3(grid_for(f.nn * groups)), dim3(kBlock), 0,
+                     s, f.nx, f.ny, f.nz, c.f[0], c.f[1], c.f[2], groups, xc, fixed, xf);
+}
+
+template <typename T, int NC, int KC>
+static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const void* dinv0, int k,
+                    const T* r, T* z, T* w0, hipStream_t s) {
+  const int L = (int)mg->lv.size() - 1;
+  const int groups = k / KC;
+  const int deg = mg->degree;
+  const uint8_t* fixed0 = op->n_fixed > 0 ? op->fixed : nullptr;
+  int stt;
+  // downward sweep
+  for (int l = 0; l < L; ++l) {
+    MGLevel& v = mg->lv[l];
+    const T* b = l == 0 ? r : (const T*)v.b;
+    T* x = l == 0 ? z : (T*)v.x;
+    T* w = l == 0 ? w0 : (T*)v.w;
+    T* d = (T*)v.d;
+    const T* dinv = l == 0 ? (const T*)dinv0 : (const T*)v.dinv;
+    const int64_t total = v.ndof * k;
+    const unsigned ge = grid_for(total);
+    const double* cf = mg->coef + (size_t)l * 2 * deg;
+    hipLaunchKernelGGL(k_cheb<T>, dim3(ge), dim3(kBlock), 0, s, total, v.ndof, KC, cf, b,
+                       (const T*)nullptr, dinv, d, x, 1);
+    for (int st = 1; st < deg; ++st) {
+      if ((stt = level_apply<T>(mg, op, E, l, k, KC, x, w, s))) return stt;
+      hipLaunchKernelGGL(k_cheb<T>, dim3(ge), dim3(kBlock), 0, s, total, v.ndof, KC, cf + 2 * st,
+                         b, (const T*)w, dinv, d, x, 0);
+    }
+    if ((stt = level_apply<T>(mg, op, E, l, k, KC, x, w, s))) return stt;
+    launch_restrict<T, NC, KC>(v, mg->lv[l + 1], groups, b, w, l == 0 ? fixed0 : nullptr,
+                               (T*)mg->lv[l + 1].b, s);
+    SKT_LAUNCH_CHECK();
+  }
+  // coarsest
+  {
+    const MGLevel& c = mg->lv[L];
+    const int64_t waves = c.ndof * groups;
+    hipLaunchKernelGGL((k_dense_apply<T, KC>), dim3((unsigned)((waves * kWave + 255) / 256)),
+                       dim3(256), 0, s, c.ndof, groups, (const T*)mg->Ainv, (const T*)c.b,
+                       (T*)c.x);
+    SKT_LAUNCH_CHECK();
+  }
+  // upward sweep
+  for (int l = L - 1; l >= 0; --l) {
+    MGLevel& v = mg->lv[l];
+    const T* b = l == 0 ? r : (const T*)v.b;
+    T* x = l == 0 ? z : (T*)v.x;
+    T* w = l == 0 ? w0 : (T*)v.w;
+    T* d = (T*)v.d;
+    const T* dinv = l == 0 ? (const T*)dinv0 : (const T*)v.dinv;
+    const int64_t total = v.ndof * k;
+    const unsigned ge = grid_for(total);
+    const double* cf = mg->coef + (size_t)l * 2 * deg;
+    launch_prolong<T, NC, KC>(v, mg->lv[l + 1], groups, (const T*)mg->lv[l + 1].x,
+                              l == 0 ? fixed0 : nullptr, x, s);
+    for (int st = 0; st < deg; ++st) {
+      if ((stt = level_apply<T>(mg, op, E, l, k, KC, x, w, s))) return stt;
+      hipLaunchKernelGGL(k_cheb<T>, dim3(ge), dim3(kBlock), 0, s, total, v.ndof, KC, cf + 2 * st,
+                         b, (const T*)w, dinv, d, x, 0);
+    }
+    SKT_LAUNCH_CHECK();
+  }
+  return SKT_OK;
+}
+
+int mg_vcycle(MGData* mg, const skt_operator* op, const void* E, const void* dinv0, int ncols,
+              int kc, const void* r, void* z, void* w0, hipStream_t s) {
+  SKT_CHECK(mg->dtype >= 0, "multigrid not set up");
+  SKT_CHECK(ncols <= mg->max_cols && ncols % kc == 0, "bad column count for multigrid");
+#define SKT_VC(T_, NC_, KC_)                                                               \
+  return vcycle_t<T_, NC_, KC_>(mg, op, E, dinv0, ncols, (const T_*)r, (T_*)z, (T_*)w0, s)
+  if (mg->dtype == SKT_F64) {
+    if (op->ncomp == 3) {
+      if (kc == 1) SKT_VC(double, 3, 1);
+      if (kc == 2) SKT_VC(double, 3, 2);
+    } else {
+      if (kc == 1) SKT_VC(double, 1, 1);
+      if (kc == 2) SKT_VC(double, 1, 2);
+      if (kc == 4) SKT_VC(double, 1, 4);
+    }
+  } else {
+    if (op->ncomp == 3) {
+      if (kc == 1) SKT_VC(float, 3, 1);
+      if (kc == 2) SKT_VC(float, 3, 2);
+    } else {
+      if (kc == 1) SKT_VC(float, 1, 1);
+      if (kc == 2) SKT_VC(float, 1, 2);
+      if (kc == 4) SKT_VC(float, 1, 4);
+    }
+  }
+#undef SKT_VC
+  set_error("unsupported column block for multigrid");
+  return SKT_EINVAL;
+}
+
+}  // namespace skt

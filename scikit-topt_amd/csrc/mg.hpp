@@ -1,0 +1,77 @@
+// Geometric multigrid preconditioner for structured hex8 operators (internal).
+//
+// Level 0 is the matrix-free fine operator.  Each coarser level groups
+// fx*fy*fz fine elements into one coarse element (factors 2, 3 or 5 per axis,
+// 1 where an axis cannot be divided) and carries the exact Galerkin operator
+// P^T A P of trilinear interpolation, built element by element: the coarse
+// element matrix is sum_children P_c^T K_c P_c, with the fine Dirichlet rows
+// and columns removed first, so the hierarchy preconditions the operator on
+// the free DOFs.  Coarse levels store the assembled 27-point node stencil
+// (structure of arrays, [27*NC*NC][nodes]); the coarsest level is inverted
+// densely.  Smoothing is Chebyshev-accelerated Jacobi with the spectral bound
+// of D^-1 A estimated on the device by power iteration, so a V-cycle is a
+// fixed symmetric linear operator and never waits on the host.
+#pragma once
+
+#include <vector>
+
+#include "operator.hpp"
+
+namespace skt {
+
+struct MGLevel {
+  int64_t nx = 0, ny = 0, nz = 0;     // elements
+  int64_t nn = 0, ndof = 0;           // nodes, DOFs
+  int f[3] = {1, 1, 1};               // factor from the finer level
+  double* Kel = nullptr;              // (n_el, ND*ND) Galerkin element matrices, fp64
+  void* st = nullptr;                 // stencil [27*NC*NC][nn] in T (levels 1..L-1)
+  void* dinv = nullptr;               // (ndof) in T
+  void* b = nullptr;                  // right-hand side (ndof*k, blocked)
+  void* x = nullptr;                  // iterate
+  void* w = nullptr;                  // operator output
+  void* d = nullptr;                  // Chebyshev direction
+};
+
+struct MGData {
+  int dtype = -1;                     // SKT_F64 / SKT_F32 the buffers hold
+  int max_cols = 0;
+  std::vector<MGLevel> lv;            // lv[0]: fine level (vectors: d only)
+  double* Ad = nullptr;               // coarsest dense matrix, fp64 (n x n), ping
+  double* Ad2 = nullptr;              // pong
+  void* Ainv = nullptr;               // coarsest inverse in T
+  double* lam = nullptr;              // per level lambda_max(D^-1 A) estimates
+  double* coef = nullptr;             // per level Chebyshev (c1, c2) per step
+  double* part = nullptr;             // power-iteration partials
+  double* K0b = nullptr;              // unit element matrix, corner-bit order
+  void* pv = nullptr;                 // power-iteration vectors (2 * ndof0) in T
+  int degree = 2;
+  double ratio = 10.0;
+  int power_iters = 20;
+};
+
+struct MGConfig {
+  int degree = 2;          // Chebyshev steps per smoothing
+  int max_coarse = 2000;   // coarsest level at most this many DOFs
+  double ratio = 10.0;     // smoothing interval [lambda/ratio, lambda]
+  int power_iters = 20;
+};
+
+// Build the level structure for an operator (no numeric data yet).  Returns
+// false when the grid cannot be coarsened to at most max_coarse DOFs.
+bool mg_plan(const skt_operator* op, const MGConfig& cfg, std::vector<MGLevel>* out);
+
+// Allocate buffers for dtype and k columns (idempotent for the same dtype).
+int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype, int max_cols);
+void mg_free(MGData* mg);
+
+// Numeric setup for the moduli E (fp64) -- Galerkin hierarchy, stencils,
+// coarsest inverse, smoother bounds.  dinv0 is the fine Jacobi inverse in T.
+int mg_setup(MGData* mg, const skt_operator* op, const double* E64, const void* E, const void* dinv0,
+             hipStream_t s);
+
+// z = V(r) for ncols columns in the blocked layout [ncols/kc][ndof][kc];
+// w0 is fine-level scratch of the same size.
+int mg_vcycle(MGData* mg, const skt_operator* op, const void* E, const void* dinv0, int ncols,
+              int kc, const void* r, void* z, void* w0, hipStream_t s);
+
+}  // namespace skt

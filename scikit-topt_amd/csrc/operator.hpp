@@ -39,9 +39,12 @@ struct skt_operator {
 
 namespace skt {
 template <typename T>
-int hex_apply_modal(const skt_operator* op, int k, const T* coef, const T* E, const T* X, T* Y,
-                    hipStream_t s);
-int op_apply(skt_operator* op, int dtype, int ncols, const void* E,
+int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
+                    T* Y, hipStream_t s);
+// Y = K(E) X for ncols columns.  kc == ncols: (ndof, ncols) row-major;
+// otherwise the column-blocked layout [ncols/kc][ndof][kc] (structured hex8
+// only).
+int op_apply(skt_operator* op, int dtype, int ncols, int kc, const void* E,
              const void* X, void* Y, hipStream_t stream);
 int op_diag(skt_operator* op, int dtype, const void* E, void* diag,
             hipStream_t stream);

@@ -563,19 +563,21 @@ static int tet_apply_t(skt_operator* op, int ncols, const T* g, const T* vol, co
   return SKT_OK;
 }
 
-int op_apply(skt_operator* op, int dtype, int ncols, const void* E, const void* X,
+int op_apply(skt_operator* op, int dtype, int ncols, int kc, const void* E, const void* X,
              void* Y, hipStream_t s) {
   SKT_CHECK(op && E && X && Y, "null pointer");
   SKT_CHECK(ncols >= 1 && ncols <= 64, "ncols must be in [1, 64]");
+  SKT_CHECK(kc >= 1 && ncols % kc == 0, "column block must divide ncols");
   SKT_CHECK(X != Y, "operator apply cannot run in place");
   SKT_CHECK(dtype == SKT_F64 || dtype == SKT_F32, "dtype must be SKT_F64 or SKT_F32");
   if (op->mesh == SKT_MESH_HEX8 && op->modal_ok) {
     if (dtype == SKT_F64)
-      return hex_apply_modal<double>(op, ncols, op->modal_d, (const double*)E, (const double*)X,
-                                     (double*)Y, s);
-    return hex_apply_modal<float>(op, ncols, op->modal_f, (const float*)E, (const float*)X,
-                                  (float*)Y, s);
+      return hex_apply_modal<double>(op, ncols, kc, op->modal_d, (const double*)E,
+                                     (const double*)X, (double*)Y, s);
+    return hex_apply_modal<float>(op, ncols, kc, op->modal_f, (const float*)E,
+                                  (const float*)X, (float*)Y, s);
   }
+  SKT_CHECK(kc == ncols, "column-blocked vectors need the structured hex8 operator");
   if (op->mesh == SKT_MESH_HEX8) {
     if (dtype == SKT_F64)
       return hex_apply_t<double>(op, ncols, op->K0d, (const double*)E, (const double*)X,
@@ -806,7 +808,7 @@ int skt_operator_element_matrix(const skt_operator* op, double* host_out) {
 
 int skt_operator_apply(skt_operator* op, int dtype, int ncols, const void* E,
                        const void* X, void* Y, hipStream_t stream) {
-  return op_apply(op, dtype, ncols, E, X, Y, stream);
+  return op_apply(op, dtype, ncols, ncols, E, X, Y, stream);
 }
 
 int skt_operator_diag(skt_operator* op, int dtype, const void* E, void* diag,

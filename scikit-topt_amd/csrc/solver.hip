@@ -1,32 +1,53 @@
-// Batched Jacobi-preconditioned CG over ncols right-hand sides that share one
+// Batched preconditioned CG over ncols right-hand sides that share one
 // operator apply per iteration.  Per-column scalars (alpha, beta, r.z, r.r)
-// live on the device; dot products and norms are accumulated in float64 with
-// a fixed reduction order (no float atomics), so results are bit-reproducible.
-// The host only polls the per-column convergence flags every kCheck steps.
+// live on the device; dot products and norms are accumulated in float64
+// with a fixed reduction order (no float atomics), so results are
+// bit-reproducible.  The host only polls the per-column convergence flags
+// every few iterations.
 //
-// mode SKT_F64: float64 vectors and operator.
+// Working vectors use a column-blocked layout [ncols/kc][ndof][kc]: with kc
+// columns per block the structured operator's workgroups read and write
+// contiguous memory.  The caller's F and U stay (ndof, ncols) row-major and
+// are converted on entry and exit.  For operators without blocked support
+// (tet4) kc = ncols, which is the row-major layout itself.
+//
+// Preconditioners: Jacobi, or the geometric multigrid V-cycle (mg.hip) on
+// structured hex8 grids.
+//
+// mode SKT_F64: float64 vectors, operator and preconditioner.
 // mode SKT_F32: one float32 PCG (float64 dot products) with reliable
 // updates: when a column's recurrence residual has dropped by delta^2 in r.r
 // since the last update (or claims convergence), the float32 correction is
 // folded into the float64 solution, the true residual F - K U is recomputed
 // with the float64 operator and replaces the recurrence residual; the search
-// direction is kept, so the Krylov space is not restarted.  Convergence is
-// only ever declared on a true float64 residual.
+// direction is kept unless the replacement jumped, so the Krylov space is
+// not restarted.  Convergence is only ever declared on a true float64
+// residual.
 #include <cmath>
 #include <cstring>
 #include <vector>
 
+#include "mg.hpp"
 #include "operator.hpp"
 
 struct skt_solver {
   skt_operator* op = nullptr;
   int max_cols = 0;
   int64_t n = 0;
-  double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr, *dinv64 = nullptr;
-  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *d32 = nullptr, *dinv32 = nullptr;
+  // column-blocked working vectors
+  double *x64 = nullptr, *f64 = nullptr, *r64 = nullptr, *p64 = nullptr, *q64 = nullptr,
+         *z64 = nullptr, *dinv64 = nullptr;
+  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *z32 = nullptr, *d32 = nullptr,
+        *dinv32 = nullptr;
   double* part = nullptr;   // 4 * kParts * max_cols: area0 | area1
   double* sc = nullptr;     // max_cols * kSc
   double* host = nullptr;   // pinned copy of sc
+  // preconditioner
+  int precond = SKT_PC_AUTO;
+  int used = SKT_PC_JACOBI;  // the one the last solve ran with
+  skt::MGConfig mgcfg;
+  skt::MGData mg;
+  int mg_possible = -1;      // -1 unknown, 0 no, 1 yes
   // optional per-launch timing of the operator apply (hipEvents on the
   // solve stream, read back at the convergence checks)
   int timing = 0;
@@ -39,10 +60,11 @@ namespace skt {
 
 constexpr int kParts = 512;
 constexpr int kSc = 16;
-constexpr int kCheck = 32;
+constexpr int kCheckJacobi = 32;
+constexpr int kCheckMG = 4;
 // scalar slots
 enum {
-  S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_CONV, S_TOL2, S_RZN, S_SCALE, S_BAD,
+  S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_CONV, S_TOL2, S_BAD,
   S_RREF,    // r.r at the last reliable update (0: no updates, float64 mode)
   S_REQ,     // 1: the column asks for a reliable update
   S_DELTA2,  // reliable-update threshold on r.r / r.r at the last update
@@ -51,35 +73,88 @@ enum {
 };
 constexpr int kMaxRestarts = 8;
 
-// Column-parallel block layout: thread -> (row slice, column c = tid % cp).
-struct ColMap {
-  int cp, rows_per_block;
+// Blocked-layout geometry of one solve.
+struct Blk {
+  int64_t n;      // DOFs
+  int k, kc;      // columns, columns per block
+  int cp;         // kc rounded up to a power of two (lanes per row)
+  int groups;     // k / kc
+  unsigned nbx;   // reduction blocks per group
 };
+
+static Blk make_blk(int64_t n, int k, int kc) {
+  Blk b;
+  b.n = n; b.k = k; b.kc = kc;
+  b.cp = 1;
+  while (b.cp < kc) b.cp <<= 1;
+  b.groups = k / kc;
+  const int rpb = kBlock / b.cp;
+  b.nbx = grid_for(n, rpb, kParts);
+  return b;
+}
 
 __device__ __forceinline__ bool col_live(const double* sc, int c) {
   return sc[c * kSc + S_CONV] == 0.0;
 }
 
-// Fixed-order block reduction over the rows of each column (cp | blockDim).
-__device__ __forceinline__ void col_block_reduce(double* sh, double v, int cp, int ncols,
-                                                 double* out_part, int nb_stride) {
+// Thread -> (group g, lane j, first row) of a reduction kernel launched on
+// a (nbx, groups) grid.
+#define SKT_BLK_SETUP                                              \
+  const int j = threadIdx.x % cp;                                  \
+  const int rpb = blockDim.x / cp;                                 \
+  const int g = blockIdx.y;                                        \
+  const int c = g * kc + j;                                        \
+  const bool lane_on = j < kc;                                     \
+  const int64_t i0 = blockIdx.x * (int64_t)rpb + threadIdx.x / cp; \
+  const int64_t istep = (int64_t)gridDim.x * rpb;
+
+// Fixed-order block reduction of per-lane values into part[q][c][bx].
+__device__ __forceinline__ void blk_reduce(double* sh, double v, int cp, int kc, int k, int q,
+                                           double* part) {
   sh[threadIdx.x] = v;
   __syncthreads();
   for (int s = blockDim.x / 2; s >= cp; s >>= 1) {
     if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
-  if ((int)threadIdx.x < ncols) out_part[(int64_t)threadIdx.x * nb_stride + blockIdx.x] = sh[threadIdx.x];
+  if ((int)threadIdx.x < kc) {
+    const int col = blockIdx.y * kc + threadIdx.x;
+    part[((int64_t)q * k + col) * gridDim.x + blockIdx.x] = sh[threadIdx.x];
+  }
   __syncthreads();
 }
 
+// --------------------------------------------------------- layout moves
+
+// dst[(g n + i) kc + j] = src[i k + g kc + j]
+__global__ void k_to_blocked(int64_t n, int k, int kc, const double* __restrict__ src,
+                             double* __restrict__ dst) {
+  const int64_t total = n * k;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / k;
+    const int c = (int)(t % k);
+    dst[((c / kc) * n + i) * kc + c % kc] = src[t];
+  }
+}
+
+__global__ void k_from_blocked(int64_t n, int k, int kc, const double* __restrict__ src,
+                               double* __restrict__ dst) {
+  const int64_t total = n * k;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / k;
+    const int c = (int)(t % k);
+    dst[t] = src[((c / kc) * n + i) * kc + c % kc];
+  }
+}
+
 // Dirichlet rows of K_e are identity rows, so U[D] = F[D] (enforce semantics)
-__global__ void k_set_fixed(int64_t n, int ncols, const uint8_t* __restrict__ fixed,
+__global__ void k_set_fixed(int64_t n, int kc, int64_t total, const uint8_t* __restrict__ fixed,
                             const double* __restrict__ F, double* __restrict__ U) {
-  const int64_t total = n * ncols;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x)
-    if (fixed[t / ncols]) U[t] = F[t];
+    if (fixed[(t / kc) % n]) U[t] = F[t];
 }
 
 template <typename T>
@@ -90,197 +165,189 @@ __global__ void k_dinv(int64_t n, const T* __restrict__ diag, const uint8_t* __r
     dinv[i] = fixed[i] ? T(0) : T(1) / diag[i];
 }
 
+// ---------------------------------------------------------- PCG kernels
+
 // r = F - q (F is the enforced right-hand side); partial r.r and F.F
 template <typename T>
-__global__ void k_resid(int64_t n, int ncols, int cp, const double* __restrict__ F,
-                        const T* __restrict__ q, T* __restrict__ r,
-                        double* __restrict__ part) {
+__global__ void k_resid(int64_t n, int k, int kc, int cp, const double* __restrict__ F,
+                        const T* __restrict__ q, T* __restrict__ r, double* __restrict__ part) {
   __shared__ double sh[kBlock];
-  const int c = threadIdx.x % cp;
-  const int rpb = blockDim.x / cp;
+  SKT_BLK_SETUP
   double rr = 0.0, bb = 0.0;
-  if (c < ncols) {
-    for (int64_t i = blockIdx.x * (int64_t)rpb + threadIdx.x / cp; i < n;
-         i += (int64_t)gridDim.x * rpb) {
-      const int64_t t = i * ncols + c;
+  if (lane_on) {
+    for (int64_t i = i0; i < n; i += istep) {
+      const int64_t t = ((int64_t)g * n + i) * kc + j;
       const double f = F[t];
-      const double v = f - (double)q[t];
-      const T vt = (T)v;
+      const T vt = (T)(f - (double)q[t]);
       r[t] = vt;
       rr += (double)vt * (double)vt;
       bb += f * f;
     }
   }
-  col_block_reduce(sh, rr, cp, ncols, part, gridDim.x);
-  col_block_reduce(sh, bb, cp, ncols, part + (int64_t)ncols * gridDim.x, gridDim.x);
+  blk_reduce(sh, rr, cp, kc, k, 0, part);
+  blk_reduce(sh, bb, cp, kc, k, 1, part);
 }
 
-// p = dinv r (skipped when p is null); partial r.z
+// partial a.b
 template <typename T>
-__global__ void k_pinit(int64_t n, int ncols, int cp, const T* __restrict__ r,
-                        const T* __restrict__ dinv, T* __restrict__ p,
+__global__ void k_dot(int64_t n, int k, int kc, int cp, const T* __restrict__ a,
+                      const T* __restrict__ b, double* __restrict__ part) {
+  __shared__ double sh[kBlock];
+  SKT_BLK_SETUP
+  double v = 0.0;
+  if (lane_on) {
+    for (int64_t i = i0; i < n; i += istep) {
+      const int64_t t = ((int64_t)g * n + i) * kc + j;
+      v += (double)a[t] * (double)b[t];
+    }
+  }
+  blk_reduce(sh, v, cp, kc, k, 0, part);
+}
+
+// z = dinv r (when z is null the Jacobi z is formed inline), p = z (when p
+// is non-null); partial r.z
+template <typename T>
+__global__ void k_pinit(int64_t n, int k, int kc, int cp, const T* __restrict__ r,
+                        const T* __restrict__ dinv, const T* __restrict__ z, T* __restrict__ p,
                         double* __restrict__ part) {
   __shared__ double sh[kBlock];
-  const int c = threadIdx.x % cp;
-  const int rpb = blockDim.x / cp;
+  SKT_BLK_SETUP
   double rz = 0.0;
-  if (c < ncols) {
-    for (int64_t i = blockIdx.x * (int64_t)rpb + threadIdx.x / cp; i < n;
-         i += (int64_t)gridDim.x * rpb) {
-      const int64_t t = i * ncols + c;
-      const T z = dinv[i] * r[t];
-      if (p) p[t] = z;
-      rz += (double)r[t] * (double)z;
+  if (lane_on) {
+    for (int64_t i = i0; i < n; i += istep) {
+      const int64_t t = ((int64_t)g * n + i) * kc + j;
+      const T zv = z ? z[t] : dinv[i] * r[t];
+      if (p) p[t] = zv;
+      rz += (double)r[t] * (double)zv;
     }
   }
-  col_block_reduce(sh, rz, cp, ncols, part, gridDim.x);
+  blk_reduce(sh, rz, cp, kc, k, 0, part);
 }
 
-// scalars after initialisation: rr, bb from part[0..], rz from part2
-__global__ void k_init_final(int nb, int ncols, const double* __restrict__ part_rb,
-                             const double* __restrict__ part_rz, double rtol2,
-                             double delta2, double* __restrict__ sc) {
-  __shared__ double sh[kBlock / kWave];
-  for (int c = 0; c < ncols; ++c) {
-    double a = 0, b = 0, z = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-      a += part_rb[(int64_t)c * nb + i];
-      b += part_rb[(int64_t)(ncols + c) * nb + i];
-      z += part_rz[(int64_t)c * nb + i];
-    }
-    const double rr = block_sum(a, sh);
-    __syncthreads();
-    const double bb = block_sum(b, sh);
-    __syncthreads();
-    const double rz = block_sum(z, sh);
-    if (threadIdx.x == 0) {
-      double* s = sc + c * kSc;
-      s[S_RR] = rr;
-      s[S_BB] = bb;
-      s[S_RZ] = rz;
-      s[S_TOL2] = rtol2 * bb;
-      s[S_CONV] = (rr <= s[S_TOL2]) ? 1.0 : 0.0;
-      s[S_ALPHA] = 0.0;
-      s[S_BETA] = 0.0;
-      s[S_BAD] = 0.0;
-      s[S_RREF] = delta2 > 0.0 ? rr : 0.0;
-      s[S_REQ] = 0.0;
-      s[S_DELTA2] = delta2;
-      s[S_NRST] = 0.0;
-      if (rr != rr) { s[S_CONV] = 1.0; s[S_BAD] = 1.0; }
-    }
-    __syncthreads();
-  }
-}
-
-// partial p.q
+// x += alpha p; r -= alpha q; partial r.z (Jacobi z = dinv r; 0 when dinv
+// is null -- the preconditioned product is then taken after the V-cycle)
+// and r.r
 template <typename T>
-__global__ void k_dot_pq(int64_t n, int ncols, int cp, const T* __restrict__ p,
-                         const T* __restrict__ q, double* __restrict__ part) {
-  __shared__ double sh[kBlock];
-  const int c = threadIdx.x % cp;
-  const int rpb = blockDim.x / cp;
-  double v = 0.0;
-  if (c < ncols) {
-    for (int64_t i = blockIdx.x * (int64_t)rpb + threadIdx.x / cp; i < n;
-         i += (int64_t)gridDim.x * rpb) {
-      const int64_t t = i * ncols + c;
-      v += (double)p[t] * (double)q[t];
-    }
-  }
-  col_block_reduce(sh, v, cp, ncols, part, gridDim.x);
-}
-
-__global__ void k_alpha(int nb, int ncols, const double* __restrict__ part,
-                        double* __restrict__ sc) {
-  __shared__ double sh[kBlock / kWave];
-  for (int c = 0; c < ncols; ++c) {
-    double a = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) a += part[(int64_t)c * nb + i];
-    const double pq = block_sum(a, sh);
-    if (threadIdx.x == 0) {
-      double* s = sc + c * kSc;
-      s[S_PQ] = pq;
-      if (s[S_CONV] != 0.0) {
-        s[S_ALPHA] = 0.0;
-      } else if (!(pq > 0.0)) {  // breakdown (or NaN): stop this column
-        s[S_ALPHA] = 0.0;
-        s[S_CONV] = 1.0;
-        s[S_BAD] = 1.0;
-      } else {
-        s[S_ALPHA] = s[S_RZ] / pq;
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// x += alpha p; r -= alpha q; partial r.z (z = dinv r) and r.r
-template <typename T>
-__global__ void k_update(int64_t n, int ncols, int cp, const double* __restrict__ sc,
+__global__ void k_update(int64_t n, int k, int kc, int cp, const double* __restrict__ sc,
                          const T* __restrict__ p, const T* __restrict__ q,
                          const T* __restrict__ dinv, T* __restrict__ x, T* __restrict__ r,
                          double* __restrict__ part) {
   __shared__ double sh[kBlock];
-  const int c = threadIdx.x % cp;
-  const int rpb = blockDim.x / cp;
+  SKT_BLK_SETUP
   double rz = 0.0, rr = 0.0;
-  if (c < ncols && col_live(sc, c)) {
+  if (lane_on && col_live(sc, c)) {
     const T alpha = (T)sc[c * kSc + S_ALPHA];
-    for (int64_t i = blockIdx.x * (int64_t)rpb + threadIdx.x / cp; i < n;
-         i += (int64_t)gridDim.x * rpb) {
-      const int64_t t = i * ncols + c;
+    for (int64_t i = i0; i < n; i += istep) {
+      const int64_t t = ((int64_t)g * n + i) * kc + j;
       x[t] += alpha * p[t];
       const T rv = r[t] - alpha * q[t];
       r[t] = rv;
-      const T z = dinv[i] * rv;
-      rz += (double)rv * (double)z;
+      if (dinv) rz += (double)rv * (double)(dinv[i] * rv);
       rr += (double)rv * (double)rv;
     }
   }
-  col_block_reduce(sh, rz, cp, ncols, part, gridDim.x);
-  col_block_reduce(sh, rr, cp, ncols, part + (int64_t)ncols * gridDim.x, gridDim.x);
+  blk_reduce(sh, rz, cp, kc, k, 0, part);
+  blk_reduce(sh, rr, cp, kc, k, 1, part);
 }
 
-__global__ void k_beta(int nb, int ncols, const double* __restrict__ part,
-                       double* __restrict__ sc) {
-  __shared__ double sh[kBlock / kWave];
-  for (int c = 0; c < ncols; ++c) {
-    double a = 0, b = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-      a += part[(int64_t)c * nb + i];
-      b += part[(int64_t)(ncols + c) * nb + i];
-    }
-    const double rz = block_sum(a, sh);
-    __syncthreads();
-    const double rr = block_sum(b, sh);
-    if (threadIdx.x == 0) {
-      double* s = sc + c * kSc;
-      if (s[S_CONV] == 0.0) {
-        s[S_RR] = rr;
-        s[S_BETA] = rz / s[S_RZ];
-        s[S_RZ] = rz;
-        if (rr <= s[S_TOL2]) s[S_CONV] = 1.0;
-        if (rr < s[S_DELTA2] * s[S_RREF]) s[S_REQ] = 1.0;
-        if (rr != rr) { s[S_CONV] = 1.0; s[S_BAD] = 1.0; }
-      }
-    }
-    __syncthreads();
+// p = z + beta p (z null: Jacobi z = dinv r)
+template <typename T>
+__global__ void k_pupdate(int64_t n, int k, int kc, const double* __restrict__ sc,
+                          const T* __restrict__ r, const T* __restrict__ dinv,
+                          const T* __restrict__ z, T* __restrict__ p) {
+  const int64_t total = n * k;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t gi = t / kc;
+    const int c = (int)((gi / n) * kc + t % kc);
+    if (!col_live(sc, c)) continue;
+    const T beta = (T)sc[c * kSc + S_BETA];
+    const T zv = z ? z[t] : dinv[gi % n] * r[t];
+    p[t] = zv + beta * p[t];
   }
 }
 
-// p = dinv r + beta p
+// restart the search direction of columns flagged S_RST: p = z
 template <typename T>
-__global__ void k_pupdate(int64_t n, int ncols, const double* __restrict__ sc,
-                          const T* __restrict__ r, const T* __restrict__ dinv,
-                          T* __restrict__ p) {
-  const int64_t total = n * ncols;
+__global__ void k_restart_dir(int64_t n, int k, int kc, const double* __restrict__ sc,
+                              const T* __restrict__ r, const T* __restrict__ dinv,
+                              const T* __restrict__ z, T* __restrict__ p) {
+  const int64_t total = n * k;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(t % ncols);
-    if (!col_live(sc, c)) continue;
-    const T beta = (T)sc[c * kSc + S_BETA];
-    p[t] = dinv[t / ncols] * r[t] + beta * p[t];
+    const int64_t gi = t / kc;
+    const int c = (int)((gi / n) * kc + t % kc);
+    if (sc[c * kSc + S_RST] != 0.0) p[t] = z ? z[t] : dinv[gi % n] * r[t];
+  }
+}
+
+// Sum of the nb partials of quantity q for column c (one wavefront per
+// column, fixed order).
+__device__ __forceinline__ double col_sum(const double* part, int k, int nb, int q, int c) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kWave) a += part[((int64_t)q * k + c) * nb + i];
+  return wave_sum(a);
+}
+
+// scalars after initialisation: r.r, F.F from part_rb, r.z from part_rz
+__global__ void k_init_final(int nb, int k, const double* __restrict__ part_rb,
+                             const double* __restrict__ part_rz, double rtol2, double delta2,
+                             double* __restrict__ sc) {
+  const int c = blockIdx.x;
+  const double rr = col_sum(part_rb, k, nb, 0, c);
+  const double bb = col_sum(part_rb, k, nb, 1, c);
+  const double rz = col_sum(part_rz, k, nb, 0, c);
+  if (threadIdx.x == 0) {
+    double* s = sc + c * kSc;
+    s[S_RR] = rr;
+    s[S_BB] = bb;
+    s[S_RZ] = rz;
+    s[S_TOL2] = rtol2 * bb;
+    s[S_CONV] = (rr <= s[S_TOL2]) ? 1.0 : 0.0;
+    s[S_ALPHA] = 0.0;
+    s[S_BETA] = 0.0;
+    s[S_BAD] = 0.0;
+    s[S_RREF] = delta2 > 0.0 ? rr : 0.0;
+    s[S_REQ] = 0.0;
+    s[S_DELTA2] = delta2;
+    s[S_NRST] = 0.0;
+    s[S_RST] = 0.0;
+    if (rr != rr) { s[S_CONV] = 1.0; s[S_BAD] = 1.0; }
+  }
+}
+
+__global__ void k_alpha(int nb, int k, const double* __restrict__ part, double* __restrict__ sc) {
+  const int c = blockIdx.x;
+  const double pq = col_sum(part, k, nb, 0, c);
+  if (threadIdx.x == 0) {
+    double* s = sc + c * kSc;
+    s[S_PQ] = pq;
+    if (s[S_CONV] != 0.0) {
+      s[S_ALPHA] = 0.0;
+    } else if (!(pq > 0.0)) {  // breakdown (or NaN): stop this column
+      s[S_ALPHA] = 0.0;
+      s[S_CONV] = 1.0;
+      s[S_BAD] = 1.0;
+    } else {
+      s[S_ALPHA] = s[S_RZ] / pq;
+    }
+  }
+}
+
+__global__ void k_beta(int nb, int k, const double* __restrict__ part, double* __restrict__ sc) {
+  const int c = blockIdx.x;
+  const double rz = col_sum(part, k, nb, 0, c);
+  const double rr = col_sum(part, k, nb, 1, c);
+  if (threadIdx.x == 0) {
+    double* s = sc + c * kSc;
+    if (s[S_CONV] == 0.0) {
+      s[S_RR] = rr;
+      s[S_BETA] = rz / s[S_RZ];
+      s[S_RZ] = rz;
+      if (rr <= s[S_TOL2]) s[S_CONV] = 1.0;
+      if (rr < s[S_DELTA2] * s[S_RREF]) s[S_REQ] = 1.0;
+      if (rr != rr || rz != rz) { s[S_CONV] = 1.0; s[S_BAD] = 1.0; }
+    }
   }
 }
 
@@ -302,105 +369,104 @@ __global__ void k_to32(int64_t total, const double* __restrict__ r64, float* __r
   }
 }
 
-// reliable update, step 4 (only for columns flagged by step 3): restart the
-// search direction, p = dinv r
-template <typename T>
-__global__ void k_restart_dir(int64_t n, int ncols, const double* __restrict__ sc,
-                              const T* __restrict__ r, const T* __restrict__ dinv,
-                              T* __restrict__ p) {
-  const int64_t total = n * ncols;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(t % ncols);
-    if (sc[c * kSc + S_RST] != 0.0) p[t] = dinv[t / ncols] * r[t];
-  }
-}
-
 // reliable update, step 3: r.r (float64 residual) and r.z of its float32
 // copy replace the recurrence values; convergence is decided here only.
 // The search direction is kept unless the true residual has drifted well
 // above the recurrence one (the float32 iteration has lost accuracy, and
 // the old direction no longer fits the new residual) or the float32
 // recurrence broke down; a breakdown is retried up to kMaxRestarts times.
-__global__ void k_replace_final(int nb, int ncols, const double* __restrict__ part_rr,
+__global__ void k_replace_final(int nb, int k, const double* __restrict__ part_rr,
                                 const double* __restrict__ part_rz, double* __restrict__ sc) {
-  __shared__ double sh[kBlock / kWave];
-  for (int c = 0; c < ncols; ++c) {
-    double a = 0, z = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-      a += part_rr[(int64_t)c * nb + i];
-      z += part_rz[(int64_t)c * nb + i];
+  const int c = blockIdx.x;
+  const double rr = col_sum(part_rr, k, nb, 0, c);
+  const double rz = col_sum(part_rz, k, nb, 0, c);
+  if (threadIdx.x == 0) {
+    double* s = sc + c * kSc;
+    s[S_RST] = (rr > 4.0 * s[S_RR]) ? 1.0 : 0.0;
+    s[S_RR] = rr;
+    s[S_RZ] = rz;
+    s[S_RREF] = rr;
+    s[S_REQ] = 0.0;
+    if (s[S_BAD] != 0.0 && s[S_NRST] < kMaxRestarts) {
+      s[S_BAD] = 0.0;
+      s[S_NRST] += 1.0;
+      s[S_RST] = 1.0;
     }
-    const double rr = block_sum(a, sh);
-    __syncthreads();
-    const double rz = block_sum(z, sh);
-    if (threadIdx.x == 0) {
-      double* s = sc + c * kSc;
-      s[S_RST] = (rr > 4.0 * s[S_RR]) ? 1.0 : 0.0;
-      s[S_RR] = rr;
-      s[S_RZ] = rz;
-      s[S_RREF] = rr;
-      s[S_REQ] = 0.0;
-      if (s[S_BAD] != 0.0 && s[S_NRST] < kMaxRestarts) {
-        s[S_BAD] = 0.0;
-        s[S_NRST] += 1.0;
-        s[S_RST] = 1.0;
-      }
-      if (s[S_BAD] == 0.0) s[S_CONV] = (rr <= s[S_TOL2]) ? 1.0 : 0.0;
-      if (rr != rr) { s[S_CONV] = 1.0; s[S_BAD] = 1.0; }
-    }
-    __syncthreads();
+    if (s[S_BAD] == 0.0) s[S_CONV] = (rr <= s[S_TOL2]) ? 1.0 : 0.0;
+    if (rr != rr) { s[S_CONV] = 1.0; s[S_BAD] = 1.0; }
   }
 }
 
-static int colmap_for(int ncols, int* cp) {
-  int c = 1;
-  while (c < ncols) c <<= 1;
-  *cp = c;
-  return kBlock / c;
-}
+// ------------------------------------------------------------ host side
 
-// Copy the per-column scalars to the host and wait for them.
-static int poll(skt_solver* s, int ncols, hipStream_t st) {
-  SKT_HIP(hipMemcpyAsync(s->host, s->sc, sizeof(double) * kSc * ncols, hipMemcpyDeviceToHost, st));
+static int poll(skt_solver* s, int k, hipStream_t st) {
+  SKT_HIP(hipMemcpyAsync(s->host, s->sc, sizeof(double) * kSc * k, hipMemcpyDeviceToHost, st));
   SKT_HIP(hipStreamSynchronize(st));
   return SKT_OK;
 }
 
-static bool all_flag(const skt_solver* s, int ncols, int slot) {
-  for (int c = 0; c < ncols; ++c)
+static bool all_flag(const skt_solver* s, int k, int slot) {
+  for (int c = 0; c < k; ++c)
     if (s->host[c * kSc + slot] == 0.0) return false;
   return true;
 }
 
-static bool any_flag(const skt_solver* s, int ncols, int slot) {
-  for (int c = 0; c < ncols; ++c)
+static bool any_flag(const skt_solver* s, int k, int slot) {
+  for (int c = 0; c < k; ++c)
     if (s->host[c * kSc + slot] != 0.0) return true;
   return false;
 }
 
-// Enqueue `count` PCG iterations on (x, r, p, q) of type T with the Jacobi
-// preconditioner dinv; the scalars sc must be initialised.  Columns that
-// have stopped (S_CONV) are left untouched by every kernel.
+// column block of the solver layout for k columns
+static int kc_for(const skt_operator* op, int k) {
+  if (op->mesh != SKT_MESH_HEX8 || !op->modal_ok) return k;
+  if (op->ncomp == 3) return k % 2 == 0 ? 2 : 1;
+  return k % 4 == 0 ? 4 : (k % 2 == 0 ? 2 : 1);
+}
+
+// Vectors of one precision for the PCG loop.
 template <typename T>
-static int pcg_iterations(skt_solver* s, int dtype, int ncols, const void* E, T* x, T* r, T* p,
-                          T* q, const T* dinv, int64_t count, hipStream_t st) {
-  const int64_t n = s->n;
-  int cp;
-  const int rpb = colmap_for(ncols, &cp);
-  const unsigned nb = grid_for(n, rpb, kParts);
-  const unsigned ge = grid_for(n * ncols);
-  for (int64_t k = 0; k < count; ++k) {
-    if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * k], st));
-    int stt = op_apply(s->op, dtype, ncols, E, p, q, st);
+struct Vecs {
+  T *x, *r, *p, *q, *z, *dinv;
+  const void* E;
+  int dtype;
+};
+
+// z = M r; partial r.z into area (quantity 0)
+template <typename T>
+static int precondition(skt_solver* s, const Blk& b, const Vecs<T>& v, double* area,
+                        hipStream_t st) {
+  if (s->used == SKT_PC_MG) {
+    int stt = mg_vcycle(&s->mg, s->op, v.E, v.dinv, b.k, b.kc, v.r, v.z, v.q, st);
     if (stt) return stt;
-    if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * k + 1], st));
-    hipLaunchKernelGGL(k_dot_pq<T>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, p, q, s->part);
-    hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part, s->sc);
-    hipLaunchKernelGGL(k_update<T>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, s->sc, p, q,
-                       dinv, x, r, s->part);
-    hipLaunchKernelGGL(k_beta, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part, s->sc);
-    hipLaunchKernelGGL(k_pupdate<T>, dim3(ge), dim3(kBlock), 0, st, n, ncols, s->sc, r, dinv, p);
+    hipLaunchKernelGGL(k_dot<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
+                       b.cp, (const T*)v.r, (const T*)v.z, area);
+  }
+  return SKT_OK;
+}
+
+// Enqueue `count` PCG iterations; stopped columns (S_CONV) are untouched.
+template <typename T>
+static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t count,
+                          hipStream_t st) {
+  const bool mg = s->used == SKT_PC_MG;
+  const unsigned ge = grid_for(b.n * b.k);
+  for (int64_t it = 0; it < count; ++it) {
+    if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * it], st));
+    int stt = op_apply(s->op, v.dtype, b.k, b.kc, v.E, v.p, v.q, st);
+    if (stt) return stt;
+    if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * it + 1], st));
+    hipLaunchKernelGGL(k_dot<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
+                       b.cp, (const T*)v.p, (const T*)v.q, s->part);
+    hipLaunchKernelGGL(k_alpha, dim3(b.k), dim3(kWave), 0, st, (int)b.nbx, b.k, s->part, s->sc);
+    hipLaunchKernelGGL(k_update<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
+                       b.cp, s->sc, (const T*)v.p, (const T*)v.q,
+                       mg ? (const T*)nullptr : (const T*)v.dinv, v.x, v.r, s->part);
+    if ((stt = precondition<T>(s, b, v, s->part, st))) return stt;
+    hipLaunchKernelGGL(k_beta, dim3(b.k), dim3(kWave), 0, st, (int)b.nbx, b.k, s->part, s->sc);
+    hipLaunchKernelGGL(k_pupdate<T>, dim3(ge), dim3(kBlock), 0, st, b.n, b.k, b.kc, s->sc,
+                       (const T*)v.r, (const T*)v.dinv, mg ? (const T*)v.z : (const T*)nullptr,
+                       v.p);
   }
   SKT_LAUNCH_CHECK();
   return SKT_OK;
@@ -419,6 +485,104 @@ static int collect_timing(skt_solver* s, int dtype, int64_t count) {
   return SKT_OK;
 }
 
+// Pick the preconditioner and, for multigrid, build its hierarchy for E.
+template <typename T>
+static int setup_precond(skt_solver* s, const Blk& b, const double* E64, const Vecs<T>& v,
+                         hipStream_t st) {
+  s->used = SKT_PC_JACOBI;
+  if (s->precond == SKT_PC_JACOBI) return SKT_OK;
+  if (s->mg_possible < 0) {
+    std::vector<MGLevel> lv;
+    s->mg_possible = (s->op->mesh == SKT_MESH_HEX8 && s->op->modal_ok &&
+                      mg_plan(s->op, s->mgcfg, &lv)) ? 1 : 0;
+  }
+  const bool kc_ok = b.kc <= (s->op->ncomp == 3 ? 2 : 4);
+  if (!s->mg_possible || !kc_ok) {
+    if (s->precond == SKT_PC_MG) {
+      set_error("multigrid needs a structured hex8 grid that coarsens below max_coarse DOFs");
+      return SKT_EINVAL;
+    }
+    return SKT_OK;
+  }
+  int stt = mg_alloc(&s->mg, s->op, s->mgcfg, v.dtype, s->max_cols);
+  if (stt) return stt;
+  stt = mg_setup(&s->mg, s->op, E64, v.E, v.dinv, st);
+  if (stt) return stt;
+  s->used = SKT_PC_MG;
+  return SKT_OK;
+}
+
+template <typename T>
+static int solve_t(skt_solver* s, int mode, const Blk& b, const double* E64, const Vecs<T>& v,
+                   double rtol, int maxiter, double inner_rtol, int64_t* iters, hipStream_t st) {
+  const bool mixed = mode == SKT_F32;
+  const bool mg = s->used == SKT_PC_MG;
+  const int64_t total = b.n * b.k;
+  const unsigned ge = grid_for(total);
+  double* area1 = s->part + (size_t)2 * kParts * s->max_cols;
+  const int check = mg ? kCheckMG : kCheckJacobi;
+  // true residual of the initial guess (float64 operator)
+  int stt = op_apply(s->op, SKT_F64, b.k, b.kc, E64, s->x64, s->q64, st);
+  if (stt) return stt;
+  hipLaunchKernelGGL(k_resid<double>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k,
+                     b.kc, b.cp, (const double*)s->f64, (const double*)s->q64, s->r64, s->part);
+  if (mixed)
+    hipLaunchKernelGGL(k_to32, dim3(ge), dim3(kBlock), 0, st, total, s->r64, s->r32, s->d32);
+  if (mg) {
+    if ((stt = mg_vcycle(&s->mg, s->op, v.E, v.dinv, b.k, b.kc, v.r, v.z, v.q, st))) return stt;
+  }
+  hipLaunchKernelGGL(k_pinit<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
+                     b.cp, (const T*)v.r, (const T*)v.dinv, mg ? (const T*)v.z : (const T*)nullptr,
+                     v.p, area1);
+  const double delta2 = mixed ? inner_rtol * inner_rtol : 0.0;
+  hipLaunchKernelGGL(k_init_final, dim3(b.k), dim3(kWave), 0, st, (int)b.nbx, b.k, s->part, area1,
+                     rtol * rtol, delta2, s->sc);
+  SKT_LAUNCH_CHECK();
+  if ((stt = poll(s, b.k, st))) return stt;
+  int64_t it = 0, updates = 0;
+  bool conv = all_flag(s, b.k, S_CONV) && !any_flag(s, b.k, S_BAD);
+  while (!conv && it < maxiter && !any_flag(s, b.k, S_BAD)) {
+    const int64_t chunk = std::min<int64_t>(check, maxiter - it);
+    if ((stt = pcg_iterations<T>(s, b, v, chunk, st))) return stt;
+    it += chunk;
+    if ((stt = poll(s, b.k, st))) return stt;
+    if ((stt = collect_timing(s, v.dtype, chunk))) return stt;
+    const bool bad = any_flag(s, b.k, S_BAD);
+    const bool all_stop = all_flag(s, b.k, S_CONV);
+    if (!mixed) {
+      if (bad) break;
+      conv = all_stop;
+      continue;
+    }
+    if (!(bad || all_stop || any_flag(s, b.k, S_REQ) || it >= maxiter)) continue;
+    // reliable update of every column
+    hipLaunchKernelGGL(k_accumulate, dim3(ge), dim3(kBlock), 0, st, total, s->d32, s->x64);
+    if ((stt = op_apply(s->op, SKT_F64, b.k, b.kc, E64, s->x64, s->q64, st))) return stt;
+    hipLaunchKernelGGL(k_resid<double>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k,
+                       b.kc, b.cp, (const double*)s->f64, (const double*)s->q64, s->r64, s->part);
+    hipLaunchKernelGGL(k_to32, dim3(ge), dim3(kBlock), 0, st, total, s->r64, s->r32, s->d32);
+    if (mg) {
+      if ((stt = mg_vcycle(&s->mg, s->op, v.E, v.dinv, b.k, b.kc, v.r, v.z, v.q, st))) return stt;
+    }
+    hipLaunchKernelGGL(k_pinit<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
+                       b.cp, (const T*)v.r, (const T*)v.dinv,
+                       mg ? (const T*)v.z : (const T*)nullptr, (T*)nullptr, area1);
+    hipLaunchKernelGGL(k_replace_final, dim3(b.k), dim3(kWave), 0, st, (int)b.nbx, b.k, s->part,
+                       area1, s->sc);
+    hipLaunchKernelGGL(k_restart_dir<T>, dim3(ge), dim3(kBlock), 0, st, b.n, b.k, b.kc, s->sc,
+                       (const T*)v.r, (const T*)v.dinv, mg ? (const T*)v.z : (const T*)nullptr,
+                       v.p);
+    SKT_LAUNCH_CHECK();
+    ++updates;
+    if ((stt = poll(s, b.k, st))) return stt;
+    if (any_flag(s, b.k, S_BAD)) break;
+    conv = all_flag(s, b.k, S_CONV);
+  }
+  iters[0] = it;
+  iters[1] = updates;
+  return conv ? SKT_OK : SKT_ENOCONV;
+}
+
 }  // namespace skt
 
 using namespace skt;
@@ -434,26 +598,46 @@ int skt_solver_create(skt_operator* op, int max_cols, skt_solver** out) {
   s->max_cols = max_cols;
   s->n = op->ndof;
   const size_t nv = (size_t)s->n * max_cols;
-  SKT_HIP(hipMalloc(&s->r64, sizeof(double) * nv));
-  SKT_HIP(hipMalloc(&s->p64, sizeof(double) * nv));
-  SKT_HIP(hipMalloc(&s->q64, sizeof(double) * nv));
+  for (double** p : {&s->x64, &s->f64, &s->r64, &s->p64, &s->q64, &s->z64})
+    SKT_HIP(hipMalloc(p, sizeof(double) * nv));
+  for (float** p : {&s->r32, &s->p32, &s->q32, &s->z32, &s->d32})
+    SKT_HIP(hipMalloc(p, sizeof(float) * nv));
   SKT_HIP(hipMalloc(&s->dinv64, sizeof(double) * s->n));
-  SKT_HIP(hipMalloc(&s->r32, sizeof(float) * nv));
-  SKT_HIP(hipMalloc(&s->p32, sizeof(float) * nv));
-  SKT_HIP(hipMalloc(&s->q32, sizeof(float) * nv));
-  SKT_HIP(hipMalloc(&s->d32, sizeof(float) * nv));
   SKT_HIP(hipMalloc(&s->dinv32, sizeof(float) * s->n));
   SKT_HIP(hipMalloc(&s->part, sizeof(double) * kParts * 4 * max_cols));
-  SKT_HIP(hipMalloc(&s->sc, sizeof(double) * kSc * max_cols * 2));
-  SKT_HIP(hipHostMalloc(&s->host, sizeof(double) * kSc * max_cols * 2, hipHostMallocDefault));
+  SKT_HIP(hipMalloc(&s->sc, sizeof(double) * kSc * max_cols));
+  SKT_HIP(hipHostMalloc(&s->host, sizeof(double) * kSc * max_cols, hipHostMallocDefault));
   *out = s;
+  return SKT_OK;
+}
+
+int skt_solver_configure(skt_solver* s, int precond, int smooth_degree, int max_coarse) {
+  SKT_CHECK(s, "null solver");
+  SKT_CHECK(precond == SKT_PC_JACOBI || precond == SKT_PC_MG || precond == SKT_PC_AUTO,
+            "precond must be SKT_PC_JACOBI, SKT_PC_MG or SKT_PC_AUTO");
+  SKT_CHECK(smooth_degree >= 1 && smooth_degree <= 8, "smooth_degree must be in [1, 8]");
+  SKT_CHECK(max_coarse >= 8 && max_coarse <= 6000, "max_coarse must be in [8, 6000]");
+  s->precond = precond;
+  if (s->mgcfg.degree != smooth_degree || s->mgcfg.max_coarse != max_coarse) {
+    mg_free(&s->mg);
+    s->mg_possible = -1;
+  }
+  s->mgcfg.degree = smooth_degree;
+  s->mgcfg.max_coarse = max_coarse;
+  return SKT_OK;
+}
+
+int skt_solver_info(const skt_solver* s, int* precond_used, int* levels) {
+  SKT_CHECK(s && precond_used && levels, "null pointer");
+  *precond_used = s->used;
+  *levels = s->used == SKT_PC_MG ? (int)s->mg.lv.size() : 1;
   return SKT_OK;
 }
 
 int skt_solver_set_timing(skt_solver* s, int enable) {
   SKT_CHECK(s, "null solver");
   if (enable && !s->ev[0]) {
-    for (int i = 0; i < 2 * kCheck; ++i) SKT_HIP(hipEventCreate(&s->ev[i]));
+    for (int i = 0; i < 2 * 32; ++i) SKT_HIP(hipEventCreate(&s->ev[i]));
   }
   s->timing = enable ? 1 : 0;
   return SKT_OK;
@@ -471,11 +655,12 @@ int skt_solver_get_timing(skt_solver* s, double* apply_ms, int64_t* n_apply, int
 
 int skt_solver_destroy(skt_solver* s) {
   if (!s) return SKT_OK;
-  for (int i = 0; i < 2 * kCheck; ++i)
+  for (int i = 0; i < 2 * 32; ++i)
     if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
-  (void)hipFree(s->r64); (void)hipFree(s->p64); (void)hipFree(s->q64); (void)hipFree(s->dinv64);
-  (void)hipFree(s->r32); (void)hipFree(s->p32); (void)hipFree(s->q32); (void)hipFree(s->d32);
-  (void)hipFree(s->dinv32); (void)hipFree(s->part); (void)hipFree(s->sc);
+  mg_free(&s->mg);
+  for (double* p : {s->x64, s->f64, s->r64, s->p64, s->q64, s->z64, s->dinv64}) (void)hipFree(p);
+  for (float* p : {s->r32, s->p32, s->q32, s->z32, s->d32, s->dinv32}) (void)hipFree(p);
+  (void)hipFree(s->part); (void)hipFree(s->sc);
   (void)hipHostFree(s->host);
   delete s;
   return SKT_OK;
@@ -493,88 +678,42 @@ int skt_solver_solve(skt_solver* s, int mode, int ncols, const double* E64,
   SKT_CHECK(mode == SKT_F64 || (inner_rtol > 0.0 && inner_rtol < 1.0),
             "inner_rtol (reliable-update threshold) must be in (0, 1)");
   const int64_t n = s->n;
-  int cp;
-  const int rpb = colmap_for(ncols, &cp);
-  const unsigned nb = grid_for(n, rpb, kParts);
-  const unsigned ge = grid_for(n * ncols);
-  const unsigned gn = grid_for(n);
-  double* area1 = s->part + (size_t)2 * kParts * s->max_cols;  // r.z partials
+  const Blk b = make_blk(n, ncols, kc_for(s->op, ncols));
+  const int64_t total = n * ncols;
+  const unsigned ge = grid_for(total);
   iters[0] = 0;
   iters[1] = 0;
-  hipLaunchKernelGGL(k_set_fixed, dim3(ge), dim3(kBlock), 0, st, n, ncols, s->op->fixed, F,
-                     U);
-  const int dt = mode == SKT_F64 ? SKT_F64 : SKT_F32;
-  const void* E = mode == SKT_F64 ? (const void*)E64 : (const void*)E32;
-  // preconditioner
-  int stt = op_diag(s->op, dt, E, mode == SKT_F64 ? (void*)s->q64 : (void*)s->q32, st);
-  if (stt) return stt;
-  if (mode == SKT_F64)
-    hipLaunchKernelGGL(k_dinv<double>, dim3(gn), dim3(kBlock), 0, st, n, s->q64, s->op->fixed,
-                       s->dinv64);
-  else
-    hipLaunchKernelGGL(k_dinv<float>, dim3(gn), dim3(kBlock), 0, st, n, s->q32, s->op->fixed,
-                       s->dinv32);
-  // true residual of the initial guess
-  stt = op_apply(s->op, SKT_F64, ncols, E64, U, s->q64, st);
-  if (stt) return stt;
-  hipLaunchKernelGGL(k_resid<double>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, F, s->q64,
-                     s->r64, s->part);
-  if (mode == SKT_F64) {
-    hipLaunchKernelGGL(k_pinit<double>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, s->r64,
-                       s->dinv64, s->p64, area1);
-  } else {
-    hipLaunchKernelGGL(k_to32, dim3(ge), dim3(kBlock), 0, st, n * ncols, s->r64, s->r32, s->d32);
-    hipLaunchKernelGGL(k_pinit<float>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, s->r32,
-                       s->dinv32, s->p32, area1);
-  }
-  const double delta2 = mode == SKT_F64 ? 0.0 : inner_rtol * inner_rtol;
-  hipLaunchKernelGGL(k_init_final, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part,
-                     area1, rtol * rtol, delta2, s->sc);
+  // blocked copies of F and U; U[D] = F[D]
+  hipLaunchKernelGGL(k_to_blocked, dim3(ge), dim3(kBlock), 0, st, n, ncols, b.kc, F, s->f64);
+  hipLaunchKernelGGL(k_to_blocked, dim3(ge), dim3(kBlock), 0, st, n, ncols, b.kc, (const double*)U,
+                     s->x64);
+  hipLaunchKernelGGL(k_set_fixed, dim3(ge), dim3(kBlock), 0, st, n, b.kc, total, s->op->fixed,
+                     (const double*)s->f64, s->x64);
   SKT_LAUNCH_CHECK();
-  if ((stt = poll(s, ncols, st))) return stt;
-  int64_t it = 0, updates = 0;
-  bool conv = all_flag(s, ncols, S_CONV);
-  while (!conv && it < maxiter) {
-    const int64_t chunk = std::min<int64_t>(kCheck, maxiter - it);
-    if (mode == SKT_F64)
-      stt = pcg_iterations<double>(s, SKT_F64, ncols, E64, U, s->r64, s->p64, s->q64,
-                                   s->dinv64, chunk, st);
-    else
-      stt = pcg_iterations<float>(s, SKT_F32, ncols, E32, s->d32, s->r32, s->p32, s->q32,
-                                  s->dinv32, chunk, st);
+  int stt;
+  if (mode == SKT_F64) {
+    stt = op_diag(s->op, SKT_F64, E64, s->q64, st);
     if (stt) return stt;
-    it += chunk;
-    if ((stt = poll(s, ncols, st))) return stt;
-    if ((stt = collect_timing(s, dt, chunk))) return stt;
-    const bool bad = any_flag(s, ncols, S_BAD);
-    const bool all_stop = all_flag(s, ncols, S_CONV);
-    if (mode == SKT_F64) {
-      if (bad) break;
-      conv = all_stop;
-      continue;
-    }
-    if (!(bad || all_stop || any_flag(s, ncols, S_REQ) || it >= maxiter)) continue;
-    // reliable update of every column
-    hipLaunchKernelGGL(k_accumulate, dim3(ge), dim3(kBlock), 0, st, n * ncols, s->d32, U);
-    stt = op_apply(s->op, SKT_F64, ncols, E64, U, s->q64, st);
+    hipLaunchKernelGGL(k_dinv<double>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, s->q64,
+                       s->op->fixed, s->dinv64);
+    Vecs<double> v{s->x64, s->r64, s->p64, s->q64, s->z64, s->dinv64, E64, SKT_F64};
+    if ((stt = setup_precond<double>(s, b, E64, v, st))) return stt;
+    stt = solve_t<double>(s, mode, b, E64, v, rtol, maxiter, inner_rtol, iters, st);
+  } else {
+    stt = op_diag(s->op, SKT_F32, E32, s->q32, st);
     if (stt) return stt;
-    hipLaunchKernelGGL(k_resid<double>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, F,
-                       s->q64, s->r64, s->part);
-    hipLaunchKernelGGL(k_to32, dim3(ge), dim3(kBlock), 0, st, n * ncols, s->r64, s->r32, s->d32);
-    hipLaunchKernelGGL(k_pinit<float>, dim3(nb), dim3(kBlock), 0, st, n, ncols, cp, s->r32,
-                       s->dinv32, (float*)nullptr, area1);
-    hipLaunchKernelGGL(k_replace_final, dim3(1), dim3(kBlock), 0, st, (int)nb, ncols, s->part,
-                       area1, s->sc);
-    hipLaunchKernelGGL(k_restart_dir<float>, dim3(ge), dim3(kBlock), 0, st, n, ncols, s->sc,
-                       s->r32, s->dinv32, s->p32);
-    SKT_LAUNCH_CHECK();
-    ++updates;
-    if ((stt = poll(s, ncols, st))) return stt;
-    if (any_flag(s, ncols, S_BAD)) break;
-    conv = all_flag(s, ncols, S_CONV);
+    hipLaunchKernelGGL(k_dinv<float>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, s->q32,
+                       s->op->fixed, s->dinv32);
+    Vecs<float> v{s->d32, s->r32, s->p32, s->q32, s->z32, s->dinv32, E32, SKT_F32};
+    if ((stt = setup_precond<float>(s, b, E64, v, st))) return stt;
+    stt = solve_t<float>(s, mode, b, E64, v, rtol, maxiter, inner_rtol, iters, st);
   }
-  iters[0] = it;
-  iters[1] = updates;
+  if (stt != SKT_OK && stt != SKT_ENOCONV) return stt;
+  // solution back to the caller's layout
+  hipLaunchKernelGGL(k_from_blocked, dim3(ge), dim3(kBlock), 0, st, n, ncols, b.kc,
+                     (const double*)s->x64, U);
+  SKT_LAUNCH_CHECK();
+  SKT_HIP(hipStreamSynchronize(st));
   bool bad = false;
   for (int c = 0; c < ncols; ++c) {
     const double* h = s->host + c * kSc;
@@ -582,7 +721,7 @@ int skt_solver_solve(skt_solver* s, int mode, int ncols, const double* E64,
     bad = bad || h[S_BAD] != 0.0;
   }
   if (bad) { set_error("CG breakdown (non-positive p.Kp or NaN)"); return SKT_ENOCONV; }
-  if (!conv) { set_error("CG did not reach rtol within maxiter"); return SKT_ENOCONV; }
+  if (stt == SKT_ENOCONV) { set_error("CG did not reach rtol within maxiter"); return SKT_ENOCONV; }
   return SKT_OK;
 }
 

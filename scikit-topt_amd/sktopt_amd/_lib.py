@@ -14,6 +14,7 @@ LIB_PATH = Path(os.environ.get("SKTOPT_AMD_LIB", _HERE / "_lib" / "libsktopt_amd
 
 SKT_OK, SKT_EINVAL, SKT_EHIP, SKT_ENOCONV, SKT_ENOTIMPL = 0, 1, 2, 3, 4
 F64, F32 = 0, 1
+PC_JACOBI, PC_MG, PC_AUTO = 0, 1, 2
 SIMP, RAMP = 0, 1
 ELASTIC, HEAT = 0, 1
 RED_SUM, RED_DOT, RED_MAXABS, RED_MAXABSDIFF = 0, 1, 2, 3
@@ -25,6 +26,7 @@ class ConvergenceError(RuntimeError):
 
 _vp, _i64, _i32, _d, _u8p = C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_void_p
 _i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
 _pp = C.POINTER(C.c_void_p)
 
@@ -49,6 +51,8 @@ SIGNATURES = {
     "skt_operator_energy": [_vp, _i32, _i32, _vp, _vp, _vp, _vp],
     "skt_solver_create": [_vp, _i32, _pp],
     "skt_solver_destroy": [_vp],
+    "skt_solver_configure": [_vp, _i32, _i32, _i32],
+    "skt_solver_info": [_vp, _i32p, _i32p],
     "skt_solver_set_timing": [_vp, _i32],
     "skt_solver_get_timing": [_vp, _dp, _i64p, _i32],
     "skt_solver_solve": [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _d, _i32, _d, _i64p, _dp, _vp],

@@ -74,6 +74,7 @@ class DensityMethodConfig:
     precision: Literal["mixed", "fp64"] = "mixed"
     solver_rtol: float = 1e-8
     inner_rtol: float = 1e-1
+    preconditioner: str = "auto"
     solver_maxiter: int | None = None
     write_checkpoints: bool = False
 
@@ -85,7 +86,8 @@ class DensityMethodConfig:
     def __post_init__(self):
         self.solver_config = fea.LinearSolverConfig(
             solver=self.solver_option, rtol=self.solver_rtol, maxiter=self.solver_maxiter,
-            precision=self.precision, inner_rtol=self.inner_rtol)
+            precision=self.precision, inner_rtol=self.inner_rtol,
+            preconditioner=self.preconditioner)
         if self.solver_option not in fea.solver._SOLVER_NAMES:
             raise ValueError(f"Unknown solver: {self.solver_option}")
 

@@ -49,6 +49,9 @@ class LinearSolverConfig:
     inner_rtol: float = 1e-1          # fp32 residual drop that triggers a reliable update
     warm_start: bool = True           # start from the previous u_dofs
     on_noconv: str = "raise"          # "raise" or "warn"
+    preconditioner: str = "auto"      # "auto" (multigrid where possible), "mg", "jacobi"
+    mg_smooth_degree: int = 2         # Chebyshev-Jacobi steps per smoothing
+    mg_max_coarse: int = 2000         # DOFs of the coarsest (dense) multigrid level
 
 
 def normalize_linear_solver_config(solver="auto", rtol=1e-8, maxiter=None,
@@ -77,6 +80,9 @@ class _GpuFEM:
             raise ValueError(f"Unknown solver: {cfg.solver}")
         if cfg.precision not in ("mixed", "fp64"):
             raise ValueError("precision must be 'mixed' or 'fp64'")
+        pcs = {"auto": _lib.PC_AUTO, "mg": _lib.PC_MG, "jacobi": _lib.PC_JACOBI}
+        if cfg.preconditioner not in pcs:
+            raise ValueError("preconditioner must be 'auto', 'mg' or 'jacobi'")
         self.solver_config = cfg
         self.solver_option = cfg.solver
         dev = _dev.device()
@@ -105,6 +111,8 @@ class _GpuFEM:
         s = C.c_void_p()
         _lib.call("skt_solver_create", h, self.n_tasks, C.byref(s))
         self._solver = s
+        _lib.call("skt_solver_configure", s, pcs[cfg.preconditioner], int(cfg.mg_smooth_degree),
+                  int(cfg.mg_max_coarse))
         self.E64 = torch.empty(self.n_el, dtype=torch.float64, device=dev)
         self.E32 = torch.empty(self.n_el, dtype=torch.float32, device=dev)
         self.F0 = self._build_rhs().contiguous()        # enforced RHS, (ndof, k)
@@ -149,7 +157,11 @@ class _GpuFEM:
             _dev.ptr(self.E32) if mode == _lib.F32 else None, _dev.ptr(self.F), _dev.ptr(U),
             float(cfg.rtol), self._maxiter(), float(cfg.inner_rtol), iters, relres,
             _dev.stream())
+        pc, lev = C.c_int(), C.c_int()
+        _lib.call("skt_solver_info", self._solver, C.byref(pc), C.byref(lev))
         self.last_solve = dict(iters=int(iters[0]), updates=int(iters[1]),
+                               preconditioner=("mg" if pc.value == _lib.PC_MG else "jacobi"),
+                               levels=int(lev.value),
                                relres=list(relres), converged=(st == _lib.SKT_OK))
         self.solve_history.append(self.last_solve)
         if st == _lib.SKT_ENOCONV and cfg.on_noconv == "warn":
