@@ -8,8 +8,10 @@
 // 33 distinct coefficients (elasticity) or a 7-entry diagonal (conduction),
 // instead of a dense 24x24 (8x8) product.
 //
-// Tiling: a workgroup owns a 15x15 patch of nodes in x-y and a chunk of node
-// planes in z.  It walks the element layers bottom-up: each step loads one
+// Tiling: a workgroup owns a 15x15 patch of nodes in x-y (16 along an axis
+// for the last tile when exactly one node is left over: the missing 16th
+// element of that row lies outside the grid) and a chunk of node planes in
+// z.  It walks the element layers bottom-up: each step loads one
 // new node plane (17x17 with halo) into LDS, 256 threads compute the 16x16
 // elements of the layer that touch the patch, park the 8 corner results of
 // each element in LDS, and every owned node sums its 4 contributions from
@@ -28,7 +30,7 @@
 namespace skt {
 
 constexpr int kTE = 16;          // elements per tile edge (one ghost element)
-constexpr int kTN = kTE - 1;     // owned nodes per tile edge
+constexpr int kTN = kTE - 1;     // owned nodes per tile edge (16 on a closing tile)
 constexpr int kPW = kTE + 1;     // node plane width incl. halo
 constexpr int kPN = kPW * kPW;
 
@@ -135,22 +137,31 @@ constexpr int soa_stride(int n, int combos) {
   return s;
 }
 
-template <typename T, int NC, int KC>
+// What the kernel writes for an owned value y = (K x)_i (see OutMode).
+template <typename T>
+struct OutArgs {
+  const T* B = nullptr;        // right-hand side (same layout as X)
+  T* D = nullptr;              // Chebyshev direction (same layout), updated in place
+  const T* dinv = nullptr;     // (ndof) Jacobi inverse
+  const double* cheb = nullptr;  // (c1, c2) of this step, device
+};
+
+template <typename T, int NC, int KC, int MODE>
 __global__ void __launch_bounds__(256)
 k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, int dstride,
                   int lz, int tiles_x, int tiles_y, int n_zchunks, ModalArgs<T> coef,
                   const T* __restrict__ E, const uint8_t* __restrict__ fixed,
-                  const T* __restrict__ X, T* __restrict__ Y) {
+                  const T* __restrict__ X, T* __restrict__ Y, OutArgs<T> oa) {
   // LDS, structure-of-arrays so that consecutive lanes touch consecutive
   // words: two rotating node planes [NC*KC][kPS]; per element lane the
   // x-paired corner sums [side][cy][NC][KC][256] (side 0: node plane below the
   // element, 1: above; cy: node row); the finished output plane [NC*KC][kOS].
   constexpr int kPS = soa_stride(kPN, NC * KC);
-  constexpr int kOS = soa_stride(kTN * kTN, NC * KC);
+  constexpr int kOS = soa_stride(kTE * kTE, NC * KC);
   constexpr int kPlane = kPS * NC * KC;
   constexpr int kItems = kPN * NC * KC;
   constexpr int kPre = (kItems + 255) / 256;
-  constexpr int kOut = kTN * kTN * NC * KC;
+  constexpr int kOut = kTE * kTE * NC * KC;
   constexpr int kWr = (kOut + 255) / 256;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* planes = reinterpret_cast<T*>(smem_raw);
@@ -166,6 +177,13 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   const int64_t cbase = (int64_t)(c0 / lkc) * gstride + (c0 % lkc);
   X += cbase;
   Y += cbase;
+  const T* Bv = MODE != OUT_APPLY ? oa.B + cbase : nullptr;
+  T* Dv = MODE == OUT_CHEB ? oa.D + cbase : nullptr;
+  T c1 = T(0), c2 = T(0);
+  if (MODE == OUT_CHEB) {
+    c1 = (T)oa.cheb[0];
+    c2 = (T)oa.cheb[1];
+  }
   const int64_t x0 = (int64_t)tx * kTN, y0 = (int64_t)ty * kTN;
   const int64_t z0 = (int64_t)zc * lz;
   const int64_t z1 = (z0 + lz < nnz) ? z0 + lz : nnz;
@@ -174,11 +192,15 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   const int64_t ex = x0 - 1 + ex_l, ey = y0 - 1 + ey_l;
   const bool el_in = ex >= 0 && ex < nx && ey >= 0 && ey < ny;
   const int64_t e_base = el_in ? ex + nx * ey : 0;
-  const bool is_node = tid < kTN * kTN;
-  const int nx_l = tid % kTN, ny_l = tid / kTN;
-  const int e_lo = nx_l + kTE * ny_l;  // element lane holding row ny_l's pair
+  // owned nodes: kTN per axis, or all remaining (at most kTE) on the last tile
+  const int own_x = (int)(tx == tiles_x - 1 ? nnx - x0 : kTN);
+  const int own_y = (int)(ty == tiles_y - 1 ? nny - y0 : kTN);
+  const int nx_l = tid % kTE, ny_l = tid / kTE;
+  const bool is_node = nx_l < own_x && ny_l < own_y;
+  const int e_lo = tid;  // element lane holding row ny_l's pair (x0 - 1 + nx_l, y0 - 1 + ny_l)
+  const bool has_up = ny_l + 1 < kTE;  // element row ny_l + 1 exists in the tile
   const int own_slot = node_slot(nx_l + 1, ny_l + 1);  // owned node in the halo plane
-  const bool own_in = is_node && x0 + nx_l < nnx && y0 + ny_l < nny;
+  const bool own_in = is_node;
   const int64_t own_node = (y0 + ny_l) * nnx + (x0 + nx_l);  // in-plane index
 
   int f_off[kPre], f_lds[kPre];
@@ -195,17 +217,22 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
         f_off[r] = (int)(((gy * nnx + gx) * NC + comp) * dstride + col);
     }
   }
-  int w_off[kWr], w_lds[kWr];
+  int w_off[kWr], w_lds[kWr], w_dof[kWr];
 #pragma unroll
   for (int r = 0; r < kWr; ++r) {
     const int i = tid + 256 * r;
     w_off[r] = -1;
     w_lds[r] = 0;
+    w_dof[r] = 0;
     if (i < kOut) {
       const int col = i % KC, comp = (i / KC) % NC, nl = i / (KC * NC);
-      const int64_t gx = x0 + nl % kTN, gy = y0 + nl / kTN;
+      const int lx = nl % kTE, ly = nl / kTE;
+      const int64_t gx = x0 + lx, gy = y0 + ly;
       w_lds[r] = (comp * KC + col) * kOS + nl;
-      if (gx < nnx && gy < nny) w_off[r] = (int)(((gy * nnx + gx) * NC + comp) * dstride + col);
+      if (lx < own_x && ly < own_y) {
+        w_off[r] = (int)(((gy * nnx + gx) * NC + comp) * dstride + col);
+        w_dof[r] = (int)((gy * nnx + gx) * NC + comp);
+      }
     }
   }
 
@@ -281,10 +308,14 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
       for (int col = 0; col < KC; ++col)
 #pragma unroll
         for (int d = 0; d < NC; ++d) {
+          // element row ny_l + 1 is outside the tile only for the 16th node
+          // row of a closing tile, where it is outside the grid as well
           const T below = pairs[(((0 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
-                          pairs[(((0 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE];
+                          (has_up ? pairs[(((0 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE]
+                                  : T(0));
           const T above = pairs[(((1 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
-                          pairs[(((1 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE];
+                          (has_up ? pairs[(((1 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE]
+                                  : T(0));
           oplane[(d * KC + col) * kOS + tid] =
               fix[d] ? pb[(d * KC + col) * kPS + own_slot] : partial[col][d] + below;
           partial[col][d] = above;
@@ -292,10 +323,25 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
     }
     __syncthreads();
     if (ez >= z0) {
-      T* Yp = Y + ez * plane_stride;
+      const int64_t po = ez * plane_stride;
+      T* Yp = Y + po;
 #pragma unroll
-      for (int r = 0; r < kWr; ++r)
-        if (w_off[r] >= 0) Yp[w_off[r]] = oplane[w_lds[r]];
+      for (int r = 0; r < kWr; ++r) {
+        if (w_off[r] < 0) continue;
+        const T y = oplane[w_lds[r]];
+        if (MODE == OUT_APPLY) {
+          Yp[w_off[r]] = y;
+        } else if (MODE == OUT_RESID) {
+          Yp[w_off[r]] = Bv[po + w_off[r]] - y;
+        } else {
+          // Chebyshev step: d = c1 d + c2 dinv (b - K x); x_out = x + d
+          const T dv = c1 == T(0) ? T(0) : c1 * Dv[po + w_off[r]];
+          const T dn = dv + c2 * oa.dinv[ez * nnx * nny * NC + w_dof[r]] *
+                                (Bv[po + w_off[r]] - y);
+          Dv[po + w_off[r]] = dn;
+          Yp[w_off[r]] = X[po + w_off[r]] + dn;
+        }
+      }
     }
     // the bottom plane receives plane ez + 2 and becomes the new top
     stash(planes + bot * kPlane);
@@ -307,17 +353,19 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
 
 // Launch helper for structured grids whose element matrix has the modal
 // pattern.  k columns in the layout described above.
-template <typename T, int NC, int KC>
+template <typename T, int NC, int KC, int MODE>
 static int launch_modal(const skt_operator* op, int k, int lkc, int64_t gstride, int dstride,
-                        const T* coef, const T* E, const T* X, T* Y, hipStream_t s) {
+                        const T* coef, const T* E, const T* X, T* Y, const OutArgs<T>& oa,
+                        hipStream_t s) {
   SKT_CHECK((op->nx + 1) * (op->ny + 1) * NC * (int64_t)dstride < INT32_MAX,
             "node plane too large for 32-bit in-plane offsets");
   SKT_CHECK(k % KC == 0 && lkc % KC == 0, "column blocking does not divide the layout");
   ModalArgs<T> a;
   for (int i = 0; i < 40; ++i) a.c[i] = coef[i];
   const int64_t nnx = op->nx + 1, nny = op->ny + 1, nnz = op->nz + 1;
-  const int tiles_x = (int)((nnx + kTN - 1) / kTN);
-  const int tiles_y = (int)((nny + kTN - 1) / kTN);
+  // the last tile of an axis takes up to kTE nodes, so ceil((nn - 1) / kTN)
+  const int tiles_x = (int)((nnx - 1 + kTN - 1) / kTN);
+  const int tiles_y = (int)((nny - 1 + kTN - 1) / kTN);
   const int cblocks = k / KC;
   // z chunk length: aim for >= 4 workgroups per CU (256 CUs), but keep the
   // one-layer warm-up of each chunk below ~1/6 of its work
@@ -328,23 +376,23 @@ static int launch_modal(const skt_operator* op, int k, int lkc, int64_t gstride,
   const int64_t nblocks = per * nzc;
   const size_t shmem = sizeof(T) * ((size_t)2 * soa_stride(kPN, NC * KC) * NC * KC +
                                     (size_t)4 * NC * KC * 256 +
-                                    (size_t)soa_stride(kTN * kTN, NC * KC) * NC * KC);
+                                    (size_t)soa_stride(kTE * kTE, NC * KC) * NC * KC);
   static bool attr_set = false;
   if (!attr_set) {
-    SKT_HIP(hipFuncSetAttribute((const void*)k_hex_apply_modal<T, NC, KC>,
+    SKT_HIP(hipFuncSetAttribute((const void*)k_hex_apply_modal<T, NC, KC, MODE>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_hex_apply_modal<T, NC, KC>), dim3((unsigned)nblocks), dim3(256), shmem, s,
-                     op->nx, op->ny, op->nz, lkc, gstride, dstride, (int)lz, tiles_x, tiles_y,
-                     nzc, a, E, op->n_fixed > 0 ? op->fixed : nullptr, X, Y);
+  hipLaunchKernelGGL((k_hex_apply_modal<T, NC, KC, MODE>), dim3((unsigned)nblocks), dim3(256),
+                     shmem, s, op->nx, op->ny, op->nz, lkc, gstride, dstride, (int)lz, tiles_x,
+                     tiles_y, nzc, a, E, op->n_fixed > 0 ? op->fixed : nullptr, X, Y, oa);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }
 
-template <typename T>
-int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
-                    T* Y, hipStream_t s) {
+template <typename T, int MODE>
+static int dispatch_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E,
+                          const T* X, T* Y, const OutArgs<T>& oa, hipStream_t s) {
   // kc == k: public (ndof, k) layout; otherwise column-blocked with group kc
   const int lkc = kc;
   const int64_t gstride = (kc == k) ? 0 : op->ndof * kc;
@@ -352,17 +400,56 @@ int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const 
   // two columns per workgroup for elasticity: more columns cut LDS occupancy
   // below three workgroups per CU
   if (op->ncomp == 3) {
-    if (kc % 2 == 0) return launch_modal<T, 3, 2>(op, k, lkc, gstride, dstride, coef, E, X, Y, s);
-    return launch_modal<T, 3, 1>(op, k, lkc, gstride, dstride, coef, E, X, Y, s);
+    if (kc % 2 == 0)
+      return launch_modal<T, 3, 2, MODE>(op, k, lkc, gstride, dstride, coef, E, X, Y, oa, s);
+    return launch_modal<T, 3, 1, MODE>(op, k, lkc, gstride, dstride, coef, E, X, Y, oa, s);
   }
-  if (kc % 4 == 0) return launch_modal<T, 1, 4>(op, k, lkc, gstride, dstride, coef, E, X, Y, s);
-  if (kc % 2 == 0) return launch_modal<T, 1, 2>(op, k, lkc, gstride, dstride, coef, E, X, Y, s);
-  return launch_modal<T, 1, 1>(op, k, lkc, gstride, dstride, coef, E, X, Y, s);
+  if (kc % 4 == 0)
+    return launch_modal<T, 1, 4, MODE>(op, k, lkc, gstride, dstride, coef, E, X, Y, oa, s);
+  if (kc % 2 == 0)
+    return launch_modal<T, 1, 2, MODE>(op, k, lkc, gstride, dstride, coef, E, X, Y, oa, s);
+  return launch_modal<T, 1, 1, MODE>(op, k, lkc, gstride, dstride, coef, E, X, Y, oa, s);
+}
+
+template <typename T>
+int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
+                    T* Y, hipStream_t s) {
+  return dispatch_modal<T, OUT_APPLY>(op, k, kc, coef, E, X, Y, OutArgs<T>(), s);
+}
+
+template <typename T>
+int hex_resid_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E,
+                    const T* X, const T* B, T* R, hipStream_t s) {
+  OutArgs<T> oa;
+  oa.B = B;
+  return dispatch_modal<T, OUT_RESID>(op, k, kc, coef, E, X, R, oa, s);
+}
+
+template <typename T>
+int hex_cheb_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
+                   T* Xout, const T* B, T* D, const T* dinv, const double* cheb, hipStream_t s) {
+  OutArgs<T> oa;
+  oa.B = B;
+  oa.D = D;
+  oa.dinv = dinv;
+  oa.cheb = cheb;
+  return dispatch_modal<T, OUT_CHEB>(op, k, kc, coef, E, X, Xout, oa, s);
 }
 
 template int hex_apply_modal<float>(const skt_operator*, int, int, const float*, const float*,
                                     const float*, float*, hipStream_t);
 template int hex_apply_modal<double>(const skt_operator*, int, int, const double*, const double*,
                                      const double*, double*, hipStream_t);
+template int hex_resid_modal<float>(const skt_operator*, int, int, const float*, const float*,
+                                    const float*, const float*, float*, hipStream_t);
+template int hex_resid_modal<double>(const skt_operator*, int, int, const double*,
+                                     const double*, const double*, const double*, double*,
+                                     hipStream_t);
+template int hex_cheb_modal<float>(const skt_operator*, int, int, const float*, const float*,
+                                   const float*, float*, const float*, float*, const float*,
+                                   const double*, hipStream_t);
+template int hex_cheb_modal<double>(const skt_operator*, int, int, const double*, const double*,
+                                    const double*, double*, const double*, double*,
+                                    const double*, const double*, hipStream_t);
 
 }  // namespace skt

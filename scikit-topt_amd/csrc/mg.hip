@@ -230,14 +230,19 @@ __global__ void k_dense_fix_diag(int64_t n, double* __restrict__ A) {
 // of kGJ rows: with R the remaining indices,
 //   A_BB <- A_BB^-1, A_BR <- A_BB^-1 A_BR, A_RR <- A_RR - A_RB A_BR(new),
 //   A_RB <- -A_RB A_BB^-1.
-constexpr int kGJ = 32;
+// The three off-block updates are LDS-tiled: kTile x kTile tiles for A_RR
+// (the block dimension in two halves), kTileRC-wide strips for A_BR, A_RB.
+constexpr int kGJ = 64;
+constexpr int kTile = 64;
+constexpr int kTileRC = 32;
 
-// invert the kGJ x kGJ pivot block in LDS (plain Gauss-Jordan), write it to
-// Binv (kGJ x kGJ) and back into A
+// invert the pivot block (padded with identity) in LDS, write it to Binv
+// (kGJ x kGJ) and back into A
 __global__ void __launch_bounds__(256) k_gj_block_inv(int64_t n, int64_t b0, int nb,
                                                        double* __restrict__ A,
                                                        double* __restrict__ Binv) {
   __shared__ double M[kGJ][kGJ + 1];
+  constexpr int kPer = kGJ * kGJ / 256;
   const int tid = threadIdx.x;
   for (int i = tid; i < kGJ * kGJ; i += 256) {
     const int r = i / kGJ, c = i % kGJ;
@@ -245,27 +250,26 @@ __global__ void __launch_bounds__(256) k_gj_block_inv(int64_t n, int64_t b0, int
   }
   __syncthreads();
   for (int k = 0; k < kGJ; ++k) {
-    const double p = M[k][k];
-    double rowk = 0.0, colk = 0.0;
-    const int r = tid / kGJ, c = tid % kGJ;  // 8 rows x 32 cols per pass
-    __syncthreads();
-    // new values computed from the old pivot row / column
-    double nv[4];
+    const double ip = 1.0 / M[k][k];
+    double nv[kPer];
 #pragma unroll
-    for (int pass = 0; pass < 4; ++pass) {
-      const int rr = r + 8 * pass;
-      rowk = M[k][c];
-      colk = M[rr][k];
+    for (int e = 0; e < kPer; ++e) {
+      const int i = tid + 256 * e;
+      const int r = i / kGJ, c = i % kGJ;
+      const double rowk = M[k][c] * ip;
       double v;
-      if (rr == k && c == k) v = 1.0 / p;
-      else if (rr == k) v = rowk / p;
-      else if (c == k) v = -colk / p;
-      else v = M[rr][c] - colk * rowk / p;
-      nv[pass] = v;
+      if (r == k && c == k) v = ip;
+      else if (r == k) v = rowk;
+      else if (c == k) v = -M[r][k] * ip;
+      else v = M[r][c] - M[r][k] * rowk;
+      nv[e] = v;
     }
     __syncthreads();
 #pragma unroll
-    for (int pass = 0; pass < 4; ++pass) M[r + 8 * pass][c] = nv[pass];
+    for (int e = 0; e < kPer; ++e) {
+      const int i = tid + 256 * e;
+      M[i / kGJ][i % kGJ] = nv[e];
+    }
     __syncthreads();
   }
   for (int i = tid; i < kGJ * kGJ; i += 256) {
@@ -275,49 +279,101 @@ __global__ void __launch_bounds__(256) k_gj_block_inv(int64_t n, int64_t b0, int
   }
 }
 
-// A_BR <- Binv A_BR (rows b0..b0+nb, columns outside the block)
-__global__ void k_gj_rows(int64_t n, int64_t b0, int nb, const double* __restrict__ Binv,
-                          double* __restrict__ A) {
-  const int64_t total = n;  // one thread per column
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < total;
-       c += (int64_t)gridDim.x * blockDim.x) {
-    if (c >= b0 && c < b0 + nb) continue;
-    double col[kGJ];
-    for (int r = 0; r < nb; ++r) col[r] = A[(b0 + r) * n + c];
-    for (int r = 0; r < nb; ++r) {
-      double v = 0.0;
-      for (int q = 0; q < nb; ++q) v += Binv[r * kGJ + q] * col[q];
-      A[(b0 + r) * n + c] = v;
+// A_BR <- Binv A_BR on a strip of kTileRC columns (columns inside the block
+// are left alone)
+__global__ void __launch_bounds__(256) k_gj_rows(int64_t n, int64_t b0, int nb,
+                                                  const double* __restrict__ Binv,
+                                                  double* __restrict__ A) {
+  __shared__ double sB[kGJ][kGJ + 1];
+  __shared__ double sX[kGJ][kTileRC + 1];
+  const int tid = threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * kTileRC;
+  for (int i = tid; i < kGJ * kGJ; i += 256) sB[i / kGJ][i % kGJ] = Binv[i];
+  for (int i = tid; i < kGJ * kTileRC; i += 256) {
+    const int r = i / kTileRC, c = i % kTileRC;
+    sX[r][c] = (r < nb && c0 + c < n) ? A[(b0 + r) * n + c0 + c] : 0.0;
+  }
+  __syncthreads();
+  for (int i = tid; i < kGJ * kTileRC; i += 256) {
+    const int r = i / kTileRC, c = i % kTileRC;
+    const int64_t col = c0 + c;
+    if (r >= nb || col >= n || (col >= b0 && col < b0 + nb)) continue;
+    double v = 0.0;
+    for (int q = 0; q < nb; ++q) v += sB[r][q] * sX[q][c];
+    A[(b0 + r) * n + col] = v;
+  }
+}
+
+// A_RR <- A_RR - A_RB A_BR on a kTile x kTile tile (rows or columns inside
+// the block are left alone)
+__global__ void __launch_bounds__(256) k_gj_update(int64_t n, int64_t b0, int nb,
+                                                    double* __restrict__ A) {
+  constexpr int kH = kGJ / 2;             // block dimension in two halves
+  __shared__ double sL[kTile][kH + 1];    // A_RB rows of the tile
+  __shared__ double sU[kH][kTile + 1];    // A_BR columns of the tile
+  const int tid = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.y * kTile, c0 = (int64_t)blockIdx.x * kTile;
+  // 4 x 4 outputs per thread: rows tr + 16 a, columns tc + 16 b
+  const int tr = tid / 16, tc = tid % 16;
+  double acc[4][4] = {};
+  for (int h = 0; h < kGJ; h += kH) {
+    for (int i = tid; i < kTile * kH; i += 256) {
+      const int r = i / kH, q = i % kH;
+      sL[r][q] = (r0 + r < n && h + q < nb) ? A[(r0 + r) * n + b0 + h + q] : 0.0;
+    }
+    for (int i = tid; i < kH * kTile; i += 256) {
+      const int q = i / kTile, c = i % kTile;
+      sU[q][c] = (h + q < nb && c0 + c < n) ? A[(b0 + h + q) * n + c0 + c] : 0.0;
+    }
+    __syncthreads();
+    for (int q = 0; q < kH; ++q) {
+      double l[4], u[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) l[a] = sL[tr + 16 * a][q];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) u[b] = sU[q][tc + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] += l[a] * u[b];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t r = r0 + tr + 16 * a;
+    if (r >= n || (r >= b0 && r < b0 + nb)) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int64_t c = c0 + tc + 16 * b;
+      if (c >= n || (c >= b0 && c < b0 + nb)) continue;
+      A[r * n + c] -= acc[a][b];
     }
   }
 }
 
-// A_RR <- A_RR - A_RB A_BR for rows and columns outside the block
-__global__ void k_gj_update(int64_t n, int64_t b0, int nb, double* __restrict__ A) {
-  const int64_t total = n * n;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = t / n, c = t % n;
-    if ((r >= b0 && r < b0 + nb) || (c >= b0 && c < b0 + nb)) continue;
-    double v = A[t];
-    for (int q = 0; q < nb; ++q) v -= A[r * n + b0 + q] * A[(b0 + q) * n + c];
-    A[t] = v;
+// A_RB <- -A_RB Binv on a strip of kTileRC rows (rows inside the block are left
+// alone)
+__global__ void __launch_bounds__(256) k_gj_cols(int64_t n, int64_t b0, int nb,
+                                                  const double* __restrict__ Binv,
+                                                  double* __restrict__ A) {
+  __shared__ double sB[kGJ][kGJ + 1];
+  __shared__ double sX[kTileRC][kGJ + 1];
+  const int tid = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * kTileRC;
+  for (int i = tid; i < kGJ * kGJ; i += 256) sB[i / kGJ][i % kGJ] = Binv[i];
+  for (int i = tid; i < kTileRC * kGJ; i += 256) {
+    const int r = i / kGJ, q = i % kGJ;
+    sX[r][q] = (r0 + r < n && q < nb) ? A[(r0 + r) * n + b0 + q] : 0.0;
   }
-}
-
-// A_RB <- -A_RB Binv
-__global__ void k_gj_cols(int64_t n, int64_t b0, int nb, const double* __restrict__ Binv,
-                          double* __restrict__ A) {
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
-       r += (int64_t)gridDim.x * blockDim.x) {
-    if (r >= b0 && r < b0 + nb) continue;
-    double row[kGJ];
-    for (int q = 0; q < nb; ++q) row[q] = A[r * n + b0 + q];
-    for (int c = 0; c < nb; ++c) {
-      double v = 0.0;
-      for (int q = 0; q < nb; ++q) v += row[q] * Binv[q * kGJ + c];
-      A[r * n + b0 + c] = -v;
-    }
+  __syncthreads();
+  for (int i = tid; i < kTileRC * kGJ; i += 256) {
+    const int r = i / kGJ, c = i % kGJ;
+    const int64_t row = r0 + r;
+    if (c >= nb || row >= n || (row >= b0 && row < b0 + nb)) continue;
+    double v = 0.0;
+    for (int q = 0; q < nb; ++q) v += sX[r][q] * sB[q][c];
+    A[row * n + b0 + c] = -v;
   }
 }
 
@@ -330,50 +386,54 @@ __global__ void k_cast(int64_t n, const double* __restrict__ a, T* __restrict__ 
 
 // ------------------------------------------------------------ operators
 
-// y = S x on a stencil level, KC columns per thread (one column group)
-template <typename T, int NC, int KC>
+// y = S x on a stencil level.  A thread takes one (node, component row) and
+// up to GM column groups, so each stencil row is read once for all of them.
+template <typename T, int NC, int KC, int GM>
 __global__ void k_stencil_apply(int64_t nx, int64_t ny, int64_t nz, int groups,
                                 const T* __restrict__ st, const T* __restrict__ X,
                                 T* __restrict__ Y) {
   const int64_t nnx = nx + 1, nny = ny + 1, nnz = nz + 1;
   const int64_t nn = nnx * nny * nnz;
-  const int64_t total = nn * groups;
+  const int chunks = (groups + GM - 1) / GM;
+  const int64_t total = nn * NC * chunks;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t node = t % nn;
-    const int64_t g = t / nn;
+    const int d = (int)((t / nn) % NC);
+    const int g0 = (int)(t / (nn * NC)) * GM;
+    const int ng = groups - g0 < GM ? groups - g0 : GM;
     const int64_t i = node % nnx, j = (node / nnx) % nny, k = node / (nnx * nny);
-    const T* Xg = X + g * nn * NC * KC;
-    T acc[NC][KC];
+    T acc[GM][KC];
 #pragma unroll
-    for (int d = 0; d < NC; ++d)
+    for (int q = 0; q < GM; ++q)
 #pragma unroll
-      for (int c = 0; c < KC; ++c) acc[d][c] = T(0);
-#pragma unroll
+      for (int c = 0; c < KC; ++c) acc[q][c] = T(0);
+#pragma unroll 1
     for (int o = 0; o < 27; ++o) {
       const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
       const int64_t i2 = i + ox, j2 = j + oy, k2 = k + oz;
       if (i2 < 0 || i2 >= nnx || j2 < 0 || j2 >= nny || k2 < 0 || k2 >= nnz) continue;
       const int64_t m = i2 + nnx * (j2 + nny * k2);
-      T xv[NC][KC];
+      T a[NC];
 #pragma unroll
-      for (int e = 0; e < NC; ++e)
+      for (int e = 0; e < NC; ++e) a[e] = st[((o * NC + d) * NC + e) * nn + node];
 #pragma unroll
-        for (int c = 0; c < KC; ++c) xv[e][c] = Xg[(m * NC + e) * KC + c];
+      for (int q = 0; q < GM; ++q) {
+        if (q >= ng) break;
+        const T* Xm = X + ((int64_t)(g0 + q) * nn + m) * NC * KC;
 #pragma unroll
-      for (int d = 0; d < NC; ++d)
+        for (int e = 0; e < NC; ++e)
 #pragma unroll
-        for (int e = 0; e < NC; ++e) {
-          const T a = st[((o * NC + d) * NC + e) * nn + node];
-#pragma unroll
-          for (int c = 0; c < KC; ++c) acc[d][c] += a * xv[e][c];
-        }
+          for (int c = 0; c < KC; ++c) acc[q][c] += a[e] * Xm[e * KC + c];
+      }
     }
-    T* Yg = Y + g * nn * NC * KC;
 #pragma unroll
-    for (int d = 0; d < NC; ++d)
+    for (int q = 0; q < GM; ++q) {
+      if (q >= ng) break;
+      T* Yn = Y + (((int64_t)(g0 + q) * nn + node) * NC + d) * KC;
 #pragma unroll
-      for (int c = 0; c < KC; ++c) Yg[(node * NC + d) * KC + c] = acc[d][c];
+      for (int c = 0; c < KC; ++c) Yn[c] = acc[q][c];
+    }
   }
 }
 
@@ -396,21 +456,69 @@ __global__ void k_cheb(int64_t total, int64_t ndof, int kc, const double* __rest
   }
 }
 
-// b_c = P^T Z (b - w) for one coarse node and column group
+// Restriction b_c = P^T Z (b - w), separable in two passes.
+// Pass 1 (x, y): t[(I, J, k)] = sum_{dx,dy} w(dx) w(dy) r(f I + dx, f J + dy, k)
+// on every fine z plane k, r = b - w with Dirichlet DOFs removed.
 template <typename T, int NC, int KC>
-__global__ void k_restrict(int64_t fnx, int64_t fny, int64_t fnz, int fx, int fy, int fz,
-                           int groups, const T* __restrict__ b, const T* __restrict__ w,
-                           const uint8_t* __restrict__ fixed, T* __restrict__ bc) {
+__global__ void k_restrict_xy(int64_t fnx, int64_t fny, int64_t fnz, int fx, int fy, int groups,
+                              const T* __restrict__ b, const T* __restrict__ w,
+                              const uint8_t* __restrict__ fixed, T* __restrict__ tbuf) {
   const int64_t fnnx = fnx + 1, fnny = fny + 1, fnnz = fnz + 1;
-  const int64_t cnnx = fnx / fx + 1, cnny = fny / fy + 1, cnnz = fnz / fz + 1;
-  const int64_t cnn = cnnx * cnny * cnnz, fnn = fnnx * fnny * fnnz;
+  const int64_t cnnx = fnx / fx + 1, cnny = fny / fy + 1;
+  const int64_t tn = cnnx * cnny * fnnz, fnn = fnnx * fnny * fnnz;
+  const int64_t total = tn * groups;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t tnode = t % tn, g = t / tn;
+    const int64_t I = tnode % cnnx, J = (tnode / cnnx) % cnny, k = tnode / (cnnx * cnny);
+    const T* bg = b + g * fnn * NC * KC;
+    const T* wg = w ? w + g * fnn * NC * KC : nullptr;
+    T acc[NC][KC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int c = 0; c < KC; ++c) acc[d][c] = T(0);
+    for (int dy = -(fy - 1); dy <= fy - 1; ++dy) {
+      const int64_t j = J * fy + dy;
+      if (j < 0 || j >= fnny) continue;
+      const T wy = T(1) - (T)(dy < 0 ? -dy : dy) / (T)fy;
+      for (int dx = -(fx - 1); dx <= fx - 1; ++dx) {
+        const int64_t i = I * fx + dx;
+        if (i < 0 || i >= fnnx) continue;
+        const T wgt = (T(1) - (T)(dx < 0 ? -dx : dx) / (T)fx) * wy;
+        const int64_t fn = i + fnnx * (j + fnny * k);
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+          if (fixed && fixed[fn * NC + d]) continue;
+#pragma unroll
+          for (int c = 0; c < KC; ++c) {
+            const int64_t idx = (fn * NC + d) * KC + c;
+            const T r = wg ? bg[idx] - wg[idx] : bg[idx];
+            acc[d][c] += wgt * r;
+          }
+        }
+      }
+    }
+    T* out = tbuf + g * tn * NC * KC;
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int c = 0; c < KC; ++c) out[(tnode * NC + d) * KC + c] = acc[d][c];
+  }
+}
+
+// Pass 2 (z): b_c(I, J, K) = sum_dz w(dz) t(I, J, f K + dz)
+template <typename T, int NC, int KC>
+__global__ void k_restrict_z(int64_t cnnx, int64_t cnny, int64_t fnz, int fz, int groups,
+                             const T* __restrict__ tbuf, T* __restrict__ bc) {
+  const int64_t fnnz = fnz + 1, cnnz = fnz / fz + 1;
+  const int64_t tn = cnnx * cnny * fnnz, cnn = cnnx * cnny * cnnz;
   const int64_t total = cnn * groups;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t cnode = t % cnn, g = t / cnn;
-    const int64_t I = cnode % cnnx, J = (cnode / cnnx) % cnny, K = cnode / (cnnx * cnny);
-    const T* bg = b + g * fnn * NC * KC;
-    const T* wg = w ? w + g * fnn * NC * KC : nullptr;
+    const int64_t IJ = cnode % (cnnx * cnny), K = cnode / (cnnx * cnny);
+    const T* tg = tbuf + g * tn * NC * KC;
     T acc[NC][KC];
 #pragma unroll
     for (int d = 0; d < NC; ++d)
@@ -420,27 +528,11 @@ __global__ void k_restrict(int64_t fnx, int64_t fny, int64_t fnz, int fx, int fy
       const int64_t k = K * fz + dz;
       if (k < 0 || k >= fnnz) continue;
       const T wz = T(1) - (T)(dz < 0 ? -dz : dz) / (T)fz;
-      for (int dy = -(fy - 1); dy <= fy - 1; ++dy) {
-        const int64_t j = J * fy + dy;
-        if (j < 0 || j >= fnny) continue;
-        const T wy = T(1) - (T)(dy < 0 ? -dy : dy) / (T)fy;
-        for (int dx = -(fx - 1); dx <= fx - 1; ++dx) {
-          const int64_t i = I * fx + dx;
-          if (i < 0 || i >= fnnx) continue;
-          const T wgt = (T(1) - (T)(dx < 0 ? -dx : dx) / (T)fx) * wy * wz;
-          const int64_t fn = i + fnnx * (j + fnny * k);
+      const int64_t tnode = IJ + cnnx * cnny * k;
 #pragma unroll
-          for (int d = 0; d < NC; ++d) {
-            if (fixed && fixed[fn * NC + d]) continue;
+      for (int d = 0; d < NC; ++d)
 #pragma unroll
-            for (int c = 0; c < KC; ++c) {
-              const int64_t idx = (fn * NC + d) * KC + c;
-              const T r = wg ? bg[idx] - wg[idx] : bg[idx];
-              acc[d][c] += wgt * r;
-            }
-          }
-        }
-      }
+        for (int c = 0; c < KC; ++c) acc[d][c] += wz * tg[(tnode * NC + d) * KC + c];
     }
     T* out = bc + g * cnn * NC * KC;
 #pragma unroll
@@ -606,7 +698,8 @@ void mg_free(MGData* mg) {
   for (MGLevel& l : mg->lv) {
     (void)hipFree(l.Kel); (void)hipFree(l.st); (void)hipFree(l.dinv);
     (void)hipFree(l.b); (void)hipFree(l.x); (void)hipFree(l.w); (void)hipFree(l.d);
-    l.Kel = nullptr; l.st = l.dinv = l.b = l.x = l.w = l.d = nullptr;
+    (void)hipFree(l.rt);
+    l.Kel = nullptr; l.st = l.dinv = l.b = l.x = l.w = l.d = l.rt = nullptr;
   }
   (void)hipFree(mg->Ad); (void)hipFree(mg->Ad2); (void)hipFree(mg->Ainv);
   (void)hipFree(mg->lam); (void)hipFree(mg->coef); (void)hipFree(mg->part); (void)hipFree(mg->pv);
@@ -634,9 +727,12 @@ int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype,
     const size_t nv = (size_t)v.ndof * max_cols;
     if (l == 0) {
       SKT_HIP(hipMalloc(&v.d, ts * nv));
+      SKT_HIP(hipMalloc(&v.x, ts * nv));
       continue;
     }
     SKT_HIP(hipMalloc(&v.Kel, sizeof(double) * (size_t)(v.nx * v.ny * v.nz) * ND * ND));
+    SKT_HIP(hipMalloc(&v.rt, ts * (size_t)(v.nx + 1) * (v.ny + 1) * (lv[l - 1].nz + 1) * NC *
+                                 max_cols));
     SKT_HIP(hipMalloc(&v.b, ts * nv));
     SKT_HIP(hipMalloc(&v.x, ts * nv));
     if (l < L) {
@@ -682,10 +778,11 @@ static int level_apply(MGData* mg, const skt_operator* op, const void* E, int l,
   if (l == 0) return op_apply(const_cast<skt_operator*>(op), mg->dtype, k, kc, E, x, y, s);
   const MGLevel& v = mg->lv[l];
   const int groups = k / kc;
-  const unsigned g = grid_for(v.nn * groups);
+  constexpr int GM = 4;
+  const unsigned g = grid_for(v.nn * op->ncomp * ((groups + GM - 1) / GM));
   const T* st = (const T*)v.st;
-#define SKT_SA(NC_, KC_)                                                                   \
-  hipLaunchKernelGGL((k_stencil_apply<T, NC_, KC_>), dim3(g), dim3(kBlock), 0, s, v.nx, v.ny, \
+#define SKT_SA(NC_, KC_)                                                                       \
+  hipLaunchKernelGGL((k_stencil_apply<T, NC_, KC_, GM>), dim3(g), dim3(kBlock), 0, s, v.nx, v.ny, \
                      v.nz, groups, st, x, y)
   if (op->ncomp == 3) {
     if (kc == 1) SKT_SA(3, 1);
@@ -750,15 +847,14 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
       hipLaunchKernelGGL((k_dense_asm<1>), dim3(grid_for(n * n)), dim3(kBlock), 0, s, c.nx, c.ny,
                          c.nz, c.Kel, mg->Ad);
     hipLaunchKernelGGL(k_dense_fix_diag, dim3(grid_for(n)), dim3(kBlock), 0, s, n, mg->Ad);
+    const unsigned tiles = (unsigned)((n + kTile - 1) / kTile);
+    const unsigned strips = (unsigned)((n + kTileRC - 1) / kTileRC);
     for (int64_t b0 = 0; b0 < n; b0 += kGJ) {
       const int nb = (int)std::min<int64_t>(kGJ, n - b0);
       hipLaunchKernelGGL(k_gj_block_inv, dim3(1), dim3(256), 0, s, n, b0, nb, mg->Ad, mg->Ad2);
-      hipLaunchKernelGGL(k_gj_rows, dim3(grid_for(n)), dim3(kBlock), 0, s, n, b0, nb, mg->Ad2,
-                         mg->Ad);
-      hipLaunchKernelGGL(k_gj_update, dim3(grid_for(n * n)), dim3(kBlock), 0, s, n, b0, nb,
-                         mg->Ad);
-      hipLaunchKernelGGL(k_gj_cols, dim3(grid_for(n)), dim3(kBlock), 0, s, n, b0, nb, mg->Ad2,
-                         mg->Ad);
+      hipLaunchKernelGGL(k_gj_rows, dim3(strips), dim3(256), 0, s, n, b0, nb, mg->Ad2, mg->Ad);
+      hipLaunchKernelGGL(k_gj_update, dim3(tiles, tiles), dim3(256), 0, s, n, b0, nb, mg->Ad);
+      hipLaunchKernelGGL(k_gj_cols, dim3(strips), dim3(256), 0, s, n, b0, nb, mg->Ad2, mg->Ad);
     }
     hipLaunchKernelGGL(k_cast<T>, dim3(grid_for(n * n)), dim3(kBlock), 0, s, n * n, mg->Ad,
                        (T*)mg->Ainv);
@@ -805,8 +901,12 @@ int mg_setup(MGData* mg, const skt_operator* op, const double* E64, const void* 
 template <typename T, int NC, int KC>
 static void launch_restrict(const MGLevel& f, const MGLevel& c, int groups, const T* b,
                             const T* w, const uint8_t* fixed, T* bc, hipStream_t s) {
-  hipLaunchKernelGGL((k_restrict<T, NC, KC>), dim3(grid_for(c.nn * groups)), dim3(kBlock), 0, s,
-                     f.nx, f.ny, f.nz, c.f[0], c.f[1], c.f[2], groups, b, w, fixed, bc);
+  const int64_t tn = (c.nx + 1) * (c.ny + 1) * (f.nz + 1);
+  T* tbuf = (T*)c.rt;
+  hipLaunchKernelGGL((k_restrict_xy<T, NC, KC>), dim3(grid_for(tn * groups)), dim3(kBlock), 0, s,
+                     f.nx, f.ny, f.nz, c.f[0], c.f[1], groups, b, w, fixed, tbuf);
+  hipLaunchKernelGGL((k_restrict_z<T, NC, KC>), dim3(grid_for(c.nn * groups)), dim3(kBlock), 0,
+                     s, c.nx + 1, c.ny + 1, f.nz, c.f[2], groups, (const T*)tbuf, bc);
 }
 
 template <typename T, int NC, int KC>
@@ -816,6 +916,17 @@ static void launch_prolong(const MGLevel& f, const MGLevel& c, int groups, const
                      s, f.nx, f.ny, f.nz, c.f[0], c.f[1], c.f[2], groups, xc, fixed, xf);
 }
 
+template <typename T>
+static const T* modal_coef(const skt_operator* op);
+template <>
+const double* modal_coef<double>(const skt_operator* op) { return op->modal_d; }
+template <>
+const float* modal_coef<float>(const skt_operator* op) { return op->modal_f; }
+
+// V-cycle.  On the fine level the Chebyshev steps and the residual are fused
+// into the matrix-free apply (hex_cheb_modal / hex_resid_modal), so the
+// iterate ping-pongs between z and lv[0].x; the number of fused steps,
+// 2 * degree - 1, is odd, and starting in lv[0].x the cycle ends in z.
 template <typename T, int NC, int KC>
 static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const void* dinv0, int k,
                     const T* r, T* z, T* w0, hipStream_t s) {
@@ -823,15 +934,38 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
   const int groups = k / KC;
   const int deg = mg->degree;
   const uint8_t* fixed0 = op->n_fixed > 0 ? op->fixed : nullptr;
+  const T* coef0 = modal_coef<T>(op);
+  const T* E0 = (const T*)E;
   int stt;
-  // downward sweep
-  for (int l = 0; l < L; ++l) {
-    MGLevel& v = mg->lv[l];
-    const T* b = l == 0 ? r : (const T*)v.b;
-    T* x = l == 0 ? z : (T*)v.x;
-    T* w = l == 0 ? w0 : (T*)v.w;
+  T* cur = (T*)mg->lv[0].x;  // fine iterate (ping-pong with z)
+  T* oth = z;
+  // fine level, pre-smoothing and restricted residual
+  {
+    MGLevel& v = mg->lv[0];
+    const int64_t total = v.ndof * k;
+    const double* cf = mg->coef;
     T* d = (T*)v.d;
-    const T* dinv = l == 0 ? (const T*)dinv0 : (const T*)v.dinv;
+    hipLaunchKernelGGL(k_cheb<T>, dim3(grid_for(total)), dim3(kBlock), 0, s, total, v.ndof, KC,
+                       cf, r, (const T*)nullptr, (const T*)dinv0, d, cur, 1);
+    for (int st = 1; st < deg; ++st) {
+      if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, d, (const T*)dinv0,
+                                   cf + 2 * st, s)))
+        return stt;
+      std::swap(cur, oth);
+    }
+    if ((stt = hex_resid_modal<T>(op, k, KC, coef0, E0, cur, r, w0, s))) return stt;
+    launch_restrict<T, NC, KC>(v, mg->lv[1], groups, (const T*)w0, (const T*)nullptr, fixed0,
+                               (T*)mg->lv[1].b, s);
+    SKT_LAUNCH_CHECK();
+  }
+  // coarse levels, downward
+  for (int l = 1; l < L; ++l) {
+    MGLevel& v = mg->lv[l];
+    const T* b = (const T*)v.b;
+    T* x = (T*)v.x;
+    T* w = (T*)v.w;
+    T* d = (T*)v.d;
+    const T* dinv = (const T*)v.dinv;
     const int64_t total = v.ndof * k;
     const unsigned ge = grid_for(total);
     const double* cf = mg->coef + (size_t)l * 2 * deg;
@@ -843,8 +977,7 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
                          b, (const T*)w, dinv, d, x, 0);
     }
     if ((stt = level_apply<T>(mg, op, E, l, k, KC, x, w, s))) return stt;
-    launch_restrict<T, NC, KC>(v, mg->lv[l + 1], groups, b, w, l == 0 ? fixed0 : nullptr,
-                               (T*)mg->lv[l + 1].b, s);
+    launch_restrict<T, NC, KC>(v, mg->lv[l + 1], groups, b, w, nullptr, (T*)mg->lv[l + 1].b, s);
     SKT_LAUNCH_CHECK();
   }
   // coarsest
@@ -856,25 +989,42 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
                        (T*)c.x);
     SKT_LAUNCH_CHECK();
   }
-  // upward sweep
-  for (int l = L - 1; l >= 0; --l) {
+  // coarse levels, upward
+  for (int l = L - 1; l >= 1; --l) {
     MGLevel& v = mg->lv[l];
-    const T* b = l == 0 ? r : (const T*)v.b;
-    T* x = l == 0 ? z : (T*)v.x;
-    T* w = l == 0 ? w0 : (T*)v.w;
+    const T* b = (const T*)v.b;
+    T* x = (T*)v.x;
+    T* w = (T*)v.w;
     T* d = (T*)v.d;
-    const T* dinv = l == 0 ? (const T*)dinv0 : (const T*)v.dinv;
+    const T* dinv = (const T*)v.dinv;
     const int64_t total = v.ndof * k;
     const unsigned ge = grid_for(total);
     const double* cf = mg->coef + (size_t)l * 2 * deg;
-    launch_prolong<T, NC, KC>(v, mg->lv[l + 1], groups, (const T*)mg->lv[l + 1].x,
-                              l == 0 ? fixed0 : nullptr, x, s);
+    launch_prolong<T, NC, KC>(v, mg->lv[l + 1], groups, (const T*)mg->lv[l + 1].x, nullptr, x, s);
     for (int st = 0; st < deg; ++st) {
       if ((stt = level_apply<T>(mg, op, E, l, k, KC, x, w, s))) return stt;
       hipLaunchKernelGGL(k_cheb<T>, dim3(ge), dim3(kBlock), 0, s, total, v.ndof, KC, cf + 2 * st,
                          b, (const T*)w, dinv, d, x, 0);
     }
     SKT_LAUNCH_CHECK();
+  }
+  // fine level, correction and post-smoothing
+  {
+    MGLevel& v = mg->lv[0];
+    const double* cf = mg->coef;
+    T* d = (T*)v.d;
+    launch_prolong<T, NC, KC>(v, mg->lv[1], groups, (const T*)mg->lv[1].x, fixed0, cur, s);
+    for (int st = 0; st < deg; ++st) {
+      if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, d, (const T*)dinv0,
+                                   cf + 2 * st, s)))
+        return stt;
+      std::swap(cur, oth);
+    }
+    SKT_LAUNCH_CHECK();
+  }
+  if (cur != z) {
+    set_error("internal: multigrid iterate did not end in z");
+    return SKT_EINVAL;
   }
   return SKT_OK;
 }

@@ -30,14 +30,15 @@ struct MGLevel {
   void* x = nullptr;                  // iterate
   void* w = nullptr;                  // operator output
   void* d = nullptr;                  // Chebyshev direction
+  void* rt = nullptr;                 // restriction staging (x-y restricted fine planes)
 };
 
 struct MGData {
   int dtype = -1;                     // SKT_F64 / SKT_F32 the buffers hold
   int max_cols = 0;
-  std::vector<MGLevel> lv;            // lv[0]: fine level (vectors: d only)
+  std::vector<MGLevel> lv;            // lv[0]: fine level (vectors: d, x only)
   double* Ad = nullptr;               // coarsest dense matrix, fp64 (n x n), ping
-  double* Ad2 = nullptr;              // pong
+  double* Ad2 = nullptr;              // inverse of the current pivot block
   void* Ainv = nullptr;               // coarsest inverse in T
   double* lam = nullptr;              // per level lambda_max(D^-1 A) estimates
   double* coef = nullptr;             // per level Chebyshev (c1, c2) per step

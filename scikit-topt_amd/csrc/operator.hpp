@@ -38,9 +38,20 @@ struct skt_operator {
 };
 
 namespace skt {
+// Output modes of the structured hex8 modal kernel.
+enum { OUT_APPLY = 0, OUT_RESID = 1, OUT_CHEB = 2 };
 template <typename T>
 int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
                     T* Y, hipStream_t s);
+// R = B - K X
+template <typename T>
+int hex_resid_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E,
+                    const T* X, const T* B, T* R, hipStream_t s);
+// Chebyshev-Jacobi step: D = c1 D + c2 dinv (B - K X); Xout = X + D,
+// (c1, c2) = cheb[0..1] on the device; Xout must not alias X.
+template <typename T>
+int hex_cheb_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
+                   T* Xout, const T* B, T* D, const T* dinv, const double* cheb, hipStream_t s);
 // Y = K(E) X for ncols columns.  kc == ncols: (ndof, ncols) row-major;
 // otherwise the column-blocked layout [ncols/kc][ndof][kc] (structured hex8
 // only).

@@ -293,10 +293,14 @@ def test_oc_bisection_matches_oracle(gpu, it_seed):
 
 @pytest.mark.parametrize("grid,physics,ks", [((31, 17, 9), "elastic", (1, 2, 3, 4, 8)),
                                               ((16, 16, 16), "elastic", (8,)),
-                                              ((33, 20, 7), "heat", (1, 2, 4, 5))])
+                                              ((30, 15, 7), "elastic", (1, 2, 8)),
+                                              ((15, 45, 4), "elastic", (2,)),
+                                              ((33, 20, 7), "heat", (1, 2, 4, 5)),
+                                              ((30, 16, 6), "heat", (1, 4))])
 def test_hex8_tiled_apply_matches_assembled(gpu, grid, physics, ks):
-    """Multi-tile / multi-z-chunk grids (tiles of 15x15 nodes) for the modal
-    kernel, every column blocking, float64 and float32."""
+    """Multi-tile / multi-z-chunk grids for the modal kernel (tiles of 15x15
+    nodes; 16 along an axis whose node count is 1 mod 15, e.g. 31 and 16
+    nodes), every column blocking, float64 and float32."""
     import torch
     from sktopt_amd import _lib
     task = oprob.cantilever(*grid) if physics == "elastic" else oprob.heat_patch(grid[0], 3)
