@@ -263,6 +263,23 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   for (int64_t ez = z0 - 1; ez < z1; ++ez) {
     fetch(ez + 2);  // in flight while this layer is computed
     const T Ee = (el_in && ez >= 0 && ez < nz) ? E[e_base + nx * ny * ez] : T(0);
+    // operands of the fused output stage for plane ez, loaded now so that
+    // their latency hides behind this layer's compute
+    T ob[MODE != OUT_APPLY ? kWr : 1], od[MODE == OUT_CHEB ? kWr : 1],
+        ox[MODE == OUT_CHEB ? kWr : 1], oi[MODE == OUT_CHEB ? kWr : 1];
+    if (MODE != OUT_APPLY && ez >= z0) {
+      const int64_t po = ez * plane_stride;
+#pragma unroll
+      for (int r = 0; r < kWr; ++r) {
+        if (w_off[r] < 0) continue;
+        ob[r] = Bv[po + w_off[r]];
+        if constexpr (MODE == OUT_CHEB) {
+          od[r] = c1 == T(0) ? T(0) : Dv[po + w_off[r]];
+          ox[r] = X[po + w_off[r]];
+          oi[r] = oa.dinv[ez * nnx * nny * NC + w_dof[r]];
+        }
+      }
+    }
     __syncthreads();
     const T* pb = planes + bot * kPlane;
     const T* pt = planes + top * kPlane;
@@ -329,17 +346,15 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
       for (int r = 0; r < kWr; ++r) {
         if (w_off[r] < 0) continue;
         const T y = oplane[w_lds[r]];
-        if (MODE == OUT_APPLY) {
+        if constexpr (MODE == OUT_APPLY) {
           Yp[w_off[r]] = y;
-        } else if (MODE == OUT_RESID) {
-          Yp[w_off[r]] = Bv[po + w_off[r]] - y;
+        } else if constexpr (MODE == OUT_RESID) {
+          Yp[w_off[r]] = ob[r] - y;
         } else {
           // Chebyshev step: d = c1 d + c2 dinv (b - K x); x_out = x + d
-          const T dv = c1 == T(0) ? T(0) : c1 * Dv[po + w_off[r]];
-          const T dn = dv + c2 * oa.dinv[ez * nnx * nny * NC + w_dof[r]] *
-                                (Bv[po + w_off[r]] - y);
+          const T dn = c1 * od[r] + c2 * oi[r] * (ob[r] - y);
           Dv[po + w_off[r]] = dn;
-          Yp[w_off[r]] = X[po + w_off[r]] + dn;
+          Yp[w_off[r]] = ox[r] + dn;
         }
       }
     }

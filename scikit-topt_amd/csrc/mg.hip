@@ -236,47 +236,64 @@ constexpr int kGJ = 64;
 constexpr int kTile = 64;
 constexpr int kTileRC = 32;
 
-// invert the pivot block (padded with identity) in LDS, write it to Binv
-// (kGJ x kGJ) and back into A
+// invert the pivot block (padded with identity), write it to Binv
+// (kGJ x kGJ) and back into A.  Each thread keeps a 4 x 4 block of the
+// matrix in registers; per step the owners of the pivot row and column
+// publish them through LDS.
 __global__ void __launch_bounds__(256) k_gj_block_inv(int64_t n, int64_t b0, int nb,
                                                        double* __restrict__ A,
                                                        double* __restrict__ Binv) {
-  __shared__ double M[kGJ][kGJ + 1];
-  constexpr int kPer = kGJ * kGJ / 256;
+  static_assert(kGJ == 64, "4 x 4 register blocks of a 64 x 64 matrix");
+  __shared__ double prow[kGJ], pcol[kGJ];
   const int tid = threadIdx.x;
-  for (int i = tid; i < kGJ * kGJ; i += 256) {
-    const int r = i / kGJ, c = i % kGJ;
-    M[r][c] = (r < nb && c < nb) ? A[(b0 + r) * n + b0 + c] : (r == c ? 1.0 : 0.0);
-  }
-  __syncthreads();
+  const int tr = tid / 16, tc = tid % 16;  // rows 4 tr.., columns 4 tc..
+  double m[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = 4 * tr + i, c = 4 * tc + j;
+      m[i][j] = (r < nb && c < nb) ? A[(b0 + r) * n + b0 + c] : (r == c ? 1.0 : 0.0);
+    }
   for (int k = 0; k < kGJ; ++k) {
-    const double ip = 1.0 / M[k][k];
-    double nv[kPer];
+    const int kb = k >> 2, ki = k & 3;
+    if (tr == kb) {
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      const int i = tid + 256 * e;
-      const int r = i / kGJ, c = i % kGJ;
-      const double rowk = M[k][c] * ip;
-      double v;
-      if (r == k && c == k) v = ip;
-      else if (r == k) v = rowk;
-      else if (c == k) v = -M[r][k] * ip;
-      else v = M[r][c] - M[r][k] * rowk;
-      nv[e] = v;
+      for (int j = 0; j < 4; ++j) prow[4 * tc + j] = m[ki][j];
+    }
+    if (tc == kb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) pcol[4 * tr + i] = m[i][ki];
     }
     __syncthreads();
+    const double ip = 1.0 / pcol[k];
+    double rk[4], ck[4];
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      const int i = tid + 256 * e;
-      M[i / kGJ][i % kGJ] = nv[e];
-    }
+    for (int j = 0; j < 4; ++j) rk[j] = prow[4 * tc + j] * ip;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ck[i] = pcol[4 * tr + i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = 4 * tr + i, c = 4 * tc + j;
+        double v;
+        if (r == k && c == k) v = ip;
+        else if (r == k) v = rk[j];
+        else if (c == k) v = -ck[i] * ip;
+        else v = m[i][j] - ck[i] * rk[j];
+        m[i][j] = v;
+      }
     __syncthreads();
   }
-  for (int i = tid; i < kGJ * kGJ; i += 256) {
-    const int r = i / kGJ, c = i % kGJ;
-    Binv[i] = M[r][c];
-    if (r < nb && c < nb) A[(b0 + r) * n + b0 + c] = M[r][c];
-  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = 4 * tr + i, c = 4 * tc + j;
+      Binv[r * kGJ + c] = m[i][j];
+      if (r < nb && c < nb) A[(b0 + r) * n + b0 + c] = m[i][j];
+    }
 }
 
 // A_BR <- Binv A_BR on a strip of kTileRC columns (columns inside the block
