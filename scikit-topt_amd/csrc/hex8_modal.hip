@@ -16,8 +16,9 @@
 // elements of the layer that touch the patch, park the 8 corner results of
 // each element in LDS, and every owned node sums its 4 contributions from
 // the layer below and 4 from the layer above in a fixed order (no atomics,
-// bit-reproducible).  Completed planes are written out coalesced; Dirichlet
-// rows (identity) are applied on the way out.
+// bit-reproducible).  Each node thread writes its completed values itself
+// (contiguous per node in the column-blocked layout); Dirichlet rows
+// (identity) are applied on the way out.  Two barriers per element layer.
 //
 // Vector layout: value (dof, column c) lives at
 //   (c / lkc) * gstride + (c % lkc) + dof * dstride,
@@ -141,9 +142,9 @@ constexpr int soa_stride(int n, int combos) {
 template <typename T>
 struct OutArgs {
   const T* B = nullptr;        // right-hand side (same layout as X)
-  T* D = nullptr;              // Chebyshev direction (same layout), updated in place
   const T* dinv = nullptr;     // (ndof) Jacobi inverse
   const double* cheb = nullptr;  // (c1, c2) of this step, device
+  int prev_zero = 0;           // the iterate before X is zero (first smoothing step)
 };
 
 template <typename T, int NC, int KC, int MODE>
@@ -155,18 +156,14 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   // LDS, structure-of-arrays so that consecutive lanes touch consecutive
   // words: two rotating node planes [NC*KC][kPS]; per element lane the
   // x-paired corner sums [side][cy][NC][KC][256] (side 0: node plane below the
-  // element, 1: above; cy: node row); the finished output plane [NC*KC][kOS].
+  // element, 1: above; cy: node row).
   constexpr int kPS = soa_stride(kPN, NC * KC);
-  constexpr int kOS = soa_stride(kTE * kTE, NC * KC);
   constexpr int kPlane = kPS * NC * KC;
   constexpr int kItems = kPN * NC * KC;
   constexpr int kPre = (kItems + 255) / 256;
-  constexpr int kOut = kTE * kTE * NC * KC;
-  constexpr int kWr = (kOut + 255) / 256;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* planes = reinterpret_cast<T*>(smem_raw);
   T* pairs = planes + 2 * kPlane;
-  T* oplane = pairs + 4 * NC * KC * 256;
   const int64_t nnx = nx + 1, nny = ny + 1, nnz = nz + 1;
   const int64_t plane_stride = nnx * nny * NC * (int64_t)dstride;
   int b = blockIdx.x;
@@ -178,12 +175,14 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   X += cbase;
   Y += cbase;
   const T* Bv = MODE != OUT_APPLY ? oa.B + cbase : nullptr;
-  T* Dv = MODE == OUT_CHEB ? oa.D + cbase : nullptr;
   T c1 = T(0), c2 = T(0);
   if (MODE == OUT_CHEB) {
     c1 = (T)oa.cheb[0];
     c2 = (T)oa.cheb[1];
   }
+  // the previous iterate is read (own values only) from the output buffer
+  // it is about to be replaced in
+  const bool use_prev = MODE == OUT_CHEB && c1 != T(0) && !oa.prev_zero;
   const int64_t x0 = (int64_t)tx * kTN, y0 = (int64_t)ty * kTN;
   const int64_t z0 = (int64_t)zc * lz;
   const int64_t z1 = (z0 + lz < nnz) ? z0 + lz : nnz;
@@ -200,8 +199,9 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   const int e_lo = tid;  // element lane holding row ny_l's pair (x0 - 1 + nx_l, y0 - 1 + ny_l)
   const bool has_up = ny_l + 1 < kTE;  // element row ny_l + 1 exists in the tile
   const int own_slot = node_slot(nx_l + 1, ny_l + 1);  // owned node in the halo plane
-  const bool own_in = is_node;
   const int64_t own_node = (y0 + ny_l) * nnx + (x0 + nx_l);  // in-plane index
+  // the owned node's values: ((own_node * NC + d) * dstride + col) in a plane
+  const int64_t own_off = own_node * NC * (int64_t)dstride;
 
   int f_off[kPre], f_lds[kPre];
 #pragma unroll
@@ -215,24 +215,6 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
       f_lds[r] = (comp * KC + col) * kPS + node_slot(nl % kPW, nl / kPW);
       if (gx >= 0 && gx < nnx && gy >= 0 && gy < nny)
         f_off[r] = (int)(((gy * nnx + gx) * NC + comp) * dstride + col);
-    }
-  }
-  int w_off[kWr], w_lds[kWr], w_dof[kWr];
-#pragma unroll
-  for (int r = 0; r < kWr; ++r) {
-    const int i = tid + 256 * r;
-    w_off[r] = -1;
-    w_lds[r] = 0;
-    w_dof[r] = 0;
-    if (i < kOut) {
-      const int col = i % KC, comp = (i / KC) % NC, nl = i / (KC * NC);
-      const int lx = nl % kTE, ly = nl / kTE;
-      const int64_t gx = x0 + lx, gy = y0 + ly;
-      w_lds[r] = (comp * KC + col) * kOS + nl;
-      if (lx < own_x && ly < own_y) {
-        w_off[r] = (int)(((gy * nnx + gx) * NC + comp) * dstride + col);
-        w_dof[r] = (int)((gy * nnx + gx) * NC + comp);
-      }
     }
   }
 
@@ -263,26 +245,34 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   for (int64_t ez = z0 - 1; ez < z1; ++ez) {
     fetch(ez + 2);  // in flight while this layer is computed
     const T Ee = (el_in && ez >= 0 && ez < nz) ? E[e_base + nx * ny * ez] : T(0);
-    // operands of the fused output stage for plane ez, loaded now so that
-    // their latency hides behind this layer's compute
-    T ob[MODE != OUT_APPLY ? kWr : 1], od[MODE == OUT_CHEB ? kWr : 1],
-        ox[MODE == OUT_CHEB ? kWr : 1], oi[MODE == OUT_CHEB ? kWr : 1];
-    if (MODE != OUT_APPLY && ez >= z0) {
-      const int64_t po = ez * plane_stride;
+    // this step completes node plane ez; its owned values are written by the
+    // node threads, whose extra operands are loaded now so that their latency
+    // hides behind the layer's compute
+    const bool out = is_node && ez >= z0;
+    const int64_t po = ez * plane_stride + own_off;
+    bool fix[NC];
+    T ob[KC][NC], op[KC][NC], oi[NC];
 #pragma unroll
-      for (int r = 0; r < kWr; ++r) {
-        if (w_off[r] < 0) continue;
-        ob[r] = Bv[po + w_off[r]];
-        if constexpr (MODE == OUT_CHEB) {
-          od[r] = c1 == T(0) ? T(0) : Dv[po + w_off[r]];
-          ox[r] = X[po + w_off[r]];
-          oi[r] = oa.dinv[ez * nnx * nny * NC + w_dof[r]];
-        }
+    for (int d = 0; d < NC; ++d) {
+      fix[d] = fixed && out && fixed[(ez * nnx * nny + own_node) * NC + d];
+      if constexpr (MODE == OUT_CHEB) oi[d] = out ? oa.dinv[(ez * nnx * nny + own_node) * NC + d] : T(0);
+#pragma unroll
+      for (int c = 0; c < KC; ++c) {
+        if constexpr (MODE != OUT_APPLY) ob[c][d] = out ? Bv[po + d * dstride + c] : T(0);
+        if constexpr (MODE == OUT_CHEB)
+          op[c][d] = (out && use_prev) ? Y[po + d * dstride + c] : T(0);
       }
     }
     __syncthreads();
     const T* pb = planes + bot * kPlane;
     const T* pt = planes + top * kPlane;
+    // the owned node's x (node plane ez = bottom LDS plane), for Dirichlet
+    // rows (y = x) and the Chebyshev update
+    T xo[KC][NC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+      for (int d = 0; d < NC; ++d) xo[c][d] = pb[(d * KC + c) * kPS + own_slot];
 #pragma unroll
     for (int col = 0; col < KC; ++col) {
       T m[NC][8];
@@ -315,12 +305,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
     }
     __syncthreads();
     if (is_node) {
-      // Dirichlet rows of the plane being completed (node plane ez, held in
-      // the bottom LDS plane): y = x
-      bool fix[NC];
-#pragma unroll
-      for (int d = 0; d < NC; ++d)
-        fix[d] = fixed && own_in && ez >= z0 && fixed[(ez * nnx * nny + own_node) * NC + d];
+      T* Yp = Y + po;
 #pragma unroll
       for (int col = 0; col < KC; ++col)
 #pragma unroll
@@ -333,32 +318,25 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
           const T above = pairs[(((1 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
                           (has_up ? pairs[(((1 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE]
                                   : T(0));
-          oplane[(d * KC + col) * kOS + tid] =
-              fix[d] ? pb[(d * KC + col) * kPS + own_slot] : partial[col][d] + below;
+          const T y = fix[d] ? xo[col][d] : partial[col][d] + below;
           partial[col][d] = above;
+          if (out) {
+            const int64_t o = d * dstride + col;
+            if constexpr (MODE == OUT_APPLY) {
+              Yp[o] = y;
+            } else if constexpr (MODE == OUT_RESID) {
+              Yp[o] = ob[col][d] - y;
+            } else {
+              // Chebyshev step with the direction recovered from the two
+              // iterates: x_out = x + c1 (x - x_prev) + c2 dinv (b - K x)
+              const T dn = c1 * (xo[col][d] - op[col][d]) + c2 * oi[d] * (ob[col][d] - y);
+              Yp[o] = xo[col][d] + dn;
+            }
+          }
         }
     }
-    __syncthreads();
-    if (ez >= z0) {
-      const int64_t po = ez * plane_stride;
-      T* Yp = Y + po;
-#pragma unroll
-      for (int r = 0; r < kWr; ++r) {
-        if (w_off[r] < 0) continue;
-        const T y = oplane[w_lds[r]];
-        if constexpr (MODE == OUT_APPLY) {
-          Yp[w_off[r]] = y;
-        } else if constexpr (MODE == OUT_RESID) {
-          Yp[w_off[r]] = ob[r] - y;
-        } else {
-          // Chebyshev step: d = c1 d + c2 dinv (b - K x); x_out = x + d
-          const T dn = c1 * od[r] + c2 * oi[r] * (ob[r] - y);
-          Dv[po + w_off[r]] = dn;
-          Yp[w_off[r]] = ox[r] + dn;
-        }
-      }
-    }
-    // the bottom plane receives plane ez + 2 and becomes the new top
+    // the bottom plane receives plane ez + 2 and becomes the new top (every
+    // read of it in this step happened before the second barrier)
     stash(planes + bot * kPlane);
     const int t = bot;
     bot = top;
@@ -390,8 +368,7 @@ static int launch_modal(const skt_operator* op, int k, int lkc, int64_t gstride,
   const int nzc = (int)((nnz + lz - 1) / lz);
   const int64_t nblocks = per * nzc;
   const size_t shmem = sizeof(T) * ((size_t)2 * soa_stride(kPN, NC * KC) * NC * KC +
-                                    (size_t)4 * NC * KC * 256 +
-                                    (size_t)soa_stride(kTE * kTE, NC * KC) * NC * KC);
+                                    (size_t)4 * NC * KC * 256);
   static bool attr_set = false;
   if (!attr_set) {
     SKT_HIP(hipFuncSetAttribute((const void*)k_hex_apply_modal<T, NC, KC, MODE>,
@@ -442,12 +419,13 @@ int hex_resid_modal(const skt_operator* op, int k, int kc, const T* coef, const 
 
 template <typename T>
 int hex_cheb_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
-                   T* Xout, const T* B, T* D, const T* dinv, const double* cheb, hipStream_t s) {
+                   T* Xout, const T* B, const T* dinv, const double* cheb, int prev_zero,
+                   hipStream_t s) {
   OutArgs<T> oa;
   oa.B = B;
-  oa.D = D;
   oa.dinv = dinv;
   oa.cheb = cheb;
+  oa.prev_zero = prev_zero;
   return dispatch_modal<T, OUT_CHEB>(op, k, kc, coef, E, X, Xout, oa, s);
 }
 
@@ -461,10 +439,10 @@ template int hex_resid_modal<double>(const skt_operator*, int, int, const double
                                      const double*, const double*, const double*, double*,
                                      hipStream_t);
 template int hex_cheb_modal<float>(const skt_operator*, int, int, const float*, const float*,
-                                   const float*, float*, const float*, float*, const float*,
-                                   const double*, hipStream_t);
+                                   const float*, float*, const float*, const float*,
+                                   const double*, int, hipStream_t);
 template int hex_cheb_modal<double>(const skt_operator*, int, int, const double*, const double*,
-                                    const double*, double*, const double*, double*,
-                                    const double*, const double*, hipStream_t);
+                                    const double*, double*, const double*, const double*,
+                                    const double*, int, hipStream_t);
 
 }  // namespace skt

@@ -454,6 +454,18 @@ __global__ void k_stencil_apply(int64_t nx, int64_t ny, int64_t nz, int groups,
   }
 }
 
+// First Chebyshev step from a zero iterate on the fine level:
+// x = c2 dinv b (blocked vectors, total = ndof * k)
+template <typename T>
+__global__ void k_cheb_first(int64_t total, int64_t ndof, int kc, const double* __restrict__ coef,
+                             const T* __restrict__ b, const T* __restrict__ dinv,
+                             T* __restrict__ x) {
+  const T c2 = (T)coef[1];
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x)
+    x[t] = c2 * dinv[(t / kc) % ndof] * b[t];
+}
+
 // Chebyshev step on a level (blocked vectors, total = ndof * k):
 //   d <- c1 d + c2 dinv (b - w);  x <- x + d   (w null: the iterate is 0)
 // coef holds (c1, c2) for this step.
@@ -743,7 +755,6 @@ int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype,
     MGLevel& v = lv[l];
     const size_t nv = (size_t)v.ndof * max_cols;
     if (l == 0) {
-      SKT_HIP(hipMalloc(&v.d, ts * nv));
       SKT_HIP(hipMalloc(&v.x, ts * nv));
       continue;
     }
@@ -941,9 +952,11 @@ template <>
 const float* modal_coef<float>(const skt_operator* op) { return op->modal_f; }
 
 // V-cycle.  On the fine level the Chebyshev steps and the residual are fused
-// into the matrix-free apply (hex_cheb_modal / hex_resid_modal), so the
-// iterate ping-pongs between z and lv[0].x; the number of fused steps,
-// 2 * degree - 1, is odd, and starting in lv[0].x the cycle ends in z.
+// into the matrix-free apply (hex_cheb_modal / hex_resid_modal); the
+// iterate ping-pongs between z and lv[0].x, and the Chebyshev direction is
+// the difference of the two buffers, so no direction vector is kept.  The
+// number of fused steps, 2 * degree - 1, is odd: starting in lv[0].x the
+// cycle ends in z.
 template <typename T, int NC, int KC>
 static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const void* dinv0, int k,
                     const T* r, T* z, T* w0, hipStream_t s) {
@@ -961,12 +974,11 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     MGLevel& v = mg->lv[0];
     const int64_t total = v.ndof * k;
     const double* cf = mg->coef;
-    T* d = (T*)v.d;
-    hipLaunchKernelGGL(k_cheb<T>, dim3(grid_for(total)), dim3(kBlock), 0, s, total, v.ndof, KC,
-                       cf, r, (const T*)nullptr, (const T*)dinv0, d, cur, 1);
+    hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
+                       v.ndof, KC, cf, r, (const T*)dinv0, cur);
     for (int st = 1; st < deg; ++st) {
-      if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, d, (const T*)dinv0,
-                                   cf + 2 * st, s)))
+      if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, (const T*)dinv0,
+                                   cf + 2 * st, st == 1 ? 1 : 0, s)))
         return stt;
       std::swap(cur, oth);
     }
@@ -1029,11 +1041,11 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
   {
     MGLevel& v = mg->lv[0];
     const double* cf = mg->coef;
-    T* d = (T*)v.d;
     launch_prolong<T, NC, KC>(v, mg->lv[1], groups, (const T*)mg->lv[1].x, fixed0, cur, s);
     for (int st = 0; st < deg; ++st) {
-      if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, d, (const T*)dinv0,
-                                   cf + 2 * st, s)))
+      // step 0 has c1 = 0 and needs no previous iterate
+      if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, (const T*)dinv0,
+                                   cf + 2 * st, 0, s)))
         return stt;
       std::swap(cur, oth);
     }

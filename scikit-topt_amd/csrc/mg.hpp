@@ -36,7 +36,7 @@ struct MGLevel {
 struct MGData {
   int dtype = -1;                     // SKT_F64 / SKT_F32 the buffers hold
   int max_cols = 0;
-  std::vector<MGLevel> lv;            // lv[0]: fine level (vectors: d, x only)
+  std::vector<MGLevel> lv;            // lv[0]: fine level (vector: x only)
   double* Ad = nullptr;               // coarsest dense matrix, fp64 (n x n), ping
   double* Ad2 = nullptr;              // inverse of the current pivot block
   void* Ainv = nullptr;               // coarsest inverse in T

@@ -47,11 +47,14 @@ int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const 
 template <typename T>
 int hex_resid_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E,
                     const T* X, const T* B, T* R, hipStream_t s);
-// Chebyshev-Jacobi step: D = c1 D + c2 dinv (B - K X); Xout = X + D,
-// (c1, c2) = cheb[0..1] on the device; Xout must not alias X.
+// Chebyshev-Jacobi step with the direction taken from two iterates:
+// Xout = X + c1 (X - Xprev) + c2 dinv (B - K X), (c1, c2) = cheb[0..1] on the
+// device.  Xprev is read from Xout before it is overwritten (zero when
+// prev_zero); Xout must not alias X.
 template <typename T>
 int hex_cheb_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
-                   T* Xout, const T* B, T* D, const T* dinv, const double* cheb, hipStream_t s);
+                   T* Xout, const T* B, const T* dinv, const double* cheb, int prev_zero,
+                   hipStream_t s);
 // Y = K(E) X for ncols columns.  kc == ncols: (ndof, ncols) row-major;
 // otherwise the column-blocked layout [ncols/kc][ndof][kc] (structured hex8
 // only).

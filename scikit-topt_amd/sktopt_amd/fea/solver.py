@@ -46,7 +46,7 @@ class LinearSolverConfig:
     petsc_options: object = None
     allow_fallback_to_spsolve: bool = False
     precision: str = "mixed"          # "mixed" (fp32 PCG, fp64 reliable updates) or "fp64"
-    inner_rtol: float = 1e-1          # fp32 residual drop that triggers a reliable update
+    inner_rtol: float = 1e-2          # fp32 residual drop that triggers a reliable update
     warm_start: bool = True           # start from the previous u_dofs
     on_noconv: str = "raise"          # "raise" or "warn"
     preconditioner: str = "auto"      # "auto" (multigrid where possible), "mg", "jacobi"
