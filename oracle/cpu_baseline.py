@@ -14,8 +14,10 @@ a bounded sample and scaled:
   reference does it on every ``forward`` (cKDTree + Python loop,
   ``filters/spacial.py:51-103,143-149``) and one cached filter SpMV;
 * per OC iteration = assembly + n_loads * cg_iters * t_cg_iter
-  + (1 + n_bisect) * t_filter [+ n_loads adjoint SpMVs], with cg_iters and
-  n_bisect taken from the GPU run of the same iteration (same algorithm).
+  + (1 + n_bisect) * t_filter [+ n_loads adjoint SpMVs], with cg_iters the
+  Jacobi-CG iteration count of the same system (counted by the caller with a
+  Jacobi-preconditioned solve of the GPU state) and n_bisect the OC
+  bisection's evaluation count.
 """
 from __future__ import annotations
 
@@ -33,6 +35,7 @@ from . import transforms as otr
 
 
 def measure(wl: dict, gpu_solves: list, n_bisect: float = 38.0, nz_sample: int = 6) -> dict:
+    """gpu_solves: [{"iters": Jacobi-CG iterations per solve}, ...]."""
     with threadpool_limits(1):
         return _measure(wl, gpu_solves, n_bisect, nz_sample)
 

@@ -403,6 +403,49 @@ __global__ void k_cast(int64_t n, const double* __restrict__ a, T* __restrict__ 
 
 // ------------------------------------------------------------ operators
 
+// N consecutive values of a node (NC components x KC columns, contiguous in
+// the blocked layout) with paired loads / stores when KC is even: the
+// offset is then a multiple of 2 values, so 8-byte (float) or 16-byte
+// (double) aligned.
+template <typename T> struct Pair;
+template <> struct Pair<float> { using type = float2; };
+template <> struct Pair<double> { using type = double2; };
+
+template <typename T, int N, int KC>
+__device__ __forceinline__ void load_node(const T* __restrict__ p, T (&v)[N]) {
+  if constexpr (KC % 2 == 0) {
+    using V = typename Pair<T>::type;
+    const V* q = reinterpret_cast<const V*>(p);
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+      const V w = q[i];
+      v[2 * i] = w.x;
+      v[2 * i + 1] = w.y;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = p[i];
+  }
+}
+
+template <typename T, int N, int KC>
+__device__ __forceinline__ void store_node(T* __restrict__ p, const T (&v)[N]) {
+  if constexpr (KC % 2 == 0) {
+    using V = typename Pair<T>::type;
+    V* q = reinterpret_cast<V*>(p);
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+      V w;
+      w.x = v[2 * i];
+      w.y = v[2 * i + 1];
+      q[i] = w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) p[i] = v[i];
+  }
+}
+
 // y = S x on a stencil level.  A thread takes one (node, component row) and
 // up to GM column groups, so each stencil row is read once for all of them.
 template <typename T, int NC, int KC, int GM>
@@ -425,31 +468,32 @@ __global__ void k_stencil_apply(int64_t nx, int64_t ny, int64_t nz, int groups,
     for (int q = 0; q < GM; ++q)
 #pragma unroll
       for (int c = 0; c < KC; ++c) acc[q][c] = T(0);
-#pragma unroll 1
+    // unrolled and branch-free: a neighbour outside the grid has an exactly
+    // zero stencil block, so its address is clamped to the node itself
+#pragma unroll
     for (int o = 0; o < 27; ++o) {
       const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
       const int64_t i2 = i + ox, j2 = j + oy, k2 = k + oz;
-      if (i2 < 0 || i2 >= nnx || j2 < 0 || j2 >= nny || k2 < 0 || k2 >= nnz) continue;
-      const int64_t m = i2 + nnx * (j2 + nny * k2);
+      const bool in = i2 >= 0 && i2 < nnx && j2 >= 0 && j2 < nny && k2 >= 0 && k2 < nnz;
+      const int64_t m = in ? i2 + nnx * (j2 + nny * k2) : node;
       T a[NC];
 #pragma unroll
       for (int e = 0; e < NC; ++e) a[e] = st[((o * NC + d) * NC + e) * nn + node];
 #pragma unroll
       for (int q = 0; q < GM; ++q) {
         if (q >= ng) break;
-        const T* Xm = X + ((int64_t)(g0 + q) * nn + m) * NC * KC;
+        T xv[NC * KC];
+        load_node<T, NC * KC, KC>(X + ((int64_t)(g0 + q) * nn + m) * NC * KC, xv);
 #pragma unroll
         for (int e = 0; e < NC; ++e)
 #pragma unroll
-          for (int c = 0; c < KC; ++c) acc[q][c] += a[e] * Xm[e * KC + c];
+          for (int c = 0; c < KC; ++c) acc[q][c] += a[e] * xv[e * KC + c];
       }
     }
 #pragma unroll
     for (int q = 0; q < GM; ++q) {
       if (q >= ng) break;
-      T* Yn = Y + (((int64_t)(g0 + q) * nn + node) * NC + d) * KC;
-#pragma unroll
-      for (int c = 0; c < KC; ++c) Yn[c] = acc[q][c];
+      store_node<T, KC, KC>(Y + (((int64_t)(g0 + q) * nn + node) * NC + d) * KC, acc[q]);
     }
   }
 }
@@ -464,6 +508,91 @@ __global__ void k_cheb_first(int64_t total, int64_t ndof, int kc, const double* 
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x)
     x[t] = c2 * dinv[(t / kc) % ndof] * b[t];
+}
+
+// y = S x on a stencil level, LDS-tiled: a workgroup owns a kSX x kSY x kSZ
+// block of nodes, stages x of the block and its one-node halo for GM column
+// groups in LDS once, and each thread (one node) sums its 27 stencil blocks
+// for all NC rows and GM groups.  Stencil rows are read once per node.
+constexpr int kSX = 16, kSY = 4, kSZ = 4;
+constexpr int kHX = kSX + 2, kHY = kSY + 2, kHZ = kSZ + 2;
+
+template <typename T, int NC, int KC, int GM>
+__global__ void __launch_bounds__(256)
+k_stencil_apply_tiled(int64_t nx, int64_t ny, int64_t nz, int groups, const T* __restrict__ st,
+                      const T* __restrict__ X, T* __restrict__ Y) {
+  constexpr int V = NC * KC;          // values of one node and one group
+  constexpr int W = V * GM;           // values of one node in LDS
+  __shared__ T xs[kHZ * kHY * kHX * W];
+  const int64_t nnx = nx + 1, nny = ny + 1, nnz = nz + 1;
+  const int64_t nn = nnx * nny * nnz;
+  const int bx = (int)((nnx + kSX - 1) / kSX), by = (int)((nny + kSY - 1) / kSY),
+            bz = (int)((nnz + kSZ - 1) / kSZ);
+  int b = blockIdx.x;
+  const int tbx = b % bx; b /= bx;
+  const int tby = b % by; b /= by;
+  const int tbz = b % bz; b /= bz;
+  const int g0 = b * GM;
+  const int ng = groups - g0 < GM ? groups - g0 : GM;
+  const int64_t x0 = (int64_t)tbx * kSX, y0 = (int64_t)tby * kSY, z0 = (int64_t)tbz * kSZ;
+  // stage x (halo nodes outside the grid read as zero; their stencil
+  // entries are zero as well)
+  for (int it = threadIdx.x; it < kHZ * kHY * kHX * GM; it += 256) {
+    const int q = it % GM, hn = it / GM;
+    const int hx = hn % kHX, hy = (hn / kHX) % kHY, hz = hn / (kHX * kHY);
+    const int64_t gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+    T v[V];
+    if (q < ng && gx >= 0 && gx < nnx && gy >= 0 && gy < nny && gz >= 0 && gz < nnz) {
+      load_node<T, V, KC>(X + ((int64_t)(g0 + q) * nn + gx + nnx * (gy + nny * gz)) * V, v);
+    } else {
+#pragma unroll
+      for (int i = 0; i < V; ++i) v[i] = T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < V; ++i) xs[hn * W + q * V + i] = v[i];
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % kSX, ty = (threadIdx.x / kSX) % kSY, tz = threadIdx.x / (kSX * kSY);
+  const int64_t gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+  if (gx >= nnx || gy >= nny || gz >= nnz) return;
+  const int64_t node = gx + nnx * (gy + nny * gz);
+  T acc[GM][NC][KC];
+#pragma unroll
+  for (int q = 0; q < GM; ++q)
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int c = 0; c < KC; ++c) acc[q][d][c] = T(0);
+#pragma unroll 3
+  for (int o = 0; o < 27; ++o) {
+    const int ox = o % 3, oy = (o / 3) % 3, oz = o / 9;  // halo offsets 0..2
+    const T* xn = xs + ((tz + oz) * kHY * kHX + (ty + oy) * kHX + (tx + ox)) * W;
+    T a[NC][NC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int e = 0; e < NC; ++e) a[d][e] = st[((o * NC + d) * NC + e) * nn + node];
+#pragma unroll
+    for (int q = 0; q < GM; ++q)
+#pragma unroll
+      for (int e = 0; e < NC; ++e)
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const T xv = xn[q * V + e * KC + c];
+#pragma unroll
+          for (int d = 0; d < NC; ++d) acc[q][d][c] += a[d][e] * xv;
+        }
+  }
+#pragma unroll
+  for (int q = 0; q < GM; ++q) {
+    if (q >= ng) break;
+    T out[V];
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int c = 0; c < KC; ++c) out[d * KC + c] = acc[q][d][c];
+    store_node<T, V, KC>(Y + ((int64_t)(g0 + q) * nn + node) * V, out);
+  }
 }
 
 // Chebyshev step on a level (blocked vectors, total = ndof * k):
@@ -516,15 +645,15 @@ __global__ void k_restrict_xy(int64_t fnx, int64_t fny, int64_t fnz, int fx, int
         if (i < 0 || i >= fnnx) continue;
         const T wgt = (T(1) - (T)(dx < 0 ? -dx : dx) / (T)fx) * wy;
         const int64_t fn = i + fnnx * (j + fnny * k);
+        T bv[NC * KC], wv[NC * KC];
+        load_node<T, NC * KC, KC>(bg + fn * NC * KC, bv);
+        if (wg) load_node<T, NC * KC, KC>(wg + fn * NC * KC, wv);
 #pragma unroll
         for (int d = 0; d < NC; ++d) {
           if (fixed && fixed[fn * NC + d]) continue;
 #pragma unroll
-          for (int c = 0; c < KC; ++c) {
-            const int64_t idx = (fn * NC + d) * KC + c;
-            const T r = wg ? bg[idx] - wg[idx] : bg[idx];
-            acc[d][c] += wgt * r;
-          }
+          for (int c = 0; c < KC; ++c)
+            acc[d][c] += wgt * (wg ? bv[d * KC + c] - wv[d * KC + c] : bv[d * KC + c]);
         }
       }
     }
@@ -599,18 +728,23 @@ __global__ void k_prolong_add(int64_t fnx, int64_t fny, int64_t fnz, int fx, int
       if ((qx && ax == 0) || (qy && ay == 0) || (qz && az == 0)) continue;
       const T wgt = (qx ? sx : T(1) - sx) * (qy ? sy : T(1) - sy) * (qz ? sz : T(1) - sz);
       const int64_t cn = (I + qx) + cnnx * ((J + qy) + cnny * (K + qz));
+      T xv[NC * KC];
+      load_node<T, NC * KC, KC>(xg + cn * NC * KC, xv);
 #pragma unroll
       for (int d = 0; d < NC; ++d)
 #pragma unroll
-        for (int c = 0; c < KC; ++c) acc[d][c] += wgt * xg[(cn * NC + d) * KC + c];
+        for (int c = 0; c < KC; ++c) acc[d][c] += wgt * xv[d * KC + c];
     }
-    T* xfg = xf + g * fnn * NC * KC;
+    T* xfn = xf + (g * fnn + fnode) * NC * KC;
+    T out[NC * KC];
+    load_node<T, NC * KC, KC>(xfn, out);
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
       if (fixed && fixed[fnode * NC + d]) continue;
 #pragma unroll
-      for (int c = 0; c < KC; ++c) xfg[(fnode * NC + d) * KC + c] += acc[d][c];
+      for (int c = 0; c < KC; ++c) out[d * KC + c] += acc[d][c];
     }
+    store_node<T, NC * KC, KC>(xfn, out);
   }
 }
 
@@ -806,12 +940,23 @@ static int level_apply(MGData* mg, const skt_operator* op, const void* E, int l,
   if (l == 0) return op_apply(const_cast<skt_operator*>(op), mg->dtype, k, kc, E, x, y, s);
   const MGLevel& v = mg->lv[l];
   const int groups = k / kc;
-  constexpr int GM = 4;
-  const unsigned g = grid_for(v.nn * op->ncomp * ((groups + GM - 1) / GM));
   const T* st = (const T*)v.st;
-#define SKT_SA(NC_, KC_)                                                                       \
-  hipLaunchKernelGGL((k_stencil_apply<T, NC_, KC_, GM>), dim3(g), dim3(kBlock), 0, s, v.nx, v.ny, \
-                     v.nz, groups, st, x, y)
+  // large levels: LDS-tiled kernel (x staged once per tile); small levels:
+  // one (node, row, column group) per thread for memory-level parallelism
+  const bool tiled = v.nn >= 20000;
+  constexpr int GMT = sizeof(T) == 4 ? 4 : 2;  // 62 KB of LDS either way for NC * KC = 6
+  const int64_t tiles = ((v.nx + kSX) / kSX) * ((v.ny + kSY) / kSY) * ((v.nz + kSZ) / kSZ);
+  const unsigned gt = (unsigned)(tiles * ((groups + GMT - 1) / GMT));
+  const unsigned g = grid_for(v.nn * op->ncomp * groups);
+#define SKT_SA(NC_, KC_)                                                                         \
+  do {                                                                                           \
+    if (tiled)                                                                                   \
+      hipLaunchKernelGGL((k_stencil_apply_tiled<T, NC_, KC_, GMT>), dim3(gt), dim3(256), 0, s,    \
+                         v.nx, v.ny, v.nz, groups, st, x, y);                                    \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_stencil_apply<T, NC_, KC_, 1>), dim3(g), dim3(kBlock), 0, s, v.nx,    \
+                         v.ny, v.nz, groups, st, x, y);                                          \
+  } while (0)
   if (op->ncomp == 3) {
     if (kc == 1) SKT_SA(3, 1);
     else if (kc == 2) SKT_SA(3, 2);
