@@ -41,6 +41,11 @@ extern "C" {
 
 #define SKT_SIMP 0
 #define SKT_RAMP 1
+/* RAMP moduli with the exact derivative (1 + p) / (1 + p (1 - rho))^2 for
+ * the dC kernels; core/derivatives.py:56-61 (SKT_RAMP) uses
+ * (den - p rho) / den^2 instead.  No reference counterpart: it serves the
+ * differentiable compliance, whose gradient must match finite differences. */
+#define SKT_RAMP_EXACT 2
 
 #define SKT_ELASTIC 0
 #define SKT_HEAT 1
@@ -173,6 +178,8 @@ int skt_operator_energy(skt_operator* op, int dtype, int ncols, const void* E,
  *   inner_rtol since the last update, the float32 correction is added to U
  *   in float64 and the true residual F - K U (float64 operator) replaces the
  *   recurrence residual; convergence is declared on true residuals only.
+ *   inner_rtol <= 0 picks it automatically (1e-2 with multigrid, 1e-1 with
+ *   Jacobi). 
  * ---------------------------------------------------------------------- */
 int skt_solver_create(skt_operator* op, int max_cols, skt_solver** out);
 int skt_solver_destroy(skt_solver* s);

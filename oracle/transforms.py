@@ -57,3 +57,14 @@ def dC_drho_ramp(rho, strain_energy, E0, Emin, p):
     dE = (E0 - Emin) * (denom - p * rho) / (denom ** 2)
     E = Emin + (E0 - Emin) * (rho / denom)
     return -2.0 * strain_energy * dE / np.maximum(E, 1e-12)
+
+
+def dC_drho_ramp_exact(rho, strain_energy, E0, Emin, p):
+    """The derivative of the compliance through ``ramp_interpolation``:
+    dE/drho = (E0 - Emin) (1 + p) / (1 + p (1 - rho))^2.  Not a reference
+    function (the reference's expression is ``dC_drho_ramp``); it is what
+    the differentiable compliance must reproduce."""
+    denom = 1.0 + p * (1.0 - rho)
+    dE = (E0 - Emin) * (1.0 + p) / (denom ** 2)
+    E = Emin + (E0 - Emin) * (rho / denom)
+    return -2.0 * strain_energy * dE / np.maximum(E, 1e-12)

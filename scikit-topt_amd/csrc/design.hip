@@ -73,8 +73,11 @@ __device__ __forceinline__ double dC_value(int kind, double rho, double U,
     if (E != E) Ec = E;
     return ((-2.0 * U) * dE) / Ec;
   } else {
+    // SKT_RAMP: the reference's expression (core/derivatives.py:56-61);
+    // SKT_RAMP_EXACT: the derivative of rho / (1 + p (1 - rho)), (1 + p) / den^2
     double den = 1.0 + p * (1.0 - rho);
-    double dE = dEm * (den - p * rho) / (den * den);
+    double dE = kind == SKT_RAMP ? dEm * (den - p * rho) / (den * den)
+                                 : dEm * (1.0 + p) / (den * den);
     double E = Emin + dEm * (rho / den);
     double Ec = E > 1e-12 ? E : 1e-12;
     if (E != E) Ec = E;
@@ -427,7 +430,7 @@ int skt_heaviside_derivative(int64_t n, const double* x, double beta,
 int skt_interpolate(int kind, int64_t n, const double* rho, double E0,
                     double Emin, double p, void* out, int out_dtype,
                     hipStream_t stream) {
-  SKT_CHECK(kind == SKT_SIMP || kind == SKT_RAMP,
+  SKT_CHECK(kind == SKT_SIMP || kind == SKT_RAMP || kind == SKT_RAMP_EXACT,
             "Interpolation method must be SIMP or RAMP.");
   SKT_CHECK(n >= 0, "n must be >= 0");
   if (n == 0) return SKT_OK;
@@ -448,7 +451,7 @@ int skt_interpolate(int kind, int64_t n, const double* rho, double E0,
 int skt_dC_drho(int kind, int64_t n, const double* rho, const double* energy,
                 int64_t energy_stride, double E0, double Emin, double p,
                 double* out, hipStream_t stream) {
-  SKT_CHECK(kind == SKT_SIMP || kind == SKT_RAMP,
+  SKT_CHECK(kind == SKT_SIMP || kind == SKT_RAMP || kind == SKT_RAMP_EXACT,
             "Interpolation method must be SIMP or RAMP.");
   SKT_CHECK(n >= 0 && energy_stride >= 1, "bad size/stride");
   if (n == 0) return SKT_OK;
@@ -464,7 +467,7 @@ int skt_sensitivity_chain(int kind, int64_t n, int ncols,
                           const double* rho_filt, double beta, double eta,
                           double E0, double Emin, double p, double scale, double* g,
                           hipStream_t stream) {
-  SKT_CHECK(kind == SKT_SIMP || kind == SKT_RAMP,
+  SKT_CHECK(kind == SKT_SIMP || kind == SKT_RAMP || kind == SKT_RAMP_EXACT,
             "Interpolation method must be SIMP or RAMP.");
   SKT_CHECK(n >= 0 && ncols >= 1, "bad size");
   if (n == 0) return SKT_OK;

@@ -382,7 +382,9 @@ __global__ void k_replace_final(int nb, int k, const double* __restrict__ part_r
   const double rz = col_sum(part_rz, k, nb, 0, c);
   if (threadIdx.x == 0) {
     double* s = sc + c * kSc;
-    s[S_RST] = (rr > 4.0 * s[S_RR]) ? 1.0 : 0.0;
+    // restart when the true residual left the recurrence behind, or grew
+    // since the previous update (the float32 iteration has diverged)
+    s[S_RST] = (rr > 4.0 * s[S_RR] || rr > s[S_RREF]) ? 1.0 : 0.0;
     s[S_RR] = rr;
     s[S_RZ] = rz;
     s[S_RREF] = rr;
@@ -534,7 +536,11 @@ static int solve_t(skt_solver* s, int mode, const Blk& b, const double* E64, con
   hipLaunchKernelGGL(k_pinit<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
                      b.cp, (const T*)v.r, (const T*)v.dinv, mg ? (const T*)v.z : (const T*)nullptr,
                      v.p, area1);
-  const double delta2 = mixed ? inner_rtol * inner_rtol : 0.0;
+  // automatic threshold: multigrid converges in few iterations between
+  // updates and tolerates 1e-2; Jacobi-PCG runs long enough for float32
+  // drift to matter and takes 1e-1
+  const double delta = inner_rtol > 0.0 ? inner_rtol : (mg ? 1e-2 : 1e-1);
+  const double delta2 = mixed ? delta * delta : 0.0;
   hipLaunchKernelGGL(k_init_final, dim3(b.k), dim3(kWave), 0, st, (int)b.nbx, b.k, s->part, area1,
                      rtol * rtol, delta2, s->sc);
   SKT_LAUNCH_CHECK();
@@ -675,8 +681,8 @@ int skt_solver_solve(skt_solver* s, int mode, int ncols, const double* E64,
   SKT_CHECK(mode == SKT_F64 || mode == SKT_F32, "mode must be SKT_F64 or SKT_F32");
   SKT_CHECK(mode == SKT_F64 || E32, "float32 moduli are required in SKT_F32 mode");
   SKT_CHECK(rtol > 0.0 && maxiter >= 1, "rtol must be > 0 and maxiter >= 1");
-  SKT_CHECK(mode == SKT_F64 || (inner_rtol > 0.0 && inner_rtol < 1.0),
-            "inner_rtol (reliable-update threshold) must be in (0, 1)");
+  SKT_CHECK(mode == SKT_F64 || inner_rtol < 1.0,
+            "inner_rtol (reliable-update threshold) must be < 1 (<= 0: automatic)");
   const int64_t n = s->n;
   const Blk b = make_blk(n, ncols, kc_for(s->op, ncols));
   const int64_t total = n * ncols;

@@ -15,7 +15,7 @@ LIB_PATH = Path(os.environ.get("SKTOPT_AMD_LIB", _HERE / "_lib" / "libsktopt_amd
 SKT_OK, SKT_EINVAL, SKT_EHIP, SKT_ENOCONV, SKT_ENOTIMPL = 0, 1, 2, 3, 4
 F64, F32 = 0, 1
 PC_JACOBI, PC_MG, PC_AUTO = 0, 1, 2
-SIMP, RAMP = 0, 1
+SIMP, RAMP, RAMP_EXACT = 0, 1, 2
 ELASTIC, HEAT = 0, 1
 RED_SUM, RED_DOT, RED_MAXABS, RED_MAXABSDIFF = 0, 1, 2, 3
 

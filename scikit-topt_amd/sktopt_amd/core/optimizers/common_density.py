@@ -73,7 +73,7 @@ class DensityMethodConfig:
     # GPU state solve
     precision: Literal["mixed", "fp64"] = "mixed"
     solver_rtol: float = 1e-8
-    inner_rtol: float = 1e-2
+    inner_rtol: float = 0.0
     preconditioner: str = "auto"
     solver_maxiter: int | None = None
     write_checkpoints: bool = False

@@ -46,7 +46,8 @@ class LinearSolverConfig:
     petsc_options: object = None
     allow_fallback_to_spsolve: bool = False
     precision: str = "mixed"          # "mixed" (fp32 PCG, fp64 reliable updates) or "fp64"
-    inner_rtol: float = 1e-2          # fp32 residual drop that triggers a reliable update
+    inner_rtol: float = 0.0           # fp32 residual drop that triggers a reliable update
+                                      # (<= 0: 1e-2 with multigrid, 1e-1 with Jacobi)
     warm_start: bool = True           # start from the previous u_dofs
     on_noconv: str = "raise"          # "raise" or "warn"
     preconditioner: str = "auto"      # "auto" (multigrid where possible), "mg", "jacobi"
