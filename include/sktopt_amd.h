@@ -136,6 +136,31 @@ int skt_hex8_create(int physics, int64_t nx, int64_t ny, int64_t nz,
                     int quad_order, const uint8_t* dof_fixed,
                     skt_operator** out);
 
+/* Structured hex8 scalar operator a M + b K (M: unit-density trilinear mass,
+ * K: unit-conductivity Laplacian), same grid conventions as skt_hex8_create.
+ * With a = 1, b = r^2 it is the nodal Helmholtz filter's bilinear form
+ * u v + r^2 grad u . grad v (filters/helmholtz_filter_nodal.py:140-142);
+ * with b = 0 the mass matrix of its right-hand side (:144-146). */
+int skt_hex8_create_scalar(int64_t nx, int64_t ny, int64_t nz, double hx,
+                           double hy, double hz, double mass_coef,
+                           double stiff_coef, const uint8_t* dof_fixed,
+                           skt_operator** out);
+
+/* filters/helmholtz_filter_nodal.py:31-59 element_to_node_density_averaging:
+ * rho_node = sum_e w_e v_e / sum_e w_e over the elements of each node,
+ * w = vol, v_e = rho_elem[e] on design elements (design[e] != 0, or all
+ * when design is NULL) and fixed_value elsewhere; same summation order as
+ * the reference's loop. */
+int skt_hex8_elem_to_node(int64_t nx, int64_t ny, int64_t nz, const double* vol,
+                          const uint8_t* design, double fixed_value,
+                          const double* rho_elem, double* rho_node,
+                          hipStream_t stream);
+
+/* helmholtz_filter_nodal.py:27-28 node_to_element_density (mean of the 8
+ * nodes); clamp_nonpositive applies the gradient's min(v, 0) (:232). */
+int skt_hex8_node_to_elem(int64_t nx, int64_t ny, int64_t nz, const double* rho_node,
+                          int clamp_nonpositive, double* rho_elem, hipStream_t stream);
+
 /* Unstructured linear tetrahedra.  coords: (3, n_nodes) float64 device
  * array (x row, y row, z row); conn: (4, n_el) int32 device array.
  * Same operator semantics as above on tet meshes. */

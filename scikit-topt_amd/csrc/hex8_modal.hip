@@ -5,7 +5,8 @@
 // the basis of the 8 corner sign patterns (1, x, y, z, xy, yz, xz, xyz):
 // per element and column y_e = E_e * H (Kmod (H^T u_e)), with H the +-1
 // butterfly (24 add/sub per component each way) and Kmod a 21x21 matrix with
-// 33 distinct coefficients (elasticity) or a 7-entry diagonal (conduction),
+// 33 distinct coefficients (elasticity) or an 8-entry diagonal (scalar:
+// conduction, mass, or a combination),
 // instead of a dense 24x24 (8x8) product.
 //
 // Tiling: a workgroup owns a 15x15 patch of nodes in x-y (16 along an axis
@@ -118,9 +119,10 @@ __device__ __forceinline__ void modal_elastic(const T* __restrict__ c, const T m
 
 template <typename T>
 __device__ __forceinline__ void modal_heat(const T* __restrict__ c, const T m[1][8], T f[1][8]) {
-  f[0][0] = T(0);
+  // scalar operators are diagonal in the modal basis; mode 0 is 0 for pure
+  // conduction and carries the mass of a*M + b*K
 #pragma unroll
-  for (int s = 1; s < 8; ++s) f[0][s] = c[s - 1] * m[0][s];
+  for (int s = 0; s < 8; ++s) f[0][s] = c[s] * m[0][s];
 }
 
 template <typename T>

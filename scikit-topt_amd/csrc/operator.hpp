@@ -38,6 +38,8 @@ struct skt_operator {
 };
 
 namespace skt {
+// Shared tail of the hex8 constructors (operators.hip).
+int hex8_finish(skt_operator* op, const uint8_t* dof_fixed, skt_operator** out);
 // Output modes of the structured hex8 modal kernel.
 enum { OUT_APPLY = 0, OUT_RESID = 1, OUT_CHEB = 2 };
 template <typename T>

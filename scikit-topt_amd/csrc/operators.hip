@@ -107,6 +107,26 @@ static void hex_K0(int physics, double nu, const double h[3], int order, double*
     }
 }
 
+// Unit-density trilinear mass matrix of an hx*hy*hz box, 2x2x2 Gauss
+// (exact for products of trilinear shape functions).
+static void hex_M0(const double h[3], double* M) {
+  std::memset(M, 0, sizeof(double) * 64);
+  std::vector<double> x, w;
+  gauss_rule(2, x, w);
+  const double detJ = h[0] * h[1] * h[2] / 8.0;
+  for (size_t i = 0; i < x.size(); ++i)
+    for (size_t j = 0; j < x.size(); ++j)
+      for (size_t k = 0; k < x.size(); ++k) {
+        double N[8];
+        for (int a = 0; a < 8; ++a)
+          N[a] = (1 + (2.0 * kCx[a] - 1) * x[i]) * (1 + (2.0 * kCy[a] - 1) * x[j]) *
+                 (1 + (2.0 * kCz[a] - 1) * x[k]) / 8.0;
+        const double wq = w[i] * w[j] * w[k] * detJ;
+        for (int a = 0; a < 8; ++a)
+          for (int b = 0; b < 8; ++b) M[a * 8 + b] += wq * N[a] * N[b];
+      }
+}
+
 // Modal coefficients: Kmod = W K0 W / 64 with W[i][S] = (-1)^popcount(i & S)
 // the symmetric Walsh-Hadamard matrix over corner bits (bit 0 x, bit 1 y,
 // bit 2 z; W W = 8 I), modal DOF 3*S + c.
@@ -141,7 +161,9 @@ static int hex_modal_coefficients(int physics, const double* K0, double* coef) {
     return Km[i * nd + j];
   };
   if (nc == 1) {
-    for (int s = 1; s < 8; ++s) coef[s - 1] = take(s, s);
+    // all 8 modes: mode 0 (constant) carries mass, not conduction
+    for (int s = 0; s < 8; ++s) coef[s] = take(s, s);
+    if (std::fabs(coef[0]) <= 1e-12 * mx) coef[0] = 0.0;
   } else {
     const int lin[3] = {3, 7, 14};
     coef[0] = take(lin[0], lin[0]); coef[1] = take(lin[1], lin[1]); coef[2] = take(lin[2], lin[2]);
@@ -674,8 +696,45 @@ int skt_hex8_create(int physics, int64_t nx, int64_t ny, int64_t nz, double hx,
   op->n_nodes = (nx + 1) * (ny + 1) * (nz + 1);
   op->n_el = nx * ny * nz;
   op->ndof = op->n_nodes * op->ncomp;
-  const int nd = 8 * op->ncomp;
   hex_K0(physics, nu, op->h, quad_order, op->K0h);
+  return hex8_finish(op, dof_fixed, out);
+}
+
+int skt_hex8_create_scalar(int64_t nx, int64_t ny, int64_t nz, double hx, double hy, double hz,
+                           double mass_coef, double stiff_coef, const uint8_t* dof_fixed,
+                           skt_operator** out) {
+  SKT_CHECK(out, "null output handle");
+  *out = nullptr;
+  SKT_CHECK(nx >= 1 && ny >= 1 && nz >= 1, "grid dimensions must be >= 1");
+  SKT_CHECK(hx > 0 && hy > 0 && hz > 0, "element sizes must be positive");
+  SKT_CHECK(mass_coef >= 0 && stiff_coef >= 0 && mass_coef + stiff_coef > 0,
+            "coefficients must be >= 0 and not both zero");
+  SKT_CHECK(dof_fixed, "dof_fixed is required");
+  SKT_CHECK((nx + 1) * (ny + 1) * (nz + 1) < (int64_t)INT32_MAX, "grid too large");
+  skt_operator* op = new skt_operator();
+  op->mesh = SKT_MESH_HEX8;
+  op->physics = SKT_HEAT;
+  op->ncomp = 1;
+  op->nx = nx; op->ny = ny; op->nz = nz;
+  op->h[0] = hx; op->h[1] = hy; op->h[2] = hz;
+  op->n_nodes = (nx + 1) * (ny + 1) * (nz + 1);
+  op->n_el = nx * ny * nz;
+  op->ndof = op->n_nodes;
+  double Kc[64], M[64];
+  hex_K0(SKT_HEAT, 0.0, op->h, 2, Kc);
+  hex_M0(op->h, M);
+  for (int i = 0; i < 64; ++i) op->K0h[i] = mass_coef * M[i] + stiff_coef * Kc[i];
+  return hex8_finish(op, dof_fixed, out);
+}
+
+}  // extern "C"
+
+namespace skt {
+// Shared tail of the hex8 constructors: modal coefficients, device copies
+// of K_e^0, Dirichlet mask and list.
+int hex8_finish(skt_operator* op, const uint8_t* dof_fixed, skt_operator** out) {
+  const int physics = op->physics;
+  const int nd = 8 * op->ncomp;
   op->modal_ok = hex_modal_coefficients(physics, op->K0h, op->modal_d);
   for (int i = 0; i < 40; ++i) op->modal_f[i] = (float)op->modal_d[i];
   std::vector<float> kf(nd * nd);
@@ -702,6 +761,9 @@ int skt_hex8_create(int physics, int64_t nx, int64_t ny, int64_t nz, double hx,
   *out = op;
   return SKT_OK;
 }
+}  // namespace skt
+
+extern "C" {
 
 int skt_tet4_create(int physics, int64_t n_nodes, int64_t n_el, const double* coords,
                     const int32_t* conn, double nu, const uint8_t* dof_fixed,

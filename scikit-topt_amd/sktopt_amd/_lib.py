@@ -52,6 +52,9 @@ SIGNATURES = {
     "skt_solver_create": [_vp, _i32, _pp],
     "skt_solver_destroy": [_vp],
     "skt_solver_configure": [_vp, _i32, _i32, _i32],
+    "skt_hex8_create_scalar": [_i64, _i64, _i64, _d, _d, _d, _d, _d, _vp, _pp],
+    "skt_hex8_elem_to_node": [_i64, _i64, _i64, _vp, _vp, _d, _vp, _vp, _vp],
+    "skt_hex8_node_to_elem": [_i64, _i64, _i64, _vp, _i32, _vp, _vp],
     "skt_solver_info": [_vp, _i32p, _i32p],
     "skt_solver_set_timing": [_vp, _i32],
     "skt_solver_get_timing": [_vp, _dp, _i64p, _i32],

@@ -1,0 +1,149 @@
+"""Nodal Helmholtz density filter on the GPU.
+
+Call surface of ``filters/helmholtz_filter_nodal.py:162-233``
+(``HelmholtzFilterNodal``): ``from_defaults(mesh, elements_volume, radius,
+design_mask)``, ``forward(v)``, ``gradient(v)``, ``update_radius(r)`` and
+``radius``.
+
+forward (``:183-201``): element -> node volume-weighted averaging with
+non-design elements counted as 1, the nodal solve ``(M + r^2 K) x = M rho``
+with ``x = 1`` on the nodes of non-design elements, node -> element mean.
+gradient (``:203-233``): the same averaging with 0 for non-design elements,
+the solve without Dirichlet nodes, the mean, then ``min(v, 0)``.  As in the
+reference the gradient is not the exact adjoint of the forward map.
+
+On structured hex8 grids ``M + r^2 K`` and ``M`` are matrix-free scalar
+operators (diagonal in the Walsh-Hadamard corner basis, the same modal
+kernel as the state operator) and the solves use the batched CG with the
+geometric-multigrid preconditioner, warm-started from the previous call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from .. import _dev, _lib
+
+
+class HelmholtzFilterNodal:
+    def __init__(self, mesh, elements_volume, radius, design_mask=None, rtol=1e-12,
+                 precision="fp64"):
+        if getattr(mesh, "kind", None) != "hex8" or not hasattr(mesh, "grid"):
+            raise NotImplementedError("the nodal Helmholtz filter is built for structured "
+                                      "hex8 meshes")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.mesh = mesh
+        self.elements_volume = elements_volume
+        self.radius = float(radius)
+        self.design_mask = None if design_mask is None else np.asarray(design_mask, dtype=bool)
+        self.rtol = float(rtol)
+        self.precision = precision
+        self.grid = tuple(int(v) for v in mesh.grid)
+        self.h = tuple(float(v) for v in mesh.h)
+        nx, ny, nz = self.grid
+        self.n_el = nx * ny * nz
+        self.n_nodes = (nx + 1) * (ny + 1) * (nz + 1)
+        dev = _dev.device()
+        self._vol = _dev.to_dev(np.asarray(elements_volume, dtype=np.float64))
+        self._design = None if self.design_mask is None else \
+            torch.as_tensor(self.design_mask.astype(np.uint8), device=dev)
+        # nodes of non-design elements carry x = 1 in the forward solve
+        fixed = np.zeros(self.n_nodes, dtype=np.uint8)
+        if self.design_mask is not None and (~self.design_mask).any():
+            fixed[np.unique(mesh.t[:, ~self.design_mask].ravel())] = 1
+        self._fixed_fwd = torch.as_tensor(fixed, device=dev)
+        self._fixed_none = torch.zeros(self.n_nodes, dtype=torch.uint8, device=dev)
+        self._ones64 = torch.ones(self.n_el, dtype=torch.float64, device=dev)
+        self._ones32 = torch.ones(self.n_el, dtype=torch.float32, device=dev)
+        self._mass = self._scalar_op(1.0, 0.0, self._fixed_none)
+        self._ops = {}       # case -> (operator, solver) for the current radius
+        self._built_radius = None
+        self._warm = {"forward": torch.zeros((self.n_nodes, 1), dtype=torch.float64, device=dev),
+                      "gradient": torch.zeros((self.n_nodes, 1), dtype=torch.float64,
+                                              device=dev)}
+        self.last_solve = {}
+
+    @classmethod
+    def from_defaults(cls, mesh, elements_volume, radius=0.3, design_mask=None):
+        return cls(mesh, elements_volume, radius, design_mask)
+
+    def update_radius(self, radius, **args):
+        self.radius = float(radius)
+
+    # --- device objects -----------------------------------------------
+    def _scalar_op(self, a, b, fixed):
+        h = C.c_void_p()
+        nx, ny, nz = self.grid
+        _lib.call("skt_hex8_create_scalar", nx, ny, nz, *self.h, float(a), float(b),
+                  _dev.ptr(fixed), C.byref(h))
+        return h
+
+    def _ensure(self):
+        if self._built_radius == self.radius and self._ops:
+            return
+        self._destroy_ops()
+        r2 = self.radius ** 2
+        for case, fixed in (("forward", self._fixed_fwd), ("gradient", self._fixed_none)):
+            op = self._scalar_op(1.0, r2, fixed)
+            s = C.c_void_p()
+            _lib.call("skt_solver_create", op, 1, C.byref(s))
+            self._ops[case] = (op, s)
+        self._built_radius = self.radius
+
+    def _destroy_ops(self):
+        lib = _lib.load()
+        for op, s in self._ops.values():
+            lib.skt_solver_destroy(s)
+            lib.skt_operator_destroy(op)
+        self._ops = {}
+
+    def __del__(self):
+        try:
+            self._destroy_ops()
+            if getattr(self, "_mass", None):
+                _lib.load().skt_operator_destroy(self._mass)
+        except Exception:
+            pass
+
+    # --- the filter ----------------------------------------------------
+    def _run(self, case, v):
+        self._ensure()
+        host = _dev.is_host(v)
+        x = _dev.to_dev(v).contiguous()
+        S = _dev.stream()
+        nx, ny, nz = self.grid
+        rn = torch.empty(self.n_nodes, dtype=torch.float64, device=x.device)
+        _lib.call("skt_hex8_elem_to_node", nx, ny, nz, _dev.ptr(self._vol),
+                  _dev.ptr(self._design) if self._design is not None else None,
+                  1.0 if case == "forward" else 0.0, _dev.ptr(x), _dev.ptr(rn), S)
+        b = torch.empty((self.n_nodes, 1), dtype=torch.float64, device=x.device)
+        _lib.call("skt_operator_apply", self._mass, _lib.F64, 1, _dev.ptr(self._ones64),
+                  _dev.ptr(rn), _dev.ptr(b), S)
+        if case == "forward":
+            b[self._fixed_fwd.bool(), 0] = 1.0     # prescribed x_D = 1 (enforced rhs)
+        op, solver = self._ops[case]
+        U = self._warm[case]
+        mode = _lib.F32 if self.precision == "mixed" else _lib.F64
+        iters = (C.c_int64 * 2)()
+        relres = (C.c_double * 1)()
+        st = _lib.load().skt_solver_solve(
+            solver, mode, 1, _dev.ptr(self._ones64),
+            _dev.ptr(self._ones32) if mode == _lib.F32 else None, _dev.ptr(b), _dev.ptr(U),
+            self.rtol, 100000, 0.0, iters, relres, S)
+        self.last_solve = dict(case=case, iters=int(iters[0]), relres=float(relres[0]))
+        _lib.check(st, "skt_solver_solve")
+        out = torch.empty(self.n_el, dtype=torch.float64, device=x.device)
+        _lib.call("skt_hex8_node_to_elem", nx, ny, nz, _dev.ptr(U), 1 if case == "gradient" else 0,
+                  _dev.ptr(out), S)
+        return out.cpu().numpy() if host else out
+
+    def forward(self, rho_element):
+        """``helmholtz_filter_nodal.py:183-201``."""
+        return self._run("forward", rho_element)
+
+    def gradient(self, v_ele):
+        """``helmholtz_filter_nodal.py:203-233`` (clamped to <= 0)."""
+        return self._run("gradient", v_ele)

@@ -1,0 +1,87 @@
+"""Nodal Helmholtz filter on the GPU against the oracle
+(filters/helmholtz_filter_nodal.py restated with scipy)."""
+import numpy as np
+import pytest
+
+from oracle import helmholtz as oh
+from oracle import problems as oprob
+
+pytestmark = pytest.mark.gpu
+
+
+def _mask(task):
+    m = np.zeros(task.n_elements, dtype=bool)
+    m[task.design_elements] = True
+    return m
+
+
+def _gpu_filter(task, r, precision="fp64"):
+    from sktopt_amd.filters import HelmholtzFilterNodal
+    from sktopt_amd.mesh import structured_hex
+    m = structured_hex(*task.mesh.grid, *task.mesh.h)
+    return HelmholtzFilterNodal(m, task.elements_volume, r, _mask(task), precision=precision), m
+
+
+def test_element_node_transfers_are_bitwise(gpu):
+    import torch
+    from sktopt_amd import _lib
+    task = oprob.cantilever(13, 7, 5)
+    nx, ny, nz = task.mesh.grid
+    rng = np.random.default_rng(0)
+    rho = rng.uniform(0, 1, task.n_elements)
+    s = torch.cuda.current_stream().cuda_stream
+    vol = torch.tensor(task.elements_volume, device="cuda")
+    des = torch.tensor(_mask(task).astype(np.uint8), device="cuda")
+    x = torch.tensor(rho, device="cuda")
+    for fixed in (1.0, 0.0):
+        rn = torch.empty(task.mesh.nnodes, dtype=torch.float64, device="cuda")
+        _lib.call("skt_hex8_elem_to_node", nx, ny, nz, vol.data_ptr(), des.data_ptr(), fixed,
+                  x.data_ptr(), rn.data_ptr(), s)
+        ref = oh.element_to_node_density_averaging(task.mesh, task.elements_volume, rho,
+                                                   _mask(task), True, fixed)
+        np.testing.assert_array_equal(rn.cpu().numpy(), ref)
+    vn = torch.tensor(rng.standard_normal(task.mesh.nnodes), device="cuda")
+    out = torch.empty(task.n_elements, dtype=torch.float64, device="cuda")
+    for clamp in (0, 1):
+        _lib.call("skt_hex8_node_to_elem", nx, ny, nz, vn.data_ptr(), clamp, out.data_ptr(), s)
+        ref = oh.node_to_element_density(task.mesh, vn.cpu().numpy())
+        if clamp:
+            ref = np.minimum(ref, 0.0)
+        np.testing.assert_array_equal(out.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("grid,r", [((12, 6, 6), 1.5), ((16, 8, 8), 2.5), ((31, 16, 8), 2.0)])
+def test_forward_and_gradient_match_oracle(gpu, grid, r):
+    task = oprob.cantilever(*grid)
+    mask = _mask(task)
+    assert (~mask).any()
+    of = oh.HelmholtzFilterNodal(task.mesh, task.elements_volume, r, mask)
+    gf, _ = _gpu_filter(task, r)
+    rng = np.random.default_rng(2)
+    rho = np.ones(task.n_elements)
+    rho[mask] = rng.uniform(0.05, 1.0, mask.sum())
+    a, b = gf.forward(rho), of.forward(rho)
+    assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+    v = -rng.uniform(0.0, 1.0, task.n_elements)
+    a, b = gf.gradient(v), of.gradient(v)
+    assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+
+
+def test_radius_update_and_mixed_precision(gpu):
+    task = oprob.cantilever(16, 8, 8)
+    gf, _ = _gpu_filter(task, 1.5, precision="mixed")
+    of = oh.HelmholtzFilterNodal(task.mesh, task.elements_volume, 1.5, _mask(task))
+    rho = np.random.default_rng(5).uniform(0.1, 1.0, task.n_elements)
+    np.testing.assert_allclose(gf.forward(rho), of.forward(rho), rtol=1e-9)
+    gf.update_radius(3.0)
+    of.update_radius(3.0)
+    np.testing.assert_allclose(gf.forward(rho), of.forward(rho), rtol=1e-9)
+
+
+def test_unstructured_mesh_is_rejected(gpu):
+    from sktopt_amd.filters import HelmholtzFilterNodal
+    from sktopt_amd.mesh import MeshTet
+    task = oprob.cantilever(3, 2, 2, kind="tet4")
+    with pytest.raises(NotImplementedError):
+        HelmholtzFilterNodal(MeshTet(task.mesh.p, task.mesh.t), task.elements_volume, 1.5,
+                             _mask(task))
