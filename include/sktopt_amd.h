@@ -90,6 +90,16 @@ int skt_sensitivity_chain(int kind, int64_t n, int ncols,
                           double E0, double Emin, double p, double scale,
                           double* g, hipStream_t stream);
 
+/* The same chain per load case, g (n, ncols) row-major, without the mean:
+ * g[:, k] = dC(rho_proj, energy[:, k]) * H'(rho_filt)
+ * (core/optimizers/common_density.py:1075-1097), for filters whose adjoint
+ * is not linear (the nodal Helmholtz gradient clamps each call). */
+int skt_sensitivity_chain_cols(int kind, int64_t n, int ncols,
+                               const double* rho_proj, const double* energy,
+                               const double* rho_filt, double beta, double eta,
+                               double E0, double Emin, double p, double* g,
+                               hipStream_t stream);
+
 /* Elementwise helpers of the update rules (logmoc.py:186-225):
  * op 0: out = a*x + b*y (y may be NULL: out = a*x); op 1: out = a*x*y. */
 int skt_ewise(int op, int64_t n, double a, const double* x, double b,
@@ -150,16 +160,22 @@ int skt_hex8_create_scalar(int64_t nx, int64_t ny, int64_t nz, double hx,
  * rho_node = sum_e w_e v_e / sum_e w_e over the elements of each node,
  * w = vol, v_e = rho_elem[e] on design elements (design[e] != 0, or all
  * when design is NULL) and fixed_value elsewhere; same summation order as
- * the reference's loop. */
-int skt_hex8_elem_to_node(int64_t nx, int64_t ny, int64_t nz, const double* vol,
-                          const uint8_t* design, double fixed_value,
-                          const double* rho_elem, double* rho_node,
-                          hipStream_t stream);
+ * the reference's loop.  rho_elem (n_el, ncols) -> rho_node (n_nodes, ncols),
+ * row-major, one averaging per column. */
+int skt_hex8_elem_to_node(int64_t nx, int64_t ny, int64_t nz, int ncols,
+                          const double* vol, const uint8_t* design,
+                          double fixed_value, const double* rho_elem,
+                          double* rho_node, hipStream_t stream);
 
 /* helmholtz_filter_nodal.py:27-28 node_to_element_density (mean of the 8
- * nodes); clamp_nonpositive applies the gradient's min(v, 0) (:232). */
-int skt_hex8_node_to_elem(int64_t nx, int64_t ny, int64_t nz, const double* rho_node,
-                          int clamp_nonpositive, double* rho_elem, hipStream_t stream);
+ * nodes) per column of rho_node (n_nodes, ncols); clamp_nonpositive applies
+ * the gradient's min(v, 0) (:232).  mean_cols = 0: rho_elem is
+ * (n_el, ncols); mean_cols = 1: rho_elem is (n_el,) = the columns summed in
+ * order and divided by ncols, as the per-load-case accumulation of the
+ * filtered sensitivities (core/optimizers/common_density.py:1098, :1101). */
+int skt_hex8_node_to_elem(int64_t nx, int64_t ny, int64_t nz, int ncols,
+                          const double* rho_node, int clamp_nonpositive,
+                          int mean_cols, double* rho_elem, hipStream_t stream);
 
 /* Unstructured linear tetrahedra.  coords: (3, n_nodes) float64 device
  * array (x row, y row, z row); conn: (4, n_el) int32 device array.
@@ -320,6 +336,26 @@ int skt_filter_gather_design(skt_filter* f, const double* full, double* design,
                              hipStream_t stream);
 int skt_filter_scatter_design(skt_filter* f, const double* design, double* full,
                               hipStream_t stream);
+
+/* The same with an explicit list of design element ids (int32, ascending):
+ * design[r] = full[ids[r]] and full[ids[r]] = design[r]
+ * (common_density.py:1108-1109, oc.py:67-68). */
+int skt_design_gather(int64_t n_design, const int32_t* design_ids,
+                      const double* full, double* design, hipStream_t stream);
+int skt_design_scatter(int64_t n_design, const int32_t* design_ids,
+                       const double* design, double* full, hipStream_t stream);
+
+/* One OC candidate at multiplier lmid (oc.py:51-68): rho_out (n_design) =
+ * clip(rho_e * clip((-dC / (lmid + 1e-12))^eta, rate_min, rate_max),
+ * max(rho_e - move, rho_min), min(rho_e + move, rho_max)), also written to
+ * rho_full[ids] when rho_full is not NULL.  The building block of the
+ * bisection for filters that are not a sparse matrix (the host drives the
+ * bracket and runs the filter between candidates). */
+int skt_oc_candidate(int64_t n_design, const int32_t* design_ids,
+                     const double* rho_e, const double* dC, double lmid,
+                     double eta, double rho_min, double rho_max,
+                     double rate_min, double rate_max, double move_limit,
+                     double* rho_full, double* rho_out, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,7 @@ import numpy as np
 
 from . import fem as fem_mod
 from . import filter as filt_mod
+from . import helmholtz
 from . import logmoc as logmoc_mod
 from . import oc as oc_mod
 from . import scheduler as sch
@@ -46,6 +47,7 @@ class Config:
     vol_frac: Sched = field(default_factory=lambda: Sched("Constant", 0.8, 0.8))
     beta: Sched = field(default_factory=lambda: Sched("StepAccelerating", 1.0, 2.0, 3, 2.0))
     neumann_scale: Sched = field(default_factory=lambda: Sched("ConstantOne", 1.0, 1.0))
+    filter_type: str = "spacial"            # "spacial" or "helmholtz"
     filter_radius: Sched = field(default_factory=lambda: Sched("Constant", 0.01, 0.01))
     move_limit: Sched = field(default_factory=lambda: Sched("SawtoothDecay", 0.3, 0.1, 6))
     percentile: Sched = field(default_factory=lambda: Sched("None"))
@@ -99,8 +101,13 @@ class DensityLoop:
         self.dV_drho_design = vol / self.vol_design_sum
         r0 = cfg.filter_radius.init if isinstance(cfg.filter_radius.num_steps, (int, float)) \
             else cfg.filter_radius.target
-        self.filter = filt_mod.SpacialFilter(task.centers, cfg.filter_radius.init,
-                                             task.design_mask)
+        if cfg.filter_type == "helmholtz":
+            self.filter = helmholtz.HelmholtzFilterNodal(task.mesh, task.elements_volume,
+                                                         cfg.filter_radius.init,
+                                                         task.design_mask)
+        else:
+            self.filter = filt_mod.SpacialFilter(task.centers, cfg.filter_radius.init,
+                                                 task.design_mask)
         self.filter.update_radius(r0)
         self.update = oc_mod.OCUpdate(cfg) if cfg.method == "OC" \
             else logmoc_mod.LogMOCUpdate(cfg)

@@ -108,4 +108,29 @@ __global__ void reduce_partials_sum(const T* __restrict__ part, int n, T* __rest
   if (threadIdx.x == 0) out[0] = s;
 }
 
+// numpy.clip semantics (NaN propagates): min(max(x, lo), hi).
+__device__ __forceinline__ double clip_np(double x, double lo, double hi) {
+  double y = x < lo ? lo : x;
+  return y > hi ? hi : y;
+}
+
+// OC multiplicative update of one design element at multiplier lmid
+// (core/optimizers/oc.py:51-65): rate = clip((-dC / (lmid + eps))^eta,
+// rate_min, rate_max); candidate = clip(rho * rate, max(rho - move, rho_min),
+// min(rho + move, rho_max)), in the reference's operation order.
+__device__ __forceinline__ double oc_candidate(double dC, double re, double lmid, double eps,
+                                               double eta, double rate_min, double rate_max,
+                                               double move_limit, double rho_min,
+                                               double rho_max) {
+  double s = -dC;
+  s = s / (lmid + eps);
+  s = pow(s, eta);
+  s = clip_np(s, rate_min, rate_max);
+  double lo = re - move_limit;
+  lo = (lo < rho_min) ? rho_min : lo;     // np.maximum
+  double hi = re + move_limit;
+  hi = (hi > rho_max) ? rho_max : hi;     // np.minimum
+  return clip_np(re * s, lo, hi);
+}
+
 }  // namespace skt

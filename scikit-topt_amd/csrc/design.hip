@@ -113,6 +113,54 @@ __global__ void k_sens_chain(int kind, int64_t n, int ncols,
   }
 }
 
+// One chain product per load case, g (n, ncols): for filters whose adjoint
+// is applied per load case (the nodal Helmholtz gradient clamps each call).
+__global__ void k_sens_chain_cols(int kind, int64_t n, int ncols,
+                                  const double* __restrict__ rho_proj,
+                                  const double* __restrict__ energy,
+                                  const double* __restrict__ rho_filt, double beta,
+                                  double eta, double denom, double E0, double Emin,
+                                  double p, double* __restrict__ g) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double t = (rho_filt[i] - eta) * beta;
+    double c = cosh(t);
+    double dH = ((1.0 / (c * c)) * beta) / denom;
+    double r = rho_proj[i];
+    for (int k = 0; k < ncols; ++k)
+      g[i * ncols + k] = dC_value(kind, r, energy[i * ncols + k], E0, Emin, p) * dH;
+  }
+}
+
+__global__ void k_design_gather(int64_t nd, const int32_t* __restrict__ ids,
+                                const double* __restrict__ full, double* __restrict__ d) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nd;
+       r += (int64_t)gridDim.x * blockDim.x)
+    d[r] = full[ids[r]];
+}
+
+__global__ void k_design_scatter(int64_t nd, const int32_t* __restrict__ ids,
+                                 const double* __restrict__ d, double* __restrict__ full) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nd;
+       r += (int64_t)gridDim.x * blockDim.x)
+    full[ids[r]] = d[r];
+}
+
+__global__ void k_oc_candidate_at(int64_t nd, const int32_t* __restrict__ ids,
+                                  const double* __restrict__ rho_e,
+                                  const double* __restrict__ dC, double lmid, double eps,
+                                  double eta, double rate_min, double rate_max,
+                                  double move_limit, double rho_min, double rho_max,
+                                  double* __restrict__ cand_full, double* __restrict__ rho_out) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nd;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const double c = oc_candidate(dC[r], rho_e[r], lmid, eps, eta, rate_min, rate_max,
+                                  move_limit, rho_min, rho_max);
+    rho_out[r] = c;
+    if (cand_full) cand_full[ids[r]] = c;
+  }
+}
+
 __global__ void k_logmoc(int64_t n, double* __restrict__ rho,
                          const double* __restrict__ dL, double eta,
                          double move_limit, double rmin, double rmax,
@@ -476,6 +524,58 @@ int skt_sensitivity_chain(int kind, int64_t n, int ncols,
   hipLaunchKernelGGL(k_sens_chain, dim3(grid_for(n)), dim3(kBlock), 0, stream,
                      kind, n, ncols, rho_proj, energy, rho_filt, beta, eta,
                      denom, E0, Emin, p, scale, g);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_sensitivity_chain_cols(int kind, int64_t n, int ncols, const double* rho_proj,
+                               const double* energy, const double* rho_filt, double beta,
+                               double eta, double E0, double Emin, double p, double* g,
+                               hipStream_t stream) {
+  SKT_CHECK(kind == SKT_SIMP || kind == SKT_RAMP || kind == SKT_RAMP_EXACT,
+            "Interpolation method must be SIMP or RAMP.");
+  SKT_CHECK(n >= 0 && ncols >= 1, "bad size");
+  if (n == 0) return SKT_OK;
+  SKT_CHECK(rho_proj && energy && rho_filt && g, "null pointer");
+  const double denom = std::tanh(beta * eta) + std::tanh(beta * (1.0 - eta)) + 1e-12;
+  hipLaunchKernelGGL(k_sens_chain_cols, dim3(grid_for(n)), dim3(kBlock), 0, stream, kind, n,
+                     ncols, rho_proj, energy, rho_filt, beta, eta, denom, E0, Emin, p, g);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_design_gather(int64_t n_design, const int32_t* design_ids, const double* full,
+                      double* design, hipStream_t stream) {
+  SKT_CHECK(n_design >= 0, "n_design must be >= 0");
+  if (n_design == 0) return SKT_OK;
+  SKT_CHECK(design_ids && full && design, "null pointer");
+  hipLaunchKernelGGL(k_design_gather, dim3(grid_for(n_design)), dim3(kBlock), 0, stream,
+                     n_design, design_ids, full, design);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_design_scatter(int64_t n_design, const int32_t* design_ids, const double* design,
+                       double* full, hipStream_t stream) {
+  SKT_CHECK(n_design >= 0, "n_design must be >= 0");
+  if (n_design == 0) return SKT_OK;
+  SKT_CHECK(design_ids && full && design, "null pointer");
+  hipLaunchKernelGGL(k_design_scatter, dim3(grid_for(n_design)), dim3(kBlock), 0, stream,
+                     n_design, design_ids, design, full);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_oc_candidate(int64_t n_design, const int32_t* design_ids, const double* rho_e,
+                     const double* dC, double lmid, double eta, double rho_min,
+                     double rho_max, double rate_min, double rate_max, double move_limit,
+                     double* rho_full, double* rho_out, hipStream_t stream) {
+  SKT_CHECK(n_design >= 0, "n_design must be >= 0");
+  if (n_design == 0) return SKT_OK;
+  SKT_CHECK(rho_e && dC && rho_out && (!rho_full || design_ids), "null pointer");
+  hipLaunchKernelGGL(k_oc_candidate_at, dim3(grid_for(n_design)), dim3(kBlock), 0, stream,
+                     n_design, design_ids, rho_e, dC, lmid, 1e-12, eta, rate_min, rate_max,
+                     move_limit, rho_min, rho_max, rho_full, rho_out);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }

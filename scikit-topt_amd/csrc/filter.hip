@@ -256,10 +256,6 @@ __global__ void k_oc_init(double* st, double l1, double l2) {
   st[6] = 0.0;
 }
 
-__device__ __forceinline__ double clip_np(double x, double lo, double hi) {
-  double y = x < lo ? lo : x;
-  return y > hi ? hi : y;
-}
 
 __global__ void k_oc_candidate(int64_t n_design, const int32_t* __restrict__ ids,
                                const double* __restrict__ rho_e,
@@ -273,16 +269,8 @@ __global__ void k_oc_candidate(int64_t n_design, const int32_t* __restrict__ ids
   const double lmid = st[2];
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
        r += (int64_t)gridDim.x * blockDim.x) {
-    double s = -dC[r];
-    s = s / (lmid + eps);
-    s = pow(s, eta);
-    s = clip_np(s, rate_min, rate_max);
-    const double re = rho_e[r];
-    double lo = re - move_limit;
-    lo = (lo < rho_min) ? rho_min : lo;     // np.maximum
-    double hi = re + move_limit;
-    hi = (hi > rho_max) ? rho_max : hi;     // np.minimum
-    double c = clip_np(re * s, lo, hi);
+    const double c = oc_candidate(dC[r], rho_e[r], lmid, eps, eta, rate_min, rate_max,
+                                  move_limit, rho_min, rho_max);
     rho_out[r] = c;
     cand_full[ids[r]] = c;
   }

@@ -37,6 +37,7 @@ SIGNATURES = {
     "skt_interpolate": [_i32, _i64, _vp, _d, _d, _d, _vp, _i32, _vp],
     "skt_dC_drho": [_i32, _i64, _vp, _vp, _i64, _d, _d, _d, _vp, _vp],
     "skt_sensitivity_chain": [_i32, _i64, _i32, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d, _vp, _vp],
+    "skt_sensitivity_chain_cols": [_i32, _i64, _i32, _vp, _vp, _vp, _d, _d, _d, _d, _d, _vp, _vp],
     "skt_ewise": [_i32, _i64, _d, _vp, _d, _vp, _vp, _vp],
     "skt_reduce": [_i32, _i64, _vp, _vp, _vp, _vp],
     "skt_reduce_cols": [_i32, _i64, _i32, _vp, _vp, _vp, _vp],
@@ -53,8 +54,8 @@ SIGNATURES = {
     "skt_solver_destroy": [_vp],
     "skt_solver_configure": [_vp, _i32, _i32, _i32],
     "skt_hex8_create_scalar": [_i64, _i64, _i64, _d, _d, _d, _d, _d, _vp, _pp],
-    "skt_hex8_elem_to_node": [_i64, _i64, _i64, _vp, _vp, _d, _vp, _vp, _vp],
-    "skt_hex8_node_to_elem": [_i64, _i64, _i64, _vp, _i32, _vp, _vp],
+    "skt_hex8_elem_to_node": [_i64, _i64, _i64, _i32, _vp, _vp, _d, _vp, _vp, _vp],
+    "skt_hex8_node_to_elem": [_i64, _i64, _i64, _i32, _vp, _i32, _i32, _vp, _vp],
     "skt_solver_info": [_vp, _i32p, _i32p],
     "skt_solver_set_timing": [_vp, _i32],
     "skt_solver_get_timing": [_vp, _dp, _i64p, _i32],
@@ -67,6 +68,9 @@ SIGNATURES = {
     "skt_filter_gradient": [_vp, _vp, _vp, _vp],
     "skt_filter_gather_design": [_vp, _vp, _vp, _vp],
     "skt_filter_scatter_design": [_vp, _vp, _vp, _vp],
+    "skt_design_gather": [_i64, _vp, _vp, _vp, _vp],
+    "skt_design_scatter": [_i64, _vp, _vp, _vp, _vp],
+    "skt_oc_candidate": [_i64, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d, _d, _vp, _vp, _vp],
     "skt_oc_bisect": [_vp, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d,
                       _d, _d, _d, _d, _i32, _vp, _vp, _vp],
     "skt_logmoc_update": [_i64, _vp, _vp, _d, _d, _d, _d, _d, _vp],

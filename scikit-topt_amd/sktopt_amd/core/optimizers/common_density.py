@@ -168,6 +168,8 @@ class DensityMethod:
         self._vol_design_sum = float(np.sum(vol))
         self._vol_design = _dev.to_dev(vol)
         self._dV_drho_design = _dev.to_dev(vol / self._vol_design_sum)
+        self._design_ids = torch.as_tensor(np.asarray(tsk.design_elements, dtype=np.int32),
+                                           device=self.dev)
         self.kkt_residual = None
         self._state = None
         self._iter_next = None
@@ -204,9 +206,9 @@ class DensityMethod:
                 self.tsk.mesh, self.tsk.elements_volume, cfg.filter_radius.init_value,
                 design_mask=self.tsk.design_mask)
         elif cfg.filter_type == "helmholtz":
-            raise NotImplementedError(
-                "filter_type='helmholtz' (HelmholtzFilterNodal) is not built on the GPU "
-                "path; use filter_type='spacial'")
+            self.filter = filters.HelmholtzFilterNodal.from_defaults(
+                self.tsk.mesh, self.tsk.elements_volume, cfg.filter_radius.init_value,
+                design_mask=self.tsk.design_mask)
         else:
             raise ValueError("should be spacial or helmholtz")
 
@@ -299,14 +301,14 @@ class DensityMethod:
         if filter_radius != self.filter.radius:
             self.filter.update_radius(filter_radius)
         S = _dev.stream()
-        fh = self.filter.handle
+        filt = self.filter
         rho, rf, rp = st["rho"], st["rho_filtered"], st["rho_projected"]
         n_el = rho.numel()
         nd = st["rho_design"].numel()
         k = tsk.n_tasks
         # rho_prev for the change record (design entries only ever change)
-        _lib.call("skt_filter_gather_design", fh, _dev.ptr(rho), _dev.ptr(st["rho_design_prev"]), S)
-        _lib.call("skt_filter_forward", fh, _dev.ptr(rho), _dev.ptr(rf), S)
+        self.gather_design(rho, st["rho_design_prev"])
+        filt.forward_into(rho, rf)
         _lib.call("skt_heaviside_project", n_el, _dev.ptr(rf), float(beta), float(cfg.beta_eta),
                   _dev.ptr(rp), S)
         comp = self.fem.objectives_multi_load(rp, p, st["u_dofs"], force_scale=neumann_scale)
@@ -316,21 +318,31 @@ class DensityMethod:
         # sensitivity: the heat FEM always uses the SIMP form (solver_heat.py:945-947)
         kind = _lib.SIMP if isinstance(tsk, LinearHeatConduction) else self.kind
         E0 = tsk.material_coef
-        _lib.call("skt_sensitivity_chain", kind, n_el, k, _dev.ptr(rp), _dev.ptr(energy),
-                  _dev.ptr(rf), float(beta), float(cfg.beta_eta), float(E0),
-                  float(E0 * cfg.E_min_coeff), float(p), 1.0 / k, _dev.ptr(st["g"]), S)
-        _lib.call("skt_filter_gradient", fh, _dev.ptr(st["g"]), _dev.ptr(st["dC_full"]), S)
+        if filt.linear_adjoint or k == 1:
+            # sum over load cases first, then one adjoint application
+            _lib.call("skt_sensitivity_chain", kind, n_el, k, _dev.ptr(rp), _dev.ptr(energy),
+                      _dev.ptr(rf), float(beta), float(cfg.beta_eta), float(E0),
+                      float(E0 * cfg.E_min_coeff), float(p), 1.0 / k, _dev.ptr(st["g"]), S)
+            filt.gradient_into(st["g"], st["dC_full"])
+        else:
+            # one adjoint per load case (batched), averaged
+            G = st.get("g_cols")
+            if G is None:
+                G = st["g_cols"] = torch.empty((n_el, k), dtype=torch.float64, device=self.dev)
+            _lib.call("skt_sensitivity_chain_cols", kind, n_el, k, _dev.ptr(rp),
+                      _dev.ptr(energy), _dev.ptr(rf), float(beta), float(cfg.beta_eta),
+                      float(E0), float(E0 * cfg.E_min_coeff), float(p), _dev.ptr(G), S)
+            filt.gradient_mean_into(G, st["dC_full"])
         if cfg.sensitivity_filter:
             tmp = torch.empty_like(st["dC_full"])
-            _lib.call("skt_filter_forward", fh, _dev.ptr(st["dC_full"]), _dev.ptr(tmp), S)
+            filt.forward_into(st["dC_full"], tmp)
             st["dC_full"].copy_(tmp)
-        _lib.call("skt_filter_gather_design", fh, _dev.ptr(st["dC_full"]),
-                  _dev.ptr(st["dC_design"]), S)
-        _lib.call("skt_filter_gather_design", fh, _dev.ptr(rho), _dev.ptr(st["rho_design"]), S)
+        self.gather_design(st["dC_full"], st["dC_design"])
+        self.gather_design(rho, st["rho_design"])
         self.rho_update(iter_num, st["rho_design"], rp, st["dC_design"], st["u_dofs"], None,
                         None, move_limit, eta, beta, None, None, percentile, self._vol_design,
                         self._vol_design_sum, vol_frac)
-        _lib.call("skt_filter_scatter_design", fh, _dev.ptr(st["rho_design"]), _dev.ptr(rho), S)
+        self.scatter_design(st["rho_design"], rho)
         # records (one small device->host copy per iteration)
         sc = st["scal"]
         _lib.call("skt_reduce", _lib.RED_MAXABSDIFF, nd, _dev.ptr(st["rho_design"]),
@@ -370,6 +382,16 @@ class DensityMethod:
         """Subclass hook: turn the device scalar slots (copied to the host
         once per iteration) into named records."""
         return {}
+
+    def gather_design(self, full, design):
+        """``design = full[design_elements]`` on the device."""
+        _lib.call("skt_design_gather", design.numel(), _dev.ptr(self._design_ids),
+                  _dev.ptr(full), _dev.ptr(design), _dev.stream())
+
+    def scatter_design(self, design, full):
+        """``full[design_elements] = design`` on the device."""
+        _lib.call("skt_design_scatter", design.numel(), _dev.ptr(self._design_ids),
+                  _dev.ptr(design), _dev.ptr(full), _dev.stream())
 
     # --- accessors ----------------------------------------------------------
     def rho_host(self) -> np.ndarray:

@@ -69,15 +69,14 @@ class LogMOC_Optimizer(common_density.DensityMethod):
                    elements_volume_design_sum, vol_frac):
         cfg, tsk, st = self.cfg, self.tsk, self._state
         S = _dev.stream()
-        fh = self.filter.handle
+        filt = self.filter
         nd = rho_design_eles.numel()
         n_el = rho_projected.numel()
         vsum = float(elements_volume_design_sum)
         if self._buf is None:
             z = lambda m: torch.zeros(m, dtype=torch.float64, device=self.dev)  # noqa: E731
             dVp = z(n_el)
-            _lib.call("skt_filter_scatter_design", fh, _dev.ptr(self._dV_drho_design),
-                      _dev.ptr(dVp), S)
+            self.scatter_design(self._dV_drho_design, dVp)
             self._buf = dict(dC_raw=z(nd), dL=z(nd), chain=z(nd), dVp=dVp, dVf=z(n_el),
                              back=z(n_el), full=z(n_el), full2=z(n_el), tmp=z(nd),
                              s=z(1), zero1=z(1), one1=torch.ones(1, dtype=torch.float64,
@@ -92,15 +91,15 @@ class LogMOC_Optimizer(common_density.DensityMethod):
         _lib.call("skt_heaviside_derivative", n_el, _dev.ptr(st["rho_filtered"]), float(beta),
                   float(cfg.beta_eta), _dev.ptr(b["dVf"]), S)
         self._ewise(1, 1.0, b["dVf"], 0.0, b["dVp"], b["dVf"])
-        _lib.call("skt_filter_gradient", fh, _dev.ptr(b["dVf"]), _dev.ptr(b["back"]), S)
+        filt.gradient_into(b["dVf"], b["back"])
         back = b["back"]
         # np.allclose(back, 0) and any(dVf > 0): forward fallback (logmoc.py:145-153)
         if self._scalar(_lib.RED_MAXABS, n_el, back) <= 1e-8 and \
                 self._scalar(_lib.RED_MAXABS, n_el, b["dVf"]) > 0.0:
-            _lib.call("skt_filter_forward", fh, _dev.ptr(b["dVf"]), _dev.ptr(b["full"]), S)
+            filt.forward_into(b["dVf"], b["full"])
             back = b["full"]
-        _lib.call("skt_filter_gather_design", fh, _dev.ptr(back), _dev.ptr(b["chain"]), S)
-        _lib.call("skt_filter_gather_design", fh, _dev.ptr(rho_projected), _dev.ptr(b["tmp"]), S)
+        self.gather_design(back, b["chain"])
+        self.gather_design(rho_projected, b["tmp"])
         volume = self._scalar(_lib.RED_DOT, nd, b["tmp"], elements_volume_design) / vsum
         vol_error = volume - vol_frac
         chain_scale = 1.0
@@ -125,9 +124,9 @@ class LogMOC_Optimizer(common_density.DensityMethod):
         self._ewise(0, 1.0, b["dC_raw"], coeff, b["chain"], b["dL"])
         if cfg.filter_lagrangian:
             b["full"].zero_()
-            _lib.call("skt_filter_scatter_design", fh, _dev.ptr(b["dL"]), _dev.ptr(b["full"]), S)
-            _lib.call("skt_filter_forward", fh, _dev.ptr(b["full"]), _dev.ptr(b["full2"]), S)
-            _lib.call("skt_filter_gather_design", fh, _dev.ptr(b["full2"]), _dev.ptr(b["dL"]), S)
+            self.scatter_design(b["dL"], b["full"])
+            filt.forward_into(b["full"], b["full2"])
+            self.gather_design(b["full2"], b["dL"])
         if cfg.center_lagrangian:
             mean = self._scalar(_lib.RED_DOT, nd, b["dL"], elements_volume_design) / vsum
             b["tmp"].fill_(mean)

@@ -55,15 +55,15 @@ class HelmholtzFilterNodal:
         if self.design_mask is not None and (~self.design_mask).any():
             fixed[np.unique(mesh.t[:, ~self.design_mask].ravel())] = 1
         self._fixed_fwd = torch.as_tensor(fixed, device=dev)
+        self._fixed_ids = torch.as_tensor(np.nonzero(fixed)[0], dtype=torch.int64, device=dev)
+        self._n_fixed = int(fixed.sum())
         self._fixed_none = torch.zeros(self.n_nodes, dtype=torch.uint8, device=dev)
         self._ones64 = torch.ones(self.n_el, dtype=torch.float64, device=dev)
         self._ones32 = torch.ones(self.n_el, dtype=torch.float32, device=dev)
         self._mass = self._scalar_op(1.0, 0.0, self._fixed_none)
-        self._ops = {}       # case -> (operator, solver) for the current radius
+        self._ops = {}       # case -> (operator, solver, max_cols) for the current radius
         self._built_radius = None
-        self._warm = {"forward": torch.zeros((self.n_nodes, 1), dtype=torch.float64, device=dev),
-                      "gradient": torch.zeros((self.n_nodes, 1), dtype=torch.float64,
-                                              device=dev)}
+        self._warm = {}      # case -> previous nodal solution (warm start)
         self.last_solve = {}
 
     @classmethod
@@ -81,24 +81,31 @@ class HelmholtzFilterNodal:
                   _dev.ptr(fixed), C.byref(h))
         return h
 
-    def _ensure(self):
-        if self._built_radius == self.radius and self._ops:
-            return
-        self._destroy_ops()
-        r2 = self.radius ** 2
-        for case, fixed in (("forward", self._fixed_fwd), ("gradient", self._fixed_none)):
-            op = self._scalar_op(1.0, r2, fixed)
-            s = C.c_void_p()
-            _lib.call("skt_solver_create", op, 1, C.byref(s))
-            self._ops[case] = (op, s)
-        self._built_radius = self.radius
+    def _ensure(self, case, ncols):
+        if self._built_radius != self.radius:
+            self._destroy_ops()
+            self._built_radius = self.radius
+        have = self._ops.get(case)
+        if have is not None and have[2] >= ncols:
+            return have
+        if have is not None:
+            self._destroy_one(case)
+        fixed = self._fixed_fwd if case == "forward" else self._fixed_none
+        op = self._scalar_op(1.0, self.radius ** 2, fixed)
+        s = C.c_void_p()
+        _lib.call("skt_solver_create", op, int(ncols), C.byref(s))
+        self._ops[case] = (op, s, int(ncols))
+        return self._ops[case]
+
+    def _destroy_one(self, case):
+        op, s, _ = self._ops.pop(case)
+        lib = _lib.load()
+        lib.skt_solver_destroy(s)
+        lib.skt_operator_destroy(op)
 
     def _destroy_ops(self):
-        lib = _lib.load()
-        for op, s in self._ops.values():
-            lib.skt_solver_destroy(s)
-            lib.skt_operator_destroy(op)
-        self._ops = {}
+        for case in list(self._ops):
+            self._destroy_one(case)
 
     def __del__(self):
         try:
@@ -109,35 +116,46 @@ class HelmholtzFilterNodal:
             pass
 
     # --- the filter ----------------------------------------------------
-    def _run(self, case, v):
-        self._ensure()
-        host = _dev.is_host(v)
-        x = _dev.to_dev(v).contiguous()
+    def _apply(self, case, x, out, mean_cols=False):
+        """x: (n_el,) or (n_el, k) float64 device tensor; out: (n_el,) when
+        x is one column or mean_cols, else (n_el, k)."""
+        k = 1 if x.dim() == 1 else int(x.shape[1])
+        op, solver, _ = self._ensure(case, k)
         S = _dev.stream()
         nx, ny, nz = self.grid
-        rn = torch.empty(self.n_nodes, dtype=torch.float64, device=x.device)
-        _lib.call("skt_hex8_elem_to_node", nx, ny, nz, _dev.ptr(self._vol),
+        dev = x.device
+        rn = torch.empty((self.n_nodes, k), dtype=torch.float64, device=dev)
+        _lib.call("skt_hex8_elem_to_node", nx, ny, nz, k, _dev.ptr(self._vol),
                   _dev.ptr(self._design) if self._design is not None else None,
                   1.0 if case == "forward" else 0.0, _dev.ptr(x), _dev.ptr(rn), S)
-        b = torch.empty((self.n_nodes, 1), dtype=torch.float64, device=x.device)
-        _lib.call("skt_operator_apply", self._mass, _lib.F64, 1, _dev.ptr(self._ones64),
+        b = torch.empty((self.n_nodes, k), dtype=torch.float64, device=dev)
+        _lib.call("skt_operator_apply", self._mass, _lib.F64, k, _dev.ptr(self._ones64),
                   _dev.ptr(rn), _dev.ptr(b), S)
-        if case == "forward":
-            b[self._fixed_fwd.bool(), 0] = 1.0     # prescribed x_D = 1 (enforced rhs)
-        op, solver = self._ops[case]
-        U = self._warm[case]
+        if case == "forward" and self._n_fixed:
+            b.index_fill_(0, self._fixed_ids, 1.0)     # prescribed x_D = 1 (enforced rhs)
+        U = self._warm.get(case)
+        if U is None or U.shape[1] != k:
+            U = torch.zeros((self.n_nodes, k), dtype=torch.float64, device=dev)
+            self._warm[case] = U
         mode = _lib.F32 if self.precision == "mixed" else _lib.F64
         iters = (C.c_int64 * 2)()
-        relres = (C.c_double * 1)()
+        relres = (C.c_double * k)()      # one final residual per column
         st = _lib.load().skt_solver_solve(
-            solver, mode, 1, _dev.ptr(self._ones64),
+            solver, mode, k, _dev.ptr(self._ones64),
             _dev.ptr(self._ones32) if mode == _lib.F32 else None, _dev.ptr(b), _dev.ptr(U),
             self.rtol, 100000, 0.0, iters, relres, S)
-        self.last_solve = dict(case=case, iters=int(iters[0]), relres=float(relres[0]))
+        self.last_solve = dict(case=case, iters=int(iters[0]), relres=max(relres))
         _lib.check(st, "skt_solver_solve")
-        out = torch.empty(self.n_el, dtype=torch.float64, device=x.device)
-        _lib.call("skt_hex8_node_to_elem", nx, ny, nz, _dev.ptr(U), 1 if case == "gradient" else 0,
+        _lib.call("skt_hex8_node_to_elem", nx, ny, nz, k, _dev.ptr(U),
+                  1 if case == "gradient" else 0, 1 if (mean_cols or k == 1) else 0,
                   _dev.ptr(out), S)
+        return out
+
+    def _run(self, case, v):
+        host = _dev.is_host(v)
+        x = _dev.to_dev(v).contiguous()
+        out = torch.empty(x.shape, dtype=torch.float64, device=x.device)
+        self._apply(case, x, out)
         return out.cpu().numpy() if host else out
 
     def forward(self, rho_element):
@@ -147,3 +165,18 @@ class HelmholtzFilterNodal:
     def gradient(self, v_ele):
         """``helmholtz_filter_nodal.py:203-233`` (clamped to <= 0)."""
         return self._run("gradient", v_ele)
+
+    # --- device interface of the density loop ------------------------------
+    linear_adjoint = False    # the gradient clamps: applied per load case
+
+    def forward_into(self, x, y):
+        self._apply("forward", x, y)
+
+    def gradient_into(self, v, g):
+        self._apply("gradient", v, g)
+
+    def gradient_mean_into(self, G, g):
+        """g = mean over the columns of G (n_el, k) of gradient(G[:, j]),
+        the per-load-case accumulation of common_density.py:1075-1101, with
+        the k solves batched in one multi-column solve."""
+        self._apply("gradient", G, g, mean_cols=True)

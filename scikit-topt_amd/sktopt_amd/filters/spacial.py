@@ -103,6 +103,15 @@ class SpacialFilter:
         """``out = 0; out[design] = W^T v[design]`` (``spacial.py:159-166``)."""
         return self._run("skt_filter_gradient", v)
 
+    # --- device interface of the density loop ------------------------------
+    linear_adjoint = True     # a fixed sparse matrix: the load cases can be summed first
+
+    def forward_into(self, x, y):
+        _lib.call("skt_filter_forward", self.handle, _dev.ptr(x), _dev.ptr(y), _dev.stream())
+
+    def gradient_into(self, v, g):
+        _lib.call("skt_filter_gradient", self.handle, _dev.ptr(v), _dev.ptr(g), _dev.stream())
+
     def csr(self):
         """(indptr, indices, data, design_ids) as numpy, for inspection."""
         self._ensure()

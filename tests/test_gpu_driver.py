@@ -18,7 +18,7 @@ def _oracle_cfg(method="OC", vol=0.3, r=1.5, beta=1.0, emin=1e-9, iters=50, **kw
 
 
 def _gpu_opt(task_o, method="OC", vol=0.3, r=1.5, beta=1.0, emin=1e-9, iters=50,
-             precision="fp64", rtol=1e-12, tmp="/tmp/skt", **kw):
+             precision="fp64", rtol=1e-12, tmp="/tmp/skt", filter_type="spacial", **kw):
     from sktopt_amd import core, mesh as pmesh
     from sktopt_amd.tools import SchedulerConfig as SC
     pm = task_o.mesh
@@ -32,7 +32,7 @@ def _gpu_opt(task_o, method="OC", vol=0.3, r=1.5, beta=1.0, emin=1e-9, iters=50,
                                        task_o.forces[0], None, task_o.E0)
     common = dict(dst_path=tmp, max_iters=iters, p=SC.constant(target_value=3.0),
                   vol_frac=SC.constant(target_value=vol), beta=SC.constant(target_value=beta),
-                  filter_type="spacial", filter_radius=SC.constant(target_value=r),
+                  filter_type=filter_type, filter_radius=SC.constant(target_value=r),
                   move_limit=SC.constant(target_value=0.2), E_min_coeff=emin,
                   precision=precision, solver_rtol=rtol, **kw)
     if method == "OC":
@@ -111,13 +111,54 @@ def test_tet_cantilever_matches_oracle(gpu, tmp_path):
         assert np.abs(opt.rho_host() - loop.rho).max() <= 1e-8
 
 
-def test_full_vs_stepwise_bit_identical(gpu, tmp_path):
+@pytest.mark.parametrize("task_fn,ftype,r,vol,iters", [
+    (lambda: oprob.cantilever(16, 8, 8), "helmholtz", 1.5, 0.3, 5),
+    (lambda: oprob.multiload(12, 6, 6, n_loads=8, seed=0), "helmholtz", 2.0, 0.2, 3),
+])
+def test_helmholtz_oc_trajectory_matches_oracle(gpu, task_fn, ftype, r, vol, iters, tmp_path):
+    """OC with the nodal Helmholtz filter (the reference's default filter):
+    one filter solve per bisection evaluation, the clamped gradient applied
+    per load case (batched over the 8 load cases of the multi-load task)."""
+    loop = odrv.DensityLoop(_oracle_cfg(vol=vol, r=r, filter_type=ftype), task_fn())
+    opt = _gpu_opt(task_fn(), vol=vol, r=r, precision="fp64", rtol=1e-13, tmp=str(tmp_path),
+                   filter_type=ftype)
+    for it in range(1, iters + 1):
+        rec = loop.step(it)
+        opt.optimize_steps(1)
+        c = opt.recorder.latest("compliance")
+        assert abs(c - rec["compliance"]) <= 1e-9 * abs(rec["compliance"]), (it, c, rec)
+        assert opt.recorder.latest("bisection_evals") == rec["n_evals"]
+        np.testing.assert_allclose(opt.recorder.latest("lmid"), rec["lmid"], rtol=1e-6)
+        assert np.abs(opt.rho_host() - loop.rho).max() <= 1e-8, it
+
+
+def test_helmholtz_logmoc_heat_matches_oracle(gpu, tmp_path):
+    task = oprob.heat_patch(10, patch=4, source=1e-2)
+    cfg = _oracle_cfg(method="LogMOC", vol=0.3, r=2.0, emin=1e-3,
+                      eta=odrv.Sched("Constant", 0.6, 0.6), lambda_lower=-1e7,
+                      lambda_upper=1e7, filter_type="helmholtz")
+    loop = odrv.DensityLoop(cfg, task)
+    opt = _gpu_opt(oprob.heat_patch(10, patch=4, source=1e-2), method="LogMOC", vol=0.3, r=2.0,
+                   emin=1e-3, precision="fp64", rtol=1e-13, tmp=str(tmp_path),
+                   filter_type="helmholtz")
+    for it in range(1, 5):
+        rec = loop.step(it)
+        opt.optimize_steps(1)
+        c = opt.recorder.latest("compliance")
+        assert abs(c - rec["compliance"]) <= 1e-9 * abs(rec["compliance"])
+        np.testing.assert_allclose(opt.recorder.latest("lambda_v"), rec["lambda_v"], rtol=1e-7,
+                                   atol=1e-12)
+        assert np.abs(opt.rho_host() - loop.rho).max() <= 1e-8
+
+
+@pytest.mark.parametrize("ftype", ["spacial", "helmholtz"])
+def test_full_vs_stepwise_bit_identical(gpu, ftype, tmp_path):
     """tests/test_optimize_steps_equivalence.py:49-52 of the reference."""
     a = _gpu_opt(oprob.cantilever(10, 5, 5), iters=5, precision="mixed", rtol=1e-8,
-                 tmp=str(tmp_path / "a"))
+                 tmp=str(tmp_path / "a"), filter_type=ftype)
     a.optimize()
     b = _gpu_opt(oprob.cantilever(10, 5, 5), iters=5, precision="mixed", rtol=1e-8,
-                 tmp=str(tmp_path / "b"))
+                 tmp=str(tmp_path / "b"), filter_type=ftype)
     for _ in range(5):
         b.optimize_steps(1)
     np.testing.assert_array_equal(a.rho_host(), b.rho_host())

@@ -35,7 +35,7 @@ def test_element_node_transfers_are_bitwise(gpu):
     x = torch.tensor(rho, device="cuda")
     for fixed in (1.0, 0.0):
         rn = torch.empty(task.mesh.nnodes, dtype=torch.float64, device="cuda")
-        _lib.call("skt_hex8_elem_to_node", nx, ny, nz, vol.data_ptr(), des.data_ptr(), fixed,
+        _lib.call("skt_hex8_elem_to_node", nx, ny, nz, 1, vol.data_ptr(), des.data_ptr(), fixed,
                   x.data_ptr(), rn.data_ptr(), s)
         ref = oh.element_to_node_density_averaging(task.mesh, task.elements_volume, rho,
                                                    _mask(task), True, fixed)
@@ -43,11 +43,68 @@ def test_element_node_transfers_are_bitwise(gpu):
     vn = torch.tensor(rng.standard_normal(task.mesh.nnodes), device="cuda")
     out = torch.empty(task.n_elements, dtype=torch.float64, device="cuda")
     for clamp in (0, 1):
-        _lib.call("skt_hex8_node_to_elem", nx, ny, nz, vn.data_ptr(), clamp, out.data_ptr(), s)
+        _lib.call("skt_hex8_node_to_elem", nx, ny, nz, 1, vn.data_ptr(), clamp, 0,
+                  out.data_ptr(), s)
         ref = oh.node_to_element_density(task.mesh, vn.cpu().numpy())
         if clamp:
             ref = np.minimum(ref, 0.0)
         np.testing.assert_array_equal(out.cpu().numpy(), ref)
+
+
+def test_column_batched_transfers_are_bitwise(gpu):
+    """k columns at once: per-column results equal the one-column ones, and
+    the folded mean equals the per-load-case accumulation acc += g_k;
+    acc /= k of common_density.py:1098-1101."""
+    import torch
+    from sktopt_amd import _lib
+    task = oprob.cantilever(9, 6, 5)
+    nx, ny, nz = task.mesh.grid
+    mask = _mask(task)
+    k = 5
+    rng = np.random.default_rng(3)
+    s = torch.cuda.current_stream().cuda_stream
+    R = rng.standard_normal((task.n_elements, k))
+    vol = torch.tensor(task.elements_volume, device="cuda")
+    des = torch.tensor(mask.astype(np.uint8), device="cuda")
+    x = torch.tensor(R, device="cuda")
+    rn = torch.empty((task.mesh.nnodes, k), dtype=torch.float64, device="cuda")
+    _lib.call("skt_hex8_elem_to_node", nx, ny, nz, k, vol.data_ptr(), des.data_ptr(), 0.0,
+              x.data_ptr(), rn.data_ptr(), s)
+    for j in range(k):
+        ref = oh.element_to_node_density_averaging(task.mesh, task.elements_volume, R[:, j],
+                                                   mask, True, 0.0)
+        np.testing.assert_array_equal(rn[:, j].cpu().numpy(), ref)
+    V = rng.standard_normal((task.mesh.nnodes, k))
+    vn = torch.tensor(V, device="cuda")
+    cols = torch.empty((task.n_elements, k), dtype=torch.float64, device="cuda")
+    _lib.call("skt_hex8_node_to_elem", nx, ny, nz, k, vn.data_ptr(), 1, 0, cols.data_ptr(), s)
+    mean = torch.empty(task.n_elements, dtype=torch.float64, device="cuda")
+    _lib.call("skt_hex8_node_to_elem", nx, ny, nz, k, vn.data_ptr(), 1, 1, mean.data_ptr(), s)
+    acc = np.zeros(task.n_elements)
+    for j in range(k):
+        g = np.minimum(oh.node_to_element_density(task.mesh, V[:, j]), 0.0)
+        np.testing.assert_array_equal(cols[:, j].cpu().numpy(), g)
+        acc += g
+    acc /= k
+    np.testing.assert_array_equal(mean.cpu().numpy(), acc)
+
+
+def test_batched_gradient_mean_matches_per_case_loop(gpu):
+    import torch
+    task = oprob.cantilever(12, 6, 6)
+    gf, _ = _gpu_filter(task, 2.0)
+    of = oh.HelmholtzFilterNodal(task.mesh, task.elements_volume, 2.0, _mask(task))
+    k = 4
+    G = -np.random.default_rng(4).uniform(0.0, 1.0, (task.n_elements, k))
+    G[::7, 1] *= -1.0      # some positive entries: the clamp acts per case
+    out = torch.empty(task.n_elements, dtype=torch.float64, device="cuda")
+    gf.gradient_mean_into(torch.tensor(G, device="cuda"), out)
+    ref = np.zeros(task.n_elements)
+    for j in range(k):
+        ref += of.gradient(G[:, j])
+    ref /= k
+    got = out.cpu().numpy()
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
 @pytest.mark.parametrize("grid,r", [((12, 6, 6), 1.5), ((16, 8, 8), 2.5), ((31, 16, 8), 2.0)])
