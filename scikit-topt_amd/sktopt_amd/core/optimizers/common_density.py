@@ -32,6 +32,8 @@ from ... import fea, filters
 from ...fea import composer
 from ...mesh import LinearElasticity, LinearHeatConduction
 from ...tools import SchedulerConfig, Schedulers
+from ...tools.history import HistoryCollection  # noqa: F401
+from .. import visualization
 
 logger = logging.getLogger(__name__)
 
@@ -92,9 +94,32 @@ class DensityMethodConfig:
             raise ValueError(f"Unknown solver: {self.solver_option}")
 
     def export(self, path: str):
+        """``common_density.py:322-324``: ``cfg.json`` (schedules as dicts)."""
         d = {k: v for k, v in asdict(self).items()}
+        os.makedirs(path, exist_ok=True)
         with open(f"{path}/cfg.json", "w") as f:
             json.dump(d, f, indent=2, default=str)
+
+    @classmethod
+    def import_from(cls, path: str):
+        """``common_density.py:312-319``; schedule dicts become
+        ``SchedulerConfig`` again."""
+        with open(f"{path}/cfg.json", "r") as f:
+            data = json.load(f)
+        data.pop("record_timing", None)
+        keys = inspect.signature(cls).parameters.keys()
+        out = {}
+        for k, v in data.items():
+            if k not in keys:
+                continue
+            if isinstance(v, dict) and "scheduler_type" in v:
+                v = SchedulerConfig(**v)
+            out[k] = v
+        return cls(**out)
+
+    def vtu_path(self, iter_num: int):
+        """``common_density.py:338-339``."""
+        return f"{self.dst_path}/mesh_rho/info_mesh-{iter_num:08d}.vtu"
 
 
 @dataclass
@@ -106,35 +131,18 @@ class DensityMethod_OC_Config(DensityMethodConfig):
         "move_limit", 0.3, 0.1, 3))
 
 
+def _stats4(x: torch.Tensor) -> torch.Tensor:
+    """[min, mean, max, std] (population std, as numpy's) of a device array."""
+    x = x.reshape(-1)
+    return torch.stack([x.min(), x.mean(), x.max(), x.std(correction=0)])
+
+
 def interpolation_funcs(cfg):
     if cfg.interpolation == "SIMP":
         return composer.simp_interpolation, _lib.SIMP
     if cfg.interpolation == "RAMP":
         return composer.ramp_interpolation, _lib.RAMP
     raise ValueError("Interpolation method must be SIMP or RAMP.")
-
-
-class HistoryCollection:
-    """Per-iteration records (names of ``common_density.py:591-611``)."""
-
-    def __init__(self):
-        self.data: dict[str, list] = {}
-
-    def add(self, name, **_):
-        self.data.setdefault(name, [])
-
-    def feed_data(self, name, value):
-        self.data.setdefault(name, []).append(value)
-
-    def latest(self, name):
-        v = self.data.get(name, [])
-        return v[-1] if v else None
-
-    def as_object(self):
-        return type("Histories", (), {k: list(v) for k, v in self.data.items()})
-
-    def as_object_latest(self):
-        return type("Latest", (), {k: (v[-1] if v else None) for k, v in self.data.items()})
 
 
 class DensityMethod:
@@ -162,7 +170,8 @@ class DensityMethod:
             raise NotImplementedError("")
         self.schedulers = Schedulers(cfg.dst_path)
         self._schedulers_initialized = False
-        self.recorder = HistoryCollection()
+        cfg.export(cfg.dst_path)
+        self.recorder = self.add_recorder(tsk)
         self.dev = _dev.device()
         vol = tsk.elements_volume[tsk.design_elements]
         self._vol_design_sum = float(np.sum(vol))
@@ -179,6 +188,36 @@ class DensityMethod:
         self.timings = []
 
     # --- setup ------------------------------------------------------------
+    def add_recorder(self, tsk) -> HistoryCollection:
+        """History names and types of ``common_density.py:591-611``."""
+        r = HistoryCollection(self.cfg.dst_path)
+        r.add("rho_projected", plot_type="min-max-mean-std")
+        r.add("energy", plot_type="min-max-mean-std")
+        r.add("vol_error")
+        r.add("u_max", plot_type="min-max-mean-std" if tsk.n_tasks > 1 else "value")
+        r.add(self._objective_history_name(tsk), ylog=self._objective_history_ylog(tsk))
+        r.add("scaling_rate", plot_type="min-max-mean-std")
+        r.add("neumann_scale")
+        r.add("rho_change_max")
+        r.add("kkt_residual")
+        return r
+
+    @staticmethod
+    def _objective_history_name(tsk) -> str:
+        """``common_density.py:613-621``."""
+        if isinstance(tsk, LinearHeatConduction):
+            return str(getattr(tsk, "objective", "compliance"))
+        return "compliance"
+
+    @staticmethod
+    def _objective_history_ylog(tsk) -> bool:
+        if isinstance(tsk, LinearHeatConduction):
+            return getattr(tsk, "objective", "compliance") == "compliance"
+        return True
+
+    def params_latest(self):
+        return self.recorder.as_object_latest()
+
     def init_schedulers(self, export: bool = True):
         cfg = self.cfg
         s = self.schedulers
@@ -283,13 +322,15 @@ class DensityMethod:
             self._completed = True
             return
         limit = end if max_steps is None else min(start + max_steps, end)
+        self._iter_limit = limit
+        converged = False
         for it in range(start, limit):
             converged = self.step(it)
             self._iter_next = it + 1
             if converged:
-                self._completed = True
                 break
-        if self._iter_next >= end:
+        if converged or self._iter_next >= end:
+            self._finalize()
             self._completed = True
 
     def step(self, iter_num: int) -> bool:
@@ -347,21 +388,33 @@ class DensityMethod:
         sc = st["scal"]
         _lib.call("skt_reduce", _lib.RED_MAXABSDIFF, nd, _dev.ptr(st["rho_design"]),
                   _dev.ptr(st["rho_design_prev"]), _dev.ptr(sc[0:1]), S)
-        rec = torch.cat([sc[:8], comp.reshape(-1), u_max.reshape(-1)]).cpu().numpy()
+        # array records as [min, mean, max, std] taken on the device
+        rpd = st.get("rp_design")
+        if rpd is None:
+            rpd = st["rp_design"] = torch.empty_like(st["rho_design"])
+        self.gather_design(rp, rpd)
+        arrays = {"rho_projected": rpd, "energy": energy.mean(dim=1)}
+        arrays.update(self._record_arrays(st))
+        names = list(arrays)
+        stats = [_stats4(arrays[n]) for n in names]
+        rec = torch.cat([sc[:8], comp.reshape(-1), u_max.reshape(-1)] + stats).cpu().numpy()
         upd = self._update_records(rec[:8])
         rho_change_max = float(rec[0])
         comp_h = rec[8:8 + k]
         umax_h = rec[8 + k:8 + 2 * k]
+        stats_h = rec[8 + 2 * k:].reshape(-1, 4)
         compliance_avg = float(comp_h.mean())
         r = self.recorder
-        r.feed_data("compliance", compliance_avg)
-        r.feed_data("compliance_each", comp_h.copy())
         r.feed_data("rho_change_max", rho_change_max)
+        for n, sv in zip(names, stats_h):
+            r.feed_stats(n, sv)
+        r.feed_data(self._objective_history_name(tsk), compliance_avg)
         r.feed_data("u_max", float(umax_h[0]) if k == 1 else umax_h.copy())
         r.feed_data("neumann_scale", neumann_scale)
-        r.feed_data("solve_iters", self.fem.last_solve.get("iters"))
         for name, v in upd.items():
             r.feed_data(name, v)
+        r.feed_data("compliance_each", comp_h.copy())
+        r.feed_data("solve_iters", self.fem.last_solve.get("iters"))
         converged = False
         if cfg.check_convergence:
             kkt = self.kkt_residual
@@ -371,12 +424,32 @@ class DensityMethod:
             conv_kkt = abs(kkt) < cfg.tol_kkt_residual if np.isfinite(kkt) else True
             converged = conv_rho and conv_kkt
         rt = max(cfg.max_iters // max(cfg.record_times, 1), 1)
+        # save points of common_density.py:1186-1193 (the last iteration of
+        # this call, as the reference does for partial runs)
+        last = getattr(self, "_iter_limit", self._iter_end) - 1
         if cfg.write_checkpoints and (iter_num % rt == 0 or iter_num == 1 or converged or
-                                      iter_num == self._iter_end - 1):
+                                      iter_num == last):
             os.makedirs(f"{cfg.dst_path}/data", exist_ok=True)
+            visualization.export_mesh_with_info(
+                tsk.mesh, cell_data_names=["rho_projected", "energy"],
+                cell_data_values=[rp.cpu().numpy(), arrays["energy"].cpu().numpy()],
+                filepath=cfg.vtu_path(iter_num))
             np.savez_compressed(f"{cfg.dst_path}/data/{iter_num:06d}-rho.npz",
                                 rho_design_elements=self.rho_host()[tsk.design_elements])
         return converged
+
+    def _record_arrays(self, st) -> dict:
+        """Subclass hook: device arrays recorded as [min, mean, max, std]."""
+        return {}
+
+    def _finalize(self):
+        """``common_density.py:944-964``: ``histories.npz`` always; the
+        ``cubic_top.vtu`` submesh (rho > 0.5) when checkpoints are written."""
+        cfg, tsk = self.cfg, self.tsk
+        if cfg.write_checkpoints and self._state is not None:
+            visualization.export_submesh(tsk, self.rho_host(), 0.5,
+                                         f"{cfg.dst_path}/cubic_top.vtu")
+        self.recorder.export_histories(fname="histories.npz")
 
     def _update_records(self, scal: np.ndarray) -> dict:
         """Subclass hook: turn the device scalar slots (copied to the host

@@ -149,6 +149,22 @@ class LogMOC_Optimizer(common_density.DensityMethod):
         self._last = dict(lambda_v=self.lambda_v, constraint_coeff=coeff, vol_error=vol_error,
                           volume_chain_scale=chain_scale, running_scale=self.running_scale)
 
+    def add_recorder(self, tsk):
+        """``logmoc.py:75-83``."""
+        r = super().add_recorder(tsk)
+        r.add("dL", plot_type="min-max-mean-std", ylog=False)
+        r.add("-dC", plot_type="min-max-mean-std", ylog=True)
+        r.add("lambda_v", ylog=False)
+        r.add("constraint_coeff", ylog=False)
+        r.add("dV_chain", plot_type="min-max-mean-std", ylog=True)
+        r.add("volume_chain_scale", ylog=True)
+        return r
+
+    def _record_arrays(self, st) -> dict:
+        """``logmoc.py:238-245``."""
+        b = self._buf
+        return {"-dC": -b["dC_raw"], "dV_chain": b["chain"].abs(), "dL": b["dL"]}
+
     def _update_records(self, scal: np.ndarray) -> dict:
         self.kkt_residual = float(scal[5])
         d = dict(self._last)

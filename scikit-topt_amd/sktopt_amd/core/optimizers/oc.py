@@ -53,6 +53,7 @@ class OC_Optimizer(common_density.DensityMethod):
         st = self._state
         if st is None:
             raise RuntimeError("Optimizer state is not initialized.")
+        self._last_eta = float(eta)
         S = _dev.stream()
         n = rho_design_eles.numel()
         sc = st["scal"]
@@ -134,6 +135,23 @@ class OC_Optimizer(common_density.DensityMethod):
             it += 1
             lmid = 0.5 * (l1 + l2)
         res.copy_(torch.tensor([lmid, vol_error, float(n_evals), l2 - l1], dtype=torch.float64))
+
+    def add_recorder(self, tsk):
+        """``oc.py:118-122``."""
+        r = super().add_recorder(tsk)
+        r.add("-dC", plot_type="min-max-mean-std", ylog=True)
+        r.add("lmid", ylog=True)
+        return r
+
+    def _record_arrays(self, st) -> dict:
+        """``oc.py:247-250`` and the driver's ``scaling_rate`` record: the
+        normalised sensitivities and the rates at the final multiplier."""
+        cfg = self.cfg
+        dC = st["dC_design"]
+        lmid = st["scal"][S_LMID]
+        rate = torch.clamp(torch.pow(-dC / (lmid + 1e-12), self._last_eta),
+                           cfg.scaling_rate_min, cfg.scaling_rate_max)
+        return {"-dC": -dC, "scaling_rate": rate}
 
     def _update_records(self, scal: np.ndarray) -> dict:
         self.kkt_residual = float(scal[S_KKT])

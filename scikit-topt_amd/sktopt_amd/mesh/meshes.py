@@ -14,6 +14,10 @@ CORNERS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
                     [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=np.int64)
 HEX_TETS = np.array([[0, 1, 2, 6], [0, 2, 3, 6], [0, 3, 7, 6],
                      [0, 7, 4, 6], [0, 4, 5, 6], [0, 5, 1, 6]], dtype=np.int64)
+# element faces as cyclic corner lists (quads of a hex, triangles of a tet)
+HEX_FACES = np.array([[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4],
+                      [3, 7, 6, 2], [0, 4, 7, 3], [1, 2, 6, 5]], dtype=np.int64)
+TET_FACES = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]], dtype=np.int64)
 
 
 class _Mesh:
@@ -36,6 +40,50 @@ class _Mesh:
         (``filters/spacial.py:13-16``)."""
         return np.mean(self.p[:, self.t], axis=1)
 
+    # --- facets (skfem's Mesh.facets / facets_satisfying / boundary_facets) --
+    def _build_facets(self):
+        faces = HEX_FACES if self.kind == "hex8" else TET_FACES
+        allf = np.concatenate([self.t[f].T for f in faces])           # (nf_all, nen_f)
+        key = np.sort(allf, axis=1)
+        _, first, inv, cnt = np.unique(key, axis=0, return_index=True, return_inverse=True,
+                                       return_counts=True)
+        self._facets = np.ascontiguousarray(allf[first].T)           # cyclic node order
+        self._facet_count = cnt
+
+    @property
+    def facets(self) -> np.ndarray:
+        """(nodes per facet, n_facets): every facet once, numbered by its
+        sorted node tuple; nodes in cyclic order."""
+        if getattr(self, "_facets", None) is None:
+            self._build_facets()
+        return self._facets
+
+    def boundary_facets(self) -> np.ndarray:
+        self.facets
+        return np.nonzero(self._facet_count == 1)[0]
+
+    def facets_satisfying(self, test, boundaries_only=False) -> np.ndarray:
+        """Facets whose midpoint (mean of its nodes) passes ``test``."""
+        midp = np.sum(self.p[:, self.facets], axis=1) / self.facets.shape[0]
+        f = np.nonzero(test(midp))[0]
+        if boundaries_only:
+            f = np.intersect1d(f, self.boundary_facets())
+        return f
+
+    def elements_satisfying(self, test) -> np.ndarray:
+        """Elements whose midpoint (mean of its nodes) passes ``test``."""
+        midp = np.sum(self.p[:, self.t], axis=1) / self.t.shape[0]
+        return np.nonzero(test(midp))[0]
+
+    def facet_areas(self, ids=None) -> np.ndarray:
+        f = self.facets if ids is None else self.facets[:, np.asarray(ids, dtype=np.int64)]
+        P = self.p[:, f]                                     # (3, nen_f, nf)
+        if f.shape[0] == 3:
+            c = np.cross((P[:, 1] - P[:, 0]).T, (P[:, 2] - P[:, 0]).T)
+        else:                                                # planar quad: diagonals
+            c = np.cross((P[:, 2] - P[:, 0]).T, (P[:, 3] - P[:, 1]).T)
+        return 0.5 * np.linalg.norm(c, axis=1)
+
     def elements_touching(self, nodes) -> np.ndarray:
         flag = np.zeros(self.nvertices, dtype=bool)
         flag[np.asarray(nodes, dtype=np.int64)] = True
@@ -50,14 +98,23 @@ def _tet_volumes(p, t):
 
 
 class MeshHex(_Mesh):
-    """Structured box grid of identical hx*hy*hz hexahedra."""
+    """Structured box grid of identical hx*hy*hz hexahedra.  With
+    ``extent=(x_len, y_len, z_len)`` the node coordinates are
+    ``numpy.linspace(0, len, n + 1)`` per axis (the reference's
+    ``create_box_hex``, ``mesh/toy_problem.py:10-37``) and h = len / n."""
     kind = "hex8"
 
-    def __init__(self, nx, ny, nz, hx=1.0, hy=1.0, hz=1.0):
+    def __init__(self, nx, ny, nz, hx=1.0, hy=1.0, hz=1.0, extent=None):
         self.grid = (int(nx), int(ny), int(nz))
-        self.h = (float(hx), float(hy), float(hz))
         i, j, k = self.node_ijk(nx, ny, nz)
-        p = np.vstack([i * self.h[0], j * self.h[1], k * self.h[2]])
+        if extent is None:
+            self.h = (float(hx), float(hy), float(hz))
+            p = np.vstack([i * self.h[0], j * self.h[1], k * self.h[2]])
+        else:
+            L = [float(v) for v in extent]
+            self.h = (L[0] / nx, L[1] / ny, L[2] / nz)
+            xs = [np.linspace(0.0, L[a], n + 1) for a, n in enumerate(self.grid)]
+            p = np.vstack([xs[0][i], xs[1][j], xs[2][k]])
         e = np.arange(nx * ny * nz)
         ei, ej, ek = e % nx, (e // nx) % ny, e // (nx * ny)
         t = np.empty((8, e.size), dtype=np.int64)

@@ -66,6 +66,54 @@ class FEMDomain:
         return m
 
 
+def _dir_to_comp(d) -> int:
+    """'u^1' -> 0 ... (``mesh/task_elastic.py:25-33``); ints pass through."""
+    if isinstance(d, (int, np.integer)):
+        return int(d)
+    if not (isinstance(d, str) and d.startswith("u^") and d[2:].isdigit()):
+        raise ValueError(f"force_dir_type must be like 'u^1','u^2','u^3', got: {d}")
+    c = int(d[2:]) - 1
+    if c < 0:
+        raise ValueError(f"Invalid component index parsed from {d}")
+    return c
+
+
+def _as_list(x):
+    return x if isinstance(x, list) else [x]
+
+
+def assemble_surface_forces(mesh, force_facets_ids, force_dir_type, force_value,
+                            ncomp=3) -> list:
+    """``mesh/task_elastic.py:15-81``: per region, ``pressure = value / A``
+    with A the total area of the region's facets, and the consistent load
+    ``int_facets pressure v[comp] dS``.  On the planar facets of these
+    meshes each facet node receives ``pressure * area / nodes_per_facet``
+    (exact integral of a bilinear / linear shape function).  Returns one
+    (ndof,) vector per region."""
+    facets_list = _as_list(force_facets_ids)
+    dirs_list = _as_list(force_dir_type)
+    vals_list = _as_list(force_value)
+    if not (len(facets_list) == len(dirs_list) == len(vals_list)):
+        raise ValueError("Lengths of force_facets_ids, force_dir_type, and force_value "
+                         "must match when lists.")
+    out = []
+    for ids, d, val in zip(facets_list, dirs_list, vals_list):
+        comp = _dir_to_comp(d)
+        ids = np.asarray(ids, dtype=np.int64)
+        area = mesh.facet_areas(ids)
+        pressure = float(val) / float(np.sum(area))
+        nodes = mesh.facets[:, ids]                          # (nen_f, nf)
+        f = np.zeros(mesh.nvertices * ncomp)
+        share = pressure * area / nodes.shape[0]
+        np.add.at(f, ncomp * nodes.T.ravel() + comp, np.repeat(share, nodes.shape[0]))
+        out.append(f)
+    return out
+
+
+def _facet_nodes(mesh, ids):
+    return np.unique(mesh.facets[:, np.asarray(ids, dtype=np.int64)].ravel())
+
+
 class LinearElasticity(FEMDomain):
     """Linear elasticity task: ``E`` (``material_coef``), ``nu`` and one nodal
     load vector per load case (``neumann_linear``; ``n_tasks`` =
@@ -82,11 +130,41 @@ class LinearElasticity(FEMDomain):
     @classmethod
     def from_nodes(cls, mesh, dirichlet_nodes, dirichlet_dir, forces, neumann_nodes,
                    design_elements, E, nu):
-        """``dirichlet_dir``: "all" or a component index / list of them."""
-        nodes = np.asarray(dirichlet_nodes, dtype=np.int64)
-        comps = [0, 1, 2] if dirichlet_dir == "all" else np.atleast_1d(dirichlet_dir)
-        dofs = np.sort(np.concatenate([3 * nodes + int(c) for c in comps]))
+        """``dirichlet_dir``: "all", "u^1".."u^3" or a component index (or a
+        list of them); with a list of node sets, one direction per set
+        (``mesh/task_common.py:142-164``)."""
+        if isinstance(dirichlet_nodes, list):
+            dirs = dirichlet_dir if isinstance(dirichlet_dir, list) \
+                else [dirichlet_dir] * len(dirichlet_nodes)
+            sets = list(zip(dirichlet_nodes, dirs))
+        else:
+            sets = [(dirichlet_nodes, dirichlet_dir)]
+        dofs, nodes = [], []
+        for nd, d in sets:
+            nd = np.asarray(nd, dtype=np.int64)
+            comps = [0, 1, 2] if d == "all" else [_dir_to_comp(c) for c in np.atleast_1d(d)]
+            dofs.append(np.concatenate([3 * nd + c for c in comps]))
+            nodes.append(nd)
+        dofs = np.unique(np.concatenate(dofs))
+        nodes = np.unique(np.concatenate(nodes))
         return cls(mesh, nodes, dofs, forces, neumann_nodes, design_elements, E, nu)
+
+    @classmethod
+    def from_facets(cls, mesh, dirichlet_facets_ids, dirichlet_dir, force_facets_ids,
+                    force_dir_type, force_value, design_elements, E, nu):
+        """``mesh/task_elastic.py:150-250`` with ``FEMDomain.from_facets``
+        (``mesh/task_common.py:273-347``): Dirichlet nodes are the nodes of
+        the Dirichlet facets, load nodes those of all force facets, and each
+        force region becomes one load case."""
+        if isinstance(dirichlet_facets_ids, list):
+            dn = [_facet_nodes(mesh, f) for f in dirichlet_facets_ids]
+        else:
+            dn = _facet_nodes(mesh, dirichlet_facets_ids)
+        fids = np.concatenate(force_facets_ids) if isinstance(force_facets_ids, list) \
+            else force_facets_ids
+        nn = _facet_nodes(mesh, fids)
+        forces = assemble_surface_forces(mesh, force_facets_ids, force_dir_type, force_value)
+        return cls.from_nodes(mesh, dn, dirichlet_dir, forces, nn, design_elements, E, nu)
 
     @property
     def material_coef(self):

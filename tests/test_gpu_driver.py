@@ -180,3 +180,75 @@ def test_c0_matches_golden_trajectory(gpu, precision, rtol, ctol, dtol, tmp_path
     assert list(opt.recorder.as_object().bisection_evals) == list(g["n_evals"])
     rho = opt.rho_host()[g["design_elements"]]
     assert np.abs(rho - g["rho_final"]).max() <= dtol
+
+
+def test_output_files_and_restart(gpu, tmp_path):
+    """cfg.json at construction, VTU + rho checkpoints at the record
+    iterations, histories.npz and cubic_top.vtu at the end
+    (common_density.py:544,1186-1233,944-964), and a restart from a
+    checkpoint (:711-745)."""
+    import os
+    from sktopt_amd.core import visualization as vis
+    d = tmp_path / "run"
+    opt = _gpu_opt(oprob.cantilever(10, 5, 5), iters=4, precision="fp64", rtol=1e-10,
+                   tmp=str(d), write_checkpoints=True, record_times=2)
+    assert os.path.exists(d / "cfg.json")
+    opt.optimize()
+    for it in (1, 2, 4):        # it == 1, it % (4 // 2) == 0, the last one
+        assert os.path.exists(d / "data" / f"{it:06d}-rho.npz"), it
+        assert os.path.exists(d / "mesh_rho" / f"info_mesh-{it:08d}.vtu"), it
+    assert not os.path.exists(d / "data" / "000003-rho.npz")
+    assert os.path.exists(d / "cubic_top.vtu")
+    with np.load(d / "histories.npz", allow_pickle=False) as h:
+        np.testing.assert_array_equal(h["compliance"],
+                                      np.array(opt.recorder.as_object().compliance))
+        assert h["rho_projected"].shape == (4, 4)
+        assert h["-dC"].shape == (4, 4)
+        assert h["lmid"].shape == (4,)
+    _, _, _, _, cd = vis.read_vtu(str(d / "mesh_rho" / "info_mesh-00000004.vtu"))
+    rp = cd["rho_projected"]
+    design = np.asarray(opt.tsk.design_elements)
+    last = opt.recorder.latest("rho_projected")
+    np.testing.assert_allclose(last, [rp[design].min(), rp[design].mean(), rp[design].max(),
+                                      rp[design].std()], rtol=1e-12)
+    # restart from iteration 2 picks up that checkpoint
+    opt2 = _gpu_opt(oprob.cantilever(10, 5, 5), iters=4, precision="fp64", rtol=1e-10,
+                    tmp=str(d), restart=True, restart_from=2)
+    opt2._ensure_state_initialized()
+    assert opt2._iter_next == 3
+    with np.load(d / "data" / "000002-rho.npz") as z:
+        np.testing.assert_array_equal(opt2.rho_host()[design], z["rho_design_elements"])
+    opt2.optimize()
+    assert len(opt2.recorder.as_object().compliance) == 2
+
+
+def test_reference_toy_problem_matches_oracle(gpu, tmp_path):
+    """toy_test() (mesh/toy_problem.py:51-95): facet-selected clamp and
+    surface load on an 8 x 6 x 4 box, E = 210e3; the oracle gets the load
+    from its own facet quadrature."""
+    from oracle import facets as ofc, mesh as omesh
+    from sktopt_amd import core
+    from sktopt_amd.mesh import toy_problem as tp
+    from sktopt_amd.tools import SchedulerConfig as SC
+    pt = tp.toy_test()
+    m = pt.mesh
+    mid = m.p[:, m.facets].mean(axis=1)
+    box = tp.get_points_in_range((6.8, 8.1), (2.4, 3.6), (2.8, 4.0))
+    F = ofc.surface_load(m.p, m.facets[:, np.nonzero(box(mid))[0]], 2, -100.0)
+    task = oprob.Task(omesh.structured_hex(*m.grid, *m.h), "elastic", 210e3, 0.3,
+                      pt.dirichlet_nodes, pt.dirichlet_dofs, [F], pt.neumann_nodes)
+    loop = odrv.DensityLoop(_oracle_cfg(vol=0.3, r=1.5, emin=1e-3), task)
+    cfg = core.OC_Config(dst_path=str(tmp_path), max_iters=50, p=SC.constant(target_value=3.0),
+                         vol_frac=SC.constant(target_value=0.3),
+                         beta=SC.constant(target_value=1.0), filter_type="spacial",
+                         filter_radius=SC.constant(target_value=1.5),
+                         move_limit=SC.constant(target_value=0.2), E_min_coeff=1e-3,
+                         precision="fp64", solver_rtol=1e-13)
+    opt = core.OC_Optimizer(cfg, pt)
+    opt.parameterize()
+    for it in range(1, 4):
+        rec = loop.step(it)
+        opt.optimize_steps(1)
+        c = opt.recorder.latest("compliance")
+        assert abs(c - rec["compliance"]) <= 1e-9 * abs(rec["compliance"]), (it, c, rec)
+        assert np.abs(opt.rho_host() - loop.rho).max() <= 1e-8
