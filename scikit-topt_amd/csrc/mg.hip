@@ -446,12 +446,26 @@ __device__ __forceinline__ void store_node(T* __restrict__ p, const T (&v)[N]) {
   }
 }
 
+// Output of a stencil apply: mode 0 writes y = S x; mode 2 writes a fused
+// Chebyshev-Jacobi step x_out = x + c1 (x - x_prev) + c2 dinv (b - S x),
+// where x_prev is read from the output buffer itself (ping-pong iterates,
+// so no direction vector exists) unless prev_zero, and (c1, c2) = coef[0..1].
+enum { ST_APPLY = 0, ST_CHEB = 2 };
+
+template <typename T>
+struct StOut {
+  const T* b = nullptr;
+  const T* dinv = nullptr;
+  const double* coef = nullptr;
+  int prev_zero = 1;
+};
+
 // y = S x on a stencil level.  A thread takes one (node, component row) and
 // up to GM column groups, so each stencil row is read once for all of them.
-template <typename T, int NC, int KC, int GM>
+template <typename T, int NC, int KC, int GM, int MODE>
 __global__ void k_stencil_apply(int64_t nx, int64_t ny, int64_t nz, int groups,
                                 const T* __restrict__ st, const T* __restrict__ X,
-                                T* __restrict__ Y) {
+                                T* __restrict__ Y, StOut<T> oa) {
   const int64_t nnx = nx + 1, nny = ny + 1, nnz = nz + 1;
   const int64_t nn = nnx * nny * nnz;
   const int chunks = (groups + GM - 1) / GM;
@@ -490,10 +504,31 @@ __global__ void k_stencil_apply(int64_t nx, int64_t ny, int64_t nz, int groups,
           for (int c = 0; c < KC; ++c) acc[q][c] += a[e] * xv[e * KC + c];
       }
     }
+    T c1 = T(0), c2 = T(0), dv = T(0);
+    if constexpr (MODE == ST_CHEB) {
+      c1 = (T)oa.coef[0];
+      c2 = (T)oa.coef[1];
+      dv = oa.dinv[node * NC + d];
+    }
 #pragma unroll
     for (int q = 0; q < GM; ++q) {
       if (q >= ng) break;
-      store_node<T, KC, KC>(Y + (((int64_t)(g0 + q) * nn + node) * NC + d) * KC, acc[q]);
+      const int64_t off = (((int64_t)(g0 + q) * nn + node) * NC + d) * KC;
+      if constexpr (MODE == ST_CHEB) {
+        T xc[KC], bv[KC], xp[KC];
+        load_node<T, KC, KC>(X + off, xc);
+        load_node<T, KC, KC>(oa.b + off, bv);
+        if (oa.prev_zero) {
+#pragma unroll
+          for (int c = 0; c < KC; ++c) xp[c] = T(0);
+        } else {
+          load_node<T, KC, KC>(Y + off, xp);
+        }
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+          acc[q][c] = xc[c] + c1 * (xc[c] - xp[c]) + c2 * dv * (bv[c] - acc[q][c]);
+      }
+      store_node<T, KC, KC>(Y + off, acc[q]);
     }
   }
 }
@@ -511,16 +546,17 @@ __global__ void k_cheb_first(int64_t total, int64_t ndof, int kc, const double* 
 }
 
 // y = S x on a stencil level, LDS-tiled: a workgroup owns a kSX x kSY x kSZ
-// block of nodes, stages x of the block and its one-node halo for GM column
-// groups in LDS once, and each thread (one node) sums its 27 stencil blocks
-// for all NC rows and GM groups.  Stencil rows are read once per node.
+// block of nodes and stages x of the block and its one-node halo for GM
+// column groups in LDS once.  NC threads share a node, one per component
+// row, so a workgroup runs 256 NC threads: each sums its row's 27 stencil
+// blocks for all GM groups.  Stencil rows are read once per node.
 constexpr int kSX = 16, kSY = 4, kSZ = 4;
 constexpr int kHX = kSX + 2, kHY = kSY + 2, kHZ = kSZ + 2;
 
-template <typename T, int NC, int KC, int GM>
-__global__ void __launch_bounds__(256)
+template <typename T, int NC, int KC, int GM, int MODE>
+__global__ void __launch_bounds__(256 * NC)
 k_stencil_apply_tiled(int64_t nx, int64_t ny, int64_t nz, int groups, const T* __restrict__ st,
-                      const T* __restrict__ X, T* __restrict__ Y) {
+                      const T* __restrict__ X, T* __restrict__ Y, StOut<T> oa) {
   constexpr int V = NC * KC;          // values of one node and one group
   constexpr int W = V * GM;           // values of one node in LDS
   __shared__ T xs[kHZ * kHY * kHX * W];
@@ -537,7 +573,7 @@ k_stencil_apply_tiled(int64_t nx, int64_t ny, int64_t nz, int groups, const T* _
   const int64_t x0 = (int64_t)tbx * kSX, y0 = (int64_t)tby * kSY, z0 = (int64_t)tbz * kSZ;
   // stage x (halo nodes outside the grid read as zero; their stencil
   // entries are zero as well)
-  for (int it = threadIdx.x; it < kHZ * kHY * kHX * GM; it += 256) {
+  for (int it = threadIdx.x; it < kHZ * kHY * kHX * GM; it += 256 * NC) {
     const int q = it % GM, hn = it / GM;
     const int hx = hn % kHX, hy = (hn / kHX) % kHY, hz = hn / (kHX * kHY);
     const int64_t gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
@@ -552,46 +588,57 @@ k_stencil_apply_tiled(int64_t nx, int64_t ny, int64_t nz, int groups, const T* _
     for (int i = 0; i < V; ++i) xs[hn * W + q * V + i] = v[i];
   }
   __syncthreads();
-  const int tx = threadIdx.x % kSX, ty = (threadIdx.x / kSX) % kSY, tz = threadIdx.x / (kSX * kSY);
+  const int nt = threadIdx.x % 256, d = threadIdx.x / 256;
+  const int tx = nt % kSX, ty = (nt / kSX) % kSY, tz = nt / (kSX * kSY);
   const int64_t gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
   if (gx >= nnx || gy >= nny || gz >= nnz) return;
   const int64_t node = gx + nnx * (gy + nny * gz);
-  T acc[GM][NC][KC];
+  T acc[GM][KC];
 #pragma unroll
   for (int q = 0; q < GM; ++q)
 #pragma unroll
-    for (int d = 0; d < NC; ++d)
-#pragma unroll
-      for (int c = 0; c < KC; ++c) acc[q][d][c] = T(0);
+    for (int c = 0; c < KC; ++c) acc[q][c] = T(0);
 #pragma unroll 3
   for (int o = 0; o < 27; ++o) {
     const int ox = o % 3, oy = (o / 3) % 3, oz = o / 9;  // halo offsets 0..2
     const T* xn = xs + ((tz + oz) * kHY * kHX + (ty + oy) * kHX + (tx + ox)) * W;
-    T a[NC][NC];
+    T a[NC];
 #pragma unroll
-    for (int d = 0; d < NC; ++d)
-#pragma unroll
-      for (int e = 0; e < NC; ++e) a[d][e] = st[((o * NC + d) * NC + e) * nn + node];
+    for (int e = 0; e < NC; ++e) a[e] = st[((o * NC + d) * NC + e) * nn + node];
 #pragma unroll
     for (int q = 0; q < GM; ++q)
 #pragma unroll
       for (int e = 0; e < NC; ++e)
 #pragma unroll
-        for (int c = 0; c < KC; ++c) {
-          const T xv = xn[q * V + e * KC + c];
-#pragma unroll
-          for (int d = 0; d < NC; ++d) acc[q][d][c] += a[d][e] * xv;
-        }
+        for (int c = 0; c < KC; ++c) acc[q][c] += a[e] * xn[q * V + e * KC + c];
   }
+  T c1 = T(0), c2 = T(0), dv = T(0);
+  if constexpr (MODE == ST_CHEB) {
+    c1 = (T)oa.coef[0];
+    c2 = (T)oa.coef[1];
+    dv = oa.dinv[node * NC + d];
+  }
+  const T* xcen = xs + ((tz + 1) * kHY * kHX + (ty + 1) * kHX + (tx + 1)) * W;
 #pragma unroll
   for (int q = 0; q < GM; ++q) {
     if (q >= ng) break;
-    T out[V];
+    const int64_t off = ((int64_t)(g0 + q) * nn + node) * V + d * KC;
+    if constexpr (MODE == ST_CHEB) {
+      T bv[KC], xp[KC];
+      load_node<T, KC, KC>(oa.b + off, bv);
+      if (oa.prev_zero) {
 #pragma unroll
-    for (int d = 0; d < NC; ++d)
+        for (int c = 0; c < KC; ++c) xp[c] = T(0);
+      } else {
+        load_node<T, KC, KC>(Y + off, xp);
+      }
 #pragma unroll
-      for (int c = 0; c < KC; ++c) out[d * KC + c] = acc[q][d][c];
-    store_node<T, V, KC>(Y + ((int64_t)(g0 + q) * nn + node) * V, out);
+      for (int c = 0; c < KC; ++c) {
+        const T xc = xcen[q * V + d * KC + c];
+        acc[q][c] = xc + c1 * (xc - xp[c]) + c2 * dv * (bv[c] - acc[q][c]);
+      }
+    }
+    store_node<T, KC, KC>(Y + off, acc[q]);
   }
 }
 
@@ -934,9 +981,9 @@ int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype,
 }
 
 // apply of level l (k columns, group kc) in T
-template <typename T>
+template <typename T, int MODE = ST_APPLY>
 static int level_apply(MGData* mg, const skt_operator* op, const void* E, int l, int k, int kc,
-                       const T* x, T* y, hipStream_t s) {
+                       const T* x, T* y, hipStream_t s, StOut<T> oa = StOut<T>()) {
   if (l == 0) return op_apply(const_cast<skt_operator*>(op), mg->dtype, k, kc, E, x, y, s);
   const MGLevel& v = mg->lv[l];
   const int groups = k / kc;
@@ -951,11 +998,11 @@ static int level_apply(MGData* mg, const skt_operator* op, const void* E, int l,
 #define SKT_SA(NC_, KC_)                                                                         \
   do {                                                                                           \
     if (tiled)                                                                                   \
-      hipLaunchKernelGGL((k_stencil_apply_tiled<T, NC_, KC_, GMT>), dim3(gt), dim3(256), 0, s,    \
-                         v.nx, v.ny, v.nz, groups, st, x, y);                                    \
+      hipLaunchKernelGGL((k_stencil_apply_tiled<T, NC_, KC_, GMT, MODE>), dim3(gt),               \
+                         dim3(256 * NC_), 0, s, v.nx, v.ny, v.nz, groups, st, x, y, oa);         \
     else                                                                                         \
-      hipLaunchKernelGGL((k_stencil_apply<T, NC_, KC_, 1>), dim3(g), dim3(kBlock), 0, s, v.nx,    \
-                         v.ny, v.nz, groups, st, x, y);                                          \
+      hipLaunchKernelGGL((k_stencil_apply<T, NC_, KC_, 1, MODE>), dim3(g), dim3(kBlock), 0, s,    \
+                         v.nx, v.ny, v.nz, groups, st, x, y, oa);                                \
   } while (0)
   if (op->ncomp == 3) {
     if (kc == 1) SKT_SA(3, 1);
@@ -970,6 +1017,20 @@ static int level_apply(MGData* mg, const skt_operator* op, const void* E, int l,
 #undef SKT_SA
   SKT_LAUNCH_CHECK();
   return SKT_OK;
+}
+
+// One fused Chebyshev-Jacobi step on a stencil level: x_out = x + c1 (x -
+// x_out_before) + c2 dinv (b - S x).
+template <typename T>
+static int level_cheb(MGData* mg, const skt_operator* op, int l, int k, int kc, const T* x,
+                      T* xout, const double* coef, int prev_zero, hipStream_t s) {
+  const MGLevel& v = mg->lv[l];
+  StOut<T> oa;
+  oa.b = (const T*)v.b;
+  oa.dinv = (const T*)v.dinv;
+  oa.coef = coef;
+  oa.prev_zero = prev_zero;
+  return level_apply<T, ST_CHEB>(mg, op, nullptr, l, k, kc, x, xout, s, oa);
 }
 
 template <typename T>
@@ -1132,25 +1193,32 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
                                (T*)mg->lv[1].b, s);
     SKT_LAUNCH_CHECK();
   }
-  // coarse levels, downward
+  // coarse levels: each level's iterate ping-pongs between its x and d
+  // buffers through fused Chebyshev steps; it[l] holds the current one
+  T* it[32];
+  if (L >= 32) {
+    set_error("internal: too many multigrid levels");
+    return SKT_EINVAL;
+  }
   for (int l = 1; l < L; ++l) {
     MGLevel& v = mg->lv[l];
     const T* b = (const T*)v.b;
-    T* x = (T*)v.x;
+    T* A = (T*)v.x;
+    T* B = (T*)v.d;
     T* w = (T*)v.w;
-    T* d = (T*)v.d;
-    const T* dinv = (const T*)v.dinv;
     const int64_t total = v.ndof * k;
-    const unsigned ge = grid_for(total);
     const double* cf = mg->coef + (size_t)l * 2 * deg;
-    hipLaunchKernelGGL(k_cheb<T>, dim3(ge), dim3(kBlock), 0, s, total, v.ndof, KC, cf, b,
-                       (const T*)nullptr, dinv, d, x, 1);
+    // step 0 from a zero iterate, then fused steps (the iterate before
+    // step 0 is zero)
+    hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
+                       v.ndof, KC, cf, b, (const T*)v.dinv, A);
     for (int st = 1; st < deg; ++st) {
-      if ((stt = level_apply<T>(mg, op, E, l, k, KC, x, w, s))) return stt;
-      hipLaunchKernelGGL(k_cheb<T>, dim3(ge), dim3(kBlock), 0, s, total, v.ndof, KC, cf + 2 * st,
-                         b, (const T*)w, dinv, d, x, 0);
+      if ((stt = level_cheb<T>(mg, op, l, k, KC, A, B, cf + 2 * st, st == 1 ? 1 : 0, s)))
+        return stt;
+      std::swap(A, B);
     }
-    if ((stt = level_apply<T>(mg, op, E, l, k, KC, x, w, s))) return stt;
+    it[l] = A;
+    if ((stt = level_apply<T>(mg, op, E, l, k, KC, A, w, s))) return stt;
     launch_restrict<T, NC, KC>(v, mg->lv[l + 1], groups, b, w, nullptr, (T*)mg->lv[l + 1].b, s);
     SKT_LAUNCH_CHECK();
   }
@@ -1162,31 +1230,29 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
                        dim3(256), 0, s, c.ndof, groups, (const T*)mg->Ainv, (const T*)c.b,
                        (T*)c.x);
     SKT_LAUNCH_CHECK();
+    it[L] = (T*)c.x;
   }
   // coarse levels, upward
   for (int l = L - 1; l >= 1; --l) {
     MGLevel& v = mg->lv[l];
-    const T* b = (const T*)v.b;
-    T* x = (T*)v.x;
-    T* w = (T*)v.w;
-    T* d = (T*)v.d;
-    const T* dinv = (const T*)v.dinv;
-    const int64_t total = v.ndof * k;
-    const unsigned ge = grid_for(total);
+    T* A = it[l];
+    T* B = (A == (T*)v.x) ? (T*)v.d : (T*)v.x;
     const double* cf = mg->coef + (size_t)l * 2 * deg;
-    launch_prolong<T, NC, KC>(v, mg->lv[l + 1], groups, (const T*)mg->lv[l + 1].x, nullptr, x, s);
+    launch_prolong<T, NC, KC>(v, mg->lv[l + 1], groups, (const T*)it[l + 1], nullptr, A, s);
     for (int st = 0; st < deg; ++st) {
-      if ((stt = level_apply<T>(mg, op, E, l, k, KC, x, w, s))) return stt;
-      hipLaunchKernelGGL(k_cheb<T>, dim3(ge), dim3(kBlock), 0, s, total, v.ndof, KC, cf + 2 * st,
-                         b, (const T*)w, dinv, d, x, 0);
+      // step 0 has c1 = 0 and needs no previous iterate
+      if ((stt = level_cheb<T>(mg, op, l, k, KC, A, B, cf + 2 * st, st == 0 ? 1 : 0, s)))
+        return stt;
+      std::swap(A, B);
     }
+    it[l] = A;
     SKT_LAUNCH_CHECK();
   }
   // fine level, correction and post-smoothing
   {
     MGLevel& v = mg->lv[0];
     const double* cf = mg->coef;
-    launch_prolong<T, NC, KC>(v, mg->lv[1], groups, (const T*)mg->lv[1].x, fixed0, cur, s);
+    launch_prolong<T, NC, KC>(v, mg->lv[1], groups, (const T*)it[1], fixed0, cur, s);
     for (int st = 0; st < deg; ++st) {
       // step 0 has c1 = 0 and needs no previous iterate
       if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, (const T*)dinv0,
