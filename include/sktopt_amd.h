@@ -285,6 +285,16 @@ int skt_filter_info(const skt_filter* f, int64_t* n_design, int64_t* nnz);
 int skt_filter_export(const skt_filter* f, int64_t* indptr, int32_t* indices,
                       double* values, int32_t* design_ids, hipStream_t stream);
 
+/* The filter of a structured nx x ny x nz grid of hx x hy x hz boxes
+ * (elements x-fastest) in stencil form: offsets inside the support with
+ * one Gaussian weight each, times the row's 1 / row-sum.  The form is only
+ * switched on after every CSR entry has been checked equal to it (pattern
+ * and weight bits), and forward / gradient / skt_oc_bisect then read no
+ * column indices or per-entry weights; *active reports whether it is on.
+ * Same matrix as spacial.py:51-103 builds; an optimisation of this build. */
+int skt_filter_set_grid(skt_filter* f, int64_t nx, int64_t ny, int64_t nz,
+                        double hx, double hy, double hz, int* active);
+
 /* spacial.py:135-157: y = x; y[design] = W x[design]  (n_el arrays). */
 int skt_filter_forward(skt_filter* f, const double* x, double* y,
                        hipStream_t stream);

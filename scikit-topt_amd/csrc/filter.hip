@@ -9,6 +9,7 @@
 // are placed by a binary search of the mirrored entry.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -24,6 +25,18 @@ struct skt_filter {
   int32_t* gcol = nullptr;         // (nnz) full element index of the column
   double* w = nullptr;             // (nnz) row-normalised weights
   double* wt = nullptr;            // (nnz) transposed weights, same pattern
+  double* inv_rs = nullptr;        // (n_design) 1 / row sum of the Gaussian weights
+  // structured-grid form (skt_filter_set_grid): the same matrix as a
+  // stencil of element offsets with per-offset weights, used when it has
+  // been verified equal to the CSR entry by entry
+  int stencil = 0;
+  double sigma = 0.0, support = 0.0;
+  int64_t gnx = 0, gny = 0, gnz = 0;
+  int n_off = 0;
+  int R = 0;                       // stencil half-width in elements
+  int32_t* off = nullptr;          // (n_off) packed dx, dy, dz, ascending element order
+  int32_t* del = nullptr;          // (n_off) element-index delta
+  double* u_off = nullptr;         // (n_off) unnormalised Gaussian weight
   // bisection workspace
   double* cand_full = nullptr;     // (n_el)
   double* part = nullptr;          // (kPartMax)
@@ -146,7 +159,7 @@ __global__ void k_sort_weights(int64_t n_el, int64_t n_design,
                                const int32_t* __restrict__ design_ids,
                                const int64_t* __restrict__ row_ptr,
                                int32_t* __restrict__ gcol, double sigma,
-                               double* __restrict__ w) {
+                               double* __restrict__ w, double* __restrict__ inv_rs) {
 #pragma clang fp contract(off)
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
        r += (int64_t)gridDim.x * blockDim.x) {
@@ -171,6 +184,7 @@ __global__ void k_sort_weights(int64_t n_el, int64_t n_design,
     }
     if (sum == 0.0) sum = 1.0;
     const double inv = 1.0 / sum;
+    inv_rs[r] = inv;
     for (int64_t q = s0; q < s1; ++q) w[q] = inv * w[q];
   }
 }
@@ -235,6 +249,134 @@ __global__ void k_scatter_design(int64_t n_design, const int32_t* __restrict__ i
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
        r += (int64_t)gridDim.x * blockDim.x)
     full[ids[r]] = des[r];
+}
+
+// ------------------------------------------------------- structured stencil
+// On a structured nx x ny x nz grid the filter is a stencil: element e's row
+// holds the design elements e + o for the offsets o of the Gaussian support,
+// with weight inv_rs[row] * u[o].  The weights below are the same
+// floating-point operations as k_sort_weights applied to the offset's centre
+// difference, and skt_filter_set_grid only switches to this form after
+// checking every CSR entry against it.
+
+// u[o] and the inclusion flag for every offset of a (2R+1)^3 box
+__global__ void k_offset_weights(int R, double hx, double hy, double hz, double r2,
+                                 double sigma, double* __restrict__ u,
+                                 int32_t* __restrict__ inside) {
+#pragma clang fp contract(off)
+  const int w = 2 * R + 1;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < w * w * w;
+       o += gridDim.x * blockDim.x) {
+    const double dx = (double)(o % w - R) * hx, dy = (double)((o / w) % w - R) * hy,
+                 dz = (double)(o / (w * w) - R) * hz;
+    double s = dx * dx;
+    s = s + dy * dy;
+    s = s + dz * dz;
+    inside[o] = s <= r2 ? 1 : 0;
+    double d = sqrt(s);
+    double t = d / sigma;
+    u[o] = exp(-0.5 * (t * t));
+  }
+}
+
+// 1 where a CSR row differs from the stencil (pattern or weight bits)
+__global__ void k_check_stencil(int64_t n_design, int64_t nx, int64_t ny, int64_t nz, int R,
+                                const int32_t* __restrict__ design_ids,
+                                const int32_t* __restrict__ colmap,
+                                const int64_t* __restrict__ row_ptr,
+                                const int32_t* __restrict__ gcol,
+                                const double* __restrict__ w,
+                                const double* __restrict__ inv_rs,
+                                const double* __restrict__ ubox,
+                                const int32_t* __restrict__ inside,
+                                int32_t* __restrict__ bad) {
+  const int wd = 2 * R + 1;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = design_ids[r];
+    const int64_t ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+    int ok = 1;
+    // every CSR entry is a stencil offset with the stencil's weight
+    for (int64_t q = row_ptr[r]; q < row_ptr[r + 1]; ++q) {
+      const int64_t j = gcol[q];
+      const int64_t dx = j % nx - ex, dy = (j / nx) % ny - ey, dz = j / (nx * ny) - ez;
+      if (dx < -R || dx > R || dy < -R || dy > R || dz < -R || dz > R) { ok = 0; break; }
+      const int o = (int)((dx + R) + wd * ((dy + R) + wd * (dz + R)));
+      if (!inside[o] || inv_rs[r] * ubox[o] != w[q]) { ok = 0; break; }
+    }
+    // and the stencil has no design neighbour the CSR lacks
+    int64_t cnt = 0;
+    for (int dz = -R; dz <= R; ++dz)
+      for (int dy = -R; dy <= R; ++dy)
+        for (int dx = -R; dx <= R; ++dx) {
+          const int o = (dx + R) + wd * ((dy + R) + wd * (dz + R));
+          if (!inside[o]) continue;
+          const int64_t x2 = ex + dx, y2 = ey + dy, z2 = ez + dz;
+          if (x2 < 0 || x2 >= nx || y2 < 0 || y2 >= ny || z2 < 0 || z2 >= nz) continue;
+          if (colmap[x2 + nx * (y2 + ny * z2)] >= 0) ++cnt;
+        }
+    if (cnt != row_ptr[r + 1] - row_ptr[r]) ok = 0;
+    if (!ok) bad[0] = 1;
+  }
+}
+
+// Stencil row sum for element e (design row r): sum_o inv_rs[r_row] u[o] x[e+o]
+// over in-grid design neighbours, in ascending element order.  With
+// transpose, the row index of the weight is the neighbour's (W^T).  off[o]
+// packs (dx + 64) | (dy + 64) << 8 | (dz + 64) << 16 and del[o] is the
+// element-index delta.  Indices fit in 32 bits (n_el < 2^31).
+template <bool kTranspose>
+__device__ __forceinline__ double stencil_row(int e, int32_t r, int nx, int ny, int nz, int R,
+                                              int n_off, const int32_t* __restrict__ off,
+                                              const int32_t* __restrict__ del,
+                                              const double* __restrict__ u,
+                                              const int32_t* __restrict__ colmap,
+                                              const double* __restrict__ inv_rs,
+                                              const double* __restrict__ x) {
+  const int nxy = nx * ny;
+  const int ex = e % nx, ey = (e / nx) % ny, ez = e / nxy;
+  const double ir = kTranspose ? 0.0 : inv_rs[r];
+  double acc = 0.0;
+  // branch-free over all offsets (lanes near a face take the same path as
+  // interior ones): an out-of-grid neighbour reads the element itself and a
+  // selected zero, so loads never wait on a branch and several offsets are
+  // in flight at once
+#pragma unroll 4
+  for (int o = 0; o < n_off; ++o) {
+    const int pk = off[o];
+    const int x2 = ex + (pk & 255) - 64, y2 = ey + ((pk >> 8) & 255) - 64,
+              z2 = ez + ((pk >> 16) & 255) - 64;
+    const bool inb = (unsigned)x2 < (unsigned)nx && (unsigned)y2 < (unsigned)ny &&
+                     (unsigned)z2 < (unsigned)nz;
+    const int j = inb ? e + del[o] : e;
+    const int32_t rj = colmap[j];
+    const double xv = x[j];
+    double wgt;
+    if constexpr (kTranspose)
+      wgt = inv_rs[rj < 0 ? 0 : rj] * u[o];
+    else
+      wgt = ir * u[o];
+    const double term = wgt * xv;
+    acc += (inb && rj >= 0) ? term : 0.0;
+  }
+  return acc;
+}
+
+template <bool kTranspose>
+__global__ void k_stencil_filter(int n_el, int nx, int ny, int nz, int R, int n_off,
+                                 const int32_t* __restrict__ off,
+                                 const int32_t* __restrict__ del, const double* __restrict__ u,
+                                 const int32_t* __restrict__ colmap,
+                                 const double* __restrict__ inv_rs,
+                                 const double* __restrict__ x, double* __restrict__ y) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n_el; e += gridDim.x * blockDim.x) {
+    const int32_t r = colmap[e];
+    if (r < 0) {
+      y[e] = kTranspose ? 0.0 : x[e];
+      continue;
+    }
+    y[e] = stencil_row<kTranspose>(e, r, nx, ny, nz, R, n_off, off, del, u, colmap, inv_rs, x);
+  }
 }
 
 // ------------------------------------------------------------------ bisection
@@ -308,6 +450,34 @@ __global__ void k_oc_volume(int64_t n_design, const int32_t* __restrict__ ids,
   if (threadIdx.x == 0) part[blockIdx.x] = b;
 }
 
+__global__ void k_oc_volume_stencil(int n_design, const int32_t* __restrict__ ids, int nx,
+                                    int ny, int nz, int R, int n_off,
+                                    const int32_t* __restrict__ off,
+                                    const int32_t* __restrict__ del,
+                                    const double* __restrict__ u,
+                                    const int32_t* __restrict__ colmap,
+                                    const double* __restrict__ inv_rs,
+                                    const double* __restrict__ cand_full,
+                                    const double* __restrict__ vol, double beta,
+                                    double beta_eta, double tbe, double denom,
+                                    const double* __restrict__ st,
+                                    double* __restrict__ part) {
+  __shared__ double sh[kBlock / kWave];
+  if (st[5] != 0.0) return;
+  double vsum = 0.0;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n_design;
+       r += gridDim.x * blockDim.x) {
+    const double acc = stencil_row<false>(ids[r], r, nx, ny, nz, R, n_off, off, del, u,
+                                          colmap, inv_rs, cand_full);
+    double t = (acc - beta_eta) * beta;
+    t = tanh(t);
+    double proj = (tbe + t) / denom;
+    vsum += proj * vol[r];
+  }
+  double b = block_sum(vsum, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
 __global__ void k_oc_bracket(int nb, const double* __restrict__ part,
                              double vol_sum, double vol_frac, double vol_tol,
                              double tolerance, int max_iter,
@@ -375,6 +545,8 @@ int skt_filter_create(int64_t n_el, const double* centers,
   hipStream_t stream = nullptr;
   skt_filter* f = new skt_filter();
   f->n_el = n_el;
+  f->sigma = sigma;
+  f->support = support;
   int st = SKT_OK;
   // --- bounding box on the host (setup only)
   std::vector<double> hc(3 * n_el);
@@ -459,11 +631,12 @@ int skt_filter_create(int64_t n_el, const double* centers,
   SKT_HIP(hipMalloc(&f->gcol, sizeof(int32_t) * nnz));
   SKT_HIP(hipMalloc(&f->w, sizeof(double) * nnz));
   SKT_HIP(hipMalloc(&f->wt, sizeof(double) * nnz));
+  SKT_HIP(hipMalloc(&f->inv_rs, sizeof(double) * nd));
   hipLaunchKernelGGL(k_neighbours<true>, dim3(gd), dim3(kBlock), 0, stream, n_el, nd,
                      centers, f->design_ids, f->colmap, cell, ids_sorted, cstart, cend,
                      g[0], g[1], g[2], r2, (int64_t*)nullptr, f->row_ptr, f->gcol);
   hipLaunchKernelGGL(k_sort_weights, dim3(gd), dim3(kBlock), 0, stream, n_el, nd,
-                     centers, f->design_ids, f->row_ptr, f->gcol, sigma, f->w);
+                     centers, f->design_ids, f->row_ptr, f->gcol, sigma, f->w, f->inv_rs);
   hipLaunchKernelGGL(k_transpose_values, dim3(gd), dim3(kBlock), 0, stream, nd,
                      f->design_ids, f->colmap, f->row_ptr, f->gcol, f->w, f->wt);
   SKT_LAUNCH_CHECK();
@@ -484,6 +657,75 @@ int skt_filter_create(int64_t n_el, const double* centers,
   return SKT_OK;
 }
 
+int skt_filter_set_grid(skt_filter* f, int64_t nx, int64_t ny, int64_t nz, double hx,
+                        double hy, double hz, int* active) {
+  SKT_CHECK(f && active, "null pointer");
+  SKT_CHECK(nx >= 1 && ny >= 1 && nz >= 1 && nx * ny * nz == f->n_el,
+            "grid does not match the filter's element count");
+  SKT_CHECK(hx > 0.0 && hy > 0.0 && hz > 0.0, "grid spacing must be positive");
+  *active = 0;
+  f->stencil = 0;
+  (void)hipFree(f->off);
+  (void)hipFree(f->del);
+  (void)hipFree(f->u_off);
+  f->off = nullptr;
+  f->del = nullptr;
+  f->u_off = nullptr;
+  SKT_CHECK(f->n_el < (int64_t)INT32_MAX, "grid too large for the stencil form");
+  const double hmin = std::min(hx, std::min(hy, hz));
+  const int R = (int)std::floor(f->support / hmin) + 1;
+  if (R > 12) return SKT_OK;            // very wide stencils stay in CSR form
+  const int wd = 2 * R + 1, nbox = wd * wd * wd;
+  double* ubox = nullptr;
+  int32_t* inside = nullptr;
+  int32_t* bad = nullptr;
+  SKT_HIP(hipMalloc(&ubox, sizeof(double) * nbox));
+  SKT_HIP(hipMalloc(&inside, sizeof(int32_t) * nbox));
+  SKT_HIP(hipMalloc(&bad, sizeof(int32_t)));
+  SKT_HIP(hipMemset(bad, 0, sizeof(int32_t)));
+  hipLaunchKernelGGL(k_offset_weights, dim3(grid_for(nbox)), dim3(kBlock), 0, nullptr, R, hx,
+                     hy, hz, f->support * f->support, f->sigma, ubox, inside);
+  hipLaunchKernelGGL(k_check_stencil, dim3(grid_for(f->n_design)), dim3(kBlock), 0, nullptr,
+                     f->n_design, nx, ny, nz, R, f->design_ids, f->colmap, f->row_ptr,
+                     f->gcol, f->w, f->inv_rs, ubox, inside, bad);
+  SKT_LAUNCH_CHECK();
+  std::vector<double> hu(nbox);
+  std::vector<int32_t> hin(nbox);
+  int32_t hbad = 1;
+  SKT_HIP(hipMemcpy(hu.data(), ubox, sizeof(double) * nbox, hipMemcpyDeviceToHost));
+  SKT_HIP(hipMemcpy(hin.data(), inside, sizeof(int32_t) * nbox, hipMemcpyDeviceToHost));
+  SKT_HIP(hipMemcpy(&hbad, bad, sizeof(int32_t), hipMemcpyDeviceToHost));
+  (void)hipFree(ubox);
+  (void)hipFree(inside);
+  (void)hipFree(bad);
+  if (hbad) return SKT_OK;              // not the same matrix: keep the CSR form
+  std::vector<int32_t> ho, hd;
+  std::vector<double> hw;
+  for (int dz = -R; dz <= R; ++dz)      // ascending element order of the neighbours
+    for (int dy = -R; dy <= R; ++dy)
+      for (int dx = -R; dx <= R; ++dx) {
+        const int o = (dx + R) + wd * ((dy + R) + wd * (dz + R));
+        if (!hin[o]) continue;
+        ho.push_back((dx + 64) | ((dy + 64) << 8) | ((dz + 64) << 16));
+        hd.push_back((int32_t)(dx + nx * (dy + ny * dz)));
+        hw.push_back(hu[o]);
+      }
+  f->n_off = (int)hw.size();
+  f->R = R;
+  SKT_HIP(hipMalloc(&f->off, sizeof(int32_t) * ho.size()));
+  SKT_HIP(hipMalloc(&f->del, sizeof(int32_t) * hd.size()));
+  SKT_HIP(hipMalloc(&f->u_off, sizeof(double) * hw.size()));
+  SKT_HIP(hipMemcpy(f->off, ho.data(), sizeof(int32_t) * ho.size(), hipMemcpyHostToDevice));
+  SKT_HIP(hipMemcpy(f->del, hd.data(), sizeof(int32_t) * hd.size(), hipMemcpyHostToDevice));
+  SKT_HIP(hipMemcpy(f->u_off, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice));
+  f->gnx = nx;
+  f->gny = ny;
+  f->gnz = nz;
+  f->stencil = 1;
+  *active = 1;
+  return SKT_OK;
+}
+
 int skt_filter_destroy(skt_filter* f) {
   if (!f) return SKT_OK;
   (void)hipFree(f->design_ids);
@@ -492,6 +734,10 @@ int skt_filter_destroy(skt_filter* f) {
   (void)hipFree(f->gcol);
   (void)hipFree(f->w);
   (void)hipFree(f->wt);
+  (void)hipFree(f->inv_rs);
+  (void)hipFree(f->off);
+  (void)hipFree(f->del);
+  (void)hipFree(f->u_off);
   (void)hipFree(f->cand_full);
   (void)hipFree(f->part);
   (void)hipFree(f->state);
@@ -536,8 +782,13 @@ int skt_filter_export(const skt_filter* f, int64_t* indptr, int32_t* indices,
 int skt_filter_forward(skt_filter* f, const double* x, double* y, hipStream_t stream) {
   SKT_CHECK(f && x && y, "null pointer");
   SKT_CHECK(x != y, "filter forward cannot run in place");
-  hipLaunchKernelGGL(k_csr_apply, dim3(grid_for(f->n_el * kSeg)), dim3(kBlock), 0, stream,
-                     f->n_el, f->colmap, f->row_ptr, f->gcol, f->w, x, y, 0);
+  if (f->stencil)
+    hipLaunchKernelGGL(k_stencil_filter<false>, dim3(grid_for(f->n_el)), dim3(kBlock), 0,
+                       stream, (int)f->n_el, (int)f->gnx, (int)f->gny, (int)f->gnz, f->R,
+                       f->n_off, f->off, f->del, f->u_off, f->colmap, f->inv_rs, x, y);
+  else
+    hipLaunchKernelGGL(k_csr_apply, dim3(grid_for(f->n_el * kSeg)), dim3(kBlock), 0, stream,
+                       f->n_el, f->colmap, f->row_ptr, f->gcol, f->w, x, y, 0);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }
@@ -545,8 +796,13 @@ int skt_filter_forward(skt_filter* f, const double* x, double* y, hipStream_t st
 int skt_filter_gradient(skt_filter* f, const double* v, double* g, hipStream_t stream) {
   SKT_CHECK(f && v && g, "null pointer");
   SKT_CHECK(v != g, "filter gradient cannot run in place");
-  hipLaunchKernelGGL(k_csr_apply, dim3(grid_for(f->n_el * kSeg)), dim3(kBlock), 0, stream,
-                     f->n_el, f->colmap, f->row_ptr, f->gcol, f->wt, v, g, 1);
+  if (f->stencil)
+    hipLaunchKernelGGL(k_stencil_filter<true>, dim3(grid_for(f->n_el)), dim3(kBlock), 0,
+                       stream, (int)f->n_el, (int)f->gnx, (int)f->gny, (int)f->gnz, f->R,
+                       f->n_off, f->off, f->del, f->u_off, f->colmap, f->inv_rs, v, g);
+  else
+    hipLaunchKernelGGL(k_csr_apply, dim3(grid_for(f->n_el * kSeg)), dim3(kBlock), 0, stream,
+                       f->n_el, f->colmap, f->row_ptr, f->gcol, f->wt, v, g, 1);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }
@@ -587,6 +843,7 @@ int skt_oc_bisect(skt_filter* f, const double* rho_full, const double* rho_e,
   const double tbe = std::tanh(beta * beta_eta);
   const double denom = std::tanh(beta * beta_eta) + std::tanh(beta * (1.0 - beta_eta)) + 1e-12;
   const int64_t nd = f->n_design;
+  const unsigned nbs = grid_for(nd, kBlock, kPartMax);
   unsigned nb = grid_for(nd * kSeg, kBlock, kPartMax);
   hipLaunchKernelGGL(k_copy, dim3(grid_for(f->n_el)), dim3(kBlock), 0, stream, f->n_el,
                      rho_full, f->cand_full);
@@ -595,10 +852,17 @@ int skt_oc_bisect(skt_filter* f, const double* rho_full, const double* rho_e,
     hipLaunchKernelGGL(k_oc_candidate, dim3(grid_for(nd)), dim3(kBlock), 0, stream, nd,
                        f->design_ids, rho_e, dC, f->state, 1e-12, eta, rate_min, rate_max,
                        move_limit, rho_min, rho_max, f->cand_full, rho_out);
-    hipLaunchKernelGGL(k_oc_volume, dim3(nb), dim3(kBlock), 0, stream, nd, f->design_ids,
-                       f->row_ptr, f->gcol, f->w, f->cand_full, vol, beta, beta_eta, tbe,
-                       denom, f->state, f->part);
-    hipLaunchKernelGGL(k_oc_bracket, dim3(1), dim3(kBlock), 0, stream, (int)nb, f->part,
+    if (f->stencil)
+      hipLaunchKernelGGL(k_oc_volume_stencil, dim3(nbs), dim3(kBlock), 0, stream, (int)nd,
+                         f->design_ids, (int)f->gnx, (int)f->gny, (int)f->gnz, f->R, f->n_off,
+                         f->off, f->del, f->u_off, f->colmap, f->inv_rs, f->cand_full, vol,
+                         beta, beta_eta, tbe, denom, f->state, f->part);
+    else
+      hipLaunchKernelGGL(k_oc_volume, dim3(nb), dim3(kBlock), 0, stream, nd, f->design_ids,
+                         f->row_ptr, f->gcol, f->w, f->cand_full, vol, beta, beta_eta, tbe,
+                         denom, f->state, f->part);
+    hipLaunchKernelGGL(k_oc_bracket, dim3(1), dim3(kBlock), 0, stream,
+                       (int)(f->stencil ? nbs : nb), f->part,
                        vol_sum, vol_frac, vol_tol, tolerance, max_iter, f->state);
   }
   hipLaunchKernelGGL(k_oc_result, dim3(1), dim3(1), 0, stream, f->state, result);

@@ -68,6 +68,7 @@ SIGNATURES = {
     "skt_filter_gradient": [_vp, _vp, _vp, _vp],
     "skt_filter_gather_design": [_vp, _vp, _vp, _vp],
     "skt_filter_scatter_design": [_vp, _vp, _vp, _vp],
+    "skt_filter_set_grid": [_vp, _i64, _i64, _i64, _d, _d, _d, _i32p],
     "skt_design_gather": [_i64, _vp, _vp, _vp, _vp],
     "skt_design_scatter": [_i64, _vp, _vp, _vp, _vp],
     "skt_oc_candidate": [_i64, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d, _d, _vp, _vp, _vp],

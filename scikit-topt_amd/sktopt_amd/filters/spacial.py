@@ -46,6 +46,9 @@ class SpacialFilter:
         self._built_radius = None
         self.n_design = 0
         self.nnz = 0
+        # structured hex grids: apply W as a verified stencil (no CSR reads)
+        self.use_stencil = True
+        self.stencil_active = False
 
     @classmethod
     def from_defaults(cls, mesh, elements_volume, radius=0.3, design_mask=None):
@@ -71,6 +74,12 @@ class SpacialFilter:
                   _dev.ptr(self._dev_mask), float(sigma), float(support), C.byref(h))
         self._h = h
         self._built_radius = self.radius
+        self.stencil_active = False
+        if self.use_stencil and getattr(self.mesh, "kind", None) == "hex8" and \
+                getattr(self.mesh, "grid", None) is not None:
+            act = C.c_int32(0)
+            _lib.call("skt_filter_set_grid", h, *self.mesh.grid, *self.mesh.h, C.byref(act))
+            self.stencil_active = bool(act.value)
         nd, nnz = C.c_int64(), C.c_int64()
         _lib.call("skt_filter_info", h, C.byref(nd), C.byref(nnz))
         self.n_design, self.nnz = nd.value, nnz.value
