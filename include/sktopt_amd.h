@@ -100,6 +100,18 @@ int skt_sensitivity_chain_cols(int kind, int64_t n, int ncols,
                                double E0, double Emin, double p, double* g,
                                hipStream_t stream);
 
+/* Both chains with a per-load-case additive term (nullable, (n, ncols)):
+ * dC_k = dC(rho_proj, energy[:, k]) + extra[:, k] before the product with
+ * H' -- the custom sensitivity of the Robin heat compliance
+ * (fea/solver_heat.py:928-980, common_density.py:1067-1097).  per_column = 0:
+ * g (n,) = scale * sum_k dC_k H'; per_column = 1: g (n, ncols) = dC_k H'. */
+int skt_sensitivity_chain_extra(int kind, int64_t n, int ncols,
+                                const double* rho_proj, const double* energy,
+                                const double* extra, const double* rho_filt,
+                                double beta, double eta, double E0, double Emin,
+                                double p, double scale, int per_column,
+                                double* g, hipStream_t stream);
+
 /* Elementwise helpers of the update rules (logmoc.py:186-225):
  * op 0: out = a*x + b*y (y may be NULL: out = a*x); op 1: out = a*x*y. */
 int skt_ewise(int op, int64_t n, double a, const double* x, double b,
@@ -176,6 +188,51 @@ int skt_hex8_elem_to_node(int64_t nx, int64_t ny, int64_t nz, int ncols,
 int skt_hex8_node_to_elem(int64_t nx, int64_t ny, int64_t nz, int ncols,
                           const double* rho_node, int clamp_nonpositive,
                           int mean_cols, double* rho_elem, hipStream_t stream);
+
+/* Robin terms of the heat path on structured hex8 grids
+ * (fea/solver_heat.py:495-624,705-980).
+ *
+ * Nodal densities as objectives_multi_load (:738-745) and
+ * _element_to_nodal_average (:600-613) compute them: np.add.at(rho_nodal,
+ * t.ravel(), np.repeat(rho, 8)) / count, i.e. node n adds rho[(a n_el + e) /
+ * 8] for each of its corner slots a = 0..7 (element e); count (nullable) is
+ * the number of elements of the node. */
+int skt_hex8_nodal_average_pairing(int64_t nx, int64_t ny, int64_t nz,
+                                   const double* rho_elem, double* rho_node,
+                                   double* count, hipStream_t stream);
+
+/* get_robin_virtual (:495-517): the 27-point nodal stencil (27, n_nodes) of
+ * int h a |grad rho| u v and the load int h a |grad rho| T_env v, a =
+ * rho^p (1 - rho)^q, rho the trilinear interpolant of rho_node, Gauss rule
+ * of npts (1..4) points per axis. */
+int skt_hex8_robin_virtual(int64_t nx, int64_t ny, int64_t nz, double hx,
+                           double hy, double hz, int npts,
+                           const double* rho_node, double p, double q,
+                           double h, double T_env, double* stencil,
+                           double* load, hipStream_t stream);
+
+/* _robin_compliance_explicit_grad_ (:520-546) per load column of T
+ * (n_nodes, ncols), mapped to elements by _nodal_gradient_to_element_gradient
+ * (:616-624): out (n_el, ncols); work: n_nodes doubles. */
+int skt_hex8_robin_grad(int64_t nx, int64_t ny, int64_t nz, double hx,
+                        double hy, double hz, int npts, const double* rho_node,
+                        const double* count, int ncols, const double* T,
+                        double p, double q, double h, double T_env,
+                        double* work, double* out, hipStream_t stream);
+
+/* Attach (stencil != NULL, copied) or drop (NULL) an extra scalar 27-point
+ * stencil on a scalar structured hex8 operator: applies add it on free rows
+ * and the diagonal includes its centre.  The conduction plus Robin matrix of
+ * solve_multi_load (:198-253); the multigrid preconditioner keeps the
+ * conduction part. */
+int skt_operator_set_extra(skt_operator* op, const double* stencil,
+                           hipStream_t stream);
+
+/* Y = S X for a 27-point stencil (27, n_nodes), X, Y (n_nodes, ncols), no
+ * Dirichlet rows: the unenforced Robin part of T^T K T (:785-789). */
+int skt_stencil27_apply(int64_t nx, int64_t ny, int64_t nz, int ncols,
+                        const double* stencil, const double* X, double* Y,
+                        hipStream_t stream);
 
 /* Unstructured linear tetrahedra.  coords: (3, n_nodes) float64 device
  * array (x row, y row, z row); conn: (4, n_el) int32 device array.

@@ -83,10 +83,12 @@ class Config:
 
 
 class DensityLoop:
-    def __init__(self, cfg: Config, task):
+    def __init__(self, cfg: Config, task, fem=None):
         self.cfg = cfg
         self.task = task
-        if task.physics == "elastic":
+        if fem is not None:
+            self.fem = fem
+        elif task.physics == "elastic":
             self.fem = fem_mod.ElasticFEM(task, cfg.E_min_coeff,
                                           cfg.interpolation, cfg.solver,
                                           cfg.solver_rtol)

@@ -1,0 +1,251 @@
+"""Heat conduction with Robin boundaries (oracle) -- TEST INFRASTRUCTURE ONLY.
+
+Restates, for structured box hex8 grids, ``fea/solver_heat.py``:
+
+* ``get_robin_virtual`` (:495-517): the volume ("virtual") Robin forms
+  ``int h a(rho) |grad rho| u v`` and ``int h a(rho) |grad rho| T_env v`` with
+  ``a = rho^p (1 - rho)^q`` and ``rho`` the interpolant of nodal densities;
+* the nodal densities of ``objectives_multi_load`` (:738-745) and
+  ``_element_to_nodal_average`` (:600-613): ``np.add.at(rho_nodal,
+  t.ravel(), np.repeat(rho, 8))`` -- note that ``t.ravel()`` runs corner by
+  corner over all elements while ``np.repeat`` runs element by element, so
+  the k-th term added is ``rho[k // 8]`` at node ``t.ravel()[k]``; the
+  reference's pairing is reproduced as it is;
+* ``_robin_compliance_explicit_grad_`` (:520-546) and
+  ``_nodal_gradient_to_element_gradient`` (:616-624);
+* ``objectives_multi_load`` (:705-895) for ``objective='compliance'``
+  (``J = T^T K T`` with the unenforced total K, conduction + facet Robin +
+  virtual Robin) and ``'averaged_temp'`` (``J = sum(T)``, adjoint
+  ``K lam = 1`` enforced with the state's Dirichlet values);
+* ``assemble_surface_robin`` of ``mesh/task_heat.py:60-139``: the facet
+  forms ``int_G h u v`` and ``int_G h T_env v`` (constant: the density
+  field there is never set);
+
+with Gauss-Legendre quadrature of ``npts`` points per axis for the volume
+forms (the reference uses its basis' rule) and exact 2x2 Gauss on facets.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import elements as el
+from .fem import enforce, solve
+from .mesh import HEX_CORNERS
+
+_S = 2.0 * HEX_CORNERS - 1.0
+
+
+def shape(xi):
+    """N (8,) and reference gradients dN (8, 3) at xi (3,)."""
+    f = 1.0 + _S * np.asarray(xi)[None, :]
+    N = f[:, 0] * f[:, 1] * f[:, 2] / 8.0
+    dN = np.empty((8, 3))
+    dN[:, 0] = _S[:, 0] * f[:, 1] * f[:, 2] / 8.0
+    dN[:, 1] = _S[:, 1] * f[:, 0] * f[:, 2] / 8.0
+    dN[:, 2] = _S[:, 2] * f[:, 0] * f[:, 1] / 8.0
+    return N, dN
+
+
+def quadrature(npts):
+    x, w = el._gauss(npts)
+    pts = np.array([(a, b, c) for c in x for b in x for a in x])
+    wts = np.array([wa * wb * wc for wc in w for wb in w for wa in w])
+    return pts, wts
+
+
+def nodal_average_reference(mesh, rho):
+    """(rho_nodal, count) with the reference's add.at pairing."""
+    t = mesh.t
+    nen = t.shape[0]
+    rn = np.zeros(mesh.nnodes)
+    cnt = np.zeros(mesh.nnodes)
+    np.add.at(rn, t.ravel(), np.repeat(rho, nen))
+    np.add.at(cnt, t.ravel(), 1.0)
+    rn /= np.maximum(cnt, 1.0)
+    return rn, cnt
+
+
+def _element_fields(mesh, rho_nodal, npts):
+    """Per quadrature point: N (nq, 8), physical dN (nq, 8, 3), weights*detJ,
+    rho_h (n_el, nq) and |grad rho_h| (n_el, nq)."""
+    h = np.asarray(mesh.h, dtype=np.float64)
+    detJ = h[0] * h[1] * h[2] / 8.0
+    pts, wts = quadrature(npts)
+    Ns, dNs = [], []
+    for xi in pts:
+        N, dN = shape(xi)
+        Ns.append(N)
+        dNs.append(dN * (2.0 / h)[None, :])
+    Ns, dNs = np.array(Ns), np.array(dNs)
+    rl = rho_nodal[mesh.t]                          # (8, n_el)
+    rh = Ns @ rl                                    # (nq, n_el)
+    g = np.einsum("qad,ae->qed", dNs, rl)           # (nq, n_el, 3)
+    gn = np.sqrt(np.sum(g ** 2, axis=2))            # (nq, n_el)
+    return Ns, dNs, wts * detJ, rh, g, gn
+
+
+def _assemble(mesh, Ke):
+    t = mesh.t
+    rows = np.repeat(t.T, 8, axis=1).ravel()
+    cols = np.tile(t.T, (1, 8)).ravel()
+    n = mesh.nnodes
+    return sp.csr_matrix((Ke.reshape(Ke.shape[0], -1).ravel(), (rows, cols)), shape=(n, n))
+
+
+def virtual_robin(mesh, rho, p, q, h, T_env, npts=2):
+    """(B_v, L_v) of get_robin_virtual with rho the element densities."""
+    rn, _ = nodal_average_reference(mesh, rho)
+    return virtual_robin_nodal(mesh, rn, p, q, h, T_env, npts)
+
+
+def virtual_robin_nodal(mesh, rn, p, q, h, T_env, npts=2):
+    """The same from nodal densities."""
+    Ns, dNs, wd, rh, g, gn = _element_fields(mesh, rn, npts)
+    c = h * rh ** p * (1.0 - rh) ** q * gn          # (nq, n_el)
+    Ke = np.einsum("q,qe,qa,qb->eab", wd, c, Ns, Ns)
+    Le = np.einsum("q,qe,qa->ea", wd, c * T_env, Ns)
+    L = np.zeros(mesh.nnodes)
+    np.add.at(L, mesh.t.T.ravel(), Le.ravel())
+    return _assemble(mesh, Ke), L
+
+
+def facet_robin(p_nodes, facets, h, T_env, n):
+    """(B_f, L_f) of one facet region: facets (4, nf) cyclic quads."""
+    x, w = el._gauss(2)
+    sg = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], dtype=float)
+    rows, cols, vals = [], [], []
+    L = np.zeros(n)
+    for f in range(facets.shape[1]):
+        P = p_nodes[:, facets[:, f]]
+        Me = np.zeros((4, 4))
+        Le = np.zeros(4)
+        for a, wa in zip(x, w):
+            for b, wb in zip(x, w):
+                N = 0.25 * (1 + sg[:, 0] * a) * (1 + sg[:, 1] * b)
+                dNa = 0.25 * sg[:, 0] * (1 + sg[:, 1] * b)
+                dNb = 0.25 * sg[:, 1] * (1 + sg[:, 0] * a)
+                J = np.linalg.norm(np.cross(P @ dNa, P @ dNb))
+                Me += wa * wb * J * h * np.outer(N, N)
+                Le += wa * wb * J * h * T_env * N
+        for i in range(4):
+            L[facets[i, f]] += Le[i]
+            for j in range(4):
+                rows.append(facets[i, f])
+                cols.append(facets[j, f])
+                vals.append(Me[i, j])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n)), L
+
+
+def robin_explicit_grad(mesh, rho, T, p, q, h, T_env, npts=2):
+    """Element gradient (n_el,) of one load case: the nodal linear form of
+    ``_robin_compliance_explicit_grad_`` mapped by
+    ``_nodal_gradient_to_element_gradient``."""
+    rn, cnt = nodal_average_reference(mesh, rho)
+    nodal = robin_explicit_grad_nodal(mesh, rn, T, p, q, h, T_env, npts)
+    out = np.zeros(mesh.nelements)
+    for local in mesh.t:
+        out += nodal[local] / np.maximum(cnt[local], 1.0)
+    return out
+
+
+def robin_functional(mesh, rn, T, p, q, h, T_env, npts=2):
+    """int h a(rho) |grad rho| (2 T_env T - T^2): the functional whose
+    derivative in the nodal densities (T fixed) the explicit gradient is."""
+    Ns, dNs, wd, rh, g, gn = _element_fields(mesh, rn, npts)
+    Th = Ns @ T[mesh.t]
+    phi = 2.0 * T_env * Th - Th * Th
+    return float(np.einsum("q,qe->", wd, h * rh ** p * (1.0 - rh) ** q * gn * phi))
+
+
+def robin_explicit_grad_nodal(mesh, rn, T, p, q, h, T_env, npts=2):
+    """The nodal linear form of ``_robin_compliance_explicit_grad_``."""
+    Ns, dNs, wd, rh, g, gn = _element_fields(mesh, rn, npts)
+    Tl = T[mesh.t]                                   # (8, n_el)
+    Th = Ns @ Tl                                     # (nq, n_el)
+    gs = np.maximum(gn, 1e-12)
+    a = rh ** p * (1.0 - rh) ** q
+    da = p * rh ** (p - 1.0) * (1.0 - rh) ** q - q * rh ** p * (1.0 - rh) ** (q - 1.0)
+    phi = 2.0 * T_env * Th - Th * Th
+    term1 = da * gn * phi                            # multiplies v
+    gv = np.einsum("qed,qad->qea", g / gs[:, :, None], dNs)   # (grad rho / |.|) . grad N_a
+    Le = np.einsum("q,qe,qa->ea", wd, h * term1, Ns) + \
+        np.einsum("q,qe,qea->ea", wd, h * a * phi, gv)
+    nodal = np.zeros(mesh.nnodes)
+    np.add.at(nodal, mesh.t.T.ravel(), Le.ravel())
+    return nodal
+
+
+class HeatRobinFEM:
+    """Oracle of ``FEM_SimpLinearHeatConduction`` with Robin data on a
+    structured hex grid.  ``robin_facets``: list of (4, nf) facet arrays;
+    ``emit``: extra volumetric source vector added to the Robin loads."""
+
+    def __init__(self, mesh, dirichlet_dofs, dirichlet_values, k, E_min_coeff, robin_facets,
+                 h, T_env, emit=None, objective="compliance", q=4, npts=2):
+        self.mesh = mesh
+        self.D = np.asarray(dirichlet_dofs)
+        self.values = list(dirichlet_values)
+        self.k_max, self.k_min = k, k * E_min_coeff
+        self.h, self.T_env, self.q, self.npts = h, T_env, q, npts
+        self.objective = objective
+        self.K0 = el.hex8_conduction_K0(mesh.h)
+        n = mesh.nnodes
+        self.Bf = sp.csr_matrix((n, n))
+        self.Lf = np.zeros(n)
+        for fc in robin_facets:
+            B, L = facet_robin(mesh.p, fc, h, T_env, n)
+            self.Bf = self.Bf + B
+            self.Lf = self.Lf + L
+        self.emit0 = np.zeros(n) if emit is None else np.asarray(emit, float)
+        self.lam = None
+
+    def _K(self, rho, p):
+        from . import transforms as tr
+        E = tr.simp_interpolation(rho, self.k_max, self.k_min, p)
+        Kc = _assemble(self.mesh, E[:, None, None] * self.K0[None])
+        Bv, Lv = virtual_robin(self.mesh, rho, p, self.q, self.h, self.T_env, self.npts)
+        return (Kc + self.Bf + Bv).tocsr(), self.emit0 + self.Lf + Lv
+
+    def _solve(self, K, rhs, n_loads):
+        cols = []
+        for i in range(n_loads):
+            Ke, fe = enforce(K, rhs, self.D, np.full(len(self.D), self.values[i]))
+            cols.append(fe)
+        return solve(Ke, np.column_stack(cols), "spsolve", 1e-12)
+
+    def objectives_multi_load(self, rho, p, u_dofs, timer=None, force_scale=1.0):
+        K, emit = self._K(rho, p)
+        n_loads = u_dofs.shape[1]
+        U = self._solve(K, emit, n_loads)
+        u_dofs[:, :] = U
+        if self.objective == "compliance":
+            self.lam = -2.0 * U
+            return np.array([float(U[:, i] @ (K @ U[:, i])) for i in range(n_loads)])
+        if self.objective == "averaged_temp":
+            self.lam = self._solve(K, np.ones_like(emit), n_loads)
+            return np.array([float(np.sum(U[:, i])) for i in range(n_loads)])
+        raise ValueError(self.objective)
+
+    def energy_multi_load(self, rho, p, u_dofs):
+        from . import transforms as tr
+        Ul = u_dofs[self.mesh.t]                                  # (8, n_el, k)
+        if self.objective == "compliance":
+            E = tr.simp_interpolation(rho, self.k_max, self.k_min, p)
+            return 0.5 * E[:, None] * np.einsum("aek,ab,bek->ek", Ul, self.K0, Ul)
+        Ll = self.lam[self.mesh.t]
+        return np.einsum("aek,ab,bek->ek", Ul, self.K0, Ll)
+
+    def compliance_sensitivity_multi_load(self, rho, p, u_dofs):
+        from . import transforms as tr
+        E = tr.simp_interpolation(rho, self.k_max, self.k_min, p)
+        Ul = u_dofs[self.mesh.t]
+        energy = 0.5 * E[:, None] * np.einsum("aek,ab,bek->ek", Ul, self.K0, Ul)
+        out = np.column_stack([tr.dC_drho_simp(rho, energy[:, i], self.k_max, self.k_min, p)
+                               for i in range(energy.shape[1])])
+        if self.objective != "compliance":
+            return out
+        for i in range(out.shape[1]):
+            out[:, i] += robin_explicit_grad(self.mesh, rho, u_dofs[:, i], p, self.q, self.h,
+                                             self.T_env, self.npts)
+        return out

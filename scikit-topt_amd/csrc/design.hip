@@ -97,6 +97,7 @@ __global__ void k_dC(int kind, int64_t n, const double* __restrict__ rho,
 __global__ void k_sens_chain(int kind, int64_t n, int ncols,
                              const double* __restrict__ rho_proj,
                              const double* __restrict__ energy,
+                             const double* __restrict__ extra,
                              const double* __restrict__ rho_filt, double beta,
                              double eta, double denom, double E0, double Emin,
                              double p, double scale, double* __restrict__ g) {
@@ -107,8 +108,11 @@ __global__ void k_sens_chain(int kind, int64_t n, int ncols,
     double dH = ((1.0 / (c * c)) * beta) / denom;
     double r = rho_proj[i];
     double acc = 0.0;
-    for (int k = 0; k < ncols; ++k)
-      acc += dC_value(kind, r, energy[i * ncols + k], E0, Emin, p) * dH;
+    for (int k = 0; k < ncols; ++k) {
+      double d = dC_value(kind, r, energy[i * ncols + k], E0, Emin, p);
+      if (extra) d = d + extra[i * ncols + k];
+      acc += d * dH;
+    }
     g[i] = acc * scale;
   }
 }
@@ -118,6 +122,7 @@ __global__ void k_sens_chain(int kind, int64_t n, int ncols,
 __global__ void k_sens_chain_cols(int kind, int64_t n, int ncols,
                                   const double* __restrict__ rho_proj,
                                   const double* __restrict__ energy,
+                                  const double* __restrict__ extra,
                                   const double* __restrict__ rho_filt, double beta,
                                   double eta, double denom, double E0, double Emin,
                                   double p, double* __restrict__ g) {
@@ -127,8 +132,11 @@ __global__ void k_sens_chain_cols(int kind, int64_t n, int ncols,
     double c = cosh(t);
     double dH = ((1.0 / (c * c)) * beta) / denom;
     double r = rho_proj[i];
-    for (int k = 0; k < ncols; ++k)
-      g[i * ncols + k] = dC_value(kind, r, energy[i * ncols + k], E0, Emin, p) * dH;
+    for (int k = 0; k < ncols; ++k) {
+      double d = dC_value(kind, r, energy[i * ncols + k], E0, Emin, p);
+      if (extra) d = d + extra[i * ncols + k];
+      g[i * ncols + k] = d * dH;
+    }
   }
 }
 
@@ -522,8 +530,8 @@ int skt_sensitivity_chain(int kind, int64_t n, int ncols,
   SKT_CHECK(rho_proj && energy && rho_filt && g, "null pointer");
   const double denom = std::tanh(beta * eta) + std::tanh(beta * (1.0 - eta)) + 1e-12;
   hipLaunchKernelGGL(k_sens_chain, dim3(grid_for(n)), dim3(kBlock), 0, stream,
-                     kind, n, ncols, rho_proj, energy, rho_filt, beta, eta,
-                     denom, E0, Emin, p, scale, g);
+                     kind, n, ncols, rho_proj, energy, (const double*)nullptr, rho_filt, beta,
+                     eta, denom, E0, Emin, p, scale, g);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }
@@ -539,7 +547,31 @@ int skt_sensitivity_chain_cols(int kind, int64_t n, int ncols, const double* rho
   SKT_CHECK(rho_proj && energy && rho_filt && g, "null pointer");
   const double denom = std::tanh(beta * eta) + std::tanh(beta * (1.0 - eta)) + 1e-12;
   hipLaunchKernelGGL(k_sens_chain_cols, dim3(grid_for(n)), dim3(kBlock), 0, stream, kind, n,
-                     ncols, rho_proj, energy, rho_filt, beta, eta, denom, E0, Emin, p, g);
+                     ncols, rho_proj, energy, (const double*)nullptr, rho_filt, beta, eta, denom,
+                     E0, Emin, p, g);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_sensitivity_chain_extra(int kind, int64_t n, int ncols, const double* rho_proj,
+                                const double* energy, const double* extra,
+                                const double* rho_filt, double beta, double eta, double E0,
+                                double Emin, double p, double scale, int per_column, double* g,
+                                hipStream_t stream) {
+  SKT_CHECK(kind == SKT_SIMP || kind == SKT_RAMP || kind == SKT_RAMP_EXACT,
+            "Interpolation method must be SIMP or RAMP.");
+  SKT_CHECK(n >= 0 && ncols >= 1, "bad size");
+  if (n == 0) return SKT_OK;
+  SKT_CHECK(rho_proj && energy && rho_filt && g, "null pointer");
+  const double denom = std::tanh(beta * eta) + std::tanh(beta * (1.0 - eta)) + 1e-12;
+  if (per_column)
+    hipLaunchKernelGGL(k_sens_chain_cols, dim3(grid_for(n)), dim3(kBlock), 0, stream, kind, n,
+                       ncols, rho_proj, energy, extra, rho_filt, beta, eta, denom, E0, Emin, p,
+                       g);
+  else
+    hipLaunchKernelGGL(k_sens_chain, dim3(grid_for(n)), dim3(kBlock), 0, stream, kind, n, ncols,
+                       rho_proj, energy, extra, rho_filt, beta, eta, denom, E0, Emin, p, scale,
+                       g);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }

@@ -31,6 +31,9 @@ struct skt_operator {
   int32_t* nel = nullptr;
   uint8_t* nloc = nullptr;
   double lam0 = 0.0, mu0 = 0.0;  // unit-modulus Lame parameters
+  // extra scalar 27-point stencil added on free rows (Robin terms), or null
+  double* extra64 = nullptr;   // (27, n_nodes)
+  float* extra32 = nullptr;
   // Dirichlet DOFs
   uint8_t* fixed = nullptr;    // (ndof)
   int32_t* fixed_list = nullptr;  // indices of the fixed DOFs
@@ -38,6 +41,11 @@ struct skt_operator {
 };
 
 namespace skt {
+// Y += S X on free rows for the operator's extra stencil (robin.hip); no-op
+// without one.  diag += the stencil's centre entries.
+int extra_apply_add(const skt_operator* op, int dtype, int ncols, int kc, const void* X, void* Y,
+                    hipStream_t s);
+int extra_diag_add(const skt_operator* op, int dtype, void* diag, hipStream_t s);
 // Shared tail of the hex8 constructors (operators.hip).
 int hex8_finish(skt_operator* op, const uint8_t* dof_fixed, skt_operator** out);
 // Output modes of the structured hex8 modal kernel.

@@ -593,19 +593,27 @@ int op_apply(skt_operator* op, int dtype, int ncols, int kc, const void* E, cons
   SKT_CHECK(X != Y, "operator apply cannot run in place");
   SKT_CHECK(dtype == SKT_F64 || dtype == SKT_F32, "dtype must be SKT_F64 or SKT_F32");
   if (op->mesh == SKT_MESH_HEX8 && op->modal_ok) {
+    int st;
     if (dtype == SKT_F64)
-      return hex_apply_modal<double>(op, ncols, kc, op->modal_d, (const double*)E,
-                                     (const double*)X, (double*)Y, s);
-    return hex_apply_modal<float>(op, ncols, kc, op->modal_f, (const float*)E,
+      st = hex_apply_modal<double>(op, ncols, kc, op->modal_d, (const double*)E,
+                                   (const double*)X, (double*)Y, s);
+    else
+      st = hex_apply_modal<float>(op, ncols, kc, op->modal_f, (const float*)E,
                                   (const float*)X, (float*)Y, s);
+    if (st) return st;
+    return extra_apply_add(op, dtype, ncols, kc, X, Y, s);
   }
   SKT_CHECK(kc == ncols, "column-blocked vectors need the structured hex8 operator");
   if (op->mesh == SKT_MESH_HEX8) {
+    int st;
     if (dtype == SKT_F64)
-      return hex_apply_t<double>(op, ncols, op->K0d, (const double*)E, (const double*)X,
-                                 (double*)Y, s);
-    return hex_apply_t<float>(op, ncols, op->K0f, (const float*)E, (const float*)X,
+      st = hex_apply_t<double>(op, ncols, op->K0d, (const double*)E, (const double*)X,
+                               (double*)Y, s);
+    else
+      st = hex_apply_t<float>(op, ncols, op->K0f, (const float*)E, (const float*)X,
                               (float*)Y, s);
+    if (st) return st;
+    return extra_apply_add(op, dtype, ncols, kc, X, Y, s);
   }
   if (dtype == SKT_F64)
     return tet_apply_t<double>(op, ncols, op->g64, op->vol64, (const double*)E,
@@ -640,10 +648,15 @@ static int diag_t(skt_operator* op, const T* K0, const T* g, const T* vol, const
 
 int op_diag(skt_operator* op, int dtype, const void* E, void* diag, hipStream_t s) {
   SKT_CHECK(op && E && diag, "null pointer");
-  if (dtype == SKT_F64)
-    return diag_t<double>(op, op->K0d, op->g64, op->vol64, (const double*)E, (double*)diag, s);
-  SKT_CHECK(dtype == SKT_F32, "dtype must be SKT_F64 or SKT_F32");
-  return diag_t<float>(op, op->K0f, op->g32, op->vol32, (const float*)E, (float*)diag, s);
+  int st;
+  if (dtype == SKT_F64) {
+    st = diag_t<double>(op, op->K0d, op->g64, op->vol64, (const double*)E, (double*)diag, s);
+  } else {
+    SKT_CHECK(dtype == SKT_F32, "dtype must be SKT_F64 or SKT_F32");
+    st = diag_t<float>(op, op->K0f, op->g32, op->vol32, (const float*)E, (float*)diag, s);
+  }
+  if (st) return st;
+  return extra_diag_add(op, dtype, diag, s);
 }
 
 template <typename T>
@@ -848,6 +861,8 @@ int skt_operator_destroy(skt_operator* op) {
   (void)hipFree(op->nloc);
   (void)hipFree(op->fixed);
   (void)hipFree(op->fixed_list);
+  (void)hipFree(op->extra64);
+  (void)hipFree(op->extra32);
   delete op;
   return SKT_OK;
 }

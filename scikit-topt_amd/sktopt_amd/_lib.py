@@ -38,6 +38,8 @@ SIGNATURES = {
     "skt_dC_drho": [_i32, _i64, _vp, _vp, _i64, _d, _d, _d, _vp, _vp],
     "skt_sensitivity_chain": [_i32, _i64, _i32, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d, _vp, _vp],
     "skt_sensitivity_chain_cols": [_i32, _i64, _i32, _vp, _vp, _vp, _d, _d, _d, _d, _d, _vp, _vp],
+    "skt_sensitivity_chain_extra": [_i32, _i64, _i32, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d,
+                                    _i32, _vp, _vp],
     "skt_ewise": [_i32, _i64, _d, _vp, _d, _vp, _vp, _vp],
     "skt_reduce": [_i32, _i64, _vp, _vp, _vp, _vp],
     "skt_reduce_cols": [_i32, _i64, _i32, _vp, _vp, _vp, _vp],
@@ -70,6 +72,13 @@ SIGNATURES = {
     "skt_filter_scatter_design": [_vp, _vp, _vp, _vp],
     "skt_filter_set_grid": [_vp, _i64, _i64, _i64, _d, _d, _d, _i32p],
     "skt_design_gather": [_i64, _vp, _vp, _vp, _vp],
+    "skt_hex8_nodal_average_pairing": [_i64, _i64, _i64, _vp, _vp, _vp, _vp],
+    "skt_hex8_robin_virtual": [_i64, _i64, _i64, _d, _d, _d, _i32, _vp, _d, _d, _d, _d, _vp,
+                               _vp, _vp],
+    "skt_hex8_robin_grad": [_i64, _i64, _i64, _d, _d, _d, _i32, _vp, _vp, _i32, _vp, _d, _d,
+                            _d, _d, _vp, _vp, _vp],
+    "skt_operator_set_extra": [_vp, _vp, _vp],
+    "skt_stencil27_apply": [_i64, _i64, _i64, _i32, _vp, _vp, _vp, _vp],
     "skt_design_scatter": [_i64, _vp, _vp, _vp, _vp],
     "skt_oc_candidate": [_i64, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d, _d, _vp, _vp, _vp],
     "skt_oc_bisect": [_vp, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d,

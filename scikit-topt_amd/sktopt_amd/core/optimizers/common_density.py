@@ -359,20 +359,28 @@ class DensityMethod:
         # sensitivity: the heat FEM always uses the SIMP form (solver_heat.py:945-947)
         kind = _lib.SIMP if isinstance(tsk, LinearHeatConduction) else self.kind
         E0 = tsk.material_coef
+        # the heat FEM's custom sensitivity (common_density.py:1067-1081): the
+        # conduction energies, plus an explicit Robin term per load case
+        sens_e = self.fem.sensitivity_energy(rp, p, st["u_dofs"]) \
+            if hasattr(self.fem, "sensitivity_energy") else energy
+        extra = self.fem.extra_sensitivity(rp, p, st["u_dofs"]) \
+            if hasattr(self.fem, "extra_sensitivity") else None
+        args = (float(beta), float(cfg.beta_eta), float(E0), float(E0 * cfg.E_min_coeff),
+                float(p))
         if filt.linear_adjoint or k == 1:
             # sum over load cases first, then one adjoint application
-            _lib.call("skt_sensitivity_chain", kind, n_el, k, _dev.ptr(rp), _dev.ptr(energy),
-                      _dev.ptr(rf), float(beta), float(cfg.beta_eta), float(E0),
-                      float(E0 * cfg.E_min_coeff), float(p), 1.0 / k, _dev.ptr(st["g"]), S)
+            _lib.call("skt_sensitivity_chain_extra", kind, n_el, k, _dev.ptr(rp),
+                      _dev.ptr(sens_e), _dev.ptr(extra), _dev.ptr(rf), *args, 1.0 / k, 0,
+                      _dev.ptr(st["g"]), S)
             filt.gradient_into(st["g"], st["dC_full"])
         else:
             # one adjoint per load case (batched), averaged
             G = st.get("g_cols")
             if G is None:
                 G = st["g_cols"] = torch.empty((n_el, k), dtype=torch.float64, device=self.dev)
-            _lib.call("skt_sensitivity_chain_cols", kind, n_el, k, _dev.ptr(rp),
-                      _dev.ptr(energy), _dev.ptr(rf), float(beta), float(cfg.beta_eta),
-                      float(E0), float(E0 * cfg.E_min_coeff), float(p), _dev.ptr(G), S)
+            _lib.call("skt_sensitivity_chain_extra", kind, n_el, k, _dev.ptr(rp),
+                      _dev.ptr(sens_e), _dev.ptr(extra), _dev.ptr(rf), *args, 1.0, 1,
+                      _dev.ptr(G), S)
             filt.gradient_mean_into(G, st["dC_full"])
         if cfg.sensitivity_filter:
             tmp = torch.empty_like(st["dC_full"])

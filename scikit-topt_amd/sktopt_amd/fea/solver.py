@@ -247,22 +247,46 @@ class FEM_SimpLinearElasticity(_GpuFEM):
 
 
 class FEM_SimpLinearHeatConduction(_GpuFEM):
-    """GPU counterpart of ``fea/solver_heat.py:628-980`` for
-    ``objective='compliance'`` without Robin terms."""
+    """GPU counterpart of ``fea/solver_heat.py:628-980``: objectives
+    ``'compliance'`` and ``'averaged_temp'``, with or without Robin facets.
+
+    With Robin data (``task.robin_coefficient`` set) every solve uses
+    ``K = K_cond(rho) + B_facets + B_virtual(rho)`` and the load
+    ``emit + L_facets + L_virtual(rho)`` (``:724-760``, ``get_robin_virtual``
+    :495-517); the Robin matrices are a 27-point stencil attached to the
+    operator.  The volume integrals use ``npts`` Gauss points per axis
+    (``npts = 2`` integrates the basis products exactly on the reference's
+    ``intorder`` 2 and 3)."""
     physics = _lib.HEAT
 
     def __init__(self, task, E_min_coeff, density_interpolation=simp_interpolation,
-                 solver_config=None, solver_option="spsolve", q=4, quad_order=2):
-        if getattr(task, "robin_coefficient", None) is not None:
-            raise NotImplementedError("Robin boundaries are not built on the GPU path")
-        if getattr(task, "objective", "compliance") != "compliance":
-            raise NotImplementedError("only objective='compliance' is built")
+                 solver_config=None, solver_option="spsolve", q=4, quad_order=2, npts=2):
+        if getattr(task, "objective", "compliance") not in ("compliance", "averaged_temp"):
+            raise NotImplementedError("objective='heat_exchange' is not built on the GPU path")
         self.k_max = task.k * 1.0
         self.k_min = task.k * E_min_coeff
         self.E_max, self.E_min = self.k_max, self.k_min
         self.q = q
+        self.npts = int(npts)
+        self.objective = getattr(task, "objective", "compliance")
+        self.robin = getattr(task, "robin_coefficient", None) is not None
+        if self.robin and (task.mesh.kind != "hex8" or getattr(task.mesh, "grid", None) is None):
+            raise NotImplementedError("Robin boundaries are built for structured hex8 grids")
         super().__init__(task, E_min_coeff, density_interpolation, solver_config,
                          solver_option, quad_order)
+        self.lam = None
+        self._lam_warm = None
+        self._D = torch.as_tensor(np.asarray(task.dirichlet_dofs, dtype=np.int64),
+                                  device=self.device)
+        if self.robin:
+            dev, nn = self.device, task.mesh.nvertices
+            z = lambda *sh: torch.zeros(sh, dtype=torch.float64, device=dev)  # noqa: E731
+            self._st_facet = _dev.to_dev(task.robin_stencil)
+            self._L_facet = _dev.to_dev(task.robin_load)
+            self._emit = _dev.to_dev(task.emit)
+            self._rn, self._cnt, self._work = z(nn), z(nn), z(nn)
+            self._st_virt, self._st_tot = z(27, nn), z(27, nn)
+            self._L_virt, self._emit_tot = z(nn), z(nn)
 
     def _build_rhs(self):
         cols = []
@@ -272,23 +296,124 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
             cols.append(f)
         return _dev.to_dev(np.column_stack(cols))
 
+    def _enforced(self, base):
+        """(ndof, k): base in every column, b_D = the column's Dirichlet value."""
+        F = base[:, None].repeat(1, self.n_tasks).contiguous()
+        for c, val in enumerate(self.task.dirichlet_values):
+            F[:, c].index_fill_(0, self._D, float(val))
+        return F
+
+    def _robin_update(self, rho, p):
+        """Virtual Robin stencil and load for the current densities
+        (``solver_heat.py:724-760``), attached to the operator; the enforced
+        right-hand side is rebuilt from the total load."""
+        t = self.task
+        nx, ny, nz = t.mesh.grid
+        S = _dev.stream()
+        _lib.call("skt_hex8_nodal_average_pairing", nx, ny, nz, _dev.ptr(rho), _dev.ptr(self._rn),
+                  _dev.ptr(self._cnt), S)
+        _lib.call("skt_hex8_robin_virtual", nx, ny, nz, *t.mesh.h, self.npts, _dev.ptr(self._rn),
+                  float(p), float(self.q), float(t.robin_coefficient), float(t.robin_bc_value),
+                  _dev.ptr(self._st_virt), _dev.ptr(self._L_virt), S)
+        _lib.call("skt_ewise", 0, self._st_tot.numel(), 1.0, _dev.ptr(self._st_facet), 1.0,
+                  _dev.ptr(self._st_virt), _dev.ptr(self._st_tot), S)
+        _lib.call("skt_operator_set_extra", self._op, _dev.ptr(self._st_tot), S)
+        # emit + facet loads + virtual load, in the reference's list order
+        _lib.call("skt_ewise", 0, self._emit.numel(), 1.0, _dev.ptr(self._emit), 1.0,
+                  _dev.ptr(self._L_facet), _dev.ptr(self._emit_tot), S)
+        _lib.call("skt_ewise", 0, self._emit.numel(), 1.0, _dev.ptr(self._emit_tot), 1.0,
+                  _dev.ptr(self._L_virt), _dev.ptr(self._emit_tot), S)
+        self.F = self._enforced(self._emit_tot)
+
+    def _solve_rhs(self, F, U):
+        saved = self.F
+        self.F = F
+        try:
+            self.solve(U)
+        finally:
+            self.F = saved
+
     def _objectives_dev(self, rho, p, U, force_scale=1.0):
-        """``J_k = T_k^T K T_k`` with the unenforced K (``solver_heat.py:785-789``),
-        i.e. twice the sum of the element energies."""
+        """``compliance``: ``J_k = T_k^T K T_k`` with the unenforced total K
+        (``solver_heat.py:785-789``): twice the conduction energies plus the
+        Robin stencil's ``T^T S T``.  ``averaged_temp``: ``J_k = sum(T_k)``
+        and the adjoint ``K lam = 1`` enforced with the state's Dirichlet
+        values (``:869-882``)."""
         self._moduli(rho, p)
+        if self.robin:
+            self._robin_update(rho, p)
         self.solve(U)
+        if self.objective == "averaged_temp":
+            J = self.col_reduce(_lib.RED_SUM, U)
+            if self._lam_warm is None:
+                self._lam_warm = torch.zeros_like(U)
+            ones = torch.ones(self.ndof, dtype=torch.float64, device=self.device)
+            self._solve_rhs(self._enforced(ones), self._lam_warm)
+            self.lam = self._lam_warm
+            return J
         en = torch.empty((self.n_el, self.n_tasks), dtype=torch.float64, device=self.device)
         _lib.call("skt_operator_energy", self._op, _lib.F64, self.n_tasks, _dev.ptr(self.E64),
                   _dev.ptr(U), _dev.ptr(en), _dev.stream())
-        return 2.0 * self.col_reduce(_lib.RED_SUM, en)
+        J = 2.0 * self.col_reduce(_lib.RED_SUM, en)
+        self.lam = None
+        if self.robin:
+            nx, ny, nz = self.task.mesh.grid
+            SU = torch.empty_like(U)
+            _lib.call("skt_stencil27_apply", nx, ny, nz, self.n_tasks, _dev.ptr(self._st_tot),
+                      _dev.ptr(U), _dev.ptr(SU), _dev.stream())
+            J = J + self.col_reduce(_lib.RED_DOT, U, SU)
+        return J
+
+    def _cross_energy(self, A, B):
+        """(n_el, k) ``int_e grad A . grad B`` with unit conductivity."""
+        ones = torch.ones(self.n_el, dtype=torch.float64, device=self.device)
+        def e(X):
+            out = torch.empty((self.n_el, self.n_tasks), dtype=torch.float64, device=self.device)
+            _lib.call("skt_operator_energy", self._op, _lib.F64, self.n_tasks, _dev.ptr(ones),
+                      _dev.ptr(X.contiguous()), _dev.ptr(out), _dev.stream())
+            return out
+        return e(A + B) - e(A) - e(B)
+
+    def energy(self, rho, p, U):
+        """``energy_multi_load`` (``solver_heat.py:897-925``): conduction
+        energies for ``compliance``; ``int grad T . grad lam`` for
+        ``averaged_temp``."""
+        if self.objective == "averaged_temp":
+            if self.lam is None:
+                raise RuntimeError("adjoint field λ_all is not computed.")
+            return self._cross_energy(U, self.lam)
+        return super().energy(rho, p, U)
+
+    def sensitivity_energy(self, rho, p, U):
+        """The energies the sensitivity uses: always the conduction ones
+        (``solver_heat.py:928-950``)."""
+        return _GpuFEM.energy(self, rho, p, U)
+
+    def extra_sensitivity(self, rho, p, U):
+        """(n_el, k) explicit Robin term of the compliance sensitivity
+        (``solver_heat.py:955-980``), or None."""
+        if not (self.robin and self.objective == "compliance"):
+            return None
+        t = self.task
+        nx, ny, nz = t.mesh.grid
+        S = _dev.stream()
+        _lib.call("skt_hex8_nodal_average_pairing", nx, ny, nz, _dev.ptr(rho), _dev.ptr(self._rn),
+                  _dev.ptr(self._cnt), S)
+        out = torch.empty((self.n_el, self.n_tasks), dtype=torch.float64, device=self.device)
+        _lib.call("skt_hex8_robin_grad", nx, ny, nz, *t.mesh.h, self.npts, _dev.ptr(self._rn),
+                  _dev.ptr(self._cnt), self.n_tasks, _dev.ptr(U.contiguous()), float(p),
+                  float(self.q), float(t.robin_coefficient), float(t.robin_bc_value),
+                  _dev.ptr(self._work), _dev.ptr(out), S)
+        return out
 
     def compliance_sensitivity_multi_load(self, rho, p, u_dofs):
-        """Always the SIMP sensitivity of the energies
-        (``solver_heat.py:928-950``)."""
+        """SIMP sensitivity of the conduction energies (``solver_heat.py:928-950``)
+        plus, for the Robin compliance, the explicit Robin term (``:955-980``)."""
         host = _dev.is_host(rho) or _dev.is_host(u_dofs)
         r = _dev.to_dev(rho)
         U = _dev.to_dev(u_dofs).reshape(self.ndof, self.n_tasks).contiguous()
-        en = self.energy(r, p, U)
+        en = self.sensitivity_energy(r, p, U)
+        extra = self.extra_sensitivity(r, p, U)
         out = torch.empty_like(en)
         for k in range(self.n_tasks):
             col = en[:, k].contiguous()
@@ -296,4 +421,6 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
             _lib.call("skt_dC_drho", _lib.SIMP, self.n_el, _dev.ptr(r), _dev.ptr(col), 1,
                       self.k_max, self.k_min, float(p), _dev.ptr(tmp), _dev.stream())
             out[:, k] = tmp
+        if extra is not None:
+            out += extra
         return out.cpu().numpy() if host else out

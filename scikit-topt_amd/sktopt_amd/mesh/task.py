@@ -179,24 +179,109 @@ class LinearElasticity(FEMDomain):
         return self.neumann_linear
 
 
+_QUAD_MASS = np.array([[4, 2, 1, 2], [2, 4, 2, 1], [1, 2, 4, 2], [2, 1, 2, 4]],
+                      dtype=np.float64) / 36.0
+
+
+def assemble_surface_robin(mesh, robin_facets_ids, robin_coefficient, robin_bc_value):
+    """``mesh/task_heat.py:60-139`` on a structured hex grid: the facet forms
+    ``int_G h u v`` and ``int_G h T_env v`` of every Robin region, summed,
+    as a 27-point nodal stencil (27, n_nodes) and a load vector.  Facets are
+    planar parallelograms: mass ``h A / 36 [[4 2 1 2] ...]`` in cyclic node
+    order and load ``h T_env A / 4`` per node.  (The reference's density
+    argument never reaches these forms, so they do not change with rho.)"""
+    if getattr(mesh, "kind", None) != "hex8" or getattr(mesh, "grid", None) is None:
+        raise NotImplementedError("Robin boundaries are built for structured hex8 grids")
+    facets_list = _as_list(robin_facets_ids)
+    hs = [float(robin_coefficient)] * len(facets_list) if not isinstance(
+        robin_coefficient, list) else [float(v) for v in robin_coefficient]
+    ts = [float(robin_bc_value)] * len(facets_list) if not isinstance(
+        robin_bc_value, list) else [float(v) for v in robin_bc_value]
+    if not (len(facets_list) == len(hs) == len(ts)):
+        raise ValueError("Lengths of robin_facets_ids and robin_value must match when lists.")
+    nx, ny, nz = mesh.grid
+    nn = mesh.nvertices
+    i, j, k = mesh.node_ijk(nx, ny, nz)
+    st = np.zeros((27, nn))
+    load = np.zeros(nn)
+    for ids, h, te in zip(facets_list, hs, ts):
+        ids = np.asarray(ids, dtype=np.int64)
+        f = mesh.facets[:, ids]                               # (4, nf) cyclic
+        area = mesh.facet_areas(ids)
+        for a in range(4):
+            np.add.at(load, f[a], h * te * area / 4.0)
+            for b in range(4):
+                na, nb = f[a], f[b]
+                o = (i[nb] - i[na] + 1) + 3 * (j[nb] - j[na] + 1) + 9 * (k[nb] - k[na] + 1)
+                np.add.at(st, (o, na), h * area * _QUAD_MASS[a, b])
+    return st, load
+
+
 class LinearHeatConduction(FEMDomain):
-    """Steady heat conduction with ``objective='compliance'``: conductivity
+    """Steady heat conduction (``mesh/task_heat.py:141-250``): conductivity
     ``k``, Dirichlet temperature sets (one value per load case,
-    ``mesh/task_heat.py:165-169``) and a nodal heat-source vector ``emit``
-    (the reference injects sources through ``robin_linear``,
-    ``fea/solver_heat.py:230-235``)."""
+    ``:165-169``), a nodal heat-source vector ``emit`` (the reference
+    injects sources through ``robin_linear``, ``fea/solver_heat.py:230-235``)
+    and optional Robin facets with coefficient ``robin_coefficient`` and
+    ambient value ``robin_bc_value``.  Objectives ``'compliance'`` and
+    ``'averaged_temp'``; ``'heat_exchange'`` is not built."""
 
     def __init__(self, mesh, dirichlet_nodes, dirichlet_values, emit, design_elements, k,
-                 objective="compliance"):
-        if objective != "compliance":
-            raise NotImplementedError("only objective='compliance' is built")
+                 objective="compliance", robin_facets_ids=None, robin_coefficient=None,
+                 robin_bc_value=None, design_robin_boundary=None, avg_temp_weight=0.0):
+        if objective not in ("compliance", "heat_exchange", "averaged_temp"):
+            raise ValueError("objective must be one of 'compliance', 'heat_exchange', or "
+                             "'averaged_temp'")
+        if objective == "heat_exchange":
+            raise NotImplementedError("objective='heat_exchange' is not built on the GPU path")
         vals = dirichlet_values if isinstance(dirichlet_values, list) else [dirichlet_values]
+        if robin_facets_ids is None:
+            robin_coefficient = robin_bc_value = design_robin_boundary = None
+        self.robin_facets_ids = robin_facets_ids
+        self.robin_coefficient = robin_coefficient
+        self.robin_bc_value = robin_bc_value
+        self.design_robin_boundary = design_robin_boundary
         super().__init__(mesh, 1, dirichlet_nodes, dirichlet_nodes, vals, None,
                          design_elements)
+        if robin_facets_ids is not None:
+            fids = np.concatenate(robin_facets_ids) if isinstance(robin_facets_ids, list) \
+                else np.asarray(robin_facets_ids)
+            self.robin_nodes = _facet_nodes(mesh, fids)
+            self.robin_elements = mesh.elements_touching(self.robin_nodes)
+            if design_robin_boundary is False:
+                d = _setdiff(self.design_elements, self.robin_elements)
+                if len(d) == 0:
+                    raise ValueError("design_elements is empty")
+                self.design_elements = d
+                self.fixed_elements = _setdiff(self.all_elements, d)
+            if isinstance(robin_coefficient, list) or isinstance(robin_bc_value, list):
+                raise NotImplementedError("one scalar Robin coefficient and ambient value "
+                                          "(the virtual Robin terms take scalars)")
+            self.robin_stencil, self.robin_load = assemble_surface_robin(
+                mesh, robin_facets_ids, robin_coefficient, robin_bc_value)
+        else:
+            self.robin_nodes = None
+            self.robin_elements = None
         self.k = float(k)
         self.objective = objective
-        self.emit = np.asarray(emit, dtype=np.float64)
-        self.robin_coefficient = None
+        self.avg_temp_weight = float(avg_temp_weight)
+        self.emit = np.zeros(mesh.nvertices) if emit is None \
+            else np.asarray(emit, dtype=np.float64)
+
+    @classmethod
+    def from_facets(cls, mesh, dirichlet_facets_ids, dirichlet_values, robin_facets_ids,
+                    robin_coefficient, robin_bc_value, design_robin_boundary, design_elements,
+                    k, objective="compliance", avg_temp_weight=0.0, emit=None):
+        """``mesh/task_heat.py:170-250`` with nodes taken from facets
+        (``mesh/task_common.py:273-347``).  One Dirichlet node set; a list of
+        values makes one load case per value."""
+        if isinstance(dirichlet_facets_ids, list):
+            dn = np.unique(np.concatenate([_facet_nodes(mesh, f) for f in dirichlet_facets_ids]))
+        else:
+            dn = _facet_nodes(mesh, dirichlet_facets_ids)
+        return cls(mesh, dn, dirichlet_values, emit, design_elements, k, objective,
+                   robin_facets_ids, robin_coefficient, robin_bc_value, design_robin_boundary,
+                   avg_temp_weight)
 
     @property
     def material_coef(self):

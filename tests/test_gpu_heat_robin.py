@@ -1,0 +1,144 @@
+"""Robin heat conduction on the GPU against the oracle
+(fea/solver_heat.py:495-980, mesh/task_heat.py:60-139)."""
+import numpy as np
+import pytest
+
+from oracle import driver as odrv
+from oracle import heat_robin as hr
+from oracle import mesh as om
+from oracle import problems as oprob
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x):
+    import torch
+    return torch.tensor(np.ascontiguousarray(x), dtype=torch.float64, device="cuda")
+
+
+def _s():
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _task(nx=10, ny=10, nz=4, objective="compliance", h=4e-2, design_robin=True):
+    from sktopt_amd.mesh import structured_hex, toy_problem as tp
+    from sktopt_amd.mesh.task import LinearHeatConduction
+    m = structured_hex(nx, ny, nz)
+    d = m.facets_satisfying(tp.get_points_in_range((nx, nx), (0, 2), (0, nz)))
+    r0 = m.facets_satisfying(tp.get_points_in_range((0, 0), (0, ny), (0, nz)))
+    r1 = m.facets_satisfying(tp.get_points_in_range((0, nx), (ny, ny), (0, nz)))
+    return LinearHeatConduction.from_facets(m, d, 600.0, [r0, r1], h, 300.0, design_robin,
+                                            np.arange(m.nelements), 10.0, objective=objective)
+
+
+def _oracle_fem(t, objective):
+    om_ = om.structured_hex(*t.mesh.grid, *t.mesh.h)
+    facets = [t.mesh.facets[:, np.asarray(f)] for f in t.robin_facets_ids]
+    return om_, hr.HeatRobinFEM(om_, t.dirichlet_dofs, t.dirichlet_values, t.k, 1e-3, facets,
+                                t.robin_coefficient, t.robin_bc_value, emit=t.emit,
+                                objective=objective)
+
+
+def test_robin_kernels_match_oracle(gpu):
+    import torch
+    from sktopt_amd import _lib
+    t = _task(7, 5, 3)
+    m = om.structured_hex(*t.mesh.grid)
+    nx, ny, nz = m.grid
+    rng = np.random.default_rng(3)
+    rho = rng.uniform(0.05, 1.0, m.nelements)
+    rn = torch.empty(m.nnodes, dtype=torch.float64, device="cuda")
+    cnt = torch.empty_like(rn)
+    _lib.call("skt_hex8_nodal_average_pairing", nx, ny, nz, _t(rho).data_ptr(), rn.data_ptr(),
+              cnt.data_ptr(), _s())
+    rn_ref, cnt_ref = hr.nodal_average_reference(m, rho)
+    np.testing.assert_array_equal(rn.cpu().numpy(), rn_ref)
+    np.testing.assert_array_equal(cnt.cpu().numpy(), cnt_ref)
+    for npts in (1, 2, 3):
+        st = torch.empty((27, m.nnodes), dtype=torch.float64, device="cuda")
+        L = torch.empty(m.nnodes, dtype=torch.float64, device="cuda")
+        _lib.call("skt_hex8_robin_virtual", nx, ny, nz, 1.0, 1.0, 1.0, npts, rn.data_ptr(),
+                  3.0, 4.0, 0.04, 300.0, st.data_ptr(), L.data_ptr(), _s())
+        B, Lr = hr.virtual_robin_nodal(m, rn_ref, 3.0, 4.0, 0.04, 300.0, npts)
+        x = rng.standard_normal((m.nnodes, 2))
+        y = torch.empty((m.nnodes, 2), dtype=torch.float64, device="cuda")
+        _lib.call("skt_stencil27_apply", nx, ny, nz, 2, st.data_ptr(), _t(x).data_ptr(),
+                  y.data_ptr(), _s())
+        ref = B @ x
+        np.testing.assert_allclose(y.cpu().numpy(), ref, rtol=1e-12,
+                                   atol=1e-13 * np.abs(ref).max())
+        np.testing.assert_allclose(L.cpu().numpy(), Lr, rtol=1e-12, atol=1e-13 * np.abs(Lr).max())
+    T = 300.0 + 100.0 * rng.uniform(size=(m.nnodes, 2))
+    out = torch.empty((m.nelements, 2), dtype=torch.float64, device="cuda")
+    work = torch.empty(m.nnodes, dtype=torch.float64, device="cuda")
+    _lib.call("skt_hex8_robin_grad", nx, ny, nz, 1.0, 1.0, 1.0, 2, rn.data_ptr(), cnt.data_ptr(),
+              2, _t(T).data_ptr(), 3.0, 4.0, 0.04, 300.0, work.data_ptr(), out.data_ptr(), _s())
+    for c in range(2):
+        ref = hr.robin_explicit_grad(m, rho, T[:, c], 3.0, 4.0, 0.04, 300.0)
+        np.testing.assert_allclose(out[:, c].cpu().numpy(), ref, rtol=1e-11,
+                                   atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("objective,precision", [("compliance", "fp64"),
+                                                 ("averaged_temp", "fp64"),
+                                                 ("compliance", "mixed")])
+def test_robin_objectives_match_oracle(gpu, objective, precision):
+    from sktopt_amd import fea
+    t = _task(objective=objective)
+    _, ofem = _oracle_fem(t, objective)
+    cfg = fea.LinearSolverConfig(rtol=1e-12 if precision == "fp64" else 1e-10,
+                                 precision=precision)
+    fem = fea.FEM_SimpLinearHeatConduction(t, 1e-3, solver_config=cfg)
+    rho = np.random.default_rng(5).uniform(0.1, 1.0, t.mesh.nelements)
+    U = np.zeros((t.ndof, 1))
+    Uo = np.zeros((t.ndof, 1))
+    J = fem.objectives_multi_load(rho, 3.0, U)
+    Jo = ofem.objectives_multi_load(rho, 3.0, Uo)
+    tol = 1e-9 if precision == "fp64" else 1e-7
+    np.testing.assert_allclose(J, Jo, rtol=tol)
+    np.testing.assert_allclose(U, Uo, rtol=tol, atol=tol * np.abs(Uo).max())
+    e = fem.energy_multi_load(rho, 3.0, U)
+    eo = ofem.energy_multi_load(rho, 3.0, Uo)
+    np.testing.assert_allclose(e, eo, rtol=1e-6, atol=1e-8 * np.abs(eo).max())
+    s = fem.compliance_sensitivity_multi_load(rho, 3.0, U)
+    so = ofem.compliance_sensitivity_multi_load(rho, 3.0, Uo)
+    np.testing.assert_allclose(s, so, rtol=1e-6, atol=1e-8 * np.abs(so).max())
+
+
+@pytest.mark.parametrize("method,objective", [("OC", "compliance"), ("LogMOC", "compliance"),
+                                              ("OC", "averaged_temp")])
+def test_robin_heat_trajectory_matches_oracle(gpu, method, objective, tmp_path):
+    from sktopt_amd import core
+    from sktopt_amd.tools import SchedulerConfig as SC
+    t = _task(objective=objective)
+    om_, ofem = _oracle_fem(t, objective)
+    otask = oprob.Task(om_, "heat", 10.0, 0.0, t.dirichlet_nodes, t.dirichlet_dofs, [t.emit],
+                       None, dirichlet_values=list(t.dirichlet_values))
+    S = odrv.Sched
+    kw = dict(lambda_lower=-1e7, lambda_upper=1e7) if method == "LogMOC" else {}
+    ocfg = odrv.Config(method=method, max_iters=50, p=S("Constant", 3.0, 3.0),
+                       vol_frac=S("Constant", 0.3, 0.3), beta=S("Constant", 1.0, 1.0),
+                       filter_radius=S("Constant", 1.5, 1.5), move_limit=S("Constant", 0.2, 0.2),
+                       eta=S("Constant", 0.5, 0.5), E_min_coeff=1e-3, **kw)
+    loop = odrv.DensityLoop(ocfg, otask, fem=ofem)
+    common = dict(dst_path=str(tmp_path), max_iters=50, p=SC.constant(target_value=3.0),
+                  vol_frac=SC.constant(target_value=0.3), beta=SC.constant(target_value=1.0),
+                  filter_type="spacial", filter_radius=SC.constant(target_value=1.5),
+                  move_limit=SC.constant(target_value=0.2), E_min_coeff=1e-3,
+                  precision="fp64", solver_rtol=1e-13,
+                  eta=SC.constant(target_value=0.5))
+    if method == "OC":
+        opt = core.OC_Optimizer(core.OC_Config(**common), t)
+    else:
+        opt = core.LogMOC_Optimizer(core.LogMOC_Config(**common, **kw), t)
+    opt.parameterize()
+    # the Robin compliance sensitivity is a difference of the conduction and
+    # Robin terms (phi = 2 T_env T - T^2 ~ -1e5 here), so solver-level
+    # differences (1e-9 in T) reach the densities amplified to ~1e-7
+    for it in range(1, 4):
+        rec = loop.step(it)
+        opt.optimize_steps(1)
+        c = opt.recorder.latest(objective)
+        assert abs(c - rec["compliance"]) <= 1e-8 * abs(rec["compliance"]), (it, c, rec)
+        assert np.abs(opt.rho_host() - loop.rho).max() <= 2e-6, it

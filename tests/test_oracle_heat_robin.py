@@ -1,0 +1,109 @@
+"""Pins of the Robin heat oracle (fea/solver_heat.py:306-980,
+mesh/task_heat.py:60-139) against what the mathematics fixes."""
+import numpy as np
+import pytest
+
+from oracle import heat_robin as hr
+from oracle import helmholtz as oh
+from oracle import mesh as om
+from oracle import problems as oprob
+from oracle.fem import HeatFEM
+
+
+def test_nodal_average_pairing_equals_the_reference_statement():
+    """k-th term of np.add.at(rn, t.ravel(), np.repeat(rho, 8)) is rho[k // 8]
+    at node t.ravel()[k] -- written out as a plain loop."""
+    m = om.structured_hex(4, 3, 2)
+    rho = np.random.default_rng(0).uniform(size=m.nelements)
+    rn, cnt = hr.nodal_average_reference(m, rho)
+    flat = m.t.ravel()
+    acc = np.zeros(m.nnodes)
+    c = np.zeros(m.nnodes)
+    for k in range(flat.size):
+        acc[flat[k]] += rho[k // 8]
+        c[flat[k]] += 1.0
+    np.testing.assert_array_equal(cnt, c)
+    np.testing.assert_array_equal(rn, acc / np.maximum(c, 1.0))
+    # constant fields survive the pairing
+    rn1, _ = hr.nodal_average_reference(m, np.full(m.nelements, 0.3))
+    np.testing.assert_allclose(rn1, 0.3, rtol=1e-15)
+
+
+def test_virtual_robin_reduces_to_a_mass_matrix():
+    """p = q = 0 and a linear nodal field: h a |grad rho| = h |beta| constant,
+    so B = h |beta| M and L = h |beta| T_env M 1."""
+    m = om.structured_hex(3, 2, 2, 0.5, 1.0, 2.0)
+    beta = 0.37
+    rn = 0.2 + beta * m.p[0]
+    B, L = hr.virtual_robin_nodal(m, rn, 0.0, 0.0, 2.5, 300.0, npts=2)
+    _, M = oh.helmholtz_matrices(m, 0.0)
+    np.testing.assert_allclose(B.toarray(), 2.5 * beta * M.toarray(), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(L, 2.5 * beta * 300.0 * (M @ np.ones(m.nnodes)), rtol=1e-13)
+    # p = 1: the integrand is cubic along x, exact with 2 points per axis
+    B2, _ = hr.virtual_robin_nodal(m, rn, 1.0, 0.0, 1.0, 1.0, npts=2)
+    B4, _ = hr.virtual_robin_nodal(m, rn, 1.0, 0.0, 1.0, 1.0, npts=4)
+    np.testing.assert_allclose(B2.toarray(), B4.toarray(), rtol=1e-12, atol=1e-15)
+
+
+def test_facet_robin_integrals():
+    m = om.structured_hex(2, 2, 1)
+    # the z = 1 face: 4 unit squares in cyclic order
+    nid = lambda i, j, k: i + 3 * (j + 3 * k)  # noqa: E731
+    fc = np.array([[nid(i, j, 1), nid(i + 1, j, 1), nid(i + 1, j + 1, 1), nid(i, j + 1, 1)]
+                   for j in range(2) for i in range(2)]).T
+    B, L = hr.facet_robin(m.p, fc, 2.0, 5.0, m.nnodes)
+    np.testing.assert_allclose(B.sum(), 2.0 * 4.0, rtol=1e-14)
+    np.testing.assert_allclose(L.sum(), 2.0 * 5.0 * 4.0, rtol=1e-14)
+    one, _ = hr.facet_robin(m.p, fc[:, :1], 36.0, 0.0, m.nnodes)
+    loc = one.toarray()[np.ix_(fc[:, 0], fc[:, 0])]
+    ref = np.array([[4, 2, 1, 2], [2, 4, 2, 1], [1, 2, 4, 2], [2, 1, 2, 4]], dtype=float)
+    np.testing.assert_allclose(loc, ref, rtol=1e-14)
+
+
+def test_explicit_gradient_is_the_nodal_derivative():
+    """The nodal form of _robin_compliance_explicit_grad_ is the derivative of
+    int h a(rho) |grad rho| (2 T_env T - T^2) in the nodal densities."""
+    m = om.structured_hex(3, 2, 2)
+    rng = np.random.default_rng(4)
+    rn = 0.3 + 0.4 * rng.uniform(size=m.nnodes)
+    T = 300.0 + 50.0 * rng.uniform(size=m.nnodes)
+    g = hr.robin_explicit_grad_nodal(m, rn, T, 3.0, 4.0, 0.7, 310.0)
+    eps = 1e-6
+    for n in rng.choice(m.nnodes, size=6, replace=False):
+        a, b = rn.copy(), rn.copy()
+        a[n] += eps
+        b[n] -= eps
+        fd = (hr.robin_functional(m, a, T, 3.0, 4.0, 0.7, 310.0) -
+              hr.robin_functional(m, b, T, 3.0, 4.0, 0.7, 310.0)) / (2 * eps)
+        np.testing.assert_allclose(g[n], fd, rtol=1e-6, atol=1e-6 * np.abs(g).max())
+
+
+def test_without_robin_equals_the_plain_heat_oracle():
+    task = oprob.heat_patch(6, patch=3, source=1e-2)
+    fem0 = HeatFEM(task, 1e-3)
+    fem1 = hr.HeatRobinFEM(task.mesh, task.dirichlet_dofs, task.dirichlet_values, task.E0,
+                           1e-3, [], 0.0, 0.0, emit=task.forces[0])
+    rho = np.random.default_rng(1).uniform(0.1, 1.0, task.n_elements)
+    U0 = np.zeros((task.ndof, 1))
+    U1 = np.zeros((task.ndof, 1))
+    c0 = fem0.objectives_multi_load(rho, 3.0, U0)
+    c1 = fem1.objectives_multi_load(rho, 3.0, U1)
+    np.testing.assert_allclose(c1, c0, rtol=1e-12)
+    s0 = fem0.compliance_sensitivity_multi_load(rho, 3.0, U0)
+    np.testing.assert_allclose(fem1.compliance_sensitivity_multi_load(rho, 3.0, U1), s0,
+                               rtol=1e-12, atol=1e-12 * np.abs(s0).max())
+
+
+def test_averaged_temp_adjoint():
+    """J = sum(T) and lam solves K lam = 1 (enforced with the state's values):
+    for a zero Dirichlet value, sum(T) = lam . F on the free dofs."""
+    task = oprob.heat_patch(5, patch=3, source=1e-2)
+    fem = hr.HeatRobinFEM(task.mesh, task.dirichlet_dofs, [0.0], task.E0, 1e-3, [], 0.0,
+                          0.0, emit=task.forces[0], objective="averaged_temp")
+    rho = np.random.default_rng(2).uniform(0.1, 1.0, task.n_elements)
+    U = np.zeros((task.ndof, 1))
+    J = fem.objectives_multi_load(rho, 3.0, U)
+    free = np.setdiff1d(np.arange(task.ndof), task.dirichlet_dofs)
+    np.testing.assert_allclose(J[0], fem.lam[free, 0] @ task.forces[0][free], rtol=1e-10)
+    e = fem.energy_multi_load(rho, 3.0, U)
+    assert e.shape == (task.n_elements, 1)
