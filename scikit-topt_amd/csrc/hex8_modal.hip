@@ -27,6 +27,8 @@
 // and the solver's column-blocked [k/kc][ndof][kc] layout (lkc = kc,
 // gstride = ndof*kc, dstride = kc).  A workgroup handles KC columns; with
 // the blocked layout its loads and stores are contiguous.
+#include <cstdlib>
+
 #include "operator.hpp"
 
 namespace skt {
@@ -62,6 +64,18 @@ __device__ __forceinline__ double from_right(double v) {
 // (bit 0: x, bit 1: y, bit 2: z): out[S] = sum_i (-1)^popcount(i & S) v[i].
 // The matrix is symmetric and squares to 8 I, so the same butterfly maps
 // modal forces back to corners.
+// 2-stage butterfly on the 4 corners of a node-plane face (bit 0: x, bit 1:
+// y): the x-y part of wht8.
+template <typename T>
+__device__ __forceinline__ void wht4(T v[4]) {
+  T p = v[0] + v[1], q = v[0] - v[1];
+  T r = v[2] + v[3], t = v[2] - v[3];
+  v[0] = p + r;
+  v[2] = p - r;
+  v[1] = q + t;
+  v[3] = q - t;
+}
+
 template <typename T>
 __device__ __forceinline__ void wht8(T v[8]) {
 #pragma unroll
@@ -149,7 +163,16 @@ struct OutArgs {
   int prev_zero = 0;           // the iterate before X is zero (first smoothing step)
 };
 
-template <typename T, int NC, int KC, int MODE>
+// SF (sum-factorised), bit 0: the x-y butterfly of each element's face on a
+// node plane is computed once per plane and carried in registers to the
+// next layer (the element thread keeps its footprint as the z-stream
+// advances), so a layer reads 4 corners instead of 8 and does 16 instead of
+// 24 add/sub per component forward.  Bit 1: the backward butterfly
+// accumulates the face forces of the layer below and this layer in modal
+// space and finishes each node plane with one 2-D butterfly, leaving half
+// the pair traffic.  The carries cost registers, so which form is fastest
+// depends on the precision and output mode.
+template <typename T, int NC, int KC, int MODE, int SF>
 __global__ void __launch_bounds__(256)
 k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, int dstride,
                   int lz, int tiles_x, int tiles_y, int n_zchunks, ModalArgs<T> coef,
@@ -244,6 +267,31 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   stash(planes + bot * kPlane);
   fetch(z0);
   stash(planes + top * kPlane);
+  // SF carries: face butterfly of the element's footprint on the bottom
+  // plane, and the top-face forces of the layer below
+  constexpr bool kSFF = (SF & 1) != 0, kSFB = (SF & 2) != 0;
+  T Fb[kSFF ? KC : 1][NC][4], Gc[kSFB ? KC : 1][NC][4];
+  if constexpr (kSFF) {
+    __syncthreads();
+#pragma unroll
+    for (int col = 0; col < KC; ++col)
+#pragma unroll
+      for (int d = 0; d < NC; ++d) {
+#pragma unroll
+        for (int i2 = 0; i2 < 4; ++i2)
+          Fb[col][d][i2] = planes[bot * kPlane + (d * KC + col) * kPS +
+                                  node_slot(ex_l + (i2 & 1), ey_l + (i2 >> 1))];
+        wht4(Fb[col][d]);
+      }
+  }
+  if constexpr (kSFB) {
+#pragma unroll
+    for (int col = 0; col < KC; ++col)
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+#pragma unroll
+        for (int i2 = 0; i2 < 4; ++i2) Gc[col][d][i2] = T(0);
+  }
   for (int64_t ez = z0 - 1; ez < z1; ++ez) {
     fetch(ez + 2);  // in flight while this layer is computed
     const T Ee = (el_in && ez >= 0 && ez < nz) ? E[e_base + nx * ny * ez] : T(0);
@@ -278,30 +326,70 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
 #pragma unroll
     for (int col = 0; col < KC; ++col) {
       T m[NC][8];
+      if constexpr (kSFF) {
+        // z stage from the carried bottom face and this top face
 #pragma unroll
-      for (int idx = 0; idx < 8; ++idx) {
-        const int cx = idx & 1, cy = (idx >> 1) & 1, cz = idx >> 2;
-        const int nl = node_slot(ex_l + cx, ey_l + cy);
-        const T* p = cz ? pt : pb;
+        for (int d = 0; d < NC; ++d) {
+          T ft[4];
 #pragma unroll
-        for (int d = 0; d < NC; ++d) m[d][idx] = p[(d * KC + col) * kPS + nl] * Ee;
+          for (int i2 = 0; i2 < 4; ++i2)
+            ft[i2] = pt[(d * KC + col) * kPS + node_slot(ex_l + (i2 & 1), ey_l + (i2 >> 1))];
+          wht4(ft);
+#pragma unroll
+          for (int i2 = 0; i2 < 4; ++i2) {
+            m[d][i2] = (Fb[col][d][i2] + ft[i2]) * Ee;
+            m[d][i2 | 4] = (Fb[col][d][i2] - ft[i2]) * Ee;
+            Fb[col][d][i2] = ft[i2];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int idx = 0; idx < 8; ++idx) {
+          const int cx = idx & 1, cy = (idx >> 1) & 1, cz = idx >> 2;
+          const int nl = node_slot(ex_l + cx, ey_l + cy);
+          const T* p = cz ? pt : pb;
+#pragma unroll
+          for (int d = 0; d < NC; ++d) m[d][idx] = p[(d * KC + col) * kPS + nl] * Ee;
+        }
+#pragma unroll
+        for (int d = 0; d < NC; ++d) wht8(m[d]);
       }
-#pragma unroll
-      for (int d = 0; d < NC; ++d) wht8(m[d]);
       T f[NC][8];
       if constexpr (NC == 3) modal_elastic(coef.c, m, f);
       else modal_heat(coef.c, m, f);
-#pragma unroll
-      for (int d = 0; d < NC; ++d) wht8(f[d]);
-      // node (ex+1, ey+cy) of plane side cz gets this element's corner cx=1
-      // plus the right-hand element's corner cx=0
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {  // q = cy | cz << 1
-        const int cy = q & 1, cz = q >> 1;
+      if constexpr (kSFB) {
+        // bottom-face forces of this layer plus top-face forces of the layer
+        // below complete node plane ez; the top-face forces are carried
 #pragma unroll
         for (int d = 0; d < NC; ++d) {
-          const T v = f[d][1 | (cy << 1) | (cz << 2)] + from_right(f[d][(cy << 1) | (cz << 2)]);
-          pairs[(((cz * 2 + cy) * NC + d) * KC + col) * 256 + tid] = v;
+          T g[4];
+#pragma unroll
+          for (int i2 = 0; i2 < 4; ++i2) {
+            g[i2] = Gc[col][d][i2] + (f[d][i2] + f[d][i2 | 4]);
+            Gc[col][d][i2] = f[d][i2] - f[d][i2 | 4];
+          }
+          wht4(g);
+          // node (ex+1, ey+cy) gets this element's corner cx=1 plus the
+          // right-hand element's corner cx=0
+#pragma unroll
+          for (int cy = 0; cy < 2; ++cy) {
+            const T v = g[1 | (cy << 1)] + from_right(g[cy << 1]);
+            pairs[((cy * NC + d) * KC + col) * 256 + tid] = v;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int d = 0; d < NC; ++d) wht8(f[d]);
+        // node (ex+1, ey+cy) of plane side cz gets this element's corner cx=1
+        // plus the right-hand element's corner cx=0
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {  // q = cy | cz << 1
+          const int cy = q & 1, cz = q >> 1;
+#pragma unroll
+          for (int d = 0; d < NC; ++d) {
+            const T v = f[d][1 | (cy << 1) | (cz << 2)] + from_right(f[d][(cy << 1) | (cz << 2)]);
+            pairs[(((cz * 2 + cy) * NC + d) * KC + col) * 256 + tid] = v;
+          }
         }
       }
     }
@@ -314,14 +402,21 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
         for (int d = 0; d < NC; ++d) {
           // element row ny_l + 1 is outside the tile only for the 16th node
           // row of a closing tile, where it is outside the grid as well
-          const T below = pairs[(((0 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
-                          (has_up ? pairs[(((0 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE]
-                                  : T(0));
-          const T above = pairs[(((1 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
-                          (has_up ? pairs[(((1 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE]
-                                  : T(0));
-          const T y = fix[d] ? xo[col][d] : partial[col][d] + below;
-          partial[col][d] = above;
+          T y;
+          if constexpr (kSFB) {
+            const T tot = pairs[((1 * NC + d) * KC + col) * 256 + e_lo] +
+                          (has_up ? pairs[((0 * NC + d) * KC + col) * 256 + e_lo + kTE] : T(0));
+            y = fix[d] ? xo[col][d] : tot;
+          } else {
+            const T below = pairs[(((0 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
+                            (has_up ? pairs[(((0 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE]
+                                    : T(0));
+            const T above = pairs[(((1 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
+                            (has_up ? pairs[(((1 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE]
+                                    : T(0));
+            y = fix[d] ? xo[col][d] : partial[col][d] + below;
+            partial[col][d] = above;
+          }
           if (out) {
             const int64_t o = d * dstride + col;
             if constexpr (MODE == OUT_APPLY) {
@@ -348,7 +443,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
 
 // Launch helper for structured grids whose element matrix has the modal
 // pattern.  k columns in the layout described above.
-template <typename T, int NC, int KC, int MODE>
+template <typename T, int NC, int KC, int MODE, int SF>
 static int launch_modal(const skt_operator* op, int k, int lkc, int64_t gstride, int dstride,
                         const T* coef, const T* E, const T* X, T* Y, const OutArgs<T>& oa,
                         hipStream_t s) {
@@ -370,18 +465,33 @@ static int launch_modal(const skt_operator* op, int k, int lkc, int64_t gstride,
   const int nzc = (int)((nnz + lz - 1) / lz);
   const int64_t nblocks = per * nzc;
   const size_t shmem = sizeof(T) * ((size_t)2 * soa_stride(kPN, NC * KC) * NC * KC +
-                                    (size_t)4 * NC * KC * 256);
+                                    (size_t)((SF & 2) ? 2 : 4) * NC * KC * 256);
   static bool attr_set = false;
   if (!attr_set) {
-    SKT_HIP(hipFuncSetAttribute((const void*)k_hex_apply_modal<T, NC, KC, MODE>,
+    SKT_HIP(hipFuncSetAttribute((const void*)k_hex_apply_modal<T, NC, KC, MODE, SF>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_hex_apply_modal<T, NC, KC, MODE>), dim3((unsigned)nblocks), dim3(256),
+  hipLaunchKernelGGL((k_hex_apply_modal<T, NC, KC, MODE, SF>), dim3((unsigned)nblocks), dim3(256),
                      shmem, s, op->nx, op->ny, op->nz, lkc, gstride, dstride, (int)lz, tiles_x,
                      tiles_y, nzc, a, E, op->n_fixed > 0 ? op->fixed : nullptr, X, Y, oa);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
+}
+
+// Butterfly form per precision and output mode: the carried (sum-factorised)
+// form pays off where the extra registers still leave two waves per SIMD --
+// the float64 plain apply (118 against 129 us at 120x60x60, 8 columns);
+// float32 and the fused residual / Chebyshev outputs keep the element-wise
+// butterflies.  SKT_MODAL_SF=0/1 forces one form everywhere.
+template <typename T, int MODE>
+static bool modal_sf() {
+  static const int forced = [] {
+    const char* v = std::getenv("SKT_MODAL_SF");
+    return (v && (v[0] == '0' || v[0] == '1') && v[1] == 0) ? v[0] - '0' : -1;
+  }();
+  if (forced >= 0) return forced == 1;
+  return sizeof(T) == 8 && MODE == OUT_APPLY;
 }
 
 template <typename T, int MODE>
@@ -393,16 +503,20 @@ static int dispatch_modal(const skt_operator* op, int k, int kc, const T* coef, 
   const int dstride = (kc == k) ? k : kc;
   // two columns per workgroup for elasticity: more columns cut LDS occupancy
   // below three workgroups per CU
+#define SKT_LM(NC_, KC_)                                                                      \
+  return modal_sf<T, MODE>()                                                                  \
+             ? launch_modal<T, NC_, KC_, MODE, 3>(op, k, lkc, gstride, dstride, coef, E, X, Y, \
+                                                  oa, s)                                       \
+             : launch_modal<T, NC_, KC_, MODE, 0>(op, k, lkc, gstride, dstride, coef, E, X, Y, \
+                                                  oa, s)
   if (op->ncomp == 3) {
-    if (kc % 2 == 0)
-      return launch_modal<T, 3, 2, MODE>(op, k, lkc, gstride, dstride, coef, E, X, Y, oa, s);
-    return launch_modal<T, 3, 1, MODE>(op, k, lkc, gstride, dstride, coef, E, X, Y, oa, s);
+    if (kc % 2 == 0) SKT_LM(3, 2);
+    SKT_LM(3, 1);
   }
-  if (kc % 4 == 0)
-    return launch_modal<T, 1, 4, MODE>(op, k, lkc, gstride, dstride, coef, E, X, Y, oa, s);
-  if (kc % 2 == 0)
-    return launch_modal<T, 1, 2, MODE>(op, k, lkc, gstride, dstride, coef, E, X, Y, oa, s);
-  return launch_modal<T, 1, 1, MODE>(op, k, lkc, gstride, dstride, coef, E, X, Y, oa, s);
+  if (kc % 4 == 0) SKT_LM(1, 4);
+  if (kc % 2 == 0) SKT_LM(1, 2);
+  SKT_LM(1, 1);
+#undef SKT_LM
 }
 
 template <typename T>
