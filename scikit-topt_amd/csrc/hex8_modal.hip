@@ -161,6 +161,7 @@ struct OutArgs {
   const T* dinv = nullptr;     // (ndof) Jacobi inverse
   const double* cheb = nullptr;  // (c1, c2) of this step, device
   int prev_zero = 0;           // the iterate before X is zero (first smoothing step)
+  const double* cheb0 = nullptr;  // OUT_CHEB0: coefficients of the implied first step
 };
 
 // SF (sum-factorised), bit 0: the x-y butterfly of each element's face on a
@@ -199,12 +200,14 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   const int64_t cbase = (int64_t)(c0 / lkc) * gstride + (c0 % lkc);
   X += cbase;
   Y += cbase;
+  constexpr bool kCheb = MODE == OUT_CHEB || MODE == OUT_CHEB0;
   const T* Bv = MODE != OUT_APPLY ? oa.B + cbase : nullptr;
-  T c1 = T(0), c2 = T(0);
-  if (MODE == OUT_CHEB) {
+  T c1 = T(0), c2 = T(0), s0 = T(0);
+  if (kCheb) {
     c1 = (T)oa.cheb[0];
     c2 = (T)oa.cheb[1];
   }
+  if (MODE == OUT_CHEB0) s0 = (T)oa.cheb0[1];
   // the previous iterate is read (own values only) from the output buffer
   // it is about to be replaced in
   const bool use_prev = MODE == OUT_CHEB && c1 != T(0) && !oa.prev_zero;
@@ -229,31 +232,49 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   const int64_t own_off = own_node * NC * (int64_t)dstride;
 
   int f_off[kPre], f_lds[kPre];
+  int f_dof[MODE == OUT_CHEB0 ? kPre : 1];
 #pragma unroll
   for (int r = 0; r < kPre; ++r) {
     const int i = tid + 256 * r;
     f_off[r] = -1;
     f_lds[r] = -1;
+    if constexpr (MODE == OUT_CHEB0) f_dof[r] = 0;
     if (i < kItems) {
       const int col = i % KC, comp = (i / KC) % NC, nl = i / (KC * NC);
       const int64_t gx = x0 - 1 + nl % kPW, gy = y0 - 1 + nl / kPW;
       f_lds[r] = (comp * KC + col) * kPS + node_slot(nl % kPW, nl / kPW);
-      if (gx >= 0 && gx < nnx && gy >= 0 && gy < nny)
+      if (gx >= 0 && gx < nnx && gy >= 0 && gy < nny) {
         f_off[r] = (int)(((gy * nnx + gx) * NC + comp) * dstride + col);
+        if constexpr (MODE == OUT_CHEB0) f_dof[r] = (int)((gy * nnx + gx) * NC + comp);
+      }
     }
   }
 
+  // prefetched plane values stay in flight until stash (the compute of the
+  // current layer runs meanwhile); OUT_CHEB0 also fetches dinv and forms the
+  // implied first iterate s0 dinv b only at stash time
   T pre[kPre];
+  T pred[MODE == OUT_CHEB0 ? kPre : 1];
   auto fetch = [&](int64_t gz) {
     const bool zin = gz >= 0 && gz < nnz;
     const T* Xp = X + (zin ? gz : 0) * plane_stride;
 #pragma unroll
     for (int r = 0; r < kPre; ++r) pre[r] = (zin && f_off[r] >= 0) ? Xp[f_off[r]] : T(0);
+    if constexpr (MODE == OUT_CHEB0) {
+      const T* Dp = oa.dinv + (zin ? gz : 0) * nnx * nny * NC;
+#pragma unroll
+      for (int r = 0; r < kPre; ++r) pred[r] = (zin && f_off[r] >= 0) ? Dp[f_dof[r]] : T(0);
+    }
   };
   auto stash = [&](T* dst) {
 #pragma unroll
-    for (int r = 0; r < kPre; ++r)
-      if (f_lds[r] >= 0) dst[f_lds[r]] = pre[r];
+    for (int r = 0; r < kPre; ++r) {
+      if (f_lds[r] < 0) continue;
+      if constexpr (MODE == OUT_CHEB0)
+        dst[f_lds[r]] = s0 * pred[r] * pre[r];
+      else
+        dst[f_lds[r]] = pre[r];
+    }
   };
 
   T partial[KC][NC];
@@ -305,7 +326,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
       fix[d] = fixed && out && fixed[(ez * nnx * nny + own_node) * NC + d];
-      if constexpr (MODE == OUT_CHEB) oi[d] = out ? oa.dinv[(ez * nnx * nny + own_node) * NC + d] : T(0);
+      if constexpr (kCheb) oi[d] = out ? oa.dinv[(ez * nnx * nny + own_node) * NC + d] : T(0);
 #pragma unroll
       for (int c = 0; c < KC; ++c) {
         if constexpr (MODE != OUT_APPLY) ob[c][d] = out ? Bv[po + d * dstride + c] : T(0);
@@ -426,7 +447,9 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
             } else {
               // Chebyshev step with the direction recovered from the two
               // iterates: x_out = x + c1 (x - x_prev) + c2 dinv (b - K x)
-              const T dn = c1 * (xo[col][d] - op[col][d]) + c2 * oi[d] * (ob[col][d] - y);
+              // (x_prev = 0 for the implied first step of OUT_CHEB0)
+              const T xp = MODE == OUT_CHEB ? op[col][d] : T(0);
+              const T dn = c1 * (xo[col][d] - xp) + c2 * oi[d] * (ob[col][d] - y);
               Yp[o] = xo[col][d] + dn;
             }
           }
@@ -534,6 +557,19 @@ int hex_resid_modal(const skt_operator* op, int k, int kc, const T* coef, const 
 }
 
 template <typename T>
+int hex_cheb0_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* B,
+                    T* Xout, const T* dinv, const double* cheb0, const double* cheb,
+                    hipStream_t s) {
+  OutArgs<T> oa;
+  oa.B = B;
+  oa.dinv = dinv;
+  oa.cheb = cheb;
+  oa.cheb0 = cheb0;
+  oa.prev_zero = 1;
+  return dispatch_modal<T, OUT_CHEB0>(op, k, kc, coef, E, B, Xout, oa, s);
+}
+
+template <typename T>
 int hex_cheb_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
                    T* Xout, const T* B, const T* dinv, const double* cheb, int prev_zero,
                    hipStream_t s) {
@@ -554,6 +590,12 @@ template int hex_resid_modal<float>(const skt_operator*, int, int, const float*,
 template int hex_resid_modal<double>(const skt_operator*, int, int, const double*,
                                      const double*, const double*, const double*, double*,
                                      hipStream_t);
+template int hex_cheb0_modal<float>(const skt_operator*, int, int, const float*, const float*,
+                                    const float*, float*, const float*, const double*,
+                                    const double*, hipStream_t);
+template int hex_cheb0_modal<double>(const skt_operator*, int, int, const double*,
+                                     const double*, const double*, double*, const double*,
+                                     const double*, const double*, hipStream_t);
 template int hex_cheb_modal<float>(const skt_operator*, int, int, const float*, const float*,
                                    const float*, float*, const float*, const float*,
                                    const double*, int, hipStream_t);

@@ -913,10 +913,13 @@ void mg_free(MGData* mg) {
   }
   (void)hipFree(mg->Ad); (void)hipFree(mg->Ad2); (void)hipFree(mg->Ainv);
   (void)hipFree(mg->lam); (void)hipFree(mg->coef); (void)hipFree(mg->part); (void)hipFree(mg->pv);
+  (void)hipFree(mg->pvw);
   (void)hipFree(mg->K0b);
   mg->Ad = mg->Ad2 = nullptr; mg->Ainv = nullptr; mg->lam = mg->coef = mg->part = nullptr;
   mg->K0b = nullptr;
   mg->pv = nullptr;
+  mg->pvw = nullptr;
+  mg->warm = 0;
   mg->lv.clear();
   mg->dtype = -1;
 }
@@ -959,6 +962,10 @@ int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype,
   SKT_HIP(hipMalloc(&mg->coef, sizeof(double) * (L + 1) * 2 * std::max(cfg.degree, 1)));
   SKT_HIP(hipMalloc(&mg->part, sizeof(double) * 1024));
   SKT_HIP(hipMalloc(&mg->pv, ts * 2 * lv[0].ndof));
+  int64_t npw = 0;
+  for (int l = 0; l < L; ++l) npw += lv[l].ndof;
+  SKT_HIP(hipMalloc(&mg->pvw, ts * npw));
+  mg->warm = 0;
   // unit element matrix in corner-bit order
   std::vector<double> K0b((size_t)ND * ND);
   for (int A = 0; A < 8; ++A)
@@ -1096,19 +1103,25 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   }
   // smoother bounds: power iteration on D^-1 A (one column) per level
   double* inv = mg->lam + (L + 1);  // scratch scalar slot
+  int64_t off = 0;
   for (int l = 0; l < L; ++l) {
     const MGLevel& v = mg->lv[l];
     const int64_t n = v.ndof;
-    T* pv0 = (T*)mg->pv;
-    T* pv1 = pv0 + mg->lv[0].ndof;
+    T* pv0 = (T*)mg->pvw + off;      // this level's vector, kept for the next set-up
+    T* pv1 = (T*)mg->pv;
+    off += n;
     const T* dinv = l == 0 ? (const T*)dinv0 : (const T*)v.dinv;
     const unsigned nb = std::min<unsigned>(grid_for(n), 1024u);
-    hipLaunchKernelGGL(k_pow_init<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, dinv, pv0);
-    hipLaunchKernelGGL(k_pow_norm<T>, dim3(nb), dim3(kBlock), 0, s, n, dinv, pv0, 0, mg->part);
-    hipLaunchKernelGGL(k_pow_finish, dim3(1), dim3(kBlock), 0, s, (int)nb, mg->part,
-                       (double*)nullptr, inv);
-    hipLaunchKernelGGL(k_pow_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, inv, pv0, pv0);
-    for (int it = 0; it < mg->power_iters; ++it) {
+    if (!mg->warm) {
+      hipLaunchKernelGGL(k_pow_init<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, dinv, pv0);
+      hipLaunchKernelGGL(k_pow_norm<T>, dim3(nb), dim3(kBlock), 0, s, n, dinv, pv0, 0, mg->part);
+      hipLaunchKernelGGL(k_pow_finish, dim3(1), dim3(kBlock), 0, s, (int)nb, mg->part,
+                         (double*)nullptr, inv);
+      hipLaunchKernelGGL(k_pow_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, inv, pv0,
+                         pv0);
+    }
+    const int steps = mg->warm ? mg->power_iters_warm : mg->power_iters;
+    for (int it = 0; it < steps; ++it) {
       int stt = level_apply<T>(mg, op, E, l, 1, 1, pv0, pv1, s);
       if (stt) return stt;
       hipLaunchKernelGGL(k_pow_norm<T>, dim3(nb), dim3(kBlock), 0, s, n, dinv, pv1, 1, mg->part);
@@ -1119,6 +1132,7 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
     }
     SKT_LAUNCH_CHECK();
   }
+  mg->warm = 1;
   hipLaunchKernelGGL(k_cheb_coef, dim3(1), dim3(64), 0, s, L, mg->degree, mg->ratio, mg->lam,
                      mg->coef);
   SKT_LAUNCH_CHECK();
@@ -1180,11 +1194,19 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     MGLevel& v = mg->lv[0];
     const int64_t total = v.ndof * k;
     const double* cf = mg->coef;
-    hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
-                       v.ndof, KC, cf, r, (const T*)dinv0, cur);
-    for (int st = 1; st < deg; ++st) {
+    if (deg >= 2) {
+      // steps 0 and 1 in one pass (step 0's iterate is formed on the fly)
+      if ((stt = hex_cheb0_modal<T>(op, k, KC, coef0, E0, r, oth, (const T*)dinv0, cf, cf + 2,
+                                    s)))
+        return stt;
+      std::swap(cur, oth);
+    } else {
+      hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
+                         v.ndof, KC, cf, r, (const T*)dinv0, cur);
+    }
+    for (int st = 2; st < deg; ++st) {
       if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, (const T*)dinv0,
-                                   cf + 2 * st, st == 1 ? 1 : 0, s)))
+                                   cf + 2 * st, 0, s)))
         return stt;
       std::swap(cur, oth);
     }

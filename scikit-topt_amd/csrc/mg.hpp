@@ -10,7 +10,10 @@
 // (structure of arrays, [27*NC*NC][nodes]); the coarsest level is inverted
 // densely.  Smoothing is Chebyshev-accelerated Jacobi with the spectral bound
 // of D^-1 A estimated on the device by power iteration, so a V-cycle is a
-// fixed symmetric linear operator and never waits on the host.
+// fixed symmetric linear operator and never waits on the host.  Set-ups for
+// new moduli restart the power iteration from the previous set-up's vector
+// (the moduli of successive optimisation steps differ little), with fewer
+// steps.
 #pragma once
 
 #include <vector>
@@ -44,10 +47,14 @@ struct MGData {
   double* coef = nullptr;             // per level Chebyshev (c1, c2) per step
   double* part = nullptr;             // power-iteration partials
   double* K0b = nullptr;              // unit element matrix, corner-bit order
-  void* pv = nullptr;                 // power-iteration vectors (2 * ndof0) in T
+  void* pv = nullptr;                 // power-iteration scratch (2 * ndof0) in T
+  void* pvw = nullptr;                // per level (levels 0..L-1, concatenated) the last
+                                      // normalised power-iteration vector in T
+  int warm = 0;                       // pvw holds vectors from an earlier set-up
   int degree = 2;
   double ratio = 10.0;
-  int power_iters = 20;
+  int power_iters = 20;               // steps from the hashed start vector
+  int power_iters_warm = 12;          // steps from the previous set-up's vector
 };
 
 struct MGConfig {

@@ -49,7 +49,10 @@ int extra_diag_add(const skt_operator* op, int dtype, void* diag, hipStream_t s)
 // Shared tail of the hex8 constructors (operators.hip).
 int hex8_finish(skt_operator* op, const uint8_t* dof_fixed, skt_operator** out);
 // Output modes of the structured hex8 modal kernel.
-enum { OUT_APPLY = 0, OUT_RESID = 1, OUT_CHEB = 2 };
+// OUT_CHEB0: the first two Chebyshev steps from a zero iterate in one pass:
+// X is the right-hand side b, the first iterate x1 = s0 dinv b is formed on
+// the fly (s0 = cheb0[1]) and x_out = x1 + c1 x1 + c2 dinv (b - K x1).
+enum { OUT_APPLY = 0, OUT_RESID = 1, OUT_CHEB = 2, OUT_CHEB0 = 3 };
 template <typename T>
 int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
                     T* Y, hipStream_t s);
@@ -57,6 +60,11 @@ int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const 
 template <typename T>
 int hex_resid_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E,
                     const T* X, const T* B, T* R, hipStream_t s);
+// x_out = x1 + c1 x1 + c2 dinv (b - K x1) with x1 = cheb0[1] dinv b (OUT_CHEB0)
+template <typename T>
+int hex_cheb0_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* B,
+                    T* Xout, const T* dinv, const double* cheb0, const double* cheb,
+                    hipStream_t s);
 // Chebyshev-Jacobi step with the direction taken from two iterates:
 // Xout = X + c1 (X - Xprev) + c2 dinv (B - K X), (c1, c2) = cheb[0..1] on the
 // device.  Xprev is read from Xout before it is overwritten (zero when
