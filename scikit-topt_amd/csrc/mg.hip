@@ -36,7 +36,15 @@ bool mg_plan(const skt_operator* op, const MGConfig& cfg, std::vector<MGLevel>* 
   out->push_back(l0);
   while (out->back().ndof > cfg.max_coarse) {
     const MGLevel& f = out->back();
-    const int fac[3] = {axis_factor(f.nx), axis_factor(f.ny), axis_factor(f.nz)};
+    int fac[3] = {axis_factor(f.nx), axis_factor(f.ny), axis_factor(f.nz)};
+    // an axis is not collapsed to a single element while a longer axis
+    // (more than 5 elements) still coarsens: e.g. 15x5x5 -> 5x5x5, not 5x1x1
+    const int64_t n3[3] = {f.nx, f.ny, f.nz};
+    bool longer = false;
+    for (int a = 0; a < 3; ++a) longer = longer || (n3[a] > 5 && fac[a] > 1);
+    if (longer)
+      for (int a = 0; a < 3; ++a)
+        if (fac[a] > 1 && n3[a] / fac[a] == 1) fac[a] = 1;
     if (fac[0] == 1 && fac[1] == 1 && fac[2] == 1) return false;
     MGLevel c;
     c.nx = f.nx / fac[0]; c.ny = f.ny / fac[1]; c.nz = f.nz / fac[2];
