@@ -59,7 +59,7 @@ def make_optimizer(wl, precision, rtol, iters, dst):
     return opt
 
 
-APPLY_KERNEL = "skt::k_hex_apply_modal<float, 3, 2, 0, 0>"   # plain apply (output mode 0)
+APPLY_KERNEL = "skt::k_hex_apply_modal<float, 3, 2, 0, 0, true>"  # plain apply + fused p.Kp
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hex8_modal_apply_f32_120x60x60_8col_pmc.txt")
 COPY_DOUBLES = 1 << 25            # 256 MiB read + 256 MiB written per copy
 

@@ -162,6 +162,10 @@ struct OutArgs {
   const double* cheb = nullptr;  // (c1, c2) of this step, device
   int prev_zero = 0;           // the iterate before X is zero (first smoothing step)
   const double* cheb0 = nullptr;  // OUT_CHEB0: coefficients of the implied first step
+  // DOT kernels: per-workgroup partial sums of the fused dot product,
+  // dot[c * dot_nb + bx] for column c and workgroup bx of its column block
+  double* dot = nullptr;
+  int dot_nb = 0;
 };
 
 // SF (sum-factorised), bit 0: the x-y butterfly of each element's face on a
@@ -173,8 +177,20 @@ struct OutArgs {
 // space and finishes each node plane with one 2-D butterfly, leaving half
 // the pair traffic.  The carries cost registers, so which form is fastest
 // depends on the precision and output mode.
-template <typename T, int NC, int KC, int MODE, int SF>
-__global__ void __launch_bounds__(256)
+//
+// DOT: the kernel also leaves the per-column partial sums of x . y
+// (OUT_APPLY: the CG's p . K p) or b . x_out (OUT_CHEB: r . z of a V-cycle's
+// last smoothing step) over its owned nodes, reduced in a fixed order.
+// Waves per SIMD the compiler must leave room for: the LDS of a float32
+// elasticity workgroup allows four per CU, and the fused r.z of the float32
+// smoothing step would otherwise take it just past 128 VGPRs (three waves).
+template <typename T, int MODE, int SF, bool DOT>
+constexpr int modal_min_waves() {
+  return (sizeof(T) == 4 && SF == 0 && MODE == OUT_CHEB && DOT) ? 4 : 1;
+}
+
+template <typename T, int NC, int KC, int MODE, int SF, bool DOT>
+__global__ void __launch_bounds__(256, (modal_min_waves<T, MODE, SF, DOT>()))
 k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, int dstride,
                   int lz, int tiles_x, int tiles_y, int n_zchunks, ModalArgs<T> coef,
                   const T* __restrict__ E, const uint8_t* __restrict__ fixed,
@@ -282,6 +298,9 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   for (int c = 0; c < KC; ++c)
 #pragma unroll
     for (int d = 0; d < NC; ++d) partial[c][d] = T(0);
+  double dacc[KC];
+#pragma unroll
+  for (int c = 0; c < KC; ++c) dacc[c] = 0.0;
 
   int bot = 0, top = 1;
   fetch(z0 - 1);
@@ -442,6 +461,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
             const int64_t o = d * dstride + col;
             if constexpr (MODE == OUT_APPLY) {
               Yp[o] = y;
+              if constexpr (DOT) dacc[col] += (double)xo[col][d] * (double)y;
             } else if constexpr (MODE == OUT_RESID) {
               Yp[o] = ob[col][d] - y;
             } else {
@@ -450,7 +470,9 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
               // (x_prev = 0 for the implied first step of OUT_CHEB0)
               const T xp = MODE == OUT_CHEB ? op[col][d] : T(0);
               const T dn = c1 * (xo[col][d] - xp) + c2 * oi[d] * (ob[col][d] - y);
-              Yp[o] = xo[col][d] + dn;
+              const T xn = xo[col][d] + dn;
+              Yp[o] = xn;
+              if constexpr (DOT) dacc[col] += (double)ob[col][d] * (double)xn;
             }
           }
         }
@@ -462,11 +484,68 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
     bot = top;
     top = t;
   }
+  if constexpr (DOT) {
+    // wavefront sums, then the four wavefronts in order; the pair area is
+    // free once every node thread has read it
+    double* red = reinterpret_cast<double*>(pairs);
+    const int lane = tid & (kWave - 1), w = tid / kWave;
+#pragma unroll
+    for (int c = 0; c < KC; ++c) dacc[c] = wave_sum(dacc[c]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < KC; ++c) red[w * KC + c] = dacc[c];
+    }
+    __syncthreads();
+    if (tid < KC) {
+      double v = 0.0;
+      for (int i = 0; i < 256 / kWave; ++i) v += red[i * KC + tid];
+      oa.dot[(int64_t)(c0 + tid) * oa.dot_nb + (int)(blockIdx.x % (unsigned)oa.dot_nb)] = v;
+    }
+  }
+}
+
+// Launch geometry: (tiles_x, tiles_y) node tiles per plane, nzc z chunks of
+// lz node planes, one workgroup per (tile, chunk, column block) with the
+// column block slowest-varying.
+struct ModalGrid {
+  int tiles_x, tiles_y, nzc;
+  int64_t lz, nblocks;
+};
+
+static ModalGrid modal_grid(const skt_operator* op, int k, int kc_block) {
+  ModalGrid g;
+  const int64_t nnx = op->nx + 1, nny = op->ny + 1, nnz = op->nz + 1;
+  // the last tile of an axis takes up to kTE nodes, so ceil((nn - 1) / kTN)
+  g.tiles_x = (int)((nnx - 1 + kTN - 1) / kTN);
+  g.tiles_y = (int)((nny - 1 + kTN - 1) / kTN);
+  const int cblocks = k / kc_block;
+  // z chunk length: aim for >= 4 workgroups per CU (256 CUs), but keep the
+  // one-layer warm-up of each chunk below ~1/6 of its work
+  g.lz = nnz;
+  const int64_t per = (int64_t)g.tiles_x * g.tiles_y * cblocks;
+  while (g.lz > 6 && per * ((nnz + g.lz - 1) / g.lz) < 1024) g.lz = (g.lz + 1) / 2;
+  g.nzc = (int)((nnz + g.lz - 1) / g.lz);
+  g.nblocks = per * g.nzc;
+  return g;
+}
+
+// Columns per workgroup for a layout with column group kc: two for
+// elasticity (more columns cut LDS occupancy below three workgroups per CU),
+// up to four for heat.
+static int modal_kc(const skt_operator* op, int kc) {
+  if (op->ncomp == 3) return kc % 2 == 0 ? 2 : 1;
+  return kc % 4 == 0 ? 4 : (kc % 2 == 0 ? 2 : 1);
+}
+
+int hex_modal_dot_slots(const skt_operator* op, int k, int kc) {
+  const ModalGrid g = modal_grid(op, k, modal_kc(op, kc));
+  return g.tiles_x * g.tiles_y * g.nzc;
 }
 
 // Launch helper for structured grids whose element matrix has the modal
 // pattern.  k columns in the layout described above.
-template <typename T, int NC, int KC, int MODE, int SF>
+template <typename T, int NC, int KC, int MODE, int SF, bool DOT>
 static int launch_modal(const skt_operator* op, int k, int lkc, int64_t gstride, int dstride,
                         const T* coef, const T* E, const T* X, T* Y, const OutArgs<T>& oa,
                         hipStream_t s) {
@@ -475,29 +554,25 @@ static int launch_modal(const skt_operator* op, int k, int lkc, int64_t gstride,
   SKT_CHECK(k % KC == 0 && lkc % KC == 0, "column blocking does not divide the layout");
   ModalArgs<T> a;
   for (int i = 0; i < 40; ++i) a.c[i] = coef[i];
-  const int64_t nnx = op->nx + 1, nny = op->ny + 1, nnz = op->nz + 1;
-  // the last tile of an axis takes up to kTE nodes, so ceil((nn - 1) / kTN)
-  const int tiles_x = (int)((nnx - 1 + kTN - 1) / kTN);
-  const int tiles_y = (int)((nny - 1 + kTN - 1) / kTN);
-  const int cblocks = k / KC;
-  // z chunk length: aim for >= 4 workgroups per CU (256 CUs), but keep the
-  // one-layer warm-up of each chunk below ~1/6 of its work
-  int64_t lz = nnz;
-  const int64_t per = (int64_t)tiles_x * tiles_y * cblocks;
-  while (lz > 6 && per * ((nnz + lz - 1) / lz) < 1024) lz = (lz + 1) / 2;
-  const int nzc = (int)((nnz + lz - 1) / lz);
-  const int64_t nblocks = per * nzc;
+  const ModalGrid gr = modal_grid(op, k, KC);
+  if (DOT) {
+    // column blocks are the slowest-varying workgroup index, so workgroup
+    // b % (nblocks / cblocks) is the partial's slot within its column
+    SKT_CHECK(oa.dot && oa.dot_nb == gr.tiles_x * gr.tiles_y * gr.nzc,
+              "internal: dot partials do not match the launch");
+  }
   const size_t shmem = sizeof(T) * ((size_t)2 * soa_stride(kPN, NC * KC) * NC * KC +
                                     (size_t)((SF & 2) ? 2 : 4) * NC * KC * 256);
   static bool attr_set = false;
   if (!attr_set) {
-    SKT_HIP(hipFuncSetAttribute((const void*)k_hex_apply_modal<T, NC, KC, MODE, SF>,
+    SKT_HIP(hipFuncSetAttribute((const void*)k_hex_apply_modal<T, NC, KC, MODE, SF, DOT>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_hex_apply_modal<T, NC, KC, MODE, SF>), dim3((unsigned)nblocks), dim3(256),
-                     shmem, s, op->nx, op->ny, op->nz, lkc, gstride, dstride, (int)lz, tiles_x,
-                     tiles_y, nzc, a, E, op->n_fixed > 0 ? op->fixed : nullptr, X, Y, oa);
+  hipLaunchKernelGGL((k_hex_apply_modal<T, NC, KC, MODE, SF, DOT>), dim3((unsigned)gr.nblocks),
+                     dim3(256), shmem, s, op->nx, op->ny, op->nz, lkc, gstride, dstride,
+                     (int)gr.lz, gr.tiles_x, gr.tiles_y, gr.nzc, a, E,
+                     op->n_fixed > 0 ? op->fixed : nullptr, X, Y, oa);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }
@@ -517,6 +592,29 @@ static bool modal_sf() {
   return sizeof(T) == 8 && MODE == OUT_APPLY;
 }
 
+// Butterfly form and fused dot product of one (NC, KC) instance.
+template <typename T, int MODE, int NC, int KC>
+static int launch_sel(const skt_operator* op, int k, int lkc, int64_t gstride, int dstride,
+                      const T* coef, const T* E, const T* X, T* Y, const OutArgs<T>& oa,
+                      hipStream_t s) {
+  const bool sf = modal_sf<T, MODE>();
+  if (oa.dot) {
+    if constexpr (MODE == OUT_APPLY || MODE == OUT_CHEB) {
+      return sf ? launch_modal<T, NC, KC, MODE, 3, true>(op, k, lkc, gstride, dstride, coef, E,
+                                                         X, Y, oa, s)
+                : launch_modal<T, NC, KC, MODE, 0, true>(op, k, lkc, gstride, dstride, coef, E,
+                                                         X, Y, oa, s);
+    } else {
+      set_error("internal: no fused dot product for this output mode");
+      return SKT_EINVAL;
+    }
+  }
+  return sf ? launch_modal<T, NC, KC, MODE, 3, false>(op, k, lkc, gstride, dstride, coef, E, X,
+                                                      Y, oa, s)
+            : launch_modal<T, NC, KC, MODE, 0, false>(op, k, lkc, gstride, dstride, coef, E, X,
+                                                      Y, oa, s);
+}
+
 template <typename T, int MODE>
 static int dispatch_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E,
                           const T* X, T* Y, const OutArgs<T>& oa, hipStream_t s) {
@@ -524,28 +622,28 @@ static int dispatch_modal(const skt_operator* op, int k, int kc, const T* coef, 
   const int lkc = kc;
   const int64_t gstride = (kc == k) ? 0 : op->ndof * kc;
   const int dstride = (kc == k) ? k : kc;
-  // two columns per workgroup for elasticity: more columns cut LDS occupancy
-  // below three workgroups per CU
-#define SKT_LM(NC_, KC_)                                                                      \
-  return modal_sf<T, MODE>()                                                                  \
-             ? launch_modal<T, NC_, KC_, MODE, 3>(op, k, lkc, gstride, dstride, coef, E, X, Y, \
-                                                  oa, s)                                       \
-             : launch_modal<T, NC_, KC_, MODE, 0>(op, k, lkc, gstride, dstride, coef, E, X, Y, \
-                                                  oa, s)
+#define SKT_LM(NC_, KC_) \
+  return launch_sel<T, MODE, NC_, KC_>(op, k, lkc, gstride, dstride, coef, E, X, Y, oa, s)
+  const int kb = modal_kc(op, kc);
   if (op->ncomp == 3) {
-    if (kc % 2 == 0) SKT_LM(3, 2);
+    if (kb == 2) SKT_LM(3, 2);
     SKT_LM(3, 1);
   }
-  if (kc % 4 == 0) SKT_LM(1, 4);
-  if (kc % 2 == 0) SKT_LM(1, 2);
+  if (kb == 4) SKT_LM(1, 4);
+  if (kb == 2) SKT_LM(1, 2);
   SKT_LM(1, 1);
 #undef SKT_LM
 }
 
 template <typename T>
 int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
-                    T* Y, hipStream_t s) {
-  return dispatch_modal<T, OUT_APPLY>(op, k, kc, coef, E, X, Y, OutArgs<T>(), s);
+                    T* Y, hipStream_t s, double* dot) {
+  OutArgs<T> oa;
+  if (dot) {
+    oa.dot = dot;
+    oa.dot_nb = hex_modal_dot_slots(op, k, kc);
+  }
+  return dispatch_modal<T, OUT_APPLY>(op, k, kc, coef, E, X, Y, oa, s);
 }
 
 template <typename T>
@@ -572,19 +670,23 @@ int hex_cheb0_modal(const skt_operator* op, int k, int kc, const T* coef, const 
 template <typename T>
 int hex_cheb_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
                    T* Xout, const T* B, const T* dinv, const double* cheb, int prev_zero,
-                   hipStream_t s) {
+                   hipStream_t s, double* dot) {
   OutArgs<T> oa;
   oa.B = B;
   oa.dinv = dinv;
   oa.cheb = cheb;
   oa.prev_zero = prev_zero;
+  if (dot) {
+    oa.dot = dot;
+    oa.dot_nb = hex_modal_dot_slots(op, k, kc);
+  }
   return dispatch_modal<T, OUT_CHEB>(op, k, kc, coef, E, X, Xout, oa, s);
 }
 
 template int hex_apply_modal<float>(const skt_operator*, int, int, const float*, const float*,
-                                    const float*, float*, hipStream_t);
+                                    const float*, float*, hipStream_t, double*);
 template int hex_apply_modal<double>(const skt_operator*, int, int, const double*, const double*,
-                                     const double*, double*, hipStream_t);
+                                     const double*, double*, hipStream_t, double*);
 template int hex_resid_modal<float>(const skt_operator*, int, int, const float*, const float*,
                                     const float*, const float*, float*, hipStream_t);
 template int hex_resid_modal<double>(const skt_operator*, int, int, const double*,
@@ -598,9 +700,9 @@ template int hex_cheb0_modal<double>(const skt_operator*, int, int, const double
                                      const double*, const double*, hipStream_t);
 template int hex_cheb_modal<float>(const skt_operator*, int, int, const float*, const float*,
                                    const float*, float*, const float*, const float*,
-                                   const double*, int, hipStream_t);
+                                   const double*, int, hipStream_t, double*);
 template int hex_cheb_modal<double>(const skt_operator*, int, int, const double*, const double*,
                                     const double*, double*, const double*, const double*,
-                                    const double*, int, hipStream_t);
+                                    const double*, int, hipStream_t, double*);
 
 }  // namespace skt

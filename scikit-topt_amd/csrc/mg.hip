@@ -246,13 +246,19 @@ constexpr int kTileRC = 32;
 
 // invert the pivot block (padded with identity), write it to Binv
 // (kGJ x kGJ) and back into A.  Each thread keeps a 4 x 4 block of the
-// matrix in registers; per step the owners of the pivot row and column
-// publish them through LDS.
+// matrix in registers; per pivot k the owners of row k and column k publish
+// them through LDS (double-buffered, so one barrier per pivot), and every
+// thread applies a_ij -= a_ik a_kj / a_kk with column k zeroed first and row
+// k replaced afterwards:
+//   row k:            a_kj / a_kk (j != k),  1 / a_kk (j = k)
+//   column k, i != k: -a_ik / a_kk
+// The pivot index inside a thread's 4 x 4 block is a compile-time constant
+// of the unrolled inner loop, so there is no per-element branching.
 __global__ void __launch_bounds__(256) k_gj_block_inv(int64_t n, int64_t b0, int nb,
                                                        double* __restrict__ A,
                                                        double* __restrict__ Binv) {
   static_assert(kGJ == 64, "4 x 4 register blocks of a 64 x 64 matrix");
-  __shared__ double prow[kGJ], pcol[kGJ];
+  __shared__ double prow[2][kGJ], pcol[2][kGJ];
   const int tid = threadIdx.x;
   const int tr = tid / 16, tc = tid % 16;  // rows 4 tr.., columns 4 tc..
   double m[4][4];
@@ -263,36 +269,40 @@ __global__ void __launch_bounds__(256) k_gj_block_inv(int64_t n, int64_t b0, int
       const int r = 4 * tr + i, c = 4 * tc + j;
       m[i][j] = (r < nb && c < nb) ? A[(b0 + r) * n + b0 + c] : (r == c ? 1.0 : 0.0);
     }
-  for (int k = 0; k < kGJ; ++k) {
-    const int kb = k >> 2, ki = k & 3;
-    if (tr == kb) {
+  for (int kb = 0; kb < kGJ / 4; ++kb) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) prow[4 * tc + j] = m[ki][j];
-    }
-    if (tc == kb) {
+    for (int ki = 0; ki < 4; ++ki) {
+      const int k = 4 * kb + ki, buf = ki & 1;
+      const bool own_row = tr == kb, own_col = tc == kb;
+      if (own_row) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) pcol[4 * tr + i] = m[i][ki];
-    }
-    __syncthreads();
-    const double ip = 1.0 / pcol[k];
-    double rk[4], ck[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) rk[j] = prow[4 * tc + j] * ip;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) ck[i] = pcol[4 * tr + i];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int r = 4 * tr + i, c = 4 * tc + j;
-        double v;
-        if (r == k && c == k) v = ip;
-        else if (r == k) v = rk[j];
-        else if (c == k) v = -ck[i] * ip;
-        else v = m[i][j] - ck[i] * rk[j];
-        m[i][j] = v;
+        for (int j = 0; j < 4; ++j) prow[buf][4 * tc + j] = m[ki][j];
       }
-    __syncthreads();
+      if (own_col) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) pcol[buf][4 * tr + i] = m[i][ki];
+      }
+      __syncthreads();
+      const double ip = 1.0 / pcol[buf][k];
+      double rk[4], ck[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) rk[j] = prow[buf][4 * tc + j] * ip;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ck[i] = pcol[buf][4 * tr + i];
+      if (own_col) {
+        rk[ki] = ip;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m[i][ki] = 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m[i][j] -= ck[i] * rk[j];
+      if (own_row) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m[ki][j] = rk[j];
+      }
+    }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -305,10 +315,13 @@ __global__ void __launch_bounds__(256) k_gj_block_inv(int64_t n, int64_t b0, int
 }
 
 // A_BR <- Binv A_BR on a strip of kTileRC columns (columns inside the block
-// are left alone)
+// are left alone).  Binv is the inverse padded with identity and the strip
+// rows past nb are zero, so the products run over all kGJ rows; each thread
+// keeps 8 outputs (one column, rows tid / 32 + 8 m) in flight.
 __global__ void __launch_bounds__(256) k_gj_rows(int64_t n, int64_t b0, int nb,
                                                   const double* __restrict__ Binv,
                                                   double* __restrict__ A) {
+  static_assert(kGJ * kTileRC == 8 * 256, "8 outputs per thread");
   __shared__ double sB[kGJ][kGJ + 1];
   __shared__ double sX[kGJ][kTileRC + 1];
   const int tid = threadIdx.x;
@@ -319,13 +332,20 @@ __global__ void __launch_bounds__(256) k_gj_rows(int64_t n, int64_t b0, int nb,
     sX[r][c] = (r < nb && c0 + c < n) ? A[(b0 + r) * n + c0 + c] : 0.0;
   }
   __syncthreads();
-  for (int i = tid; i < kGJ * kTileRC; i += 256) {
-    const int r = i / kTileRC, c = i % kTileRC;
-    const int64_t col = c0 + c;
-    if (r >= nb || col >= n || (col >= b0 && col < b0 + nb)) continue;
-    double v = 0.0;
-    for (int q = 0; q < nb; ++q) v += sB[r][q] * sX[q][c];
-    A[(b0 + r) * n + col] = v;
+  const int c = tid % kTileRC, rb = tid / kTileRC;
+  double acc[8] = {};
+#pragma unroll 8
+  for (int q = 0; q < kGJ; ++q) {
+    const double x = sX[q][c];
+#pragma unroll
+    for (int mm = 0; mm < 8; ++mm) acc[mm] += sB[rb + 8 * mm][q] * x;
+  }
+  const int64_t col = c0 + c;
+  if (col >= n || (col >= b0 && col < b0 + nb)) return;
+#pragma unroll
+  for (int mm = 0; mm < 8; ++mm) {
+    const int r = rb + 8 * mm;
+    if (r < nb) A[(b0 + r) * n + col] = acc[mm];
   }
 }
 
@@ -377,11 +397,13 @@ __global__ void __launch_bounds__(256) k_gj_update(int64_t n, int64_t b0, int nb
   }
 }
 
-// A_RB <- -A_RB Binv on a strip of kTileRC rows (rows inside the block are left
-// alone)
+// A_RB <- -A_RB Binv on a strip of kTileRC rows (rows inside the block are
+// left alone); each thread keeps 8 outputs (one column, rows tid / 64 + 4 m)
+// in flight, the products over all kGJ columns as in k_gj_rows.
 __global__ void __launch_bounds__(256) k_gj_cols(int64_t n, int64_t b0, int nb,
                                                   const double* __restrict__ Binv,
                                                   double* __restrict__ A) {
+  static_assert(kGJ * kTileRC == 8 * 256, "8 outputs per thread");
   __shared__ double sB[kGJ][kGJ + 1];
   __shared__ double sX[kTileRC][kGJ + 1];
   const int tid = threadIdx.x;
@@ -392,13 +414,20 @@ __global__ void __launch_bounds__(256) k_gj_cols(int64_t n, int64_t b0, int nb,
     sX[r][q] = (r0 + r < n && q < nb) ? A[(r0 + r) * n + b0 + q] : 0.0;
   }
   __syncthreads();
-  for (int i = tid; i < kTileRC * kGJ; i += 256) {
-    const int r = i / kGJ, c = i % kGJ;
-    const int64_t row = r0 + r;
-    if (c >= nb || row >= n || (row >= b0 && row < b0 + nb)) continue;
-    double v = 0.0;
-    for (int q = 0; q < nb; ++q) v += sX[r][q] * sB[q][c];
-    A[row * n + b0 + c] = -v;
+  const int c = tid % kGJ, rb = tid / kGJ;
+  double acc[8] = {};
+#pragma unroll 8
+  for (int q = 0; q < kGJ; ++q) {
+    const double b = sB[q][c];
+#pragma unroll
+    for (int mm = 0; mm < 8; ++mm) acc[mm] += sX[rb + 4 * mm][q] * b;
+  }
+  if (c >= nb) return;
+#pragma unroll
+  for (int mm = 0; mm < 8; ++mm) {
+    const int64_t row = r0 + rb + 4 * mm;
+    if (row >= n || (row >= b0 && row < b0 + nb)) continue;
+    A[row * n + b0 + c] = -acc[mm];
   }
 }
 
@@ -1187,7 +1216,8 @@ const float* modal_coef<float>(const skt_operator* op) { return op->modal_f; }
 // cycle ends in z.
 template <typename T, int NC, int KC>
 static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const void* dinv0, int k,
-                    const T* r, T* z, T* w0, hipStream_t s) {
+                    const T* r, T* z, T* w0, hipStream_t s, double* dot, int max_slots,
+                    int* slots) {
   const int L = (int)mg->lv.size() - 1;
   const int groups = k / KC;
   const int deg = mg->degree;
@@ -1283,13 +1313,17 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     MGLevel& v = mg->lv[0];
     const double* cf = mg->coef;
     launch_prolong<T, NC, KC>(v, mg->lv[1], groups, (const T*)it[1], fixed0, cur, s);
+    // the last step can leave the partials of r . z
+    const int ns = hex_modal_dot_slots(op, k, KC);
+    double* last_dot = (dot && slots && ns <= max_slots) ? dot : nullptr;
     for (int st = 0; st < deg; ++st) {
       // step 0 has c1 = 0 and needs no previous iterate
       if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, (const T*)dinv0,
-                                   cf + 2 * st, 0, s)))
+                                   cf + 2 * st, 0, s, st == deg - 1 ? last_dot : nullptr)))
         return stt;
       std::swap(cur, oth);
     }
+    if (slots) *slots = (last_dot && deg >= 1) ? ns : 0;
     SKT_LAUNCH_CHECK();
   }
   if (cur != z) {
@@ -1300,11 +1334,13 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
 }
 
 int mg_vcycle(MGData* mg, const skt_operator* op, const void* E, const void* dinv0, int ncols,
-              int kc, const void* r, void* z, void* w0, hipStream_t s) {
+              int kc, const void* r, void* z, void* w0, hipStream_t s, double* dot,
+              int max_slots, int* slots) {
   SKT_CHECK(mg->dtype >= 0, "multigrid not set up");
   SKT_CHECK(ncols <= mg->max_cols && ncols % kc == 0, "bad column count for multigrid");
 #define SKT_VC(T_, NC_, KC_)                                                               \
-  return vcycle_t<T_, NC_, KC_>(mg, op, E, dinv0, ncols, (const T_*)r, (T_*)z, (T_*)w0, s)
+  return vcycle_t<T_, NC_, KC_>(mg, op, E, dinv0, ncols, (const T_*)r, (T_*)z, (T_*)w0, s, dot, \
+                                max_slots, slots)
   if (mg->dtype == SKT_F64) {
     if (op->ncomp == 3) {
       if (kc == 1) SKT_VC(double, 3, 1);

@@ -78,8 +78,11 @@ int mg_setup(MGData* mg, const skt_operator* op, const double* E64, const void* 
              hipStream_t s);
 
 // z = V(r) for ncols columns in the blocked layout [ncols/kc][ndof][kc];
-// w0 is fine-level scratch of the same size.
+// w0 is fine-level scratch of the same size.  With dot, the last smoothing
+// step also leaves the per-column partials of r . z there when they fit in
+// max_slots per column (*slots = their count, else 0).
 int mg_vcycle(MGData* mg, const skt_operator* op, const void* E, const void* dinv0, int ncols,
-              int kc, const void* r, void* z, void* w0, hipStream_t s);
+              int kc, const void* r, void* z, void* w0, hipStream_t s, double* dot = nullptr,
+              int max_slots = 0, int* slots = nullptr);
 
 }  // namespace skt

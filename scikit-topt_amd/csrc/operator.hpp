@@ -53,9 +53,13 @@ int hex8_finish(skt_operator* op, const uint8_t* dof_fixed, skt_operator** out);
 // X is the right-hand side b, the first iterate x1 = s0 dinv b is formed on
 // the fly (s0 = cheb0[1]) and x_out = x1 + c1 x1 + c2 dinv (b - K x1).
 enum { OUT_APPLY = 0, OUT_RESID = 1, OUT_CHEB = 2, OUT_CHEB0 = 3 };
+// Partial sums per column left by a fused dot product (hex_apply_modal and
+// hex_cheb_modal with `dot`): dot[c * slots + i], i < slots, for column c.
+int hex_modal_dot_slots(const skt_operator* op, int k, int kc);
+// Y = K X; with `dot`, also the partials of X . Y per column
 template <typename T>
 int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
-                    T* Y, hipStream_t s);
+                    T* Y, hipStream_t s, double* dot = nullptr);
 // R = B - K X
 template <typename T>
 int hex_resid_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E,
@@ -68,16 +72,22 @@ int hex_cheb0_modal(const skt_operator* op, int k, int kc, const T* coef, const 
 // Chebyshev-Jacobi step with the direction taken from two iterates:
 // Xout = X + c1 (X - Xprev) + c2 dinv (B - K X), (c1, c2) = cheb[0..1] on the
 // device.  Xprev is read from Xout before it is overwritten (zero when
-// prev_zero); Xout must not alias X.
+// prev_zero); Xout must not alias X.  With `dot`, also the partials of
+// B . Xout per column.
 template <typename T>
 int hex_cheb_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
                    T* Xout, const T* B, const T* dinv, const double* cheb, int prev_zero,
-                   hipStream_t s);
+                   hipStream_t s, double* dot = nullptr);
 // Y = K(E) X for ncols columns.  kc == ncols: (ndof, ncols) row-major;
 // otherwise the column-blocked layout [ncols/kc][ndof][kc] (structured hex8
 // only).
 int op_apply(skt_operator* op, int dtype, int ncols, int kc, const void* E,
              const void* X, void* Y, hipStream_t stream);
+// op_apply that also leaves the per-column partials of X . Y in dot (see
+// hex_modal_dot_slots) where the operator can fuse them: *slots is then
+// their count per column, else 0 and nothing is written to dot.
+int op_apply_dot(skt_operator* op, int dtype, int ncols, int kc, const void* E, const void* X,
+                 void* Y, double* dot, int max_slots, int* slots, hipStream_t stream);
 int op_diag(skt_operator* op, int dtype, const void* E, void* diag,
             hipStream_t stream);
 }  // namespace skt

@@ -622,6 +622,27 @@ int op_apply(skt_operator* op, int dtype, int ncols, int kc, const void* E, cons
                             (const float*)X, (float*)Y, s);
 }
 
+int op_apply_dot(skt_operator* op, int dtype, int ncols, int kc, const void* E, const void* X,
+                 void* Y, double* dot, int max_slots, int* slots, hipStream_t s) {
+  SKT_CHECK(slots, "null pointer");
+  *slots = 0;
+  const bool fuse = op && dot && op->mesh == SKT_MESH_HEX8 && op->modal_ok && !op->extra64 &&
+                    ncols >= 1 && kc >= 1 && ncols % kc == 0 &&
+                    hex_modal_dot_slots(op, ncols, kc) <= max_slots;
+  if (!fuse) return op_apply(op, dtype, ncols, kc, E, X, Y, s);
+  SKT_CHECK(E && X && Y, "null pointer");
+  SKT_CHECK(ncols <= 64, "ncols must be in [1, 64]");
+  SKT_CHECK(X != Y, "operator apply cannot run in place");
+  SKT_CHECK(dtype == SKT_F64 || dtype == SKT_F32, "dtype must be SKT_F64 or SKT_F32");
+  const int st = dtype == SKT_F64
+                     ? hex_apply_modal<double>(op, ncols, kc, op->modal_d, (const double*)E,
+                                               (const double*)X, (double*)Y, s, dot)
+                     : hex_apply_modal<float>(op, ncols, kc, op->modal_f, (const float*)E,
+                                              (const float*)X, (float*)Y, s, dot);
+  if (st == SKT_OK) *slots = hex_modal_dot_slots(op, ncols, kc);
+  return st;
+}
+
 template <typename T>
 static int diag_t(skt_operator* op, const T* K0, const T* g, const T* vol, const T* E,
                   T* d, hipStream_t s) {

@@ -250,34 +250,32 @@ __global__ void k_update(int64_t n, int k, int kc, int cp, const double* __restr
   blk_reduce(sh, rr, cp, kc, k, 1, part);
 }
 
-// p = z + beta p (z null: Jacobi z = dinv r)
+// p = z + beta p (z null: Jacobi z = dinv r); (nbx, groups) grid as the
+// reduction kernels, so the row and column come without 64-bit divisions
 template <typename T>
-__global__ void k_pupdate(int64_t n, int k, int kc, const double* __restrict__ sc,
+__global__ void k_pupdate(int64_t n, int k, int kc, int cp, const double* __restrict__ sc,
                           const T* __restrict__ r, const T* __restrict__ dinv,
                           const T* __restrict__ z, T* __restrict__ p) {
-  const int64_t total = n * k;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t gi = t / kc;
-    const int c = (int)((gi / n) * kc + t % kc);
-    if (!col_live(sc, c)) continue;
-    const T beta = (T)sc[c * kSc + S_BETA];
-    const T zv = z ? z[t] : dinv[gi % n] * r[t];
+  SKT_BLK_SETUP
+  if (!lane_on || !col_live(sc, c)) return;
+  const T beta = (T)sc[c * kSc + S_BETA];
+  for (int64_t i = i0; i < n; i += istep) {
+    const int64_t t = ((int64_t)g * n + i) * kc + j;
+    const T zv = z ? z[t] : dinv[i] * r[t];
     p[t] = zv + beta * p[t];
   }
 }
 
 // restart the search direction of columns flagged S_RST: p = z
 template <typename T>
-__global__ void k_restart_dir(int64_t n, int k, int kc, const double* __restrict__ sc,
+__global__ void k_restart_dir(int64_t n, int k, int kc, int cp, const double* __restrict__ sc,
                               const T* __restrict__ r, const T* __restrict__ dinv,
                               const T* __restrict__ z, T* __restrict__ p) {
-  const int64_t total = n * k;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t gi = t / kc;
-    const int c = (int)((gi / n) * kc + t % kc);
-    if (sc[c * kSc + S_RST] != 0.0) p[t] = z ? z[t] : dinv[gi % n] * r[t];
+  SKT_BLK_SETUP
+  if (!lane_on || sc[c * kSc + S_RST] == 0.0) return;
+  for (int64_t i = i0; i < n; i += istep) {
+    const int64_t t = ((int64_t)g * n + i) * kc + j;
+    p[t] = z ? z[t] : dinv[i] * r[t];
   }
 }
 
@@ -334,10 +332,13 @@ __global__ void k_alpha(int nb, int k, const double* __restrict__ part, double* 
   }
 }
 
-__global__ void k_beta(int nb, int k, const double* __restrict__ part, double* __restrict__ sc) {
+// r.z from quantity 0 of part_rz (nbz partials per column), r.r from
+// quantity 1 of part_rr (nbr partials)
+__global__ void k_beta(int nbz, const double* __restrict__ part_rz, int nbr,
+                       const double* __restrict__ part_rr, int k, double* __restrict__ sc) {
   const int c = blockIdx.x;
-  const double rz = col_sum(part, k, nb, 0, c);
-  const double rr = col_sum(part, k, nb, 1, c);
+  const double rz = col_sum(part_rz, k, nbz, 0, c);
+  const double rr = col_sum(part_rr, k, nbr, 1, c);
   if (threadIdx.x == 0) {
     double* s = sc + c * kSc;
     if (s[S_CONV] == 0.0) {
@@ -434,15 +435,27 @@ struct Vecs {
   int dtype;
 };
 
-// z = M r; partial r.z into area (quantity 0)
+// z = M r and the partials of r.z (quantity 0): fused into the V-cycle's
+// last smoothing step when they fit (into `fused`, *nb = their count per
+// column), else a separate product into `area` (*nb = b.nbx).  Jacobi: z is
+// formed inline by the vector kernels and r.z is already in `area`.
 template <typename T>
 static int precondition(skt_solver* s, const Blk& b, const Vecs<T>& v, double* area,
-                        hipStream_t st) {
+                        double* fused, const double** rz_part, int* nb, hipStream_t st) {
+  *rz_part = area;
+  *nb = (int)b.nbx;
   if (s->used == SKT_PC_MG) {
-    int stt = mg_vcycle(&s->mg, s->op, v.E, v.dinv, b.k, b.kc, v.r, v.z, v.q, st);
+    int slots = 0;
+    int stt = mg_vcycle(&s->mg, s->op, v.E, v.dinv, b.k, b.kc, v.r, v.z, v.q, st, fused,
+                        2 * kParts, &slots);
     if (stt) return stt;
-    hipLaunchKernelGGL(k_dot<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
-                       b.cp, (const T*)v.r, (const T*)v.z, area);
+    if (slots > 0) {
+      *rz_part = fused;
+      *nb = slots;
+    } else {
+      hipLaunchKernelGGL(k_dot<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
+                         b.cp, (const T*)v.r, (const T*)v.z, area);
+    }
   }
   return SKT_OK;
 }
@@ -452,21 +465,33 @@ template <typename T>
 static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t count,
                           hipStream_t st) {
   const bool mg = s->used == SKT_PC_MG;
-  const unsigned ge = grid_for(b.n * b.k);
+  // area0 (s->part) takes the p.q partials and k_update's r.z / r.r; the
+  // fused r.z of the V-cycle goes to area1, which is idle between the
+  // initialisation and the reliable updates
+  double* area1 = s->part + (size_t)2 * kParts * s->max_cols;
   for (int64_t it = 0; it < count; ++it) {
     if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * it], st));
-    int stt = op_apply(s->op, v.dtype, b.k, b.kc, v.E, v.p, v.q, st);
+    int pq_nb = 0;
+    int stt = op_apply_dot(s->op, v.dtype, b.k, b.kc, v.E, v.p, v.q, s->part, 2 * kParts,
+                           &pq_nb, st);
     if (stt) return stt;
     if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * it + 1], st));
-    hipLaunchKernelGGL(k_dot<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
-                       b.cp, (const T*)v.p, (const T*)v.q, s->part);
-    hipLaunchKernelGGL(k_alpha, dim3(b.k), dim3(kWave), 0, st, (int)b.nbx, b.k, s->part, s->sc);
+    if (pq_nb == 0) {
+      hipLaunchKernelGGL(k_dot<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
+                         b.cp, (const T*)v.p, (const T*)v.q, s->part);
+      pq_nb = (int)b.nbx;
+    }
+    hipLaunchKernelGGL(k_alpha, dim3(b.k), dim3(kWave), 0, st, pq_nb, b.k, s->part, s->sc);
     hipLaunchKernelGGL(k_update<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
                        b.cp, s->sc, (const T*)v.p, (const T*)v.q,
                        mg ? (const T*)nullptr : (const T*)v.dinv, v.x, v.r, s->part);
-    if ((stt = precondition<T>(s, b, v, s->part, st))) return stt;
-    hipLaunchKernelGGL(k_beta, dim3(b.k), dim3(kWave), 0, st, (int)b.nbx, b.k, s->part, s->sc);
-    hipLaunchKernelGGL(k_pupdate<T>, dim3(ge), dim3(kBlock), 0, st, b.n, b.k, b.kc, s->sc,
+    const double* rz_part = nullptr;
+    int rz_nb = 0;
+    if ((stt = precondition<T>(s, b, v, s->part, area1, &rz_part, &rz_nb, st))) return stt;
+    hipLaunchKernelGGL(k_beta, dim3(b.k), dim3(kWave), 0, st, rz_nb, rz_part, (int)b.nbx,
+                       (const double*)s->part, b.k, s->sc);
+    hipLaunchKernelGGL(k_pupdate<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
+                       b.cp, s->sc,
                        (const T*)v.r, (const T*)v.dinv, mg ? (const T*)v.z : (const T*)nullptr,
                        v.p);
   }
@@ -575,7 +600,8 @@ static int solve_t(skt_solver* s, int mode, const Blk& b, const double* E64, con
                        mg ? (const T*)v.z : (const T*)nullptr, (T*)nullptr, area1);
     hipLaunchKernelGGL(k_replace_final, dim3(b.k), dim3(kWave), 0, st, (int)b.nbx, b.k, s->part,
                        area1, s->sc);
-    hipLaunchKernelGGL(k_restart_dir<T>, dim3(ge), dim3(kBlock), 0, st, b.n, b.k, b.kc, s->sc,
+    hipLaunchKernelGGL(k_restart_dir<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k,
+                       b.kc, b.cp, s->sc,
                        (const T*)v.r, (const T*)v.dinv, mg ? (const T*)v.z : (const T*)nullptr,
                        v.p);
     SKT_LAUNCH_CHECK();
