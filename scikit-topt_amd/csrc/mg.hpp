@@ -27,7 +27,8 @@ struct MGLevel {
   int64_t nn = 0, ndof = 0;           // nodes, DOFs
   int f[3] = {1, 1, 1};               // factor from the finer level
   double* Kel = nullptr;              // (n_el, ND*ND) Galerkin element matrices, fp64
-  void* st = nullptr;                 // stencil [27*NC*NC][nn] in T (levels 1..L-1)
+  void* st = nullptr;                 // stencil in T (levels 1..L-1), node- or tile-major
+  int tiled = 0;                      // tile-major stencil, LDS-tiled apply
   void* dinv = nullptr;               // (ndof) in T
   void* b = nullptr;                  // right-hand side (ndof*k, blocked)
   void* x = nullptr;                  // iterate
@@ -35,6 +36,10 @@ struct MGLevel {
   void* d = nullptr;                  // Chebyshev direction
   void* rt = nullptr;                 // restriction staging (x-y restricted fine planes)
 };
+
+// Levels with at least this many nodes use the LDS-tiled stencil apply and
+// the tile-major stencil storage.
+constexpr int64_t kTiledNodes = 20000;
 
 struct MGData {
   int dtype = -1;                     // SKT_F64 / SKT_F32 the buffers hold
