@@ -753,21 +753,41 @@ __global__ void k_cheb(int64_t total, int64_t ndof, int kc, const double* __rest
   }
 }
 
+// Transfers run on row grids: blockIdx.x and threadIdx.x give the node along
+// x, blockIdx.y (strided) the row (y, z, column group), so the index
+// arithmetic is a few 32-bit divisions per row rather than 64-bit ones per
+// value.  (Splitting a node's components over three lanes made the
+// prolongation twice as slow: the row of 3 nx pieces fills 256-lane
+// workgroups poorly and every lane repeats the weights.)  Row counts and row
+// lengths fit in 32 bits.
+struct RowGrid {
+  dim3 grid, block;
+};
+
+static RowGrid row_grid(int64_t row_len, int64_t rows) {
+  RowGrid g;
+  const int bx = row_len >= 256 ? 256 : (int)((row_len + 63) / 64) * 64;
+  g.block = dim3((unsigned)bx);
+  g.grid = dim3((unsigned)((row_len + bx - 1) / bx), (unsigned)std::min<int64_t>(rows, 65535));
+  return g;
+}
+
 // Restriction b_c = P^T Z (b - w), separable in two passes.
 // Pass 1 (x, y): t[(I, J, k)] = sum_{dx,dy} w(dx) w(dy) r(f I + dx, f J + dy, k)
-// on every fine z plane k, r = b - w with Dirichlet DOFs removed.
+// on every fine z plane k, r = b - w with Dirichlet DOFs removed.  Thread:
+// coarse I; row: (J, k, group).
 template <typename T, int NC, int KC>
-__global__ void k_restrict_xy(int64_t fnx, int64_t fny, int64_t fnz, int fx, int fy, int groups,
+__global__ void k_restrict_xy(int fnx, int fny, int fnz, int fx, int fy, int groups,
                               const T* __restrict__ b, const T* __restrict__ w,
                               const uint8_t* __restrict__ fixed, T* __restrict__ tbuf) {
-  const int64_t fnnx = fnx + 1, fnny = fny + 1, fnnz = fnz + 1;
-  const int64_t cnnx = fnx / fx + 1, cnny = fny / fy + 1;
-  const int64_t tn = cnnx * cnny * fnnz, fnn = fnnx * fnny * fnnz;
-  const int64_t total = tn * groups;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t tnode = t % tn, g = t / tn;
-    const int64_t I = tnode % cnnx, J = (tnode / cnnx) % cnny, k = tnode / (cnnx * cnny);
+  const int fnnx = fnx + 1, fnny = fny + 1, fnnz = fnz + 1;
+  const int cnnx = fnx / fx + 1, cnny = fny / fy + 1;
+  const int64_t tn = (int64_t)cnnx * cnny * fnnz, fnn = (int64_t)fnnx * fnny * fnnz;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= cnnx) return;
+  const int nrows = cnny * fnnz * groups;
+  for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+    const int J = row % cnny, kg = row / cnny, k = kg % fnnz, g = kg / fnnz;
     const T* bg = b + g * fnn * NC * KC;
     const T* wg = w ? w + g * fnn * NC * KC : nullptr;
     T acc[NC][KC];
@@ -776,14 +796,14 @@ __global__ void k_restrict_xy(int64_t fnx, int64_t fny, int64_t fnz, int fx, int
 #pragma unroll
       for (int c = 0; c < KC; ++c) acc[d][c] = T(0);
     for (int dy = -(fy - 1); dy <= fy - 1; ++dy) {
-      const int64_t j = J * fy + dy;
+      const int j = J * fy + dy;
       if (j < 0 || j >= fnny) continue;
       const T wy = T(1) - (T)(dy < 0 ? -dy : dy) / (T)fy;
       for (int dx = -(fx - 1); dx <= fx - 1; ++dx) {
-        const int64_t i = I * fx + dx;
+        const int i = I * fx + dx;
         if (i < 0 || i >= fnnx) continue;
         const T wgt = (T(1) - (T)(dx < 0 ? -dx : dx) / (T)fx) * wy;
-        const int64_t fn = i + fnnx * (j + fnny * k);
+        const int64_t fn = i + fnnx * (j + (int64_t)fnny * k);
         T bv[NC * KC], wv[NC * KC];
         load_node<T, NC * KC, KC>(bg + fn * NC * KC, bv);
         if (wg) load_node<T, NC * KC, KC>(wg + fn * NC * KC, wv);
@@ -796,6 +816,7 @@ __global__ void k_restrict_xy(int64_t fnx, int64_t fny, int64_t fnz, int fx, int
         }
       }
     }
+    const int64_t tnode = I + cnnx * (J + (int64_t)cnny * k);
     T* out = tbuf + g * tn * NC * KC;
 #pragma unroll
     for (int d = 0; d < NC; ++d)
@@ -804,17 +825,19 @@ __global__ void k_restrict_xy(int64_t fnx, int64_t fny, int64_t fnz, int fx, int
   }
 }
 
-// Pass 2 (z): b_c(I, J, K) = sum_dz w(dz) t(I, J, f K + dz)
+// Pass 2 (z): b_c(I, J, K) = sum_dz w(dz) t(I, J, f K + dz).  Thread: (I, J);
+// row: (K, group).
 template <typename T, int NC, int KC>
-__global__ void k_restrict_z(int64_t cnnx, int64_t cnny, int64_t fnz, int fz, int groups,
+__global__ void k_restrict_z(int cnnx, int cnny, int fnz, int fz, int groups,
                              const T* __restrict__ tbuf, T* __restrict__ bc) {
-  const int64_t fnnz = fnz + 1, cnnz = fnz / fz + 1;
-  const int64_t tn = cnnx * cnny * fnnz, cnn = cnnx * cnny * cnnz;
-  const int64_t total = cnn * groups;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t cnode = t % cnn, g = t / cnn;
-    const int64_t IJ = cnode % (cnnx * cnny), K = cnode / (cnnx * cnny);
+  const int fnnz = fnz + 1, cnnz = fnz / fz + 1;
+  const int64_t plane = (int64_t)cnnx * cnny;
+  const int64_t tn = plane * fnnz, cnn = plane * cnnz;
+  const int64_t IJ = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (IJ >= plane) return;
+  const int nrows = cnnz * groups;
+  for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+    const int K = row % cnnz, g = row / cnnz;
     const T* tg = tbuf + g * tn * NC * KC;
     T acc[NC][KC];
 #pragma unroll
@@ -822,15 +845,16 @@ __global__ void k_restrict_z(int64_t cnnx, int64_t cnny, int64_t fnz, int fz, in
 #pragma unroll
       for (int c = 0; c < KC; ++c) acc[d][c] = T(0);
     for (int dz = -(fz - 1); dz <= fz - 1; ++dz) {
-      const int64_t k = K * fz + dz;
+      const int k = K * fz + dz;
       if (k < 0 || k >= fnnz) continue;
       const T wz = T(1) - (T)(dz < 0 ? -dz : dz) / (T)fz;
-      const int64_t tnode = IJ + cnnx * cnny * k;
+      const int64_t tnode = IJ + plane * k;
 #pragma unroll
       for (int d = 0; d < NC; ++d)
 #pragma unroll
         for (int c = 0; c < KC; ++c) acc[d][c] += wz * tg[(tnode * NC + d) * KC + c];
     }
+    const int64_t cnode = IJ + plane * K;
     T* out = bc + g * cnn * NC * KC;
 #pragma unroll
     for (int d = 0; d < NC; ++d)
@@ -839,22 +863,24 @@ __global__ void k_restrict_z(int64_t cnnx, int64_t cnny, int64_t fnz, int fz, in
   }
 }
 
-// x_f += Z P x_c for one fine node and column group
+// x_f += Z P x_c.  Thread: fine i; row: (j, k, group).
 template <typename T, int NC, int KC>
-__global__ void k_prolong_add(int64_t fnx, int64_t fny, int64_t fnz, int fx, int fy, int fz,
-                              int groups, const T* __restrict__ xc,
-                              const uint8_t* __restrict__ fixed, T* __restrict__ xf) {
-  const int64_t fnnx = fnx + 1, fnny = fny + 1, fnnz = fnz + 1;
-  const int64_t cnnx = fnx / fx + 1, cnny = fny / fy + 1, cnnz = fnz / fz + 1;
-  const int64_t cnn = cnnx * cnny * cnnz, fnn = fnnx * fnny * fnnz;
-  const int64_t total = fnn * groups;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t fnode = t % fnn, g = t / fnn;
-    const int64_t i = fnode % fnnx, j = (fnode / fnnx) % fnny, k = fnode / (fnnx * fnny);
-    const int64_t I = i / fx, J = j / fy, K = k / fz;
-    const int ax = (int)(i % fx), ay = (int)(j % fy), az = (int)(k % fz);
-    const T sx = (T)ax / (T)fx, sy = (T)ay / (T)fy, sz = (T)az / (T)fz;
+__global__ void k_prolong_add(int fnx, int fny, int fnz, int fx, int fy, int fz, int groups,
+                              const T* __restrict__ xc, const uint8_t* __restrict__ fixed,
+                              T* __restrict__ xf) {
+  const int fnnx = fnx + 1, fnny = fny + 1, fnnz = fnz + 1;
+  const int cnnx = fnx / fx + 1, cnny = fny / fy + 1, cnnz = fnz / fz + 1;
+  const int64_t cnn = (int64_t)cnnx * cnny * cnnz, fnn = (int64_t)fnnx * fnny * fnnz;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= fnnx) return;
+  const int I = i / fx, ax = i - I * fx;
+  const T sx = (T)ax / (T)fx;
+  const int nrows = fnny * fnnz * groups;
+  for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+    const int j = row % fnny, kg = row / fnny, k = kg % fnnz, g = kg / fnnz;
+    const int J = j / fy, ay = j - J * fy, K = k / fz, az = k - K * fz;
+    const T sy = (T)ay / (T)fy, sz = (T)az / (T)fz;
+    const int64_t fnode = i + fnnx * (j + (int64_t)fnny * k);
     const T* xg = xc + g * cnn * NC * KC;
     T acc[NC][KC];
 #pragma unroll
@@ -866,7 +892,7 @@ __global__ void k_prolong_add(int64_t fnx, int64_t fny, int64_t fnz, int fx, int
       const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
       if ((qx && ax == 0) || (qy && ay == 0) || (qz && az == 0)) continue;
       const T wgt = (qx ? sx : T(1) - sx) * (qy ? sy : T(1) - sy) * (qz ? sz : T(1) - sz);
-      const int64_t cn = (I + qx) + cnnx * ((J + qy) + cnny * (K + qz));
+      const int64_t cn = (I + qx) + cnnx * ((J + qy) + (int64_t)cnny * (K + qz));
       T xv[NC * KC];
       load_node<T, NC * KC, KC>(xg + cn * NC * KC, xv);
 #pragma unroll
@@ -1030,6 +1056,9 @@ static int64_t tiled_nodes() {
 
 int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype, int max_cols) {
   if (mg->dtype == dtype && mg->max_cols == max_cols) return SKT_OK;
+  SKT_CHECK((op->nx + 1) * (op->ny + 1) < INT32_MAX &&
+                (op->ny + 1) * (op->nz + 1) * (int64_t)max_cols < INT32_MAX,
+            "grid too large for the multigrid transfers' 32-bit row indices");
   mg_free(mg);
   std::vector<MGLevel> lv;
   if (!mg_plan(op, cfg, &lv)) {
@@ -1260,19 +1289,21 @@ int mg_setup(MGData* mg, const skt_operator* op, const double* E64, const void* 
 template <typename T, int NC, int KC>
 static void launch_restrict(const MGLevel& f, const MGLevel& c, int groups, const T* b,
                             const T* w, const uint8_t* fixed, T* bc, hipStream_t s) {
-  const int64_t tn = (c.nx + 1) * (c.ny + 1) * (f.nz + 1);
   T* tbuf = (T*)c.rt;
-  hipLaunchKernelGGL((k_restrict_xy<T, NC, KC>), dim3(grid_for(tn * groups)), dim3(kBlock), 0, s,
-                     f.nx, f.ny, f.nz, c.f[0], c.f[1], groups, b, w, fixed, tbuf);
-  hipLaunchKernelGGL((k_restrict_z<T, NC, KC>), dim3(grid_for(c.nn * groups)), dim3(kBlock), 0,
-                     s, c.nx + 1, c.ny + 1, f.nz, c.f[2], groups, (const T*)tbuf, bc);
+  const RowGrid gxy = row_grid(c.nx + 1, (c.ny + 1) * (f.nz + 1) * groups);
+  hipLaunchKernelGGL((k_restrict_xy<T, NC, KC>), gxy.grid, gxy.block, 0, s, (int)f.nx, (int)f.ny,
+                     (int)f.nz, c.f[0], c.f[1], groups, b, w, fixed, tbuf);
+  const RowGrid gz = row_grid((c.nx + 1) * (c.ny + 1), (c.nz + 1) * groups);
+  hipLaunchKernelGGL((k_restrict_z<T, NC, KC>), gz.grid, gz.block, 0, s, (int)(c.nx + 1),
+                     (int)(c.ny + 1), (int)f.nz, c.f[2], groups, (const T*)tbuf, bc);
 }
 
 template <typename T, int NC, int KC>
 static void launch_prolong(const MGLevel& f, const MGLevel& c, int groups, const T* xc,
                            const uint8_t* fixed, T* xf, hipStream_t s) {
-  hipLaunchKernelGGL((k_prolong_add<T, NC, KC>), dim3(grid_for(f.nn * groups)), dim3(kBlock), 0,
-                     s, f.nx, f.ny, f.nz, c.f[0], c.f[1], c.f[2], groups, xc, fixed, xf);
+  const RowGrid gp = row_grid(f.nx + 1, (f.ny + 1) * (f.nz + 1) * groups);
+  hipLaunchKernelGGL((k_prolong_add<T, NC, KC>), gp.grid, gp.block, 0, s, (int)f.nx, (int)f.ny,
+                     (int)f.nz, c.f[0], c.f[1], c.f[2], groups, xc, fixed, xf);
 }
 
 template <typename T>
