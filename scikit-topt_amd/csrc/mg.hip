@@ -72,16 +72,56 @@ __device__ __forceinline__ double tri_weight(int q, int Q, int a, int b, int c, 
   return wx * wy * wz;
 }
 
+// acc += P_j^T sK P_j for child (a, b, c) of a coarse element divided by
+// (fx, fy, fz); sK holds the child's ND x ND matrix, 64 threads, acc[k] is
+// entry tid + 64 k.
+template <int NC>
+__device__ __forceinline__ void galerkin_child(const double* sK, double* sT, double* sP, int a,
+                                               int b, int c, int fx, int fy, int fz,
+                                               double* acc) {
+  constexpr int ND = 8 * NC;
+  constexpr int NN = ND * ND;
+  constexpr int PER = (NN + 63) / 64;
+  const int tid = threadIdx.x;
+  sP[tid] = tri_weight(tid >> 3, tid & 7, a, b, c, fx, fy, fz);  // sP[q*8+Q]
+  __syncthreads();
+  // T[(q,d)][(R,e)] = sum_r K[(q,d)][(r,e)] P[r][R]
+  for (int i = tid; i < NN; i += 64) {
+    const int row = i / ND, col = i % ND;
+    const int R = col / NC, e = col % NC;
+    double t = 0.0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) t += sK[row * ND + r * NC + e] * sP[r * 8 + R];
+    sT[i] = t;
+  }
+  __syncthreads();
+  // Kc[(Q,d)][(R,e)] += sum_q P[q][Q] T[(q,d)][(R,e)]
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = tid + 64 * k;
+    if (i < NN) {
+      const int row = i / ND, col = i % ND;
+      const int Q = row / NC, d = row % NC;
+      double t = 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) t += sP[q * 8 + Q] * sT[(q * NC + d) * ND + col];
+      acc[k] += t;
+    }
+  }
+  __syncthreads();
+}
+
 // One workgroup of 64 threads per coarse element:
 //   Kc = sum_children P_j^T K_j P_j,
 // K_j = E_j Z_j K0 Z_j on the finest level (Z_j removes Dirichlet DOFs), or
-// the finer level's Galerkin element matrix.
+// the finer level's Galerkin element matrix.  With `only`, elements whose
+// flag is 0 are left to k_galerkin_interior.
 template <int NC>
 __global__ void __launch_bounds__(64)
 k_galerkin(int64_t ncx, int64_t ncy, int64_t ncz, int fx, int fy, int fz,
            const double* __restrict__ Kf, const double* __restrict__ K0b,
            const double* __restrict__ E, const uint8_t* __restrict__ fixed,
-           double* __restrict__ Kc) {
+           const uint8_t* __restrict__ only, double* __restrict__ Kc) {
   constexpr int ND = 8 * NC;
   constexpr int NN = ND * ND;
   constexpr int PER = (NN + 63) / 64;
@@ -89,6 +129,7 @@ k_galerkin(int64_t ncx, int64_t ncy, int64_t ncz, int fx, int fy, int fz,
   __shared__ double sT[NN];
   __shared__ double sP[64];
   const int64_t ce = blockIdx.x;
+  if (only && !only[ce]) return;
   const int64_t CX = ce % ncx, CY = (ce / ncx) % ncy, CZ = ce / (ncx * ncy);
   const int64_t nfx = ncx * fx, nfy = ncy * fy;
   const int64_t nfnx = nfx + 1, nfny = nfy + 1;
@@ -120,37 +161,93 @@ k_galerkin(int64_t ncx, int64_t ncy, int64_t ncz, int fx, int fy, int fz,
             sK[i] = v;
           }
         }
-        sP[tid] = tri_weight(tid >> 3, tid & 7, a, b, c, fx, fy, fz);  // sP[q*8+Q]
-        __syncthreads();
-        // T[(q,d)][(R,e)] = sum_r K[(q,d)][(r,e)] P[r][R]
-        for (int i = tid; i < NN; i += 64) {
-          const int row = i / ND, col = i % ND;
-          const int R = col / NC, e = col % NC;
-          double t = 0.0;
-#pragma unroll
-          for (int r = 0; r < 8; ++r) t += sK[row * ND + r * NC + e] * sP[r * 8 + R];
-          sT[i] = t;
-        }
-        __syncthreads();
-        // Kc[(Q,d)][(R,e)] += sum_q P[q][Q] T[(q,d)][(R,e)]
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-          const int i = tid + 64 * k;
-          if (i < NN) {
-            const int row = i / ND, col = i % ND;
-            const int Q = row / NC, d = row % NC;
-            double t = 0.0;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) t += sP[q * 8 + Q] * sT[(q * NC + d) * ND + col];
-            acc[k] += t;
-          }
-        }
-        __syncthreads();
+        galerkin_child<NC>(sK, sT, sP, a, b, c, fx, fy, fz, acc);
       }
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int i = tid + 64 * k;
     if (i < NN) Kc[ce * NN + i] = acc[k];
+  }
+}
+
+// On the finest level K_j = E_j K0 wherever the child has no Dirichlet DOF,
+// so P_j^T K_j P_j = E_j M_j with M_j = P_j^T K0 P_j fixed per child
+// position j = a + fx (b + fy c).  One workgroup per position.
+template <int NC>
+__global__ void __launch_bounds__(64)
+k_galerkin_children(int fx, int fy, int fz, const double* __restrict__ K0b,
+                    double* __restrict__ M) {
+  constexpr int ND = 8 * NC;
+  constexpr int NN = ND * ND;
+  constexpr int PER = (NN + 63) / 64;
+  __shared__ double sK[NN];
+  __shared__ double sT[NN];
+  __shared__ double sP[64];
+  const int j = blockIdx.x;
+  const int a = j % fx, b = (j / fx) % fy, c = j / (fx * fy);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < NN; i += 64) sK[i] = K0b[i];
+  double acc[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) acc[i] = 0.0;
+  galerkin_child<NC>(sK, sT, sP, a, b, c, fx, fy, fz, acc);
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = tid + 64 * k;
+    if (i < NN) M[(int64_t)j * NN + i] = acc[k];
+  }
+}
+
+// 1 for the coarse elements of the first coarse level whose fine nodes hold
+// a Dirichlet DOF (their children need the masked product), else 0.
+template <int NC>
+__global__ void k_galerkin_flags(int64_t ncx, int64_t ncy, int64_t ncz, int fx, int fy, int fz,
+                                 const uint8_t* __restrict__ fixed, uint8_t* __restrict__ flag) {
+  const int64_t nce = ncx * ncy * ncz;
+  const int64_t nfnx = ncx * fx + 1, nfny = ncy * fy + 1;
+  for (int64_t ce = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; ce < nce;
+       ce += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t CX = ce % ncx, CY = (ce / ncx) % ncy, CZ = ce / (ncx * ncy);
+    uint8_t any = 0;
+    for (int k = 0; k <= fz; ++k)
+      for (int j = 0; j <= fy; ++j)
+        for (int i = 0; i <= fx; ++i) {
+          const int64_t n = (CX * fx + i) + nfnx * ((CY * fy + j) + nfny * (CZ * fz + k));
+#pragma unroll
+          for (int d = 0; d < NC; ++d) any |= fixed[n * NC + d];
+        }
+    flag[ce] = any ? 1 : 0;
+  }
+}
+
+// Kc = sum_j E_j M_j for the coarse elements without Dirichlet DOFs (flag
+// null: all of them).  A workgroup of NN threads (one per entry) walks over
+// coarse elements, with the element's child moduli staged in LDS; the
+// children are summed in the order k_galerkin takes them.
+constexpr int kMaxChildren = 125;  // factor 5 on every axis
+template <int NC>
+__global__ void __launch_bounds__(64 * NC * NC)
+k_galerkin_interior(int64_t ncx, int64_t ncy, int64_t ncz, int fx, int fy, int fz,
+                    const double* __restrict__ M, const double* __restrict__ E,
+                    const uint8_t* __restrict__ flag, double* __restrict__ Kc) {
+  constexpr int NN = 64 * NC * NC;
+  __shared__ double sE[kMaxChildren];
+  const int64_t nce = ncx * ncy * ncz;
+  const int64_t nfx = ncx * fx, nfy = ncy * fy;
+  const int nch = fx * fy * fz;
+  const int tid = threadIdx.x;
+  for (int64_t ce = blockIdx.x; ce < nce; ce += gridDim.x) {
+    if (flag && flag[ce]) continue;  // uniform over the workgroup
+    const int64_t CX = ce % ncx, CY = (ce / ncx) % ncy, CZ = ce / (ncx * ncy);
+    for (int j = tid; j < nch; j += NN) {
+      const int a = j % fx, b = (j / fx) % fy, c = j / (fx * fy);
+      sE[j] = E[(CX * fx + a) + nfx * ((CY * fy + b) + nfy * (CZ * fz + c))];
+    }
+    __syncthreads();
+    double acc = 0.0;
+    for (int j = 0; j < nch; ++j) acc += sE[j] * M[(int64_t)j * NN + tid];
+    Kc[ce * NN + tid] = acc;
+    __syncthreads();
   }
 }
 
@@ -1033,6 +1130,10 @@ void mg_free(MGData* mg) {
   (void)hipFree(mg->lam); (void)hipFree(mg->coef); (void)hipFree(mg->part); (void)hipFree(mg->pv);
   (void)hipFree(mg->pvw);
   (void)hipFree(mg->K0b);
+  (void)hipFree(mg->Mch);
+  (void)hipFree(mg->gflag);
+  mg->Mch = nullptr;
+  mg->gflag = nullptr;
   mg->Ad = mg->Ad2 = nullptr; mg->Ainv = nullptr; mg->lam = mg->coef = mg->part = nullptr;
   mg->K0b = nullptr;
   mg->pv = nullptr;
@@ -1182,18 +1283,62 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   const int NC = op->ncomp, ND = 8 * NC;
   const int L = (int)mg->lv.size() - 1;
   const double* dK0 = mg->K0b;
-  // Galerkin hierarchy
-  for (int l = 1; l <= L; ++l) {
+  // Galerkin hierarchy.  The first coarse level takes E_j M_j for its
+  // interior elements (M_j and the boundary flags depend on the operator
+  // only, so they are built once) and the masked product for the rest.
+  {
+    const MGLevel& c = mg->lv[1];
+    const int nch = c.f[0] * c.f[1] * c.f[2];
+    const int64_t nce = c.nx * c.ny * c.nz;
+    if (!mg->Mch) {
+      SKT_HIP(hipMalloc(&mg->Mch, sizeof(double) * (size_t)nch * ND * ND));
+      if (NC == 3)
+        hipLaunchKernelGGL((k_galerkin_children<3>), dim3(nch), dim3(64), 0, s, c.f[0], c.f[1],
+                           c.f[2], dK0, mg->Mch);
+      else
+        hipLaunchKernelGGL((k_galerkin_children<1>), dim3(nch), dim3(64), 0, s, c.f[0], c.f[1],
+                           c.f[2], dK0, mg->Mch);
+      if (op->n_fixed > 0) {
+        SKT_HIP(hipMalloc(&mg->gflag, (size_t)nce));
+        if (NC == 3)
+          hipLaunchKernelGGL((k_galerkin_flags<3>), dim3(grid_for(nce)), dim3(kBlock), 0, s,
+                             c.nx, c.ny, c.nz, c.f[0], c.f[1], c.f[2], op->fixed, mg->gflag);
+        else
+          hipLaunchKernelGGL((k_galerkin_flags<1>), dim3(grid_for(nce)), dim3(kBlock), 0, s,
+                             c.nx, c.ny, c.nz, c.f[0], c.f[1], c.f[2], op->fixed, mg->gflag);
+      }
+      SKT_LAUNCH_CHECK();
+    }
+    SKT_CHECK(nch <= kMaxChildren, "internal: coarsening factor above 5");
+    const unsigned gi = (unsigned)std::min<int64_t>(nce, 8192);
+    if (NC == 3)
+      hipLaunchKernelGGL((k_galerkin_interior<3>), dim3(gi), dim3(64 * 9), 0, s, c.nx, c.ny, c.nz,
+                         c.f[0], c.f[1], c.f[2], mg->Mch, E64, mg->gflag, c.Kel);
+    else
+      hipLaunchKernelGGL((k_galerkin_interior<1>), dim3(gi), dim3(64), 0, s, c.nx, c.ny, c.nz,
+                         c.f[0], c.f[1], c.f[2], mg->Mch, E64, mg->gflag, c.Kel);
+    if (mg->gflag) {
+      if (NC == 3)
+        hipLaunchKernelGGL((k_galerkin<3>), dim3((unsigned)nce), dim3(64), 0, s, c.nx, c.ny,
+                           c.nz, c.f[0], c.f[1], c.f[2], nullptr, dK0, E64, op->fixed,
+                           mg->gflag, c.Kel);
+      else
+        hipLaunchKernelGGL((k_galerkin<1>), dim3((unsigned)nce), dim3(64), 0, s, c.nx, c.ny,
+                           c.nz, c.f[0], c.f[1], c.f[2], nullptr, dK0, E64, op->fixed,
+                           mg->gflag, c.Kel);
+    }
+    SKT_LAUNCH_CHECK();
+  }
+  for (int l = 2; l <= L; ++l) {
     const MGLevel& c = mg->lv[l];
     const int64_t nce = c.nx * c.ny * c.nz;
-    const double* Kf = l == 1 ? nullptr : mg->lv[l - 1].Kel;
-    const uint8_t* fx = (l == 1 && op->n_fixed > 0) ? op->fixed : nullptr;
+    const double* Kf = mg->lv[l - 1].Kel;
     if (NC == 3)
       hipLaunchKernelGGL((k_galerkin<3>), dim3((unsigned)nce), dim3(64), 0, s, c.nx, c.ny, c.nz,
-                         c.f[0], c.f[1], c.f[2], Kf, dK0, E64, fx, c.Kel);
+                         c.f[0], c.f[1], c.f[2], Kf, dK0, E64, nullptr, nullptr, c.Kel);
     else
       hipLaunchKernelGGL((k_galerkin<1>), dim3((unsigned)nce), dim3(64), 0, s, c.nx, c.ny, c.nz,
-                         c.f[0], c.f[1], c.f[2], Kf, dK0, E64, fx, c.Kel);
+                         c.f[0], c.f[1], c.f[2], Kf, dK0, E64, nullptr, nullptr, c.Kel);
     SKT_LAUNCH_CHECK();
   }
   // stencils and Jacobi inverses of the intermediate levels

@@ -52,6 +52,8 @@ struct MGData {
   double* coef = nullptr;             // per level Chebyshev (c1, c2) per step
   double* part = nullptr;             // power-iteration partials
   double* K0b = nullptr;              // unit element matrix, corner-bit order
+  double* Mch = nullptr;              // P_j^T K0 P_j per child position of level 1
+  uint8_t* gflag = nullptr;           // level-1 elements with Dirichlet DOFs below
   void* pv = nullptr;                 // power-iteration scratch (2 * ndof0) in T
   void* pvw = nullptr;                // per level (levels 0..L-1, concatenated) the last
                                       // normalised power-iteration vector in T
