@@ -1120,6 +1120,7 @@ __global__ void k_cheb_coef(int nlev, int degree, double ratio, const double* __
 static size_t tsize(int dtype) { return dtype == SKT_F64 ? sizeof(double) : sizeof(float); }
 
 void mg_free(MGData* mg) {
+  if (mg->side) (void)hipStreamSynchronize(mg->side);  // the set-up's side work is done
   for (MGLevel& l : mg->lv) {
     (void)hipFree(l.Kel); (void)hipFree(l.st); (void)hipFree(l.dinv);
     (void)hipFree(l.b); (void)hipFree(l.x); (void)hipFree(l.w); (void)hipFree(l.d);
@@ -1130,6 +1131,13 @@ void mg_free(MGData* mg) {
   (void)hipFree(mg->lam); (void)hipFree(mg->coef); (void)hipFree(mg->part); (void)hipFree(mg->pv);
   (void)hipFree(mg->pvw);
   (void)hipFree(mg->K0b);
+  if (mg->side) {
+    (void)hipStreamDestroy(mg->side);
+    (void)hipEventDestroy(mg->ev_fork);
+    (void)hipEventDestroy(mg->ev_join);
+    mg->side = nullptr;
+    mg->ev_fork = mg->ev_join = nullptr;
+  }
   (void)hipFree(mg->Mch);
   (void)hipFree(mg->gflag);
   mg->Mch = nullptr;
@@ -1373,18 +1381,30 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
       hipLaunchKernelGGL((k_dense_asm<1>), dim3(grid_for(n * n)), dim3(kBlock), 0, s, c.nx, c.ny,
                          c.nz, c.Kel, mg->Ad);
     hipLaunchKernelGGL(k_dense_fix_diag, dim3(grid_for(n)), dim3(kBlock), 0, s, n, mg->Ad);
+    // The Gauss-Jordan chain (mostly small, latency-bound launches) runs on
+    // a side stream while the power iterations below fill the GPU; it only
+    // touches Ad, Ad2 and Ainv, and the set-up joins it before returning.
+    if (!mg->side) {
+      SKT_HIP(hipStreamCreateWithFlags(&mg->side, hipStreamNonBlocking));
+      SKT_HIP(hipEventCreateWithFlags(&mg->ev_fork, hipEventDisableTiming));
+      SKT_HIP(hipEventCreateWithFlags(&mg->ev_join, hipEventDisableTiming));
+    }
+    SKT_HIP(hipEventRecord(mg->ev_fork, s));
+    SKT_HIP(hipStreamWaitEvent(mg->side, mg->ev_fork, 0));
+    hipStream_t g = mg->side;
     const unsigned tiles = (unsigned)((n + kTile - 1) / kTile);
     const unsigned strips = (unsigned)((n + kTileRC - 1) / kTileRC);
     for (int64_t b0 = 0; b0 < n; b0 += kGJ) {
       const int nb = (int)std::min<int64_t>(kGJ, n - b0);
-      hipLaunchKernelGGL(k_gj_block_inv, dim3(1), dim3(256), 0, s, n, b0, nb, mg->Ad, mg->Ad2);
-      hipLaunchKernelGGL(k_gj_rows, dim3(strips), dim3(256), 0, s, n, b0, nb, mg->Ad2, mg->Ad);
-      hipLaunchKernelGGL(k_gj_update, dim3(tiles, tiles), dim3(256), 0, s, n, b0, nb, mg->Ad);
-      hipLaunchKernelGGL(k_gj_cols, dim3(strips), dim3(256), 0, s, n, b0, nb, mg->Ad2, mg->Ad);
+      hipLaunchKernelGGL(k_gj_block_inv, dim3(1), dim3(256), 0, g, n, b0, nb, mg->Ad, mg->Ad2);
+      hipLaunchKernelGGL(k_gj_rows, dim3(strips), dim3(256), 0, g, n, b0, nb, mg->Ad2, mg->Ad);
+      hipLaunchKernelGGL(k_gj_update, dim3(tiles, tiles), dim3(256), 0, g, n, b0, nb, mg->Ad);
+      hipLaunchKernelGGL(k_gj_cols, dim3(strips), dim3(256), 0, g, n, b0, nb, mg->Ad2, mg->Ad);
     }
-    hipLaunchKernelGGL(k_cast<T>, dim3(grid_for(n * n)), dim3(kBlock), 0, s, n * n, mg->Ad,
+    hipLaunchKernelGGL(k_cast<T>, dim3(grid_for(n * n)), dim3(kBlock), 0, g, n * n, mg->Ad,
                        (T*)mg->Ainv);
     SKT_LAUNCH_CHECK();
+    SKT_HIP(hipEventRecord(mg->ev_join, g));
   }
   // smoother bounds: power iteration on D^-1 A (one column) per level
   double* inv = mg->lam + (L + 1);  // scratch scalar slot
@@ -1421,6 +1441,7 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   hipLaunchKernelGGL(k_cheb_coef, dim3(1), dim3(64), 0, s, L, mg->degree, mg->ratio, mg->lam,
                      mg->coef);
   SKT_LAUNCH_CHECK();
+  SKT_HIP(hipStreamWaitEvent(s, mg->ev_join, 0));
   return SKT_OK;
 }
 
