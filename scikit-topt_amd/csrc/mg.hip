@@ -1134,9 +1134,10 @@ void mg_free(MGData* mg) {
   if (mg->side) {
     (void)hipStreamDestroy(mg->side);
     (void)hipEventDestroy(mg->ev_fork);
+    (void)hipEventDestroy(mg->ev_st);
     (void)hipEventDestroy(mg->ev_join);
     mg->side = nullptr;
-    mg->ev_fork = mg->ev_join = nullptr;
+    mg->ev_fork = mg->ev_st = mg->ev_join = nullptr;
   }
   (void)hipFree(mg->Mch);
   (void)hipFree(mg->gflag);
@@ -1291,6 +1292,21 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   const int NC = op->ncomp, ND = 8 * NC;
   const int L = (int)mg->lv.size() - 1;
   const double* dK0 = mg->K0b;
+  // The hierarchy (Galerkin products, stencils, the coarsest inverse) is
+  // built on a side stream while the main stream runs the power iterations:
+  // the fine level's needs only the fine operator, the coarse levels' wait
+  // for the stencils (ev_st), and the set-up joins the side stream (ev_join)
+  // before returning.  The side stream touches the level matrices,
+  // stencils, Jacobi inverses and the coarsest buffers only.
+  if (!mg->side) {
+    SKT_HIP(hipStreamCreateWithFlags(&mg->side, hipStreamNonBlocking));
+    SKT_HIP(hipEventCreateWithFlags(&mg->ev_fork, hipEventDisableTiming));
+    SKT_HIP(hipEventCreateWithFlags(&mg->ev_st, hipEventDisableTiming));
+    SKT_HIP(hipEventCreateWithFlags(&mg->ev_join, hipEventDisableTiming));
+  }
+  SKT_HIP(hipEventRecord(mg->ev_fork, s));
+  SKT_HIP(hipStreamWaitEvent(mg->side, mg->ev_fork, 0));
+  hipStream_t g = mg->side;
   // Galerkin hierarchy.  The first coarse level takes E_j M_j for its
   // interior elements (M_j and the boundary flags depend on the operator
   // only, so they are built once) and the masked product for the rest.
@@ -1301,18 +1317,18 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
     if (!mg->Mch) {
       SKT_HIP(hipMalloc(&mg->Mch, sizeof(double) * (size_t)nch * ND * ND));
       if (NC == 3)
-        hipLaunchKernelGGL((k_galerkin_children<3>), dim3(nch), dim3(64), 0, s, c.f[0], c.f[1],
+        hipLaunchKernelGGL((k_galerkin_children<3>), dim3(nch), dim3(64), 0, g, c.f[0], c.f[1],
                            c.f[2], dK0, mg->Mch);
       else
-        hipLaunchKernelGGL((k_galerkin_children<1>), dim3(nch), dim3(64), 0, s, c.f[0], c.f[1],
+        hipLaunchKernelGGL((k_galerkin_children<1>), dim3(nch), dim3(64), 0, g, c.f[0], c.f[1],
                            c.f[2], dK0, mg->Mch);
       if (op->n_fixed > 0) {
         SKT_HIP(hipMalloc(&mg->gflag, (size_t)nce));
         if (NC == 3)
-          hipLaunchKernelGGL((k_galerkin_flags<3>), dim3(grid_for(nce)), dim3(kBlock), 0, s,
+          hipLaunchKernelGGL((k_galerkin_flags<3>), dim3(grid_for(nce)), dim3(kBlock), 0, g,
                              c.nx, c.ny, c.nz, c.f[0], c.f[1], c.f[2], op->fixed, mg->gflag);
         else
-          hipLaunchKernelGGL((k_galerkin_flags<1>), dim3(grid_for(nce)), dim3(kBlock), 0, s,
+          hipLaunchKernelGGL((k_galerkin_flags<1>), dim3(grid_for(nce)), dim3(kBlock), 0, g,
                              c.nx, c.ny, c.nz, c.f[0], c.f[1], c.f[2], op->fixed, mg->gflag);
       }
       SKT_LAUNCH_CHECK();
@@ -1320,18 +1336,18 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
     SKT_CHECK(nch <= kMaxChildren, "internal: coarsening factor above 5");
     const unsigned gi = (unsigned)std::min<int64_t>(nce, 8192);
     if (NC == 3)
-      hipLaunchKernelGGL((k_galerkin_interior<3>), dim3(gi), dim3(64 * 9), 0, s, c.nx, c.ny, c.nz,
+      hipLaunchKernelGGL((k_galerkin_interior<3>), dim3(gi), dim3(64 * 9), 0, g, c.nx, c.ny, c.nz,
                          c.f[0], c.f[1], c.f[2], mg->Mch, E64, mg->gflag, c.Kel);
     else
-      hipLaunchKernelGGL((k_galerkin_interior<1>), dim3(gi), dim3(64), 0, s, c.nx, c.ny, c.nz,
+      hipLaunchKernelGGL((k_galerkin_interior<1>), dim3(gi), dim3(64), 0, g, c.nx, c.ny, c.nz,
                          c.f[0], c.f[1], c.f[2], mg->Mch, E64, mg->gflag, c.Kel);
     if (mg->gflag) {
       if (NC == 3)
-        hipLaunchKernelGGL((k_galerkin<3>), dim3((unsigned)nce), dim3(64), 0, s, c.nx, c.ny,
+        hipLaunchKernelGGL((k_galerkin<3>), dim3((unsigned)nce), dim3(64), 0, g, c.nx, c.ny,
                            c.nz, c.f[0], c.f[1], c.f[2], nullptr, dK0, E64, op->fixed,
                            mg->gflag, c.Kel);
       else
-        hipLaunchKernelGGL((k_galerkin<1>), dim3((unsigned)nce), dim3(64), 0, s, c.nx, c.ny,
+        hipLaunchKernelGGL((k_galerkin<1>), dim3((unsigned)nce), dim3(64), 0, g, c.nx, c.ny,
                            c.nz, c.f[0], c.f[1], c.f[2], nullptr, dK0, E64, op->fixed,
                            mg->gflag, c.Kel);
     }
@@ -1342,10 +1358,10 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
     const int64_t nce = c.nx * c.ny * c.nz;
     const double* Kf = mg->lv[l - 1].Kel;
     if (NC == 3)
-      hipLaunchKernelGGL((k_galerkin<3>), dim3((unsigned)nce), dim3(64), 0, s, c.nx, c.ny, c.nz,
+      hipLaunchKernelGGL((k_galerkin<3>), dim3((unsigned)nce), dim3(64), 0, g, c.nx, c.ny, c.nz,
                          c.f[0], c.f[1], c.f[2], Kf, dK0, E64, nullptr, nullptr, c.Kel);
     else
-      hipLaunchKernelGGL((k_galerkin<1>), dim3((unsigned)nce), dim3(64), 0, s, c.nx, c.ny, c.nz,
+      hipLaunchKernelGGL((k_galerkin<1>), dim3((unsigned)nce), dim3(64), 0, g, c.nx, c.ny, c.nz,
                          c.f[0], c.f[1], c.f[2], Kf, dK0, E64, nullptr, nullptr, c.Kel);
     SKT_LAUNCH_CHECK();
   }
@@ -1355,9 +1371,9 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
     const int64_t tot = (v.tiled ? st_tiles(v.nx, v.ny, v.nz) * 256 : v.nn) * 27 * NC * NC;
 #define SKT_ASM(NC_, TL_)                                                                        \
   do {                                                                                           \
-    hipLaunchKernelGGL((k_stencil_asm<T, NC_, TL_>), dim3(grid_for(tot)), dim3(kBlock), 0, s,     \
+    hipLaunchKernelGGL((k_stencil_asm<T, NC_, TL_>), dim3(grid_for(tot)), dim3(kBlock), 0, g,     \
                        v.nx, v.ny, v.nz, v.Kel, (T*)v.st);                                        \
-    hipLaunchKernelGGL((k_stencil_dinv<T, NC_, TL_>), dim3(grid_for(v.ndof)), dim3(kBlock), 0, s, \
+    hipLaunchKernelGGL((k_stencil_dinv<T, NC_, TL_>), dim3(grid_for(v.ndof)), dim3(kBlock), 0, g, \
                        v.nx, v.ny, v.nz, (const T*)v.st, (T*)v.dinv);                             \
   } while (0)
     if (NC == 3) {
@@ -1370,28 +1386,18 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
 #undef SKT_ASM
     SKT_LAUNCH_CHECK();
   }
+  SKT_HIP(hipEventRecord(mg->ev_st, g));
   // coarsest: dense matrix, blocked Gauss-Jordan inverse, cast to T
   {
     const MGLevel& c = mg->lv[L];
     const int64_t n = c.ndof;
     if (NC == 3)
-      hipLaunchKernelGGL((k_dense_asm<3>), dim3(grid_for(n * n)), dim3(kBlock), 0, s, c.nx, c.ny,
+      hipLaunchKernelGGL((k_dense_asm<3>), dim3(grid_for(n * n)), dim3(kBlock), 0, g, c.nx, c.ny,
                          c.nz, c.Kel, mg->Ad);
     else
-      hipLaunchKernelGGL((k_dense_asm<1>), dim3(grid_for(n * n)), dim3(kBlock), 0, s, c.nx, c.ny,
+      hipLaunchKernelGGL((k_dense_asm<1>), dim3(grid_for(n * n)), dim3(kBlock), 0, g, c.nx, c.ny,
                          c.nz, c.Kel, mg->Ad);
-    hipLaunchKernelGGL(k_dense_fix_diag, dim3(grid_for(n)), dim3(kBlock), 0, s, n, mg->Ad);
-    // The Gauss-Jordan chain (mostly small, latency-bound launches) runs on
-    // a side stream while the power iterations below fill the GPU; it only
-    // touches Ad, Ad2 and Ainv, and the set-up joins it before returning.
-    if (!mg->side) {
-      SKT_HIP(hipStreamCreateWithFlags(&mg->side, hipStreamNonBlocking));
-      SKT_HIP(hipEventCreateWithFlags(&mg->ev_fork, hipEventDisableTiming));
-      SKT_HIP(hipEventCreateWithFlags(&mg->ev_join, hipEventDisableTiming));
-    }
-    SKT_HIP(hipEventRecord(mg->ev_fork, s));
-    SKT_HIP(hipStreamWaitEvent(mg->side, mg->ev_fork, 0));
-    hipStream_t g = mg->side;
+    hipLaunchKernelGGL(k_dense_fix_diag, dim3(grid_for(n)), dim3(kBlock), 0, g, n, mg->Ad);
     const unsigned tiles = (unsigned)((n + kTile - 1) / kTile);
     const unsigned strips = (unsigned)((n + kTileRC - 1) / kTileRC);
     for (int64_t b0 = 0; b0 < n; b0 += kGJ) {
@@ -1410,6 +1416,7 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   double* inv = mg->lam + (L + 1);  // scratch scalar slot
   int64_t off = 0;
   for (int l = 0; l < L; ++l) {
+    if (l == 1) SKT_HIP(hipStreamWaitEvent(s, mg->ev_st, 0));
     const MGLevel& v = mg->lv[l];
     const int64_t n = v.ndof;
     T* pv0 = (T*)mg->pvw + off;      // this level's vector, kept for the next set-up

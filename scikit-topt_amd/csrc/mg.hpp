@@ -54,8 +54,8 @@ struct MGData {
   double* K0b = nullptr;              // unit element matrix, corner-bit order
   double* Mch = nullptr;              // P_j^T K0 P_j per child position of level 1
   uint8_t* gflag = nullptr;           // level-1 elements with Dirichlet DOFs below
-  hipStream_t side = nullptr;         // set-up: coarsest inverse beside the power iterations
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t side = nullptr;         // set-up: hierarchy beside the power iterations
+  hipEvent_t ev_fork = nullptr, ev_st = nullptr, ev_join = nullptr;
   void* pv = nullptr;                 // power-iteration scratch (2 * ndof0) in T
   void* pvw = nullptr;                // per level (levels 0..L-1, concatenated) the last
                                       // normalised power-iteration vector in T
