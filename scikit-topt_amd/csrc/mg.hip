@@ -276,56 +276,75 @@ __device__ __forceinline__ double couple(const double* __restrict__ Kel, int64_t
 // Stencil storage.  Coefficient c = (o * NC + d) * NC + e of a node couples
 // its component d to component e of the neighbour at offset
 // o = (ox+1) + 3 (oy+1) + 9 (oz+1).
-// * Node-major levels (small ones): st[c * nn + node].
-// * Tile-major levels (those applied by the LDS-tiled kernel): nodes are
-//   grouped in kSX x kSY x kSZ tiles (tile-major, x fastest in both) and
-//   st[(tile * 27 NC^2 + c) * 256 + local], padding nodes past the grid
-//   holding zeros.  A wavefront's 16 x 4 nodes then read 256 contiguous
-//   bytes per coefficient instead of four 64-byte pieces of separate rows;
-//   the level-1 apply at 60x30x30 went from 28.6 to 13.9 us.
-constexpr int kSX = 16, kSY = 4, kSZ = 4;
-constexpr int kHX = kSX + 2, kHY = kSY + 2, kHZ = kSZ + 2;
-static_assert(kSX * kSY * kSZ == 256, "one tile-major node block is 256 nodes");
+// * Node-major: st[c * nn + node].
+// * Tile-major (levels applied by the LDS-tiled kernel): nodes are grouped
+//   in tsx x tsy x tsz tiles (tile-major, x fastest in both) and
+//   st[(tile * 27 NC^2 + c) * ts + local], ts = tsx tsy tsz, padding nodes
+//   past the grid holding zeros.  A wavefront's nodes then read contiguous
+//   bytes per coefficient instead of 64-byte pieces of separate node rows;
+//   the 60x30x30 level's apply went from 28.6 to 13.9 us.
+// Tile shapes: 16x4x4 for levels of at least kTiledNodes nodes, 16x2x2
+// (more, smaller workgroups) below: the 30x15x15 level's apply takes 6.9 us
+// with it, 11.4 us with 16x4x4 tiles and 17.5 us node-major.
+struct TileShape {
+  int x, y, z;
+};
+constexpr TileShape kTileBig = {16, 4, 4}, kTileSmall = {16, 2, 2};
+enum { ST_NODE = 0, ST_TILE_BIG = 1, ST_TILE_SMALL = 2 };
 
-__host__ __device__ inline int64_t st_tiles(int64_t nx, int64_t ny, int64_t nz) {
-  return ((nx + kSX) / kSX) * ((ny + kSY) / kSY) * ((nz + kSZ) / kSZ);
+__host__ __device__ inline TileShape tile_shape(int layout) {
+  return layout == ST_TILE_BIG ? kTileBig : kTileSmall;
+}
+
+__host__ __device__ inline int64_t st_tiles(int64_t nx, int64_t ny, int64_t nz, TileShape t) {
+  return ((nx + t.x) / t.x) * ((ny + t.y) / t.y) * ((nz + t.z) / t.z);
+}
+
+// nodes the storage holds (tile padding included)
+__host__ __device__ inline int64_t st_nodes(int64_t nx, int64_t ny, int64_t nz, int layout) {
+  if (layout == ST_NODE) return (nx + 1) * (ny + 1) * (nz + 1);
+  const TileShape t = tile_shape(layout);
+  return st_tiles(nx, ny, nz, t) * (t.x * t.y * t.z);
 }
 
 // offset of coefficient c of node (i, j, k)
-template <int NC, bool TILED>
-__device__ __forceinline__ int64_t st_index(int64_t nx, int64_t ny, int64_t nz, int64_t i,
-                                            int64_t j, int64_t k, int c) {
+template <int NC>
+__device__ __forceinline__ int64_t st_index(int64_t nx, int64_t ny, int64_t nz, int layout,
+                                            int64_t i, int64_t j, int64_t k, int c) {
   const int64_t nnx = nx + 1, nny = ny + 1;
-  if constexpr (TILED) {
-    const int64_t bx = (nx + kSX) / kSX, by = (ny + kSY) / kSY;
-    const int64_t tile = i / kSX + bx * (j / kSY + by * (k / kSZ));
-    const int local = (int)(i % kSX + kSX * (j % kSY + kSY * (k % kSZ)));
-    return (tile * (27 * NC * NC) + c) * 256 + local;
-  } else {
-    return c * (nnx * nny * (nz + 1)) + i + nnx * (j + nny * k);
+  if (layout != ST_NODE) {
+    const TileShape t = tile_shape(layout);
+    const int ts = t.x * t.y * t.z;
+    const int64_t bx = (nx + t.x) / t.x, by = (ny + t.y) / t.y;
+    const int64_t tile = i / t.x + bx * (j / t.y + by * (k / t.z));
+    const int local = (int)(i % t.x + t.x * (j % t.y + t.y * (k % t.z)));
+    return (tile * (27 * NC * NC) + c) * ts + local;
   }
+  return c * (nnx * nny * (nz + 1)) + i + nnx * (j + nny * k);
 }
 
 // One entry per thread over the whole storage (tile padding included).
-template <typename T, int NC, bool TILED>
-__global__ void k_stencil_asm(int64_t nx, int64_t ny, int64_t nz, const double* __restrict__ Kel,
-                              T* __restrict__ st) {
+template <typename T, int NC>
+__global__ void k_stencil_asm(int64_t nx, int64_t ny, int64_t nz, int layout,
+                              const double* __restrict__ Kel, T* __restrict__ st) {
   constexpr int NS = 27 * NC * NC;
   const int64_t nnx = nx + 1, nny = ny + 1, nnz = nz + 1;
   const int64_t nn = nnx * nny * nnz;
-  const int64_t total = TILED ? st_tiles(nx, ny, nz) * NS * 256 : nn * NS;
-  const int64_t bx = (nx + kSX) / kSX, by = (ny + kSY) / kSY;
+  const int64_t total = st_nodes(nx, ny, nz, layout) * NS;
+  const TileShape ts = tile_shape(layout);
+  const int tsn = ts.x * ts.y * ts.z;
+  const int64_t bx = (nx + ts.x) / ts.x, by = (ny + ts.y) / ts.y;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     int64_t i, j, k;
     int c;
-    if constexpr (TILED) {
-      const int local = (int)(t % 256);
-      c = (int)((t / 256) % NS);
-      const int64_t tile = t / (256 * NS);
-      i = (tile % bx) * kSX + local % kSX;
-      j = ((tile / bx) % by) * kSY + (local / kSX) % kSY;
-      k = (tile / (bx * by)) * kSZ + local / (kSX * kSY);
+    if (layout != ST_NODE) {
+      const int local = (int)(t % tsn);
+      c = (int)((t / tsn) % NS);
+      const int64_t tile = t / ((int64_t)tsn * NS);
+      i = (tile % bx) * ts.x + local % ts.x;
+      j = ((tile / bx) % by) * ts.y + (local / ts.x) % ts.y;
+      k = (tile / (bx * by)) * ts.z + local / (ts.x * ts.y);
       if (i >= nnx || j >= nny || k >= nnz) {
         st[t] = T(0);
         continue;
@@ -345,16 +364,16 @@ __global__ void k_stencil_asm(int64_t nx, int64_t ny, int64_t nz, const double* 
 
 // Jacobi inverse of a stencil level; rows with no stiffness (DOFs that no
 // free fine DOF depends on) get 0 and stay 0 through the cycle.
-template <typename T, int NC, bool TILED>
-__global__ void k_stencil_dinv(int64_t nx, int64_t ny, int64_t nz, const T* __restrict__ st,
-                               T* __restrict__ dinv) {
+template <typename T, int NC>
+__global__ void k_stencil_dinv(int64_t nx, int64_t ny, int64_t nz, int layout,
+                               const T* __restrict__ st, T* __restrict__ dinv) {
   const int64_t nnx = nx + 1, nny = ny + 1, nn = nnx * nny * (nz + 1);
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nn * NC;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t node = t / NC;
     const int d = (int)(t % NC);
     const int64_t i = node % nnx, j = (node / nnx) % nny, k = node / (nnx * nny);
-    const T v = st[st_index<NC, TILED>(nx, ny, nz, i, j, k, (13 * NC + d) * NC + d)];
+    const T v = st[st_index<NC>(nx, ny, nz, layout, i, j, k, (13 * NC + d) * NC + d)];
     dinv[t] = v > T(0) ? T(1) / v : T(0);
   }
 }
@@ -735,36 +754,38 @@ __global__ void k_cheb_first(int64_t total, int64_t ndof, int kc, const double* 
     x[t] = c2 * dinv[(t / kc) % ndof] * b[t];
 }
 
-// y = S x on a stencil level, LDS-tiled: a workgroup owns a kSX x kSY x kSZ
+// y = S x on a stencil level, LDS-tiled: a workgroup owns a SX x SY x SZ
 // block of nodes and stages x of the block and its one-node halo for GM
 // column groups in LDS once.  NC threads share a node, one per component
-// row, so a workgroup runs 256 NC threads: each sums its row's 27 stencil
-// blocks for all GM groups.  Stencil rows are read once per node.
-template <typename T, int NC, int KC, int GM, int MODE>
-__global__ void __launch_bounds__(256 * NC)
+// row, so a workgroup runs TS NC threads (TS = SX SY SZ): each sums its
+// row's 27 stencil blocks for all GM groups.  Stencil rows are read once per
+// node, from the tile-major storage of the same tile shape.
+template <typename T, int NC, int KC, int GM, int MODE, int SX, int SY, int SZ>
+__global__ void __launch_bounds__(SX * SY * SZ * NC)
 k_stencil_apply_tiled(int64_t nx, int64_t ny, int64_t nz, int groups, const T* __restrict__ st,
                       const T* __restrict__ X, T* __restrict__ Y, StOut<T> oa) {
+  constexpr int HX = SX + 2, HY = SY + 2, HZ = SZ + 2, TS = SX * SY * SZ;
   constexpr int V = NC * KC;          // values of one node and one group
   constexpr int W = V * GM;           // values of one node in LDS
-  __shared__ T xs[kHZ * kHY * kHX * W];
+  __shared__ T xs[HZ * HY * HX * W];
   const int64_t nnx = nx + 1, nny = ny + 1, nnz = nz + 1;
   const int64_t nn = nnx * nny * nnz;
-  const int bx = (int)((nnx + kSX - 1) / kSX), by = (int)((nny + kSY - 1) / kSY),
-            bz = (int)((nnz + kSZ - 1) / kSZ);
+  const int bx = (int)((nnx + SX - 1) / SX), by = (int)((nny + SY - 1) / SY),
+            bz = (int)((nnz + SZ - 1) / SZ);
   int b = blockIdx.x;
   const int tbx = b % bx; b /= bx;
   const int tby = b % by; b /= by;
   const int tbz = b % bz; b /= bz;
   const int g0 = b * GM;
   const int ng = groups - g0 < GM ? groups - g0 : GM;
-  const int64_t x0 = (int64_t)tbx * kSX, y0 = (int64_t)tby * kSY, z0 = (int64_t)tbz * kSZ;
+  const int64_t x0 = (int64_t)tbx * SX, y0 = (int64_t)tby * SY, z0 = (int64_t)tbz * SZ;
   // this tile's coefficients (tile-major storage)
-  st += (int64_t)(tbx + bx * (tby + by * tbz)) * (27 * NC * NC) * 256;
+  st += (int64_t)(tbx + bx * (tby + by * tbz)) * (27 * NC * NC) * TS;
   // stage x (halo nodes outside the grid read as zero; their stencil
   // entries are zero as well)
-  for (int it = threadIdx.x; it < kHZ * kHY * kHX * GM; it += 256 * NC) {
+  for (int it = threadIdx.x; it < HZ * HY * HX * GM; it += TS * NC) {
     const int q = it % GM, hn = it / GM;
-    const int hx = hn % kHX, hy = (hn / kHX) % kHY, hz = hn / (kHX * kHY);
+    const int hx = hn % HX, hy = (hn / HX) % HY, hz = hn / (HX * HY);
     const int64_t gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
     T v[V];
     if (q < ng && gx >= 0 && gx < nnx && gy >= 0 && gy < nny && gz >= 0 && gz < nnz) {
@@ -777,8 +798,8 @@ k_stencil_apply_tiled(int64_t nx, int64_t ny, int64_t nz, int groups, const T* _
     for (int i = 0; i < V; ++i) xs[hn * W + q * V + i] = v[i];
   }
   __syncthreads();
-  const int nt = threadIdx.x % 256, d = threadIdx.x / 256;
-  const int tx = nt % kSX, ty = (nt / kSX) % kSY, tz = nt / (kSX * kSY);
+  const int nt = threadIdx.x % TS, d = threadIdx.x / TS;
+  const int tx = nt % SX, ty = (nt / SX) % SY, tz = nt / (SX * SY);
   const int64_t gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
   if (gx >= nnx || gy >= nny || gz >= nnz) return;
   const int64_t node = gx + nnx * (gy + nny * gz);
@@ -790,10 +811,10 @@ k_stencil_apply_tiled(int64_t nx, int64_t ny, int64_t nz, int groups, const T* _
 #pragma unroll 3
   for (int o = 0; o < 27; ++o) {
     const int ox = o % 3, oy = (o / 3) % 3, oz = o / 9;  // halo offsets 0..2
-    const T* xn = xs + ((tz + oz) * kHY * kHX + (ty + oy) * kHX + (tx + ox)) * W;
+    const T* xn = xs + ((tz + oz) * HY * HX + (ty + oy) * HX + (tx + ox)) * W;
     T a[NC];
 #pragma unroll
-    for (int e = 0; e < NC; ++e) a[e] = st[((o * NC + d) * NC + e) * 256 + nt];
+    for (int e = 0; e < NC; ++e) a[e] = st[((o * NC + d) * NC + e) * TS + nt];
 #pragma unroll
     for (int q = 0; q < GM; ++q)
 #pragma unroll
@@ -807,7 +828,7 @@ k_stencil_apply_tiled(int64_t nx, int64_t ny, int64_t nz, int groups, const T* _
     c2 = (T)oa.coef[1];
     dv = oa.dinv[node * NC + d];
   }
-  const T* xcen = xs + ((tz + 1) * kHY * kHX + (ty + 1) * kHX + (tx + 1)) * W;
+  const T* xcen = xs + ((tz + 1) * HY * HX + (ty + 1) * HX + (tx + 1)) * W;
 #pragma unroll
   for (int q = 0; q < GM; ++q) {
     if (q >= ng) break;
@@ -1152,16 +1173,16 @@ void mg_free(MGData* mg) {
   mg->dtype = -1;
 }
 
-// Node count from which a level is LDS-tiled; SKT_MG_TILED_NODES overrides
-// it (0 tiles every stencil level), read when a hierarchy is allocated.
-static int64_t tiled_nodes() {
-  const char* v = std::getenv("SKT_MG_TILED_NODES");
-  if (v && *v) {
-    char* end = nullptr;
-    const long long n = std::strtoll(v, &end, 10);
-    if (end && *end == 0 && n >= 0) return (int64_t)n;
-  }
-  return kTiledNodes;
+// Stencil layout of a level with nn nodes: big tiles from kTiledNodes
+// nodes, small tiles below.  SKT_MG_STENCIL_LAYOUT=node|big|small forces one
+// layout on every level (read when a hierarchy is allocated; the three sum
+// in the same order, which a test checks).
+static int stencil_layout(int64_t nn) {
+  const char* v = std::getenv("SKT_MG_STENCIL_LAYOUT");
+  if (v && std::strcmp(v, "node") == 0) return ST_NODE;
+  if (v && std::strcmp(v, "big") == 0) return ST_TILE_BIG;
+  if (v && std::strcmp(v, "small") == 0) return ST_TILE_SMALL;
+  return nn >= kTiledNodes ? ST_TILE_BIG : ST_TILE_SMALL;
 }
 
 int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype, int max_cols) {
@@ -1191,9 +1212,8 @@ int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype,
     SKT_HIP(hipMalloc(&v.b, ts * nv));
     SKT_HIP(hipMalloc(&v.x, ts * nv));
     if (l < L) {
-      v.tiled = v.nn >= tiled_nodes() ? 1 : 0;
-      const int64_t st_nodes = v.tiled ? st_tiles(v.nx, v.ny, v.nz) * 256 : v.nn;
-      SKT_HIP(hipMalloc(&v.st, ts * (size_t)st_nodes * 27 * NC * NC));
+      v.layout = stencil_layout(v.nn);
+      SKT_HIP(hipMalloc(&v.st, ts * (size_t)st_nodes(v.nx, v.ny, v.nz, v.layout) * 27 * NC * NC));
       SKT_HIP(hipMalloc(&v.dinv, ts * v.ndof));
       SKT_HIP(hipMalloc(&v.w, ts * nv));
       SKT_HIP(hipMalloc(&v.d, ts * nv));
@@ -1240,22 +1260,29 @@ static int level_apply(MGData* mg, const skt_operator* op, const void* E, int l,
   const MGLevel& v = mg->lv[l];
   const int groups = k / kc;
   const T* st = (const T*)v.st;
-  // large levels: LDS-tiled kernel (x staged once per tile, tile-major
-  // stencil); small levels: one (node, row, column group) per thread for
-  // memory-level parallelism
-  const bool tiled = v.tiled != 0;
-  constexpr int GMT = sizeof(T) == 4 ? 4 : 2;  // 62 KB of LDS either way for NC * KC = 6
-  const int64_t tiles = st_tiles(v.nx, v.ny, v.nz);
-  const unsigned gt = (unsigned)(tiles * ((groups + GMT - 1) / GMT));
+  // tile-major levels: LDS-tiled kernel of the level's tile shape (x staged
+  // once per tile); node-major: one (node, row, column group) per thread
+  constexpr int GMT = sizeof(T) == 4 ? 4 : 2;  // <= 62 KB of LDS for NC * KC = 6
+  const unsigned gmb = (unsigned)((groups + GMT - 1) / GMT);
   const unsigned g = grid_for(v.nn * op->ncomp * groups);
 #define SKT_SA(NC_, KC_)                                                                         \
   do {                                                                                           \
-    if (tiled)                                                                                   \
-      hipLaunchKernelGGL((k_stencil_apply_tiled<T, NC_, KC_, GMT, MODE>), dim3(gt),               \
-                         dim3(256 * NC_), 0, s, v.nx, v.ny, v.nz, groups, st, x, y, oa);         \
-    else                                                                                         \
+    if (v.layout == ST_TILE_BIG) {                                                               \
+      const unsigned gt = (unsigned)(st_tiles(v.nx, v.ny, v.nz, kTileBig) * gmb);                \
+      hipLaunchKernelGGL((k_stencil_apply_tiled<T, NC_, KC_, GMT, MODE, kTileBig.x, kTileBig.y,   \
+                                                kTileBig.z>),                                    \
+                         dim3(gt), dim3(kTileBig.x * kTileBig.y * kTileBig.z * NC_), 0, s, v.nx,  \
+                         v.ny, v.nz, groups, st, x, y, oa);                                      \
+    } else if (v.layout == ST_TILE_SMALL) {                                                      \
+      const unsigned gt = (unsigned)(st_tiles(v.nx, v.ny, v.nz, kTileSmall) * gmb);              \
+      hipLaunchKernelGGL((k_stencil_apply_tiled<T, NC_, KC_, GMT, MODE, kTileSmall.x,             \
+                                                kTileSmall.y, kTileSmall.z>),                    \
+                         dim3(gt), dim3(kTileSmall.x * kTileSmall.y * kTileSmall.z * NC_), 0, s,  \
+                         v.nx, v.ny, v.nz, groups, st, x, y, oa);                                \
+    } else {                                                                                     \
       hipLaunchKernelGGL((k_stencil_apply<T, NC_, KC_, 1, MODE>), dim3(g), dim3(kBlock), 0, s,    \
                          v.nx, v.ny, v.nz, groups, st, x, y, oa);                                \
+    }                                                                                            \
   } while (0)
   if (op->ncomp == 3) {
     if (kc == 1) SKT_SA(3, 1);
@@ -1368,22 +1395,18 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   // stencils and Jacobi inverses of the intermediate levels
   for (int l = 1; l < L; ++l) {
     const MGLevel& v = mg->lv[l];
-    const int64_t tot = (v.tiled ? st_tiles(v.nx, v.ny, v.nz) * 256 : v.nn) * 27 * NC * NC;
-#define SKT_ASM(NC_, TL_)                                                                        \
-  do {                                                                                           \
-    hipLaunchKernelGGL((k_stencil_asm<T, NC_, TL_>), dim3(grid_for(tot)), dim3(kBlock), 0, g,     \
-                       v.nx, v.ny, v.nz, v.Kel, (T*)v.st);                                        \
-    hipLaunchKernelGGL((k_stencil_dinv<T, NC_, TL_>), dim3(grid_for(v.ndof)), dim3(kBlock), 0, g, \
-                       v.nx, v.ny, v.nz, (const T*)v.st, (T*)v.dinv);                             \
-  } while (0)
+    const int64_t tot = st_nodes(v.nx, v.ny, v.nz, v.layout) * 27 * NC * NC;
     if (NC == 3) {
-      if (v.tiled) SKT_ASM(3, true);
-      else SKT_ASM(3, false);
+      hipLaunchKernelGGL((k_stencil_asm<T, 3>), dim3(grid_for(tot)), dim3(kBlock), 0, g, v.nx,
+                         v.ny, v.nz, v.layout, v.Kel, (T*)v.st);
+      hipLaunchKernelGGL((k_stencil_dinv<T, 3>), dim3(grid_for(v.ndof)), dim3(kBlock), 0, g,
+                         v.nx, v.ny, v.nz, v.layout, (const T*)v.st, (T*)v.dinv);
     } else {
-      if (v.tiled) SKT_ASM(1, true);
-      else SKT_ASM(1, false);
+      hipLaunchKernelGGL((k_stencil_asm<T, 1>), dim3(grid_for(tot)), dim3(kBlock), 0, g, v.nx,
+                         v.ny, v.nz, v.layout, v.Kel, (T*)v.st);
+      hipLaunchKernelGGL((k_stencil_dinv<T, 1>), dim3(grid_for(v.ndof)), dim3(kBlock), 0, g,
+                         v.nx, v.ny, v.nz, v.layout, (const T*)v.st, (T*)v.dinv);
     }
-#undef SKT_ASM
     SKT_LAUNCH_CHECK();
   }
   SKT_HIP(hipEventRecord(mg->ev_st, g));

@@ -28,7 +28,7 @@ struct MGLevel {
   int f[3] = {1, 1, 1};               // factor from the finer level
   double* Kel = nullptr;              // (n_el, ND*ND) Galerkin element matrices, fp64
   void* st = nullptr;                 // stencil in T (levels 1..L-1), node- or tile-major
-  int tiled = 0;                      // tile-major stencil, LDS-tiled apply
+  int layout = 0;                     // stencil layout (node-major, big or small tiles)
   void* dinv = nullptr;               // (ndof) in T
   void* b = nullptr;                  // right-hand side (ndof*k, blocked)
   void* x = nullptr;                  // iterate
@@ -37,8 +37,8 @@ struct MGLevel {
   void* rt = nullptr;                 // restriction staging (x-y restricted fine planes)
 };
 
-// Levels with at least this many nodes use the LDS-tiled stencil apply and
-// the tile-major stencil storage.
+// Levels with at least this many nodes use the big stencil tiles, smaller
+// ones the small tiles.
 constexpr int64_t kTiledNodes = 20000;
 
 struct MGData {

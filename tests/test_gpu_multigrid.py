@@ -99,22 +99,24 @@ def test_mg_request_on_unstructured_mesh_is_rejected(gpu):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_mg_tiled_stencil_levels_match_node_major(gpu, precision, monkeypatch):
-    """Coarse levels above a node-count threshold use the LDS-tiled stencil
-    apply with tile-major coefficient storage, smaller ones the node-major
-    kernel.  Both sum the 27 blocks in the same order, so forcing every level
-    onto the tiled path (SKT_MG_TILED_NODES=0) must reproduce the node-major
-    solve exactly, including the iteration count."""
+def test_mg_stencil_layouts_match(gpu, precision, monkeypatch):
+    """Coarse-level stencils are stored tile-major (big tiles on large
+    levels, small tiles below) and applied by the LDS-tiled kernel of that
+    tile shape, or node-major with one thread per row.  All three sum the 27
+    blocks in the same order, so forcing one layout on every level
+    (SKT_MG_STENCIL_LAYOUT) must reproduce the same solve exactly, including
+    the iteration count."""
     task = oprob.multiload(32, 16, 16, n_loads=2, seed=7)
     rng = np.random.default_rng(11)
     rho = rng.uniform(0.05, 1.0, task.n_elements)
     out = {}
-    for mode in ("0", "1000000000"):
-        monkeypatch.setenv("SKT_MG_TILED_NODES", mode)
+    for mode in ("node", "big", "small"):
+        monkeypatch.setenv("SKT_MG_STENCIL_LAYOUT", mode)
         fem = _fem(task, "elastic", precision, 1e-10, "mg", max_coarse=100)
         U = np.zeros((task.ndof, 2))
         fem.objectives_multi_load(rho, 3.0, U)
         assert fem.last_solve["levels"] >= 3
         out[mode] = (U, fem.last_solve["iters"])
-    assert out["0"][1] == out["1000000000"][1]
-    np.testing.assert_array_equal(out["0"][0], out["1000000000"][0])
+    for mode in ("big", "small"):
+        assert out[mode][1] == out["node"][1], mode
+        np.testing.assert_array_equal(out[mode][0], out["node"][0])
