@@ -426,9 +426,9 @@ constexpr int kTileRC = 32;
 //   column k, i != k: -a_ik / a_kk
 // The pivot index inside a thread's 4 x 4 block is a compile-time constant
 // of the unrolled inner loop, so there is no per-element branching.
-__global__ void __launch_bounds__(256) k_gj_block_inv(int64_t n, int64_t b0, int nb,
-                                                       double* __restrict__ A,
-                                                       double* __restrict__ Binv) {
+__device__ __forceinline__ void gj_block_inv(int64_t n, int64_t b0, int nb,
+                                             double* __restrict__ A,
+                                             double* __restrict__ Binv) {
   static_assert(kGJ == 64, "4 x 4 register blocks of a 64 x 64 matrix");
   __shared__ double prow[2][kGJ], pcol[2][kGJ];
   const int tid = threadIdx.x;
@@ -484,6 +484,12 @@ __global__ void __launch_bounds__(256) k_gj_block_inv(int64_t n, int64_t b0, int
       Binv[r * kGJ + c] = m[i][j];
       if (r < nb && c < nb) A[(b0 + r) * n + b0 + c] = m[i][j];
     }
+}
+
+__global__ void __launch_bounds__(256) k_gj_block_inv(int64_t n, int64_t b0, int nb,
+                                                       double* __restrict__ A,
+                                                       double* __restrict__ Binv) {
+  gj_block_inv(n, b0, nb, A, Binv);
 }
 
 // A_BR <- Binv A_BR on a strip of kTileRC columns (columns inside the block
@@ -572,14 +578,14 @@ __global__ void __launch_bounds__(256) k_gj_update(int64_t n, int64_t b0, int nb
 // A_RB <- -A_RB Binv on a strip of kTileRC rows (rows inside the block are
 // left alone); each thread keeps 8 outputs (one column, rows tid / 64 + 4 m)
 // in flight, the products over all kGJ columns as in k_gj_rows.
-__global__ void __launch_bounds__(256) k_gj_cols(int64_t n, int64_t b0, int nb,
-                                                  const double* __restrict__ Binv,
-                                                  double* __restrict__ A) {
+__device__ __forceinline__ void gj_cols(int64_t n, int64_t b0, int nb, int strip,
+                                        const double* __restrict__ Binv,
+                                        double* __restrict__ A) {
   static_assert(kGJ * kTileRC == 8 * 256, "8 outputs per thread");
   __shared__ double sB[kGJ][kGJ + 1];
   __shared__ double sX[kTileRC][kGJ + 1];
   const int tid = threadIdx.x;
-  const int64_t r0 = (int64_t)blockIdx.x * kTileRC;
+  const int64_t r0 = (int64_t)strip * kTileRC;
   for (int i = tid; i < kGJ * kGJ; i += 256) sB[i / kGJ][i % kGJ] = Binv[i];
   for (int i = tid; i < kTileRC * kGJ; i += 256) {
     const int r = i / kGJ, q = i % kGJ;
@@ -601,6 +607,22 @@ __global__ void __launch_bounds__(256) k_gj_cols(int64_t n, int64_t b0, int nb,
     if (row >= n || (row >= b0 && row < b0 + nb)) continue;
     A[row * n + b0 + c] = -acc[mm];
   }
+}
+
+// Column strips of pivot block b0 (Binv) and, in the one extra workgroup,
+// the inversion of the next pivot block b1 (nb1 > 0) into Binv1: that block
+// is final once the trailing update of b0 has run, and neither touches the
+// other's entries or inverse buffer.
+__global__ void __launch_bounds__(256) k_gj_cols_next(int64_t n, int64_t b0, int nb,
+                                                       const double* __restrict__ Binv,
+                                                       int64_t b1, int nb1,
+                                                       double* __restrict__ Binv1,
+                                                       double* __restrict__ A) {
+  if (blockIdx.x + 1 == gridDim.x) {
+    if (nb1 > 0) gj_block_inv(n, b1, nb1, A, Binv1);
+    return;
+  }
+  gj_cols(n, b0, nb, blockIdx.x, Binv, A);
 }
 
 template <typename T>
@@ -1091,18 +1113,28 @@ __global__ void k_pow_norm(int64_t n, const T* __restrict__ dinv, T* __restrict_
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// norm = sqrt(sum part); lam[level] = norm (input had unit norm); inv stored
+// norm = sqrt(sum part); inv_out = 1 / norm.  With lam_out, lam = norm *
+// inv_in (inv_in: 1 / the norm of the vector the operator was applied to).
 __global__ void k_pow_finish(int nb, const double* __restrict__ part, double* __restrict__ lam_out,
-                             double* __restrict__ inv_out) {
+                             const double* __restrict__ inv_in, double* __restrict__ inv_out) {
   __shared__ double sh[kBlock / kWave];
   double a = 0.0;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) a += part[i];
   const double s = block_sum(a, sh);
   if (threadIdx.x == 0) {
     const double nrm = sqrt(s);
-    if (lam_out) *lam_out = nrm;
+    if (lam_out) *lam_out = nrm * *inv_in;
     *inv_out = nrm > 0.0 ? 1.0 / nrm : 0.0;
   }
+}
+
+// v = dinv y (an unnormalised power step)
+template <typename T>
+__global__ void k_pow_dscale(int64_t n, const T* __restrict__ dinv, const T* __restrict__ y,
+                             T* __restrict__ v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = dinv[i] * y[i];
 }
 
 template <typename T>
@@ -1221,7 +1253,7 @@ int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype,
   }
   const int64_t nc = lv[L].ndof;
   SKT_HIP(hipMalloc(&mg->Ad, sizeof(double) * nc * nc));
-  SKT_HIP(hipMalloc(&mg->Ad2, sizeof(double) * kGJ * kGJ));
+  SKT_HIP(hipMalloc(&mg->Ad2, sizeof(double) * 2 * kGJ * kGJ));
   SKT_HIP(hipMalloc(&mg->Ainv, ts * nc * nc));
   SKT_HIP(hipMalloc(&mg->lam, sizeof(double) * (L + 1) * 2));
   SKT_HIP(hipMalloc(&mg->coef, sizeof(double) * (L + 1) * 2 * std::max(cfg.degree, 1)));
@@ -1410,8 +1442,12 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
     SKT_LAUNCH_CHECK();
   }
   SKT_HIP(hipEventRecord(mg->ev_st, g));
-  // coarsest: dense matrix, blocked Gauss-Jordan inverse, cast to T
-  {
+  // coarsest: dense matrix, blocked Gauss-Jordan inverse, cast to T (side
+  // stream).  Enqueued after the fine level's power iteration, so that the
+  // host's many small launches for it do not hold back the main stream.
+  bool coarsest_queued = false;
+  auto queue_coarsest = [&]() -> int {
+    coarsest_queued = true;
     const MGLevel& c = mg->lv[L];
     const int64_t n = c.ndof;
     if (NC == 3)
@@ -1423,23 +1459,35 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
     hipLaunchKernelGGL(k_dense_fix_diag, dim3(grid_for(n)), dim3(kBlock), 0, g, n, mg->Ad);
     const unsigned tiles = (unsigned)((n + kTile - 1) / kTile);
     const unsigned strips = (unsigned)((n + kTileRC - 1) / kTileRC);
-    for (int64_t b0 = 0; b0 < n; b0 += kGJ) {
+    // pivot block inverses alternate between the two halves of Ad2
+    hipLaunchKernelGGL(k_gj_block_inv, dim3(1), dim3(256), 0, g, n, (int64_t)0,
+                       (int)std::min<int64_t>(kGJ, n), mg->Ad, mg->Ad2);
+    int par = 0;
+    for (int64_t b0 = 0; b0 < n; b0 += kGJ, par ^= 1) {
       const int nb = (int)std::min<int64_t>(kGJ, n - b0);
-      hipLaunchKernelGGL(k_gj_block_inv, dim3(1), dim3(256), 0, g, n, b0, nb, mg->Ad, mg->Ad2);
-      hipLaunchKernelGGL(k_gj_rows, dim3(strips), dim3(256), 0, g, n, b0, nb, mg->Ad2, mg->Ad);
+      const int64_t b1 = b0 + kGJ;
+      const int nb1 = b1 < n ? (int)std::min<int64_t>(kGJ, n - b1) : 0;
+      double* bi = mg->Ad2 + par * kGJ * kGJ;
+      double* bi1 = mg->Ad2 + (par ^ 1) * kGJ * kGJ;
+      hipLaunchKernelGGL(k_gj_rows, dim3(strips), dim3(256), 0, g, n, b0, nb, bi, mg->Ad);
       hipLaunchKernelGGL(k_gj_update, dim3(tiles, tiles), dim3(256), 0, g, n, b0, nb, mg->Ad);
-      hipLaunchKernelGGL(k_gj_cols, dim3(strips), dim3(256), 0, g, n, b0, nb, mg->Ad2, mg->Ad);
+      hipLaunchKernelGGL(k_gj_cols_next, dim3(strips + 1), dim3(256), 0, g, n, b0, nb,
+                         (const double*)bi, b1, nb1, bi1, mg->Ad);
     }
     hipLaunchKernelGGL(k_cast<T>, dim3(grid_for(n * n)), dim3(kBlock), 0, g, n * n, mg->Ad,
                        (T*)mg->Ainv);
     SKT_LAUNCH_CHECK();
     SKT_HIP(hipEventRecord(mg->ev_join, g));
-  }
+    return SKT_OK;
+  };
   // smoother bounds: power iteration on D^-1 A (one column) per level
   double* inv = mg->lam + (L + 1);  // scratch scalar slot
   int64_t off = 0;
   for (int l = 0; l < L; ++l) {
-    if (l == 1) SKT_HIP(hipStreamWaitEvent(s, mg->ev_st, 0));
+    if (l == 1) {
+      if (int stt = queue_coarsest()) return stt;
+      SKT_HIP(hipStreamWaitEvent(s, mg->ev_st, 0));
+    }
     const MGLevel& v = mg->lv[l];
     const int64_t n = v.ndof;
     T* pv0 = (T*)mg->pvw + off;      // this level's vector, kept for the next set-up
@@ -1447,25 +1495,38 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
     off += n;
     const T* dinv = l == 0 ? (const T*)dinv0 : (const T*)v.dinv;
     const unsigned nb = std::min<unsigned>(grid_for(n), 1024u);
-    if (!mg->warm) {
+    if (!mg->warm)
       hipLaunchKernelGGL(k_pow_init<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, dinv, pv0);
-      hipLaunchKernelGGL(k_pow_norm<T>, dim3(nb), dim3(kBlock), 0, s, n, dinv, pv0, 0, mg->part);
-      hipLaunchKernelGGL(k_pow_finish, dim3(1), dim3(kBlock), 0, s, (int)nb, mg->part,
-                         (double*)nullptr, inv);
-      hipLaunchKernelGGL(k_pow_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, inv, pv0,
-                         pv0);
-    }
+    // The iterate is left unnormalised between steps (it grows by about
+    // lambda per step, far from the float range over these few steps); the
+    // last step's input and output norms give lambda, and the normalised
+    // result is kept for the next set-up.
     const int steps = mg->warm ? mg->power_iters_warm : mg->power_iters;
     for (int it = 0; it < steps; ++it) {
+      const bool last = it + 1 == steps;
+      if (last) {
+        hipLaunchKernelGGL(k_pow_norm<T>, dim3(nb), dim3(kBlock), 0, s, n, dinv, pv0, 0,
+                           mg->part);
+        hipLaunchKernelGGL(k_pow_finish, dim3(1), dim3(kBlock), 0, s, (int)nb, mg->part,
+                           (double*)nullptr, (const double*)nullptr, inv);
+      }
       int stt = level_apply<T>(mg, op, E, l, 1, 1, pv0, pv1, s);
       if (stt) return stt;
+      if (!last) {
+        hipLaunchKernelGGL(k_pow_dscale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, dinv, pv1,
+                           pv0);
+        continue;
+      }
       hipLaunchKernelGGL(k_pow_norm<T>, dim3(nb), dim3(kBlock), 0, s, n, dinv, pv1, 1, mg->part);
       hipLaunchKernelGGL(k_pow_finish, dim3(1), dim3(kBlock), 0, s, (int)nb, mg->part,
-                         mg->lam + l, inv);
+                         mg->lam + l, (const double*)inv, inv);
       hipLaunchKernelGGL(k_pow_scale<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, inv, pv1,
                          pv0);
     }
     SKT_LAUNCH_CHECK();
+  }
+  if (!coarsest_queued) {
+    if (int stt = queue_coarsest()) return stt;
   }
   mg->warm = 1;
   hipLaunchKernelGGL(k_cheb_coef, dim3(1), dim3(64), 0, s, L, mg->degree, mg->ratio, mg->lam,

@@ -46,7 +46,7 @@ struct MGData {
   int max_cols = 0;
   std::vector<MGLevel> lv;            // lv[0]: fine level (vector: x only)
   double* Ad = nullptr;               // coarsest dense matrix, fp64 (n x n), ping
-  double* Ad2 = nullptr;              // inverse of the current pivot block
+  double* Ad2 = nullptr;              // inverses of the current and next pivot blocks
   void* Ainv = nullptr;               // coarsest inverse in T
   double* lam = nullptr;              // per level lambda_max(D^-1 A) estimates
   double* coef = nullptr;             // per level Chebyshev (c1, c2) per step
