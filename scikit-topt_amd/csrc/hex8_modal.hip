@@ -181,16 +181,8 @@ struct OutArgs {
 // DOT: the kernel also leaves the per-column partial sums of x . y
 // (OUT_APPLY: the CG's p . K p) or b . x_out (OUT_CHEB: r . z of a V-cycle's
 // last smoothing step) over its owned nodes, reduced in a fixed order.
-// Waves per SIMD the compiler must leave room for: the LDS of a float32
-// elasticity workgroup allows four per CU, and the fused r.z of the float32
-// smoothing step would otherwise take it just past 128 VGPRs (three waves).
-template <typename T, int MODE, int SF, bool DOT>
-constexpr int modal_min_waves() {
-  return (sizeof(T) == 4 && SF == 0 && MODE == OUT_CHEB && DOT) ? 4 : 1;
-}
-
 template <typename T, int NC, int KC, int MODE, int SF, bool DOT>
-__global__ void __launch_bounds__(256, (modal_min_waves<T, MODE, SF, DOT>()))
+__global__ void __launch_bounds__(256)
 k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, int dstride,
                   int lz, int tiles_x, int tiles_y, int n_zchunks, ModalArgs<T> coef,
                   const T* __restrict__ E, const uint8_t* __restrict__ fixed,
@@ -217,6 +209,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   X += cbase;
   Y += cbase;
   constexpr bool kCheb = MODE == OUT_CHEB || MODE == OUT_CHEB0;
+  constexpr bool kLate = sizeof(T) == 4 && MODE == OUT_CHEB && DOT;
   const T* Bv = MODE != OUT_APPLY ? oa.B + cbase : nullptr;
   T c1 = T(0), c2 = T(0), s0 = T(0);
   if (kCheb) {
@@ -341,15 +334,19 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
     const bool out = is_node && ez >= z0;
     const int64_t po = ez * plane_stride + own_off;
     bool fix[NC];
+    // kLate (the float32 smoothing step with the fused r.z): the previous
+    // iterate and dinv are loaded in the epilogue instead, which keeps the
+    // kernel within 128 VGPRs (four waves per SIMD) without spills
     T ob[KC][NC], op[KC][NC], oi[NC];
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
       fix[d] = fixed && out && fixed[(ez * nnx * nny + own_node) * NC + d];
-      if constexpr (kCheb) oi[d] = out ? oa.dinv[(ez * nnx * nny + own_node) * NC + d] : T(0);
+      if constexpr (kCheb && !kLate)
+        oi[d] = out ? oa.dinv[(ez * nnx * nny + own_node) * NC + d] : T(0);
 #pragma unroll
       for (int c = 0; c < KC; ++c) {
         if constexpr (MODE != OUT_APPLY) ob[c][d] = out ? Bv[po + d * dstride + c] : T(0);
-        if constexpr (MODE == OUT_CHEB)
+        if constexpr (MODE == OUT_CHEB && !kLate)
           op[c][d] = (out && use_prev) ? Y[po + d * dstride + c] : T(0);
       }
     }
@@ -468,8 +465,13 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
               // Chebyshev step with the direction recovered from the two
               // iterates: x_out = x + c1 (x - x_prev) + c2 dinv (b - K x)
               // (x_prev = 0 for the implied first step of OUT_CHEB0)
-              const T xp = MODE == OUT_CHEB ? op[col][d] : T(0);
-              const T dn = c1 * (xo[col][d] - xp) + c2 * oi[d] * (ob[col][d] - y);
+              T xp = MODE == OUT_CHEB ? op[col][d] : T(0);
+              T di = oi[d];
+              if constexpr (kLate) {
+                xp = use_prev ? Yp[o] : T(0);
+                di = oa.dinv[(ez * nnx * nny + own_node) * NC + d];
+              }
+              const T dn = c1 * (xo[col][d] - xp) + c2 * di * (ob[col][d] - y);
               const T xn = xo[col][d] + dn;
               Yp[o] = xn;
               if constexpr (DOT) dacc[col] += (double)ob[col][d] * (double)xn;
