@@ -69,8 +69,11 @@ enum {
   S_REQ,     // 1: the column asks for a reliable update
   S_DELTA2,  // reliable-update threshold on r.r / r.r at the last update
   S_NRST,    // float32 breakdowns restarted so far (mixed mode)
-  S_RST      // 1: restart this column's search direction at the update
+  S_RST,     // 1: restart this column's search direction at the update
+  S_TRR,     // r.r of the last true (float64) residual
+  S_TCONV    // 1: that true residual meets the tolerance
 };
+static_assert(S_TCONV < 16, "scalar slots");
 constexpr int kMaxRestarts = 8;
 
 // Blocked-layout geometry of one solve.
@@ -370,6 +373,51 @@ __global__ void k_to32(int64_t total, const double* __restrict__ r64, float* __r
   }
 }
 
+// convergence test on the true residual alone (the float32 recurrence of
+// every column claims convergence): S_TRR, S_TCONV
+__global__ void k_true_conv(int nb, int k, const double* __restrict__ part_rr,
+                            double* __restrict__ sc) {
+  const int c = blockIdx.x;
+  const double rr = col_sum(part_rr, k, nb, 0, c);
+  if (threadIdx.x == 0) {
+    double* s = sc + c * kSc;
+    s[S_TRR] = rr;
+    s[S_TCONV] = (rr <= s[S_TOL2]) ? 1.0 : 0.0;
+  }
+}
+
+// reliable update inside a mixed-precision iteration: after x += alpha p,
+// r -= alpha q the residual has been replaced by the true one (r.r of it in
+// part_rr, quantity 0) and preconditioned (r.z in part_rz); this is k_beta
+// for that iteration.  The search direction restarts (beta = 0) where the
+// true residual left the recurrence behind or grew since the last update
+// (as in k_replace_final), where the column had stopped on its recurrence,
+// and after a breakdown (up to kMaxRestarts times); convergence is decided
+// on the true residual.
+__global__ void k_beta_replace(int nbz, const double* __restrict__ part_rz, int nbr,
+                               const double* __restrict__ part_rr, int k,
+                               double* __restrict__ sc) {
+  const int c = blockIdx.x;
+  const double rz = col_sum(part_rz, k, nbz, 0, c);
+  const double rr = col_sum(part_rr, k, nbr, 0, c);
+  if (threadIdx.x == 0) {
+    double* s = sc + c * kSc;
+    bool rst = rr > 4.0 * s[S_RR] || rr > s[S_RREF] || s[S_CONV] != 0.0;
+    if (s[S_BAD] != 0.0 && s[S_NRST] < kMaxRestarts) {
+      s[S_BAD] = 0.0;
+      s[S_NRST] += 1.0;
+      rst = true;
+    }
+    s[S_BETA] = rst ? 0.0 : rz / s[S_RZ];
+    s[S_RZ] = rz;
+    s[S_RR] = rr;
+    s[S_RREF] = rr;
+    s[S_REQ] = 0.0;
+    if (s[S_BAD] == 0.0) s[S_CONV] = (rr <= s[S_TOL2]) ? 1.0 : 0.0;
+    if (rr != rr || rz != rz) { s[S_CONV] = 1.0; s[S_BAD] = 1.0; }
+  }
+}
+
 // reliable update, step 3: r.r (float64 residual) and r.z of its float32
 // copy replace the recurrence values; convergence is decided here only.
 // The search direction is kept unless the true residual has drifted well
@@ -461,9 +509,12 @@ static int precondition(skt_solver* s, const Blk& b, const Vecs<T>& v, double* a
 }
 
 // Enqueue `count` PCG iterations; stopped columns (S_CONV) are untouched.
+// replace_first (mixed precision): the first iteration replaces the
+// updated residual by the true one before preconditioning it (a reliable
+// update that reuses the iteration's V-cycle).
 template <typename T>
 static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t count,
-                          hipStream_t st) {
+                          bool replace_first, const double* E64, hipStream_t st) {
   const bool mg = s->used == SKT_PC_MG;
   // area0 (s->part) takes the p.q partials and k_update's r.z / r.r; the
   // fused r.z of the V-cycle goes to area1, which is idle between the
@@ -487,9 +538,30 @@ static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t
                        mg ? (const T*)nullptr : (const T*)v.dinv, v.x, v.r, s->part);
     const double* rz_part = nullptr;
     int rz_nb = 0;
-    if ((stt = precondition<T>(s, b, v, s->part, area1, &rz_part, &rz_nb, st))) return stt;
-    hipLaunchKernelGGL(k_beta, dim3(b.k), dim3(kWave), 0, st, rz_nb, rz_part, (int)b.nbx,
-                       (const double*)s->part, b.k, s->sc);
+    if (it == 0 && replace_first) {
+      // U += d; r = F - K U (float64, r.r partials into area0); d = 0
+      const int64_t total = b.n * b.k;
+      const unsigned ge = grid_for(total);
+      hipLaunchKernelGGL(k_accumulate, dim3(ge), dim3(kBlock), 0, st, total, s->d32, s->x64);
+      if ((stt = op_apply(s->op, SKT_F64, b.k, b.kc, E64, s->x64, s->q64, st))) return stt;
+      hipLaunchKernelGGL(k_resid<double>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k,
+                         b.kc, b.cp, (const double*)s->f64, (const double*)s->q64, s->r64,
+                         s->part);
+      hipLaunchKernelGGL(k_to32, dim3(ge), dim3(kBlock), 0, st, total, s->r64, s->r32, s->d32);
+      // r.z partials into area1 whether fused or not (area0 holds r.r);
+      // Jacobi forms z = dinv r inline
+      if ((stt = precondition<T>(s, b, v, area1, area1, &rz_part, &rz_nb, st))) return stt;
+      if (!mg)
+        hipLaunchKernelGGL(k_pinit<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k,
+                           b.kc, b.cp, (const T*)v.r, (const T*)v.dinv, (const T*)nullptr,
+                           (T*)nullptr, area1);
+      hipLaunchKernelGGL(k_beta_replace, dim3(b.k), dim3(kWave), 0, st, rz_nb, rz_part,
+                         (int)b.nbx, (const double*)s->part, b.k, s->sc);
+    } else {
+      if ((stt = precondition<T>(s, b, v, s->part, area1, &rz_part, &rz_nb, st))) return stt;
+      hipLaunchKernelGGL(k_beta, dim3(b.k), dim3(kWave), 0, st, rz_nb, rz_part, (int)b.nbx,
+                         (const double*)s->part, b.k, s->sc);
+    }
     hipLaunchKernelGGL(k_pupdate<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
                        b.cp, s->sc,
                        (const T*)v.r, (const T*)v.dinv, mg ? (const T*)v.z : (const T*)nullptr,
@@ -571,10 +643,13 @@ static int solve_t(skt_solver* s, int mode, const Blk& b, const double* E64, con
   SKT_LAUNCH_CHECK();
   if ((stt = poll(s, b.k, st))) return stt;
   int64_t it = 0, updates = 0;
+  bool replace_next = false;
   bool conv = all_flag(s, b.k, S_CONV) && !any_flag(s, b.k, S_BAD);
   while (!conv && it < maxiter && !any_flag(s, b.k, S_BAD)) {
     const int64_t chunk = std::min<int64_t>(check, maxiter - it);
-    if ((stt = pcg_iterations<T>(s, b, v, chunk, st))) return stt;
+    if ((stt = pcg_iterations<T>(s, b, v, chunk, replace_next, E64, st))) return stt;
+    if (replace_next) ++updates;
+    replace_next = false;
     it += chunk;
     if ((stt = poll(s, b.k, st))) return stt;
     if ((stt = collect_timing(s, v.dtype, chunk))) return stt;
@@ -586,11 +661,30 @@ static int solve_t(skt_solver* s, int mode, const Blk& b, const double* E64, con
       continue;
     }
     if (!(bad || all_stop || any_flag(s, b.k, S_REQ) || it >= maxiter)) continue;
+    if (!(bad || all_stop || it >= maxiter)) {
+      // only reliable-update requests: done inside the next iteration
+      replace_next = true;
+      continue;
+    }
     // reliable update of every column
     hipLaunchKernelGGL(k_accumulate, dim3(ge), dim3(kBlock), 0, st, total, s->d32, s->x64);
     if ((stt = op_apply(s->op, SKT_F64, b.k, b.kc, E64, s->x64, s->q64, st))) return stt;
     hipLaunchKernelGGL(k_resid<double>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k,
                        b.kc, b.cp, (const double*)s->f64, (const double*)s->q64, s->r64, s->part);
+    if (all_stop && !bad) {
+      // every recurrence claims convergence: test the true residuals first
+      // and stop without preconditioning them when they all pass
+      hipLaunchKernelGGL(k_true_conv, dim3(b.k), dim3(kWave), 0, st, (int)b.nbx, b.k, s->part,
+                         s->sc);
+      SKT_LAUNCH_CHECK();
+      if ((stt = poll(s, b.k, st))) return stt;
+      if (all_flag(s, b.k, S_TCONV)) {
+        ++updates;
+        for (int c = 0; c < b.k; ++c) s->host[c * kSc + S_RR] = s->host[c * kSc + S_TRR];
+        conv = true;
+        break;
+      }
+    }
     hipLaunchKernelGGL(k_to32, dim3(ge), dim3(kBlock), 0, st, total, s->r64, s->r32, s->d32);
     if (mg) {
       if ((stt = mg_vcycle(&s->mg, s->op, v.E, v.dinv, b.k, b.kc, v.r, v.z, v.q, st))) return stt;
