@@ -166,6 +166,9 @@ struct OutArgs {
   // dot[c * dot_nb + bx] for column c and workgroup bx of its column block
   double* dot = nullptr;
   int dot_nb = 0;
+  // column stop flags (kStopStride apart): a workgroup whose columns have all
+  // stopped returns at once
+  const double* stop = nullptr;
 };
 
 // SF (sum-factorised), bit 0: the x-y butterfly of each element's face on a
@@ -205,6 +208,12 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   const int ty = b % tiles_y; b /= tiles_y;
   const int zc = b % n_zchunks; b /= n_zchunks;
   const int c0 = b * KC;
+  if (oa.stop) {
+    bool all = true;
+#pragma unroll
+    for (int c = 0; c < KC; ++c) all = all && oa.stop[(c0 + c) * kStopStride] != 0.0;
+    if (all) return;
+  }
   const int64_t cbase = (int64_t)(c0 / lkc) * gstride + (c0 % lkc);
   X += cbase;
   Y += cbase;
@@ -639,8 +648,9 @@ static int dispatch_modal(const skt_operator* op, int k, int kc, const T* coef, 
 
 template <typename T>
 int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
-                    T* Y, hipStream_t s, double* dot) {
+                    T* Y, hipStream_t s, double* dot, const double* stop) {
   OutArgs<T> oa;
+  oa.stop = stop;
   if (dot) {
     oa.dot = dot;
     oa.dot_nb = hex_modal_dot_slots(op, k, kc);
@@ -650,8 +660,9 @@ int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const 
 
 template <typename T>
 int hex_resid_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E,
-                    const T* X, const T* B, T* R, hipStream_t s) {
+                    const T* X, const T* B, T* R, hipStream_t s, const double* stop) {
   OutArgs<T> oa;
+  oa.stop = stop;
   oa.B = B;
   return dispatch_modal<T, OUT_RESID>(op, k, kc, coef, E, X, R, oa, s);
 }
@@ -659,8 +670,9 @@ int hex_resid_modal(const skt_operator* op, int k, int kc, const T* coef, const 
 template <typename T>
 int hex_cheb0_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* B,
                     T* Xout, const T* dinv, const double* cheb0, const double* cheb,
-                    hipStream_t s) {
+                    hipStream_t s, const double* stop) {
   OutArgs<T> oa;
+  oa.stop = stop;
   oa.B = B;
   oa.dinv = dinv;
   oa.cheb = cheb;
@@ -672,8 +684,9 @@ int hex_cheb0_modal(const skt_operator* op, int k, int kc, const T* coef, const 
 template <typename T>
 int hex_cheb_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
                    T* Xout, const T* B, const T* dinv, const double* cheb, int prev_zero,
-                   hipStream_t s, double* dot) {
+                   hipStream_t s, double* dot, const double* stop) {
   OutArgs<T> oa;
+  oa.stop = stop;
   oa.B = B;
   oa.dinv = dinv;
   oa.cheb = cheb;
@@ -686,25 +699,27 @@ int hex_cheb_modal(const skt_operator* op, int k, int kc, const T* coef, const T
 }
 
 template int hex_apply_modal<float>(const skt_operator*, int, int, const float*, const float*,
-                                    const float*, float*, hipStream_t, double*);
+                                    const float*, float*, hipStream_t, double*, const double*);
 template int hex_apply_modal<double>(const skt_operator*, int, int, const double*, const double*,
-                                     const double*, double*, hipStream_t, double*);
+                                     const double*, double*, hipStream_t, double*,
+                                     const double*);
 template int hex_resid_modal<float>(const skt_operator*, int, int, const float*, const float*,
-                                    const float*, const float*, float*, hipStream_t);
+                                    const float*, const float*, float*, hipStream_t,
+                                    const double*);
 template int hex_resid_modal<double>(const skt_operator*, int, int, const double*,
                                      const double*, const double*, const double*, double*,
-                                     hipStream_t);
+                                     hipStream_t, const double*);
 template int hex_cheb0_modal<float>(const skt_operator*, int, int, const float*, const float*,
                                     const float*, float*, const float*, const double*,
-                                    const double*, hipStream_t);
+                                    const double*, hipStream_t, const double*);
 template int hex_cheb0_modal<double>(const skt_operator*, int, int, const double*,
                                      const double*, const double*, double*, const double*,
-                                     const double*, const double*, hipStream_t);
+                                     const double*, const double*, hipStream_t, const double*);
 template int hex_cheb_modal<float>(const skt_operator*, int, int, const float*, const float*,
                                    const float*, float*, const float*, const float*,
-                                   const double*, int, hipStream_t, double*);
+                                   const double*, int, hipStream_t, double*, const double*);
 template int hex_cheb_modal<double>(const skt_operator*, int, int, const double*, const double*,
                                     const double*, double*, const double*, const double*,
-                                    const double*, int, hipStream_t, double*);
+                                    const double*, int, hipStream_t, double*, const double*);
 
 }  // namespace skt

@@ -689,6 +689,7 @@ struct StOut {
   const T* dinv = nullptr;
   const double* coef = nullptr;
   int prev_zero = 1;
+  const double* stop = nullptr;  // column stop flags (tiled kernel: skip stopped workgroups)
 };
 
 // y = S x on a stencil level.  A thread takes one (node, component row) and
@@ -800,6 +801,11 @@ k_stencil_apply_tiled(int64_t nx, int64_t ny, int64_t nz, int groups, const T* _
   const int tbz = b % bz; b /= bz;
   const int g0 = b * GM;
   const int ng = groups - g0 < GM ? groups - g0 : GM;
+  if (oa.stop) {
+    bool all = true;
+    for (int c = g0 * KC; c < (g0 + ng) * KC; ++c) all = all && oa.stop[c * kStopStride] != 0.0;
+    if (all) return;
+  }
   const int64_t x0 = (int64_t)tbx * SX, y0 = (int64_t)tby * SY, z0 = (int64_t)tbz * SZ;
   // this tile's coefficients (tile-major storage)
   st += (int64_t)(tbx + bx * (tby + by * tbz)) * (27 * NC * NC) * TS;
@@ -1335,13 +1341,15 @@ static int level_apply(MGData* mg, const skt_operator* op, const void* E, int l,
 // x_out_before) + c2 dinv (b - S x).
 template <typename T>
 static int level_cheb(MGData* mg, const skt_operator* op, int l, int k, int kc, const T* x,
-                      T* xout, const double* coef, int prev_zero, hipStream_t s) {
+                      T* xout, const double* coef, int prev_zero, hipStream_t s,
+                      const double* stop) {
   const MGLevel& v = mg->lv[l];
   StOut<T> oa;
   oa.b = (const T*)v.b;
   oa.dinv = (const T*)v.dinv;
   oa.coef = coef;
   oa.prev_zero = prev_zero;
+  oa.stop = stop;
   return level_apply<T, ST_CHEB>(mg, op, nullptr, l, k, kc, x, xout, s, oa);
 }
 
@@ -1579,7 +1587,9 @@ const float* modal_coef<float>(const skt_operator* op) { return op->modal_f; }
 template <typename T, int NC, int KC>
 static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const void* dinv0, int k,
                     const T* r, T* z, T* w0, hipStream_t s, double* dot, int max_slots,
-                    int* slots) {
+                    int* slots, const double* stop) {
+  StOut<T> ost;
+  ost.stop = stop;
   const int L = (int)mg->lv.size() - 1;
   const int groups = k / KC;
   const int deg = mg->degree;
@@ -1597,7 +1607,7 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     if (deg >= 2) {
       // steps 0 and 1 in one pass (step 0's iterate is formed on the fly)
       if ((stt = hex_cheb0_modal<T>(op, k, KC, coef0, E0, r, oth, (const T*)dinv0, cf, cf + 2,
-                                    s)))
+                                    s, stop)))
         return stt;
       std::swap(cur, oth);
     } else {
@@ -1606,11 +1616,11 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     }
     for (int st = 2; st < deg; ++st) {
       if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, (const T*)dinv0,
-                                   cf + 2 * st, 0, s)))
+                                   cf + 2 * st, 0, s, nullptr, stop)))
         return stt;
       std::swap(cur, oth);
     }
-    if ((stt = hex_resid_modal<T>(op, k, KC, coef0, E0, cur, r, w0, s))) return stt;
+    if ((stt = hex_resid_modal<T>(op, k, KC, coef0, E0, cur, r, w0, s, stop))) return stt;
     launch_restrict<T, NC, KC>(v, mg->lv[1], groups, (const T*)w0, (const T*)nullptr, fixed0,
                                (T*)mg->lv[1].b, s);
     SKT_LAUNCH_CHECK();
@@ -1635,12 +1645,12 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
                        v.ndof, KC, cf, b, (const T*)v.dinv, A);
     for (int st = 1; st < deg; ++st) {
-      if ((stt = level_cheb<T>(mg, op, l, k, KC, A, B, cf + 2 * st, st == 1 ? 1 : 0, s)))
+      if ((stt = level_cheb<T>(mg, op, l, k, KC, A, B, cf + 2 * st, st == 1 ? 1 : 0, s, stop)))
         return stt;
       std::swap(A, B);
     }
     it[l] = A;
-    if ((stt = level_apply<T>(mg, op, E, l, k, KC, A, w, s))) return stt;
+    if ((stt = level_apply<T>(mg, op, E, l, k, KC, A, w, s, ost))) return stt;
     launch_restrict<T, NC, KC>(v, mg->lv[l + 1], groups, b, w, nullptr, (T*)mg->lv[l + 1].b, s);
     SKT_LAUNCH_CHECK();
   }
@@ -1663,7 +1673,7 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     launch_prolong<T, NC, KC>(v, mg->lv[l + 1], groups, (const T*)it[l + 1], nullptr, A, s);
     for (int st = 0; st < deg; ++st) {
       // step 0 has c1 = 0 and needs no previous iterate
-      if ((stt = level_cheb<T>(mg, op, l, k, KC, A, B, cf + 2 * st, st == 0 ? 1 : 0, s)))
+      if ((stt = level_cheb<T>(mg, op, l, k, KC, A, B, cf + 2 * st, st == 0 ? 1 : 0, s, stop)))
         return stt;
       std::swap(A, B);
     }
@@ -1681,7 +1691,8 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     for (int st = 0; st < deg; ++st) {
       // step 0 has c1 = 0 and needs no previous iterate
       if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, (const T*)dinv0,
-                                   cf + 2 * st, 0, s, st == deg - 1 ? last_dot : nullptr)))
+                                   cf + 2 * st, 0, s, st == deg - 1 ? last_dot : nullptr,
+                                   stop)))
         return stt;
       std::swap(cur, oth);
     }
@@ -1697,12 +1708,12 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
 
 int mg_vcycle(MGData* mg, const skt_operator* op, const void* E, const void* dinv0, int ncols,
               int kc, const void* r, void* z, void* w0, hipStream_t s, double* dot,
-              int max_slots, int* slots) {
+              int max_slots, int* slots, const double* stop) {
   SKT_CHECK(mg->dtype >= 0, "multigrid not set up");
   SKT_CHECK(ncols <= mg->max_cols && ncols % kc == 0, "bad column count for multigrid");
 #define SKT_VC(T_, NC_, KC_)                                                               \
   return vcycle_t<T_, NC_, KC_>(mg, op, E, dinv0, ncols, (const T_*)r, (T_*)z, (T_*)w0, s, dot, \
-                                max_slots, slots)
+                                max_slots, slots, stop)
   if (mg->dtype == SKT_F64) {
     if (op->ncomp == 3) {
       if (kc == 1) SKT_VC(double, 3, 1);

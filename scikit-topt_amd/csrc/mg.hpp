@@ -90,8 +90,11 @@ int mg_setup(MGData* mg, const skt_operator* op, const double* E64, const void* 
 // w0 is fine-level scratch of the same size.  With dot, the last smoothing
 // step also leaves the per-column partials of r . z there when they fit in
 // max_slots per column (*slots = their count, else 0).
+// With stop (column stop flags, kStopStride apart), the fine-level and
+// tiled coarse-level kernels may skip workgroups whose columns have all
+// stopped; z is then stale for those columns.
 int mg_vcycle(MGData* mg, const skt_operator* op, const void* E, const void* dinv0, int ncols,
               int kc, const void* r, void* z, void* w0, hipStream_t s, double* dot = nullptr,
-              int max_slots = 0, int* slots = nullptr);
+              int max_slots = 0, int* slots = nullptr, const double* stop = nullptr);
 
 }  // namespace skt

@@ -6,6 +6,12 @@
 
 enum { SKT_MESH_HEX8 = 0, SKT_MESH_TET4 = 1 };
 
+// Column stop flags: column c is stopped where stop[c * kStopStride] != 0
+// (the solver's per-column scalar block).  Launches given such flags may
+// skip the work of workgroups whose columns have all stopped; their
+// outputs (and dot partials) for those columns are then left stale.
+constexpr int kStopStride = 16;
+
 struct skt_operator {
   int mesh = SKT_MESH_HEX8;
   int physics = SKT_ELASTIC;
@@ -59,16 +65,16 @@ int hex_modal_dot_slots(const skt_operator* op, int k, int kc);
 // Y = K X; with `dot`, also the partials of X . Y per column
 template <typename T>
 int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
-                    T* Y, hipStream_t s, double* dot = nullptr);
+                    T* Y, hipStream_t s, double* dot = nullptr, const double* stop = nullptr);
 // R = B - K X
 template <typename T>
 int hex_resid_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E,
-                    const T* X, const T* B, T* R, hipStream_t s);
+                    const T* X, const T* B, T* R, hipStream_t s, const double* stop = nullptr);
 // x_out = x1 + c1 x1 + c2 dinv (b - K x1) with x1 = cheb0[1] dinv b (OUT_CHEB0)
 template <typename T>
 int hex_cheb0_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* B,
                     T* Xout, const T* dinv, const double* cheb0, const double* cheb,
-                    hipStream_t s);
+                    hipStream_t s, const double* stop = nullptr);
 // Chebyshev-Jacobi step with the direction taken from two iterates:
 // Xout = X + c1 (X - Xprev) + c2 dinv (B - K X), (c1, c2) = cheb[0..1] on the
 // device.  Xprev is read from Xout before it is overwritten (zero when
@@ -77,7 +83,7 @@ int hex_cheb0_modal(const skt_operator* op, int k, int kc, const T* coef, const 
 template <typename T>
 int hex_cheb_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
                    T* Xout, const T* B, const T* dinv, const double* cheb, int prev_zero,
-                   hipStream_t s, double* dot = nullptr);
+                   hipStream_t s, double* dot = nullptr, const double* stop = nullptr);
 // Y = K(E) X for ncols columns.  kc == ncols: (ndof, ncols) row-major;
 // otherwise the column-blocked layout [ncols/kc][ndof][kc] (structured hex8
 // only).
@@ -87,7 +93,8 @@ int op_apply(skt_operator* op, int dtype, int ncols, int kc, const void* E,
 // hex_modal_dot_slots) where the operator can fuse them: *slots is then
 // their count per column, else 0 and nothing is written to dot.
 int op_apply_dot(skt_operator* op, int dtype, int ncols, int kc, const void* E, const void* X,
-                 void* Y, double* dot, int max_slots, int* slots, hipStream_t stream);
+                 void* Y, double* dot, int max_slots, int* slots, hipStream_t stream,
+                 const double* stop = nullptr);
 int op_diag(skt_operator* op, int dtype, const void* E, void* diag,
             hipStream_t stream);
 }  // namespace skt

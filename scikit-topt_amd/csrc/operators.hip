@@ -623,7 +623,8 @@ int op_apply(skt_operator* op, int dtype, int ncols, int kc, const void* E, cons
 }
 
 int op_apply_dot(skt_operator* op, int dtype, int ncols, int kc, const void* E, const void* X,
-                 void* Y, double* dot, int max_slots, int* slots, hipStream_t s) {
+                 void* Y, double* dot, int max_slots, int* slots, hipStream_t s,
+                 const double* stop) {
   SKT_CHECK(slots, "null pointer");
   *slots = 0;
   const bool fuse = op && dot && op->mesh == SKT_MESH_HEX8 && op->modal_ok && !op->extra64 &&
@@ -636,9 +637,9 @@ int op_apply_dot(skt_operator* op, int dtype, int ncols, int kc, const void* E, 
   SKT_CHECK(dtype == SKT_F64 || dtype == SKT_F32, "dtype must be SKT_F64 or SKT_F32");
   const int st = dtype == SKT_F64
                      ? hex_apply_modal<double>(op, ncols, kc, op->modal_d, (const double*)E,
-                                               (const double*)X, (double*)Y, s, dot)
+                                               (const double*)X, (double*)Y, s, dot, stop)
                      : hex_apply_modal<float>(op, ncols, kc, op->modal_f, (const float*)E,
-                                              (const float*)X, (float*)Y, s, dot);
+                                              (const float*)X, (float*)Y, s, dot, stop);
   if (st == SKT_OK) *slots = hex_modal_dot_slots(op, ncols, kc);
   return st;
 }

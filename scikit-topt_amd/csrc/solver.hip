@@ -60,6 +60,7 @@ namespace skt {
 
 constexpr int kParts = 512;
 constexpr int kSc = 16;
+static_assert(kSc == kStopStride, "the column stop flags are the solver's scalar blocks");
 constexpr int kCheckJacobi = 32;
 constexpr int kCheckMG = 4;
 // scalar slots
@@ -489,13 +490,14 @@ struct Vecs {
 // formed inline by the vector kernels and r.z is already in `area`.
 template <typename T>
 static int precondition(skt_solver* s, const Blk& b, const Vecs<T>& v, double* area,
-                        double* fused, const double** rz_part, int* nb, hipStream_t st) {
+                        double* fused, const double** rz_part, int* nb, const double* stop,
+                        hipStream_t st) {
   *rz_part = area;
   *nb = (int)b.nbx;
   if (s->used == SKT_PC_MG) {
     int slots = 0;
     int stt = mg_vcycle(&s->mg, s->op, v.E, v.dinv, b.k, b.kc, v.r, v.z, v.q, st, fused,
-                        2 * kParts, &slots);
+                        2 * kParts, &slots, stop);
     if (stt) return stt;
     if (slots > 0) {
       *rz_part = fused;
@@ -523,8 +525,9 @@ static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t
   for (int64_t it = 0; it < count; ++it) {
     if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * it], st));
     int pq_nb = 0;
+    // stopped columns' q is not used (alpha = 0), so their work may be skipped
     int stt = op_apply_dot(s->op, v.dtype, b.k, b.kc, v.E, v.p, v.q, s->part, 2 * kParts,
-                           &pq_nb, st);
+                           &pq_nb, st, s->sc + S_CONV);
     if (stt) return stt;
     if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * it + 1], st));
     if (pq_nb == 0) {
@@ -549,8 +552,10 @@ static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t
                          s->part);
       hipLaunchKernelGGL(k_to32, dim3(ge), dim3(kBlock), 0, st, total, s->r64, s->r32, s->d32);
       // r.z partials into area1 whether fused or not (area0 holds r.r);
-      // Jacobi forms z = dinv r inline
-      if ((stt = precondition<T>(s, b, v, area1, area1, &rz_part, &rz_nb, st))) return stt;
+      // Jacobi forms z = dinv r inline.  No column skips the V-cycle here:
+      // a stopped column may resume on its true residual.
+      if ((stt = precondition<T>(s, b, v, area1, area1, &rz_part, &rz_nb, nullptr, st)))
+        return stt;
       if (!mg)
         hipLaunchKernelGGL(k_pinit<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k,
                            b.kc, b.cp, (const T*)v.r, (const T*)v.dinv, (const T*)nullptr,
@@ -558,7 +563,11 @@ static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t
       hipLaunchKernelGGL(k_beta_replace, dim3(b.k), dim3(kWave), 0, st, rz_nb, rz_part,
                          (int)b.nbx, (const double*)s->part, b.k, s->sc);
     } else {
-      if ((stt = precondition<T>(s, b, v, s->part, area1, &rz_part, &rz_nb, st))) return stt;
+      // stopped columns' z is not used (k_beta and the direction update
+      // skip them)
+      if ((stt = precondition<T>(s, b, v, s->part, area1, &rz_part, &rz_nb, s->sc + S_CONV,
+                                 st)))
+        return stt;
       hipLaunchKernelGGL(k_beta, dim3(b.k), dim3(kWave), 0, st, rz_nb, rz_part, (int)b.nbx,
                          (const double*)s->part, b.k, s->sc);
     }
