@@ -289,7 +289,9 @@ int skt_solver_destroy(skt_solver* s);
  * guess on entry (warm start; U[D] is overwritten with F[D]) and the
  * solution on exit.
  * iters[0] = PCG iterations, iters[1] = reliable updates (0 in SKT_F64);
- * relres[c] = final ||F - K U|| / ||F|| per column (host arrays). */
+ * relres[c] = final ||F - K U|| / ||F|| per column (host arrays, valid on
+ * return).  The copy of the solution into U is stream-ordered on `stream`
+ * and may still be running when the call returns. */
 int skt_solver_solve(skt_solver* s, int mode, int ncols, const double* E64,
                      const float* E32, const double* F, double* U,
                      double rtol, int maxiter, double inner_rtol,

@@ -844,11 +844,11 @@ int skt_solver_solve(skt_solver* s, int mode, int ncols, const double* E64,
     stt = solve_t<float>(s, mode, b, E64, v, rtol, maxiter, inner_rtol, iters, st);
   }
   if (stt != SKT_OK && stt != SKT_ENOCONV) return stt;
-  // solution back to the caller's layout
+  // solution back to the caller's layout (stream-ordered; the flags and
+  // norms below come from the last poll, so the host does not wait for it)
   hipLaunchKernelGGL(k_from_blocked, dim3(ge), dim3(kBlock), 0, st, n, ncols, b.kc,
                      (const double*)s->x64, U);
   SKT_LAUNCH_CHECK();
-  SKT_HIP(hipStreamSynchronize(st));
   bool bad = false;
   for (int c = 0; c < ncols; ++c) {
     const double* h = s->host + c * kSc;

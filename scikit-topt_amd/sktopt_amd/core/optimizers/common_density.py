@@ -180,6 +180,8 @@ class DensityMethod:
         self._design_ids = torch.as_tensor(np.asarray(tsk.design_elements, dtype=np.int32),
                                            device=self.dev)
         self.kkt_residual = None
+        self._rec_host = None      # pinned host copy of an iteration's records
+        self._rec_pending = None   # (copy, event, context) not yet fed to the recorder
         self._state = None
         self._iter_next = None
         self._iter_end = None
@@ -329,6 +331,7 @@ class DensityMethod:
             self._iter_next = it + 1
             if converged:
                 break
+        self._flush_records()
         if converged or self._iter_next >= end:
             self._finalize()
             self._completed = True
@@ -392,6 +395,9 @@ class DensityMethod:
                         None, move_limit, eta, beta, None, None, percentile, self._vol_design,
                         self._vol_design_sum, vol_frac)
         self.scatter_design(st["rho_design"], rho)
+        # the previous iteration's records, while the GPU runs this one's
+        # bisection
+        self._flush_records()
         # records (one small device->host copy per iteration)
         sc = st["scal"]
         _lib.call("skt_reduce", _lib.RED_MAXABSDIFF, nd, _dev.ptr(st["rho_design"]),
@@ -405,32 +411,25 @@ class DensityMethod:
         arrays.update(self._record_arrays(st))
         names = list(arrays)
         stats = [_stats4(arrays[n]) for n in names]
-        rec = torch.cat([sc[:8], comp.reshape(-1), u_max.reshape(-1)] + stats).cpu().numpy()
-        upd = self._update_records(rec[:8])
-        rho_change_max = float(rec[0])
-        comp_h = rec[8:8 + k]
-        umax_h = rec[8 + k:8 + 2 * k]
-        stats_h = rec[8 + 2 * k:].reshape(-1, 4)
-        compliance_avg = float(comp_h.mean())
-        r = self.recorder
-        r.feed_data("rho_change_max", rho_change_max)
-        for n, sv in zip(names, stats_h):
-            r.feed_stats(n, sv)
-        r.feed_data(self._objective_history_name(tsk), compliance_avg)
-        r.feed_data("u_max", float(umax_h[0]) if k == 1 else umax_h.copy())
-        r.feed_data("neumann_scale", neumann_scale)
-        for name, v in upd.items():
-            r.feed_data(name, v)
-        r.feed_data("compliance_each", comp_h.copy())
-        r.feed_data("solve_iters", self.fem.last_solve.get("iters"))
-        converged = False
+        rec = torch.cat([sc[:8], comp.reshape(-1), u_max.reshape(-1)] + stats)
+        ctx = dict(names=names, k=k, neumann_scale=neumann_scale,
+                   solve_iters=self.fem.last_solve.get("iters"),
+                   obj=self._objective_history_name(tsk))
         if cfg.check_convergence:
-            kkt = self.kkt_residual
-            if kkt is None:
-                raise ValueError("kkt_residual is not computed in rho_update")
-            conv_rho = rho_change_max < cfg.tol_rho_change
-            conv_kkt = abs(kkt) < cfg.tol_kkt_residual if np.isfinite(kkt) else True
-            converged = conv_rho and conv_kkt
+            # the stop test needs this iteration's values now
+            converged = self._process_records(rec.cpu().numpy(), ctx)
+        else:
+            # copied asynchronously and fed to the recorder during the next
+            # iteration (or when the run returns), so the host never waits here
+            buf = self._rec_host
+            if buf is None or buf.numel() != rec.numel():
+                buf = self._rec_host = torch.empty(rec.numel(), dtype=rec.dtype,
+                                                   pin_memory=True)
+            buf.copy_(rec, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            self._rec_pending = (buf, ev, ctx)
+            converged = False
         rt = max(cfg.max_iters // max(cfg.record_times, 1), 1)
         # save points of common_density.py:1186-1193 (the last iteration of
         # this call, as the reference does for partial runs)
@@ -445,6 +444,48 @@ class DensityMethod:
             np.savez_compressed(f"{cfg.dst_path}/data/{iter_num:06d}-rho.npz",
                                 rho_design_elements=self.rho_host()[tsk.design_elements])
         return converged
+
+    def _flush_records(self):
+        """Feed the recorder with a pending iteration's records (waits for
+        their copy only)."""
+        pending = self._rec_pending
+        if pending is None:
+            return
+        buf, ev, ctx = pending
+        self._rec_pending = None
+        ev.synchronize()
+        self._process_records(buf.numpy().copy(), ctx)
+
+    def _process_records(self, rec: np.ndarray, ctx: dict) -> bool:
+        """Records of one iteration from its host copy; returns the
+        convergence flag (common_density.py:1140-1185)."""
+        cfg = self.cfg
+        k = ctx["k"]
+        upd = self._update_records(rec[:8])
+        rho_change_max = float(rec[0])
+        comp_h = rec[8:8 + k]
+        umax_h = rec[8 + k:8 + 2 * k]
+        stats_h = rec[8 + 2 * k:].reshape(-1, 4)
+        compliance_avg = float(comp_h.mean())
+        r = self.recorder
+        r.feed_data("rho_change_max", rho_change_max)
+        for n, sv in zip(ctx["names"], stats_h):
+            r.feed_stats(n, sv)
+        r.feed_data(ctx["obj"], compliance_avg)
+        r.feed_data("u_max", float(umax_h[0]) if k == 1 else umax_h.copy())
+        r.feed_data("neumann_scale", ctx["neumann_scale"])
+        for name, v in upd.items():
+            r.feed_data(name, v)
+        r.feed_data("compliance_each", comp_h.copy())
+        r.feed_data("solve_iters", ctx["solve_iters"])
+        if not cfg.check_convergence:
+            return False
+        kkt = self.kkt_residual
+        if kkt is None:
+            raise ValueError("kkt_residual is not computed in rho_update")
+        conv_rho = rho_change_max < cfg.tol_rho_change
+        conv_kkt = abs(kkt) < cfg.tol_kkt_residual if np.isfinite(kkt) else True
+        return conv_rho and conv_kkt
 
     def _record_arrays(self, st) -> dict:
         """Subclass hook: device arrays recorded as [min, mean, max, std]."""
