@@ -1179,7 +1179,9 @@ __global__ void k_cheb_coef(int nlev, int degree, double ratio, const double* __
 static size_t tsize(int dtype) { return dtype == SKT_F64 ? sizeof(double) : sizeof(float); }
 
 void mg_free(MGData* mg) {
-  if (mg->side) (void)hipStreamSynchronize(mg->side);  // the set-up's side work is done
+  // the set-up's side work is done
+  if (mg->side) (void)hipStreamSynchronize(mg->side);
+  if (mg->side2) (void)hipStreamSynchronize(mg->side2);
   for (MGLevel& l : mg->lv) {
     (void)hipFree(l.Kel); (void)hipFree(l.st); (void)hipFree(l.dinv);
     (void)hipFree(l.b); (void)hipFree(l.x); (void)hipFree(l.w); (void)hipFree(l.d);
@@ -1192,11 +1194,13 @@ void mg_free(MGData* mg) {
   (void)hipFree(mg->K0b);
   if (mg->side) {
     (void)hipStreamDestroy(mg->side);
+    (void)hipStreamDestroy(mg->side2);
     (void)hipEventDestroy(mg->ev_fork);
+    (void)hipEventDestroy(mg->ev_gal);
     (void)hipEventDestroy(mg->ev_st);
     (void)hipEventDestroy(mg->ev_join);
-    mg->side = nullptr;
-    mg->ev_fork = mg->ev_st = mg->ev_join = nullptr;
+    mg->side = mg->side2 = nullptr;
+    mg->ev_fork = mg->ev_gal = mg->ev_st = mg->ev_join = nullptr;
   }
   (void)hipFree(mg->Mch);
   (void)hipFree(mg->gflag);
@@ -1359,15 +1363,19 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   const int NC = op->ncomp, ND = 8 * NC;
   const int L = (int)mg->lv.size() - 1;
   const double* dK0 = mg->K0b;
-  // The hierarchy (Galerkin products, stencils, the coarsest inverse) is
-  // built on a side stream while the main stream runs the power iterations:
-  // the fine level's needs only the fine operator, the coarse levels' wait
-  // for the stencils (ev_st), and the set-up joins the side stream (ev_join)
-  // before returning.  The side stream touches the level matrices,
-  // stencils, Jacobi inverses and the coarsest buffers only.
+  // The hierarchy is built on side streams while the main stream runs the
+  // power iterations: `side` takes the Galerkin products and then the
+  // coarsest inverse, `side2` the stencils and Jacobi inverses once the
+  // products they need are done (ev_gal).  The fine level's power iteration
+  // needs only the fine operator, the coarse levels' wait for the stencils
+  // (ev_st), and the set-up joins the coarsest inverse (ev_join) before
+  // returning.  The side streams touch the level matrices, stencils, Jacobi
+  // inverses and the coarsest buffers only.
   if (!mg->side) {
     SKT_HIP(hipStreamCreateWithFlags(&mg->side, hipStreamNonBlocking));
+    SKT_HIP(hipStreamCreateWithFlags(&mg->side2, hipStreamNonBlocking));
     SKT_HIP(hipEventCreateWithFlags(&mg->ev_fork, hipEventDisableTiming));
+    SKT_HIP(hipEventCreateWithFlags(&mg->ev_gal, hipEventDisableTiming));
     SKT_HIP(hipEventCreateWithFlags(&mg->ev_st, hipEventDisableTiming));
     SKT_HIP(hipEventCreateWithFlags(&mg->ev_join, hipEventDisableTiming));
   }
@@ -1421,6 +1429,8 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
     SKT_LAUNCH_CHECK();
   }
   for (int l = 2; l <= L; ++l) {
+    // the stencil levels' products are done: side2 may start
+    if (l == L) SKT_HIP(hipEventRecord(mg->ev_gal, g));
     const MGLevel& c = mg->lv[l];
     const int64_t nce = c.nx * c.ny * c.nz;
     const double* Kf = mg->lv[l - 1].Kel;
@@ -1432,7 +1442,10 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
                          c.f[0], c.f[1], c.f[2], Kf, dK0, E64, nullptr, nullptr, c.Kel);
     SKT_LAUNCH_CHECK();
   }
-  // stencils and Jacobi inverses of the intermediate levels
+  if (L < 2) SKT_HIP(hipEventRecord(mg->ev_gal, g));
+  // stencils and Jacobi inverses of the intermediate levels (side2)
+  SKT_HIP(hipStreamWaitEvent(mg->side2, mg->ev_gal, 0));
+  g = mg->side2;
   for (int l = 1; l < L; ++l) {
     const MGLevel& v = mg->lv[l];
     const int64_t tot = st_nodes(v.nx, v.ny, v.nz, v.layout) * 27 * NC * NC;
@@ -1450,6 +1463,7 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
     SKT_LAUNCH_CHECK();
   }
   SKT_HIP(hipEventRecord(mg->ev_st, g));
+  g = mg->side;
   // coarsest: dense matrix, blocked Gauss-Jordan inverse, cast to T (side
   // stream).  Enqueued after the fine level's power iteration, so that the
   // host's many small launches for it do not hold back the main stream.

@@ -54,8 +54,9 @@ struct MGData {
   double* K0b = nullptr;              // unit element matrix, corner-bit order
   double* Mch = nullptr;              // P_j^T K0 P_j per child position of level 1
   uint8_t* gflag = nullptr;           // level-1 elements with Dirichlet DOFs below
-  hipStream_t side = nullptr;         // set-up: hierarchy beside the power iterations
-  hipEvent_t ev_fork = nullptr, ev_st = nullptr, ev_join = nullptr;
+  hipStream_t side = nullptr;         // set-up: Galerkin products, coarsest inverse
+  hipStream_t side2 = nullptr;        // set-up: stencils and Jacobi inverses
+  hipEvent_t ev_fork = nullptr, ev_gal = nullptr, ev_st = nullptr, ev_join = nullptr;
   void* pv = nullptr;                 // power-iteration scratch (2 * ndof0) in T
   void* pvw = nullptr;                // per level (levels 0..L-1, concatenated) the last
                                       // normalised power-iteration vector in T
@@ -63,7 +64,7 @@ struct MGData {
   int degree = 2;
   double ratio = 10.0;
   int power_iters = 20;               // steps from the hashed start vector
-  int power_iters_warm = 12;          // steps from the previous set-up's vector
+  int power_iters_warm = 8;           // steps from the previous set-up's vector
 };
 
 struct MGConfig {
