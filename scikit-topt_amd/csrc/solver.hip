@@ -49,9 +49,12 @@ struct skt_solver {
   skt::MGData mg;
   int mg_possible = -1;      // -1 unknown, 0 no, 1 yes
   // optional per-launch timing of the operator apply (hipEvents on the
-  // solve stream, read back at the convergence checks)
+  // solve stream, read back at the convergence checks); only chunks that
+  // start with every column live are timed, since the apply skips the work
+  // of stopped columns
   int timing = 0;
   hipEvent_t ev[2 * 32] = {};
+  bool timed = false;  // the last chunk's applies were timed
   double apply_ms[2] = {0, 0};
   int64_t n_apply[2] = {0, 0};
 };
@@ -522,14 +525,15 @@ static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t
   // fused r.z of the V-cycle goes to area1, which is idle between the
   // initialisation and the reliable updates
   double* area1 = s->part + (size_t)2 * kParts * s->max_cols;
+  s->timed = s->timing && !any_flag(s, b.k, S_CONV);
   for (int64_t it = 0; it < count; ++it) {
-    if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * it], st));
+    if (s->timed) SKT_HIP(hipEventRecord(s->ev[2 * it], st));
     int pq_nb = 0;
     // stopped columns' q is not used (alpha = 0), so their work may be skipped
     int stt = op_apply_dot(s->op, v.dtype, b.k, b.kc, v.E, v.p, v.q, s->part, 2 * kParts,
                            &pq_nb, st, s->sc + S_CONV);
     if (stt) return stt;
-    if (s->timing) SKT_HIP(hipEventRecord(s->ev[2 * it + 1], st));
+    if (s->timed) SKT_HIP(hipEventRecord(s->ev[2 * it + 1], st));
     if (pq_nb == 0) {
       hipLaunchKernelGGL(k_dot<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
                          b.cp, (const T*)v.p, (const T*)v.q, s->part);
@@ -583,7 +587,7 @@ static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t
 // Fold the apply timings of the last chunk into the totals (events of the
 // chunk have completed once the poll has synchronised the stream).
 static int collect_timing(skt_solver* s, int dtype, int64_t count) {
-  if (!s->timing) return SKT_OK;
+  if (!s->timed) return SKT_OK;
   for (int64_t k = 0; k < count; ++k) {
     float ms = 0.f;
     SKT_HIP(hipEventElapsedTime(&ms, s->ev[2 * k], s->ev[2 * k + 1]));
