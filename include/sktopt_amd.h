@@ -368,7 +368,9 @@ int skt_filter_gradient(skt_filter* f, const double* v, double* g,
  *   in oc.py:199-228.  Every step: candidate update, filter gather,
  *   Heaviside projection and float64 volume sum; the bracket update and
  *   exit test run on the device (no host round trip).
- * rho_full (n_el): current densities (non-design entries are carried over);
+ * rho_full (n_el): current densities (the filter couples design elements
+ *   only, so the volume depends on the design candidates alone and the
+ *   non-design entries are not read);
  * rho_e, dC, vol: (n_design,) in design order; rho_out (n_design,): the
  * candidate at the final lambda.  result (device, 4 doubles):
  * {lmid, vol_error, n_evals, l2 - l1}.

@@ -324,8 +324,11 @@ __global__ void k_check_stencil(int64_t n_design, int64_t nx, int64_t ny, int64_
 // over in-grid design neighbours, in ascending element order.  With
 // transpose, the row index of the weight is the neighbour's (W^T).  off[o]
 // packs (dx + 64) | (dy + 64) << 8 | (dz + 64) << 16 and del[o] is the
-// element-index delta.  Indices fit in 32 bits (n_el < 2^31).
-template <bool kTranspose>
+// element-index delta.  Indices fit in 32 bits (n_el < 2^31).  kDesignX: x
+// is zero on every non-design element, so the design test of the neighbour
+// (a second gather) is left out; the terms it would drop are then exact
+// zeros and the sum is the same.
+template <bool kTranspose, bool kDesignX = false>
 __device__ __forceinline__ double stencil_row(int e, int32_t r, int nx, int ny, int nz, int R,
                                               int n_off, const int32_t* __restrict__ off,
                                               const int32_t* __restrict__ del,
@@ -349,15 +352,20 @@ __device__ __forceinline__ double stencil_row(int e, int32_t r, int nx, int ny, 
     const bool inb = (unsigned)x2 < (unsigned)nx && (unsigned)y2 < (unsigned)ny &&
                      (unsigned)z2 < (unsigned)nz;
     const int j = inb ? e + del[o] : e;
-    const int32_t rj = colmap[j];
     const double xv = x[j];
-    double wgt;
-    if constexpr (kTranspose)
-      wgt = inv_rs[rj < 0 ? 0 : rj] * u[o];
-    else
-      wgt = ir * u[o];
-    const double term = wgt * xv;
-    acc += (inb && rj >= 0) ? term : 0.0;
+    if constexpr (kDesignX && !kTranspose) {
+      const double term = (ir * u[o]) * xv;
+      acc += inb ? term : 0.0;
+    } else {
+      const int32_t rj = colmap[j];
+      double wgt;
+      if constexpr (kTranspose)
+        wgt = inv_rs[rj < 0 ? 0 : rj] * u[o];
+      else
+        wgt = ir * u[o];
+      const double term = wgt * xv;
+      acc += (inb && rj >= 0) ? term : 0.0;
+    }
   }
   return acc;
 }
@@ -382,10 +390,10 @@ __global__ void k_stencil_filter(int n_el, int nx, int ny, int nz, int R, int n_
 // ------------------------------------------------------------------ bisection
 // state: [0] l1 [1] l2 [2] lmid [3] vol_error [4] iter [5] done [6] n_evals
 
-__global__ void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+__global__ void k_fill(int64_t n, double v, double* __restrict__ b) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = a[i];
+    b[i] = v;
 }
 
 __global__ void k_oc_init(double* st, double l1, double l2) {
@@ -467,7 +475,7 @@ __global__ void k_oc_volume_stencil(int n_design, const int32_t* __restrict__ id
   double vsum = 0.0;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n_design;
        r += gridDim.x * blockDim.x) {
-    const double acc = stencil_row<false>(ids[r], r, nx, ny, nz, R, n_off, off, del, u,
+    const double acc = stencil_row<false, true>(ids[r], r, nx, ny, nz, R, n_off, off, del, u,
                                           colmap, inv_rs, cand_full);
     double t = (acc - beta_eta) * beta;
     t = tanh(t);
@@ -845,8 +853,10 @@ int skt_oc_bisect(skt_filter* f, const double* rho_full, const double* rho_e,
   const int64_t nd = f->n_design;
   const unsigned nbs = grid_for(nd, kBlock, kPartMax);
   unsigned nb = grid_for(nd * kSeg, kBlock, kPartMax);
-  hipLaunchKernelGGL(k_copy, dim3(grid_for(f->n_el)), dim3(kBlock), 0, stream, f->n_el,
-                     rho_full, f->cand_full);
+  // candidates live on the design rows only; the other entries stay zero,
+  // which the stencil volume sum relies on (its design-masked row form)
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(f->n_el)), dim3(kBlock), 0, stream, f->n_el, 0.0,
+                     f->cand_full);
   hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(1), 0, stream, f->state, l1, l2);
   for (int s = 0; s < steps; ++s) {
     hipLaunchKernelGGL(k_oc_candidate, dim3(grid_for(nd)), dim3(kBlock), 0, stream, nd,
