@@ -85,6 +85,26 @@ def test_mg_multiload_columns_independent(gpu):
     assert np.abs(U - Uo).max() <= 1e-8 * np.abs(Uo).max()
 
 
+@pytest.mark.parametrize("precision,rtol", [("fp64", 1e-12), ("mixed", 1e-10)])
+def test_mg_stopped_columns_are_skipped_safely(gpu, precision, rtol):
+    """Columns 0 and 1 start from their exact solution, so they stop at the
+    first check while columns 2 and 3 iterate; the operator and V-cycle
+    then skip the workgroups of the stopped pair.  Every column must still
+    match the direct solve."""
+    task = oprob.multiload(16, 8, 8, n_loads=4, seed=3)
+    rng = np.random.default_rng(4)
+    rho = rng.uniform(0.05, 1.0, task.n_elements)
+    ofe = ofem.ElasticFEM(task, 1e-9)
+    Uo = np.zeros((task.ndof, 4))
+    ofe.objectives_multi_load(rho, 3.0, Uo)
+    fem = _fem(task, "elastic", precision, rtol, "mg")
+    U = np.zeros((task.ndof, 4))
+    U[:, :2] = Uo[:, :2]
+    fem.objectives_multi_load(rho, 3.0, U)
+    assert fem.last_solve["iters"] > 0
+    assert np.abs(U - Uo).max() <= 100 * rtol * np.abs(Uo).max()
+
+
 def test_mg_request_on_unstructured_mesh_is_rejected(gpu):
     from sktopt_amd import fea, mesh as pmesh
     task = oprob.cantilever(4, 3, 3, kind="tet4", jitter=0.2, seed=1)
