@@ -516,6 +516,63 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   }
 }
 
+// Element energies 0.5 E_e u_e^T K0 u_e in the modal basis: with m = H u
+// per component (H the 8-point butterfly above, symmetric) and f = Kmod m,
+// the element apply is H f, so u^T K0 u = m . f.  One thread per (element,
+// column), columns fastest so a node's columns are read together.
+template <typename T, int NC>
+__global__ void k_hex_energy_modal(int64_t nx, int64_t ny, int64_t nz, int ncols,
+                                   ModalArgs<T> coef, const T* __restrict__ E,
+                                   const T* __restrict__ U, double* __restrict__ energy) {
+  const int64_t nnx = nx + 1, nny = ny + 1;
+  const int64_t total = nx * ny * nz * ncols;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int col = (int)(t % ncols);
+    const int64_t e = t / ncols;
+    const int64_t ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+    T m[NC][8];
+#pragma unroll
+    for (int idx = 0; idx < 8; ++idx) {
+      const int64_t nb = (ex + (idx & 1)) + nnx * ((ey + ((idx >> 1) & 1)) + nny * (ez + (idx >> 2)));
+#pragma unroll
+      for (int d = 0; d < NC; ++d) m[d][idx] = U[(nb * NC + d) * ncols + col];
+    }
+#pragma unroll
+    for (int d = 0; d < NC; ++d) wht8(m[d]);
+    T f[NC][8];
+    if constexpr (NC == 3) modal_elastic(coef.c, m, f);
+    else modal_heat(coef.c, m, f);
+    double q = 0.0;
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) q += (double)m[d][i] * (double)f[d][i];
+    energy[e * ncols + col] = 0.5 * (double)E[e] * q;
+  }
+}
+
+template <typename T>
+int hex_energy_modal(const skt_operator* op, int ncols, const T* coef, const T* E, const T* U,
+                     double* out, hipStream_t s) {
+  ModalArgs<T> a;
+  for (int i = 0; i < 40; ++i) a.c[i] = coef[i];
+  const int64_t total = op->n_el * ncols;
+  if (op->ncomp == 3)
+    hipLaunchKernelGGL((k_hex_energy_modal<T, 3>), dim3(grid_for(total)), dim3(kBlock), 0, s,
+                       op->nx, op->ny, op->nz, ncols, a, E, U, out);
+  else
+    hipLaunchKernelGGL((k_hex_energy_modal<T, 1>), dim3(grid_for(total)), dim3(kBlock), 0, s,
+                       op->nx, op->ny, op->nz, ncols, a, E, U, out);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+template int hex_energy_modal<float>(const skt_operator*, int, const float*, const float*,
+                                     const float*, double*, hipStream_t);
+template int hex_energy_modal<double>(const skt_operator*, int, const double*, const double*,
+                                      const double*, double*, hipStream_t);
+
 // Launch geometry: (tiles_x, tiles_y) node tiles per plane, nzc z chunks of
 // lz node planes, one workgroup per (tile, chunk, column block) with the
 // column block slowest-varying.

@@ -54,6 +54,11 @@ int extra_apply_add(const skt_operator* op, int dtype, int ncols, int kc, const 
 int extra_diag_add(const skt_operator* op, int dtype, void* diag, hipStream_t s);
 // Shared tail of the hex8 constructors (operators.hip).
 int hex8_finish(skt_operator* op, const uint8_t* dof_fixed, skt_operator** out);
+// Element energies 0.5 E_e u_e^T K0 u_e ((n_el, ncols) row-major U and out)
+// through the modal form of K0.
+template <typename T>
+int hex_energy_modal(const skt_operator* op, int ncols, const T* coef, const T* E, const T* U,
+                     double* out, hipStream_t s);
 // Output modes of the structured hex8 modal kernel.
 // OUT_CHEB0: the first two Chebyshev steps from a zero iterate in one pass:
 // X is the right-hand side b, the first iterate x1 = s0 dinv b is formed on

@@ -685,6 +685,12 @@ template <typename T>
 static int energy_t(skt_operator* op, int ncols, const T* K0, const T* g, const T* vol,
                     const T* E, const T* U, double* out, hipStream_t s) {
   const int64_t total = op->n_el * ncols;
+  if (op->mesh == SKT_MESH_HEX8 && op->modal_ok) {
+    const T* coef;
+    if constexpr (sizeof(T) == 8) coef = op->modal_d;
+    else coef = op->modal_f;
+    return hex_energy_modal<T>(op, ncols, coef, E, U, out, s);
+  }
   if (op->mesh == SKT_MESH_HEX8) {
     if (op->ncomp == 3)
       hipLaunchKernelGGL((k_hex_energy<T, 3>), dim3(grid_for(total)), dim3(kBlock), 0, s,
