@@ -135,6 +135,11 @@ int skt_reduce(int op, int64_t n, const double* x, const double* y,
 int skt_reduce_cols(int op, int64_t n, int ncols, const double* x,
                     const double* y, double* out, hipStream_t stream);
 
+/* out[0..3] = [min, mean, max, std] of x (population std, numpy's np.std),
+ * the statistics a "min-max-mean-std" history keeps per iteration
+ * (tools/history.py:145-151).  n >= 1. */
+int skt_stats4(int64_t n, const double* x, double* out, hipStream_t stream);
+
 /* np.percentile(np.abs(x), q) with numpy's default 'linear' method
  * (core/optimizers/oc.py:188-189, logmoc.py:167-174,220).  Exact radix
  * select on the float64 bit patterns. */

@@ -308,6 +308,88 @@ __global__ void k_reduce_cols_final(int op, int nb, int ncols,
   }
 }
 
+// ------------------------------------------------------------------- stats
+// [min, mean, max, population std] of x.  Block b takes the contiguous
+// chunk [b*chunk, min(n, (b+1)*chunk)), whose count is c_b.  A first pass
+// gives its sum, min, max and the shift a_b = sum / c_b; a second pass over
+// the chunk (in cache) gives d1_b = sum (x - a_b) and d2_b = sum (x - a_b)^2.
+// The final block pools the chunks exactly about the global mean m:
+// M2 = sum_b d2_b + 2 (a_b - m) d1_b + c_b (a_b - m)^2, which holds for any
+// shift, so the rounding of a_b does not enter the result.
+constexpr int kStatsChunk = 4 * kBlock;
+constexpr int kStatsPart = 8;   // c, sum, a, d1, d2, min, max, (pad)
+
+__global__ void k_stats_part(int64_t n, int64_t chunk, const double* __restrict__ x,
+                             double* __restrict__ part) {
+  __shared__ double sh[kBlock / kWave];
+  __shared__ double shift_sh;
+  const int64_t a = (int64_t)blockIdx.x * chunk;
+  const int64_t b = a + chunk < n ? a + chunk : n;
+  double s = 0.0, hi = -INFINITY, nlo = -INFINITY;
+  for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+    const double v = x[i];
+    s += v;
+    hi = v > hi ? v : hi;
+    nlo = -v > nlo ? -v : nlo;
+  }
+  const double sum = block_sum(s, sh);
+  if (threadIdx.x == 0) shift_sh = sum / (double)(b - a);
+  hi = block_max(hi, sh);           // its barriers publish shift_sh
+  nlo = block_max(nlo, sh);
+  __syncthreads();
+  const double m = shift_sh;
+  double d1 = 0.0, d2 = 0.0;
+  for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+    const double d = x[i] - m;
+    d1 += d;
+    d2 += d * d;
+  }
+  d1 = block_sum(d1, sh);
+  d2 = block_sum(d2, sh);
+  if (threadIdx.x == 0) {
+    double* p = part + kStatsPart * (int64_t)blockIdx.x;
+    p[0] = (double)(b - a);
+    p[1] = sum;
+    p[2] = m;
+    p[3] = d1;
+    p[4] = d2;
+    p[5] = -nlo;
+    p[6] = hi;
+  }
+}
+
+__global__ void k_stats_final(int nb, int64_t n, const double* __restrict__ part,
+                              double* __restrict__ out) {
+  __shared__ double sh[kBlock / kWave];
+  __shared__ double mean_sh;
+  double s = 0.0, hi = -INFINITY, nlo = -INFINITY;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const double* p = part + kStatsPart * i;
+    s += p[1];
+    hi = p[6] > hi ? p[6] : hi;
+    nlo = -p[5] > nlo ? -p[5] : nlo;
+  }
+  const double sum = block_sum(s, sh);
+  if (threadIdx.x == 0) mean_sh = sum / (double)n;
+  hi = block_max(hi, sh);
+  nlo = block_max(nlo, sh);
+  __syncthreads();
+  const double m = mean_sh;
+  double m2 = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const double* p = part + kStatsPart * i;
+    const double e = p[2] - m;
+    m2 += p[4] + 2.0 * e * p[3] + p[0] * e * e;
+  }
+  const double M2 = block_sum(m2, sh);
+  if (threadIdx.x == 0) {
+    out[0] = -nlo;
+    out[1] = m;
+    out[2] = hi;
+    out[3] = sqrt(fmax(M2, 0.0) / (double)n);
+  }
+}
+
 // ---------------------------------------------------------------- percentile
 // Exact k-th smallest of |x| by an 8-bit MSB-first radix select on the
 // float64 bit patterns (non-negative doubles order like their uint64 bits).
@@ -658,6 +740,24 @@ int skt_reduce_cols(int op, int64_t n, int ncols, const double* x,
                      ncols, cp, x, y, s->part);
   hipLaunchKernelGGL(k_reduce_cols_final, dim3(1), dim3(kBlock), 0, stream, op,
                      (int)nb, ncols, s->part, out);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_stats4(int64_t n, const double* x, double* out, hipStream_t stream) {
+  SKT_CHECK(n >= 1 && x && out, "statistics of an empty array");
+  Scratch* s;
+  int st = get_scratch(&s);
+  if (st) return st;
+  // at most kMaxBlocks chunks of kStatsPart partials fit the scratch
+  int64_t chunk = (n + kMaxBlocks - 1) / kMaxBlocks;
+  if (chunk < kStatsChunk) chunk = kStatsChunk;
+  const int64_t nb = (n + chunk - 1) / chunk;
+  SKT_CHECK(nb >= 1 && nb <= kMaxBlocks, "bad statistics grid");
+  hipLaunchKernelGGL(k_stats_part, dim3((unsigned)nb), dim3(kBlock), 0, stream, n, chunk, x,
+                     s->part);
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kBlock), 0, stream, (int)nb, n,
+                     (const double*)s->part, out);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }

@@ -131,12 +131,6 @@ class DensityMethod_OC_Config(DensityMethodConfig):
         "move_limit", 0.3, 0.1, 3))
 
 
-def _stats4(x: torch.Tensor) -> torch.Tensor:
-    """[min, mean, max, std] (population std, as numpy's) of a device array."""
-    x = x.reshape(-1)
-    return torch.stack([x.min(), x.mean(), x.max(), x.std(correction=0)])
-
-
 def interpolation_funcs(cfg):
     if cfg.interpolation == "SIMP":
         return composer.simp_interpolation, _lib.SIMP
@@ -410,8 +404,15 @@ class DensityMethod:
         arrays = {"rho_projected": rpd, "energy": energy.mean(dim=1)}
         arrays.update(self._record_arrays(st))
         names = list(arrays)
-        stats = [_stats4(arrays[n]) for n in names]
-        rec = torch.cat([sc[:8], comp.reshape(-1), u_max.reshape(-1)] + stats)
+        # [min, mean, max, std] (population std, as numpy's) per array
+        sbuf = st.get("stats4")
+        if sbuf is None or sbuf.numel() != 4 * len(names):
+            sbuf = st["stats4"] = torch.empty(4 * len(names), dtype=torch.float64,
+                                              device=self.dev)
+        for i, n in enumerate(names):
+            x = arrays[n].reshape(-1).to(torch.float64).contiguous()
+            _lib.call("skt_stats4", x.numel(), _dev.ptr(x), _dev.ptr(sbuf[4 * i:4 * i + 4]), S)
+        rec = torch.cat([sc[:8], comp.reshape(-1), u_max.reshape(-1), sbuf])
         ctx = dict(names=names, k=k, neumann_scale=neumann_scale,
                    solve_iters=self.fem.last_solve.get("iters"),
                    obj=self._objective_history_name(tsk))

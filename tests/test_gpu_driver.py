@@ -81,13 +81,15 @@ def test_multiload_trajectory_matches_oracle(gpu, tmp_path):
         assert np.abs(opt.rho_host() - loop.rho).max() <= 1e-8
 
 
-def test_logmoc_heat_trajectory_matches_oracle(gpu, tmp_path):
+@pytest.mark.parametrize("dual", [dict(), dict(dual_update="augmented",
+                                             augmented_lagrangian_mu=0.5)])
+def test_logmoc_heat_trajectory_matches_oracle(gpu, dual, tmp_path):
     task = oprob.heat_patch(10, patch=4, source=1e-2)
     cfg = _oracle_cfg(method="LogMOC", vol=0.3, r=2.0, emin=1e-3, eta=odrv.Sched("Constant", 0.6, 0.6),
-                      lambda_lower=-1e7, lambda_upper=1e7)
+                      lambda_lower=-1e7, lambda_upper=1e7, **dual)
     loop = odrv.DensityLoop(cfg, task)
     opt = _gpu_opt(oprob.heat_patch(10, patch=4, source=1e-2), method="LogMOC", vol=0.3, r=2.0,
-                   emin=1e-3, precision="fp64", rtol=1e-13, tmp=str(tmp_path))
+                   emin=1e-3, precision="fp64", rtol=1e-13, tmp=str(tmp_path), **dual)
     for it in range(1, 5):
         rec = loop.step(it)
         opt.optimize_steps(1)

@@ -124,6 +124,24 @@ def test_reductions(gpu):
     np.testing.assert_array_equal(o.cpu().numpy(), np.abs(X).max(0))
 
 
+@pytest.mark.parametrize("n", [1, 2, 1023, 1024, 1025, 100001, 432000, 1_500_001])
+def test_stats4_matches_numpy(gpu, n):
+    """[min, mean, max, std] as numpy computes them; the offset case keeps
+    a large mean over a small spread (a one-pass sum of squares would lose
+    the std there), the constant case has std exactly 0."""
+    from sktopt_amd import _lib
+    rng = np.random.default_rng(n)
+    out = _t(np.zeros(4))
+    for x in (rng.standard_normal(n), 1e6 + 1e-3 * rng.uniform(size=n), np.full(n, 0.3)):
+        _lib.call("skt_stats4", n, _p(_t(x)), _p(out), _s())
+        got = out.cpu().numpy()
+        assert got[0] == x.min() and got[2] == x.max()
+        np.testing.assert_allclose(got[1], x.mean(), rtol=1e-13)
+        np.testing.assert_allclose(got[3], x.std(), rtol=1e-11, atol=1e-14 * abs(x.mean()))
+    with pytest.raises(ValueError):
+        _lib.call("skt_stats4", 0, _p(_t(np.zeros(1))), _p(out), _s())
+
+
 # ---------------------------------------------------------------- filter
 
 def _filter_case(kind, n, r):

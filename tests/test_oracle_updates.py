@@ -81,6 +81,91 @@ def test_logmoc_step_multiplicative_move_limits():
                                rtol=1e-14)
 
 
+class _StubFilter:
+    """A filter whose forward and gradient are fixed diagonal scalings."""
+
+    def __init__(self, wf, wg):
+        self.wf, self.wg = wf, wg
+
+    def forward(self, x):
+        return x * self.wf
+
+    def gradient(self, x):
+        return x * self.wg
+
+
+def _logmoc_run(cfg, filt, n_iter=6, seed=0, vol_lo=0.05, vol_hi=0.55):
+    """Drive the oracle's LogMOC step over synthetic fields; returns the
+    per-iteration (vol_error, lambda_v, coeff) and the final densities."""
+    rng = np.random.default_rng(seed)
+    n = 24
+    design = np.arange(n)
+    vol = rng.uniform(0.5, 1.5, n)
+    upd = olm.LogMOCUpdate(cfg)
+    rho = rng.uniform(0.2, 0.8, n)
+    out = []
+    for it in range(1, n_iter + 1):
+        proj = rng.uniform(vol_lo, vol_hi, n)
+        filt_rho = rng.uniform(0.0, 1.0, n)
+        dC = -rng.uniform(0.1, 1.0, n)
+        upd(it, rho, dC, proj, filt_rho, 0.2, 0.5, 2.0, None, vol, vol.sum(), 0.3, design, filt)
+        out.append((upd.last["vol_error"], upd.last["lambda_v"], upd.last["coeff"]))
+    return np.array(out), rho
+
+
+def test_logmoc_dual_updates_closed_forms():
+    """The dual variable against the closed forms of its two recursions
+    (``logmoc.py:186-200``): the augmented update is mu_p times the running
+    sum of the volume errors, the EMA is a geometrically weighted sum, both
+    clipped to [lambda_lower, lambda_upper]; the constraint coefficient adds
+    augmented_lagrangian_mu times the current error."""
+    from oracle import driver as odrv
+    ident = _StubFilter(1.0, 1.0)
+    mu, mu_aug, d = 5.0, 0.7, 0.9
+    base = dict(method="LogMOC", mu_p=mu, augmented_lagrangian_mu=mu_aug, lambda_decay=d,
+                lambda_lower=-1e7, lambda_upper=1e7)
+    # errors of both signs: projected densities around the target 0.3
+    rec, _ = _logmoc_run(odrv.Config(dual_update="augmented", **base), ident)
+    e = rec[:, 0]
+    assert (e > 0).any() and (e < 0).any()
+    np.testing.assert_allclose(rec[:, 1], mu * np.cumsum(e), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(rec[:, 2], mu * np.cumsum(e) + mu_aug * e, rtol=1e-13,
+                               atol=1e-15)
+    rec, _ = _logmoc_run(odrv.Config(dual_update="ema", **base), ident)
+    e = rec[:, 0]
+    k = np.arange(1, e.size + 1)
+    ema = [mu * (d ** (n - 1) * e[0] + (1 - d) * sum(d ** (n - j) * e[j - 1]
+                                                     for j in range(2, n + 1))) for n in k]
+    np.testing.assert_allclose(rec[:, 1], ema, rtol=1e-13, atol=1e-15)
+    # the running sum of positive errors saturates at the upper bound and stays there
+    cfg = odrv.Config(dual_update="augmented", **{**base, "lambda_upper": 2.0})
+    rec, _ = _logmoc_run(cfg, ident, vol_lo=0.5, vol_hi=0.9)
+    assert (rec[:, 0] > 0.2).all()
+    np.testing.assert_array_equal(rec[1:, 1], 2.0)
+    assert rec[0, 1] == pytest.approx(mu * rec[0, 0]) and rec[0, 1] < 2.0
+    # negative errors pin the EMA at the lower bound
+    cfg = odrv.Config(dual_update="ema", **{**base, "lambda_lower": 1e-7})
+    rec, _ = _logmoc_run(cfg, ident, vol_lo=0.0, vol_hi=0.2)
+    np.testing.assert_array_equal(rec[:, 1], 1e-7)
+
+
+def test_logmoc_zero_gradient_fallback_uses_forward():
+    """When the filter's gradient of the volume chain is all zero while the
+    chain is positive (``logmoc.py:145-153``), the step uses the forward
+    application instead: a filter with a zero gradient and forward W must
+    give the same step, bit for bit, as one whose gradient is W."""
+    from oracle import driver as odrv
+    w = np.random.default_rng(5).uniform(0.5, 2.0, 24)
+    cfg = dict(method="LogMOC", dual_update="ema", augmented_lagrangian_mu=0.3)
+    rec_a, rho_a = _logmoc_run(odrv.Config(**cfg), _StubFilter(w, 0.0))
+    rec_b, rho_b = _logmoc_run(odrv.Config(**cfg), _StubFilter(0.0 * w, w))
+    np.testing.assert_array_equal(rec_a, rec_b)
+    np.testing.assert_array_equal(rho_a, rho_b)
+    # and the fallback is not a no-op: with no chain at all the step differs
+    rec_c, rho_c = _logmoc_run(odrv.Config(**cfg), _StubFilter(0.0 * w, 0.0))
+    assert np.abs(rho_c - rho_a).max() > 1e-6
+
+
 def test_sched_hand_values():
     # Step 1 -> 3 in 3 steps over 10 iterations: step length 10/3
     got = [osch.value("Step", it, 10, 1.0, 3.0, 3) for it in range(1, 11)]
