@@ -1227,6 +1227,17 @@ static int stencil_layout(int64_t nn) {
   return nn >= kTiledNodes ? ST_TILE_BIG : ST_TILE_SMALL;
 }
 
+// Power-iteration steps from the previous set-up's vector;
+// SKT_MG_POWER_WARM=n (1..64) overrides the default.
+static int warm_power_steps(const MGData* mg) {
+  const char* v = std::getenv("SKT_MG_POWER_WARM");
+  if (v) {
+    const int n = std::atoi(v);
+    if (n >= 1 && n <= 64) return n;
+  }
+  return mg->power_iters_warm;
+}
+
 int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype, int max_cols) {
   if (mg->dtype == dtype && mg->max_cols == max_cols) return SKT_OK;
   SKT_CHECK((op->nx + 1) * (op->ny + 1) < INT32_MAX &&
@@ -1523,7 +1534,7 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
     // lambda per step, far from the float range over these few steps); the
     // last step's input and output norms give lambda, and the normalised
     // result is kept for the next set-up.
-    const int steps = mg->warm ? mg->power_iters_warm : mg->power_iters;
+    const int steps = mg->warm ? warm_power_steps(mg) : mg->power_iters;
     for (int it = 0; it < steps; ++it) {
       const bool last = it + 1 == steps;
       if (last) {

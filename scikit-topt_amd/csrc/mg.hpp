@@ -64,7 +64,7 @@ struct MGData {
   int degree = 2;
   double ratio = 10.0;
   int power_iters = 20;               // steps from the hashed start vector
-  int power_iters_warm = 8;           // steps from the previous set-up's vector
+  int power_iters_warm = 5;           // steps from the previous set-up's vector
 };
 
 struct MGConfig {
