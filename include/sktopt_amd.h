@@ -216,6 +216,22 @@ int skt_hex8_robin_virtual(int64_t nx, int64_t ny, int64_t nz, double hx,
                            double h, double T_env, double* stencil,
                            double* load, hipStream_t stream);
 
+/* Heat-exchange objective terms (fea/solver_heat.py:306-322,380-436,
+ * 791-832) on nodal densities rho_node and temperatures T (n_nodes, ncols),
+ * npts Gauss points per axis:
+ *   den (n_el)          int_e |grad rho_h|                      (J_den parts)
+ *   num (n_el, ncols)   int_e (T_c - T_env) |grad rho_h|        (J_num parts
+ *                       before the factor -T_env h)
+ *   rhs (n_nodes)       int -T_env h_eff |grad rho_h| v, h_eff the
+ *                       interpolant of h rho^p (1 - rho)^q (adjoint load
+ *                       before the division by J_den). */
+int skt_hex8_heat_exchange(int64_t nx, int64_t ny, int64_t nz, double hx,
+                           double hy, double hz, int npts,
+                           const double* rho_node, int ncols, const double* T,
+                           double p, double q, double h, double T_env,
+                           double* den, double* num, double* rhs,
+                           hipStream_t stream);
+
 /* _robin_compliance_explicit_grad_ (:520-546) per load column of T
  * (n_nodes, ncols), mapped to elements by _nodal_gradient_to_element_gradient
  * (:616-624): out (n_el, ncols); work: n_nodes doubles. */

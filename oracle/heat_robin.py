@@ -15,8 +15,11 @@ Restates, for structured box hex8 grids, ``fea/solver_heat.py``:
   ``_nodal_gradient_to_element_gradient`` (:616-624);
 * ``objectives_multi_load`` (:705-895) for ``objective='compliance'``
   (``J = T^T K T`` with the unenforced total K, conduction + facet Robin +
-  virtual Robin) and ``'averaged_temp'`` (``J = sum(T)``, adjoint
-  ``K lam = 1`` enforced with the state's Dirichlet values);
+  virtual Robin), ``'averaged_temp'`` (``J = sum(T)``, adjoint
+  ``K lam = 1`` enforced with the state's Dirichlet values) and
+  ``'heat_exchange'`` (``J = J_num / J_den``, adjoint load divided by
+  ``J_den``, optional ``avg_temp_weight`` blend; functionals on a
+  4-point-per-axis rule, see ``heat_exchange_terms``);
 * ``assemble_surface_robin`` of ``mesh/task_heat.py:60-139``: the facet
   forms ``int_G h u v`` and ``int_G h T_env v`` (constant: the density
   field there is never set);
@@ -176,19 +179,46 @@ def robin_explicit_grad_nodal(mesh, rn, T, p, q, h, T_env, npts=2):
     return nodal
 
 
+def heat_exchange_terms(mesh, rn, T, p, q, h, T_env, npts=4):
+    """(J_den, J_num (k,), rhs (nn,)) of the heat-exchange objective
+    (``solver_heat.py:306-322`` ``_hx_num_density_``, ``:380-390``
+    ``_hx_den_density_``, ``:393-408`` ``_hx_adj_rhs_``, ``:411-436``
+    ``heat_exchange_objective``): J_den = int |grad rho_h|, J_num_k =
+    -T_env h int (T_k - T_env) |grad rho_h| with the scalar h, and the
+    adjoint load int -T_env h_eff |grad rho_h| v with h_eff the interpolant
+    of the nodal h rho^p (1 - rho)^q (``:824-829``) -- not divided by J_den.
+    The reference evaluates them on ``skfem.CellBasis(mesh, elem)`` with its
+    default rule; for ElementHex1 (maximum degree 3) skfem's default
+    integration order is 2 * 3 = 6, a Gauss-Legendre rule of ceil(7 / 2) = 4
+    points per axis, the default ``npts`` here."""
+    Ns, dNs, wd, rh, g, gn = _element_fields(mesh, rn, npts)
+    den = float(np.einsum("q,qe->", wd, gn))
+    Th = np.einsum("qa,aek->qek", Ns, T[mesh.t])             # (nq, n_el, k)
+    num = -T_env * h * np.einsum("q,qek,qe->k", wd, Th - T_env, gn)
+    heff = h * rn ** p * (1.0 - rn) ** q
+    he = Ns @ heff[mesh.t]                                    # (nq, n_el)
+    Le = np.einsum("q,qe,qa->ea", wd, -T_env * he * gn, Ns)
+    rhs = np.zeros(mesh.nnodes)
+    np.add.at(rhs, mesh.t.T.ravel(), Le.ravel())
+    return den, num, rhs
+
+
 class HeatRobinFEM:
     """Oracle of ``FEM_SimpLinearHeatConduction`` with Robin data on a
     structured hex grid.  ``robin_facets``: list of (4, nf) facet arrays;
     ``emit``: extra volumetric source vector added to the Robin loads."""
 
     def __init__(self, mesh, dirichlet_dofs, dirichlet_values, k, E_min_coeff, robin_facets,
-                 h, T_env, emit=None, objective="compliance", q=4, npts=2):
+                 h, T_env, emit=None, objective="compliance", q=4, npts=2, npts_cell=4,
+                 avg_temp_weight=0.0):
         self.mesh = mesh
         self.D = np.asarray(dirichlet_dofs)
         self.values = list(dirichlet_values)
         self.k_max, self.k_min = k, k * E_min_coeff
         self.h, self.T_env, self.q, self.npts = h, T_env, q, npts
         self.objective = objective
+        self.npts_cell = npts_cell
+        self.avg_temp_weight = float(avg_temp_weight)
         self.K0 = el.hex8_conduction_K0(mesh.h)
         n = mesh.nnodes
         self.Bf = sp.csr_matrix((n, n))
@@ -225,6 +255,29 @@ class HeatRobinFEM:
         if self.objective == "averaged_temp":
             self.lam = self._solve(K, np.ones_like(emit), n_loads)
             return np.array([float(np.sum(U[:, i])) for i in range(n_loads)])
+        if self.objective == "heat_exchange":
+            # solver_heat.py:791-869
+            rn, _ = nodal_average_reference(self.mesh, rho)
+            den, num, rhs = heat_exchange_terms(self.mesh, rn, U, p, self.q, self.h,
+                                                self.T_env, self.npts_cell)
+            if den <= 1e-16:
+                self.lam = np.zeros_like(U)
+                return np.zeros(n_loads)
+            J = num / den
+            self.lam = self._solve(K, rhs / den, n_loads)
+            w = self.avg_temp_weight
+            if w != 0.0:
+                avg = U.sum(axis=0)
+                lam_avg = self._solve(K, np.ones_like(emit), n_loads)
+                if n_loads > 1:
+                    hs = max(float(np.mean(np.abs(J))), 1e-12)
+                    avs = max(float(np.mean(np.abs(avg))), 1e-12)
+                    J = J / hs + w * (avg / avs)
+                    self.lam = self.lam / hs + (w / avs) * lam_avg
+                else:
+                    J = J + w * avg
+                    self.lam = self.lam + w * lam_avg
+            return J
         raise ValueError(self.objective)
 
     def energy_multi_load(self, rho, p, u_dofs):

@@ -155,6 +155,98 @@ __global__ void k_robin_virtual(int64_t nx, int64_t ny, int64_t nz, double hx, d
   }
 }
 
+// Heat-exchange functionals per element (fea/solver_heat.py:306-322,380-436):
+// den[e] = int_e |grad rho_h| and num[e*ncols + c] = int_e (T_c - T_env)
+// |grad rho_h|, rho_h and T_h the trilinear interpolants of nodal values.
+__global__ void k_hx_elem(int64_t nx, int64_t ny, int64_t nz, double hx, double hy, double hz,
+                          Gauss g, const double* __restrict__ rn, int ncols,
+                          const double* __restrict__ T, double Tenv, double* __restrict__ den,
+                          double* __restrict__ num) {
+  const int64_t nnx = nx + 1, nny = ny + 1, ne = nx * ny * nz;
+  const double detJ = hx * hy * hz / 8.0;
+  const double sx = 2.0 / hx, sy = 2.0 / hy, sz = 2.0 / hz;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ne;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+    int64_t nd[8];
+    double r8[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      nd[b] = (ex + kCx[b]) + nnx * ((ey + kCy[b]) + nny * (ez + kCz[b]));
+      r8[b] = rn[nd[b]];
+    }
+    double dsum = 0.0;
+    for (int c = 0; c < ncols; ++c) num[e * ncols + c] = 0.0;
+    for (int c3 = 0; c3 < g.n; ++c3)
+      for (int bq = 0; bq < g.n; ++bq)
+        for (int aq = 0; aq < g.n; ++aq) {
+          QP q;
+          qp_eval(g.x[aq], g.x[bq], g.x[c3], sx, sy, sz, q);
+          const double wd = g.w[aq] * g.w[bq] * g.w[c3] * detJ;
+          double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            gx += q.dN[b][0] * r8[b];
+            gy += q.dN[b][1] * r8[b];
+            gz += q.dN[b][2] * r8[b];
+          }
+          const double wg = wd * sqrt(gx * gx + gy * gy + gz * gz);
+          dsum += wg;
+          for (int c = 0; c < ncols; ++c) {
+            double th = 0.0;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) th += q.N[b] * T[nd[b] * ncols + c];
+            num[e * ncols + c] += wg * (th - Tenv);
+          }
+        }
+    den[e] = dsum;
+  }
+}
+
+// Adjoint load of the heat-exchange objective, one node per thread
+// (fea/solver_heat.py:393-408,818-832): int -T_env h_eff |grad rho_h| v with
+// h_eff the interpolant of the nodal values h rho^p (1 - rho)^q.
+__global__ void k_hx_rhs(int64_t nx, int64_t ny, int64_t nz, double hx, double hy, double hz,
+                         Gauss g, const double* __restrict__ rn, double p, double qexp,
+                         double hcoef, double Tenv, double* __restrict__ L) {
+  const int64_t nnx = nx + 1, nny = ny + 1, nn = nnx * nny * (nz + 1);
+  const double detJ = hx * hy * hz / 8.0;
+  const double sx = 2.0 / hx, sy = 2.0 / hy, sz = 2.0 / hz;
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < nn;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = n % nnx, j = (n / nnx) % nny, k = n / (nnx * nny);
+    double lv = 0.0;
+    for (int sl = 0; sl < 8; ++sl) {          // elements of the node, ascending
+      const int64_t ex = i - 1 + (sl & 1), ey = j - 1 + ((sl >> 1) & 1), ez = k - 1 + (sl >> 2);
+      if (ex < 0 || ex >= nx || ey < 0 || ey >= ny || ez < 0 || ez >= nz) continue;
+      const int an = corner_of((int)(i - ex), (int)(j - ey), (int)(k - ez));
+      double r8[8], h8[8];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        r8[b] = rn[(ex + kCx[b]) + nnx * ((ey + kCy[b]) + nny * (ez + kCz[b]))];
+        h8[b] = hcoef * pow(r8[b], p) * pow(1.0 - r8[b], qexp);
+      }
+      for (int c = 0; c < g.n; ++c)
+        for (int bq = 0; bq < g.n; ++bq)
+          for (int aq = 0; aq < g.n; ++aq) {
+            QP q;
+            qp_eval(g.x[aq], g.x[bq], g.x[c], sx, sy, sz, q);
+            const double wd = g.w[aq] * g.w[bq] * g.w[c] * detJ;
+            double he = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+              he += q.N[b] * h8[b];
+              gx += q.dN[b][0] * r8[b];
+              gy += q.dN[b][1] * r8[b];
+              gz += q.dN[b][2] * r8[b];
+            }
+            lv += wd * (-Tenv * he * sqrt(gx * gx + gy * gy + gz * gz)) * q.N[an];
+          }
+    }
+    L[n] = lv;
+  }
+}
+
 // Nodal explicit gradient of one load column c of T (nn x ncols).
 __global__ void k_robin_grad_nodal(int64_t nx, int64_t ny, int64_t nz, double hx, double hy,
                                    double hz, Gauss g, const double* __restrict__ rn,
@@ -328,6 +420,23 @@ int skt_hex8_robin_virtual(int64_t nx, int64_t ny, int64_t nz, double hx, double
   const int64_t nn = (nx + 1) * (ny + 1) * (nz + 1);
   hipLaunchKernelGGL(k_robin_virtual, dim3(grid_for(nn)), dim3(kBlock), 0, stream, nx, ny, nz,
                      hx, hy, hz, gauss_rule(npts), rho_node, p, q, h, T_env, stencil, load);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_hex8_heat_exchange(int64_t nx, int64_t ny, int64_t nz, double hx, double hy, double hz,
+                           int npts, const double* rho_node, int ncols, const double* T,
+                           double p, double q, double h, double T_env, double* den, double* num,
+                           double* rhs, hipStream_t stream) {
+  SKT_CHECK(nx >= 1 && ny >= 1 && nz >= 1, "grid dimensions must be >= 1");
+  SKT_CHECK(npts >= 1 && npts <= 4 && ncols >= 1, "bad npts or ncols");
+  SKT_CHECK(rho_node && T && den && num && rhs, "null pointer");
+  const int64_t nn = (nx + 1) * (ny + 1) * (nz + 1), ne = nx * ny * nz;
+  const Gauss g = gauss_rule(npts);
+  hipLaunchKernelGGL(k_hx_elem, dim3(grid_for(ne)), dim3(kBlock), 0, stream, nx, ny, nz, hx, hy,
+                     hz, g, rho_node, ncols, T, T_env, den, num);
+  hipLaunchKernelGGL(k_hx_rhs, dim3(grid_for(nn)), dim3(kBlock), 0, stream, nx, ny, nz, hx, hy,
+                     hz, g, rho_node, p, q, h, T_env, rhs);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }

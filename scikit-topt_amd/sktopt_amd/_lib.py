@@ -76,6 +76,8 @@ SIGNATURES = {
     "skt_hex8_nodal_average_pairing": [_i64, _i64, _i64, _vp, _vp, _vp, _vp],
     "skt_hex8_robin_virtual": [_i64, _i64, _i64, _d, _d, _d, _i32, _vp, _d, _d, _d, _d, _vp,
                                _vp, _vp],
+    "skt_hex8_heat_exchange": [_i64, _i64, _i64, _d, _d, _d, _i32, _vp, _i32, _vp, _d, _d, _d,
+                               _d, _vp, _vp, _vp, _vp],
     "skt_hex8_robin_grad": [_i64, _i64, _i64, _d, _d, _d, _i32, _vp, _vp, _i32, _vp, _d, _d,
                             _d, _d, _vp, _vp, _vp],
     "skt_operator_set_extra": [_vp, _vp, _vp],

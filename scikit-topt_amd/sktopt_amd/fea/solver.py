@@ -248,7 +248,8 @@ class FEM_SimpLinearElasticity(_GpuFEM):
 
 class FEM_SimpLinearHeatConduction(_GpuFEM):
     """GPU counterpart of ``fea/solver_heat.py:628-980``: objectives
-    ``'compliance'`` and ``'averaged_temp'``, with or without Robin facets.
+    ``'compliance'`` and ``'averaged_temp'``, with or without Robin facets,
+    and ``'heat_exchange'`` (Robin facets required).
 
     With Robin data (``task.robin_coefficient`` set) every solve uses
     ``K = K_cond(rho) + B_facets + B_virtual(rho)`` and the load
@@ -256,13 +257,18 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
     :495-517); the Robin matrices are a 27-point stencil attached to the
     operator.  The volume integrals use ``npts`` Gauss points per axis
     (``npts = 2`` integrates the basis products exactly on the reference's
-    ``intorder`` 2 and 3)."""
+    ``intorder`` 2 and 3).  The heat-exchange functionals use ``npts_cell``
+    points per axis: the reference evaluates them on ``skfem.CellBasis`` with
+    its default rule, 4 Gauss points per axis for trilinear hexahedra."""
     physics = _lib.HEAT
 
     def __init__(self, task, E_min_coeff, density_interpolation=simp_interpolation,
-                 solver_config=None, solver_option="spsolve", q=4, quad_order=2, npts=2):
-        if getattr(task, "objective", "compliance") not in ("compliance", "averaged_temp"):
-            raise NotImplementedError("objective='heat_exchange' is not built on the GPU path")
+                 solver_config=None, solver_option="spsolve", q=4, quad_order=2, npts=2,
+                 npts_cell=4):
+        if getattr(task, "objective", "compliance") not in ("compliance", "averaged_temp",
+                                                             "heat_exchange"):
+            raise ValueError(f"Unknown objective: {task.objective}")
+        self.npts_cell = int(npts_cell)
         self.k_max = task.k * 1.0
         self.k_min = task.k * E_min_coeff
         self.E_max, self.E_min = self.k_max, self.k_min
@@ -345,12 +351,10 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
         self.solve(U)
         if self.objective == "averaged_temp":
             J = self.col_reduce(_lib.RED_SUM, U)
-            if self._lam_warm is None:
-                self._lam_warm = torch.zeros_like(U)
-            ones = torch.ones(self.ndof, dtype=torch.float64, device=self.device)
-            self._solve_rhs(self._enforced(ones), self._lam_warm)
-            self.lam = self._lam_warm
+            self.lam = self._adjoint_avg()
             return J
+        if self.objective == "heat_exchange":
+            return self._heat_exchange(rho, p, U)
         en = torch.empty((self.n_el, self.n_tasks), dtype=torch.float64, device=self.device)
         _lib.call("skt_operator_energy", self._op, _lib.F64, self.n_tasks, _dev.ptr(self.E64),
                   _dev.ptr(U), _dev.ptr(en), _dev.stream())
@@ -362,6 +366,63 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
             _lib.call("skt_stencil27_apply", nx, ny, nz, self.n_tasks, _dev.ptr(self._st_tot),
                       _dev.ptr(U), _dev.ptr(SU), _dev.stream())
             J = J + self.col_reduce(_lib.RED_DOT, U, SU)
+        return J
+
+    def _adjoint_avg(self):
+        """The averaged-temperature adjoint ``K lam = 1``, enforced with the
+        state's Dirichlet values (``solver_heat.py:869-882``)."""
+        if self._lam_warm is None:
+            self._lam_warm = torch.zeros((self.ndof, self.n_tasks), dtype=torch.float64,
+                                         device=self.device)
+        ones = torch.ones(self.ndof, dtype=torch.float64, device=self.device)
+        self._solve_rhs(self._enforced(ones), self._lam_warm)
+        return self._lam_warm
+
+    def _heat_exchange(self, rho, p, U):
+        """``objective='heat_exchange'`` (``solver_heat.py:791-869``):
+        ``J_k = -T_env h int (T_k - T_env)|grad rho_h| / int |grad rho_h|``
+        on the nodal densities of the reference's pairing, the adjoint
+        ``K lam = rhs / J_den`` enforced with the state's Dirichlet values,
+        and the ``avg_temp_weight`` blend with the averaged temperature."""
+        t = self.task
+        if not self.robin:
+            raise RuntimeError("heat_exchange objective requires Robin boundary data.")
+        nx, ny, nz = t.mesh.grid
+        S = _dev.stream()
+        k = self.n_tasks
+        dev = self.device
+        _lib.call("skt_hex8_nodal_average_pairing", nx, ny, nz, _dev.ptr(rho),
+                  _dev.ptr(self._rn), _dev.ptr(self._cnt), S)
+        den = torch.empty(self.n_el, dtype=torch.float64, device=dev)
+        num = torch.empty((self.n_el, k), dtype=torch.float64, device=dev)
+        rhs = torch.empty(self.ndof, dtype=torch.float64, device=dev)
+        h, T_env = float(t.robin_coefficient), float(t.robin_bc_value)
+        Uc = U.contiguous()
+        _lib.call("skt_hex8_heat_exchange", nx, ny, nz, *t.mesh.h, self.npts_cell,
+                  _dev.ptr(self._rn), k, _dev.ptr(Uc), float(p), float(self.q), h, T_env,
+                  _dev.ptr(den), _dev.ptr(num), _dev.ptr(rhs), S)
+        s = torch.empty(1, dtype=torch.float64, device=dev)
+        _lib.call("skt_reduce", _lib.RED_SUM, self.n_el, _dev.ptr(den), None, _dev.ptr(s), S)
+        J_den = float(s.item())
+        if J_den <= 1e-16:
+            self.lam = torch.zeros_like(U)
+            return torch.zeros(k, dtype=torch.float64, device=dev)
+        J = (-T_env * h) * self.col_reduce(_lib.RED_SUM, num) / J_den
+        lam = torch.zeros_like(U)
+        self._solve_rhs(self._enforced(rhs / J_den), lam)
+        w = t.avg_temp_weight
+        if w != 0.0:
+            avg = self.col_reduce(_lib.RED_SUM, U)
+            lam_avg = self._adjoint_avg()
+            if k > 1:
+                hs = max(float(J.abs().mean()), 1e-12)
+                avs = max(float(avg.abs().mean()), 1e-12)
+                J = J / hs + w * (avg / avs)
+                lam = lam / hs + (w / avs) * lam_avg
+            else:
+                J = J + w * avg
+                lam = lam + w * lam_avg
+        self.lam = lam
         return J
 
     def _cross_energy(self, A, B):
@@ -378,7 +439,7 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
         """``energy_multi_load`` (``solver_heat.py:897-925``): conduction
         energies for ``compliance``; ``int grad T . grad lam`` for
         ``averaged_temp``."""
-        if self.objective == "averaged_temp":
+        if self.objective in ("averaged_temp", "heat_exchange"):
             if self.lam is None:
                 raise RuntimeError("adjoint field λ_all is not computed.")
             return self._cross_energy(U, self.lam)

@@ -223,8 +223,8 @@ class LinearHeatConduction(FEMDomain):
     ``:165-169``), a nodal heat-source vector ``emit`` (the reference
     injects sources through ``robin_linear``, ``fea/solver_heat.py:230-235``)
     and optional Robin facets with coefficient ``robin_coefficient`` and
-    ambient value ``robin_bc_value``.  Objectives ``'compliance'`` and
-    ``'averaged_temp'``; ``'heat_exchange'`` is not built."""
+    ambient value ``robin_bc_value``.  Objectives ``'compliance'``,
+    ``'averaged_temp'`` and ``'heat_exchange'`` (with ``avg_temp_weight``)."""
 
     def __init__(self, mesh, dirichlet_nodes, dirichlet_values, emit, design_elements, k,
                  objective="compliance", robin_facets_ids=None, robin_coefficient=None,
@@ -232,8 +232,6 @@ class LinearHeatConduction(FEMDomain):
         if objective not in ("compliance", "heat_exchange", "averaged_temp"):
             raise ValueError("objective must be one of 'compliance', 'heat_exchange', or "
                              "'averaged_temp'")
-        if objective == "heat_exchange":
-            raise NotImplementedError("objective='heat_exchange' is not built on the GPU path")
         vals = dirichlet_values if isinstance(dirichlet_values, list) else [dirichlet_values]
         if robin_facets_ids is None:
             robin_coefficient = robin_bc_value = design_robin_boundary = None

@@ -21,15 +21,17 @@ def _s():
     return torch.cuda.current_stream().cuda_stream
 
 
-def _task(nx=10, ny=10, nz=4, objective="compliance", h=4e-2, design_robin=True):
+def _task(nx=10, ny=10, nz=4, objective="compliance", h=4e-2, design_robin=True, w=0.0,
+          values=600.0):
     from sktopt_amd.mesh import structured_hex, toy_problem as tp
     from sktopt_amd.mesh.task import LinearHeatConduction
     m = structured_hex(nx, ny, nz)
     d = m.facets_satisfying(tp.get_points_in_range((nx, nx), (0, 2), (0, nz)))
     r0 = m.facets_satisfying(tp.get_points_in_range((0, 0), (0, ny), (0, nz)))
     r1 = m.facets_satisfying(tp.get_points_in_range((0, nx), (ny, ny), (0, nz)))
-    return LinearHeatConduction.from_facets(m, d, 600.0, [r0, r1], h, 300.0, design_robin,
-                                            np.arange(m.nelements), 10.0, objective=objective)
+    return LinearHeatConduction.from_facets(m, d, values, [r0, r1], h, 300.0, design_robin,
+                                            np.arange(m.nelements), 10.0, objective=objective,
+                                            avg_temp_weight=w)
 
 
 def _oracle_fem(t, objective):
@@ -37,7 +39,7 @@ def _oracle_fem(t, objective):
     facets = [t.mesh.facets[:, np.asarray(f)] for f in t.robin_facets_ids]
     return om_, hr.HeatRobinFEM(om_, t.dirichlet_dofs, t.dirichlet_values, t.k, 1e-3, facets,
                                 t.robin_coefficient, t.robin_bc_value, emit=t.emit,
-                                objective=objective)
+                                objective=objective, avg_temp_weight=t.avg_temp_weight)
 
 
 def test_robin_kernels_match_oracle(gpu):
@@ -78,21 +80,39 @@ def test_robin_kernels_match_oracle(gpu):
         ref = hr.robin_explicit_grad(m, rho, T[:, c], 3.0, 4.0, 0.04, 300.0)
         np.testing.assert_allclose(out[:, c].cpu().numpy(), ref, rtol=1e-11,
                                    atol=1e-12 * np.abs(ref).max())
+    for npts in (2, 4):
+        den = torch.empty(m.nelements, dtype=torch.float64, device="cuda")
+        num = torch.empty((m.nelements, 2), dtype=torch.float64, device="cuda")
+        rhs = torch.empty(m.nnodes, dtype=torch.float64, device="cuda")
+        Td = _t(T)
+        _lib.call("skt_hex8_heat_exchange", nx, ny, nz, 1.0, 1.0, 1.0, npts, rn.data_ptr(), 2,
+                  Td.data_ptr(), 3.0, 4.0, 0.04, 300.0, den.data_ptr(), num.data_ptr(),
+                  rhs.data_ptr(), _s())
+        d_ref, n_ref, r_ref = hr.heat_exchange_terms(m, rn_ref, T, 3.0, 4.0, 0.04, 300.0, npts)
+        np.testing.assert_allclose(den.sum().item(), d_ref, rtol=1e-12)
+        np.testing.assert_allclose(-300.0 * 0.04 * num.sum(0).cpu().numpy(), n_ref, rtol=1e-11)
+        r = rhs.cpu().numpy()
+        np.testing.assert_allclose(r, r_ref, rtol=1e-12, atol=1e-13 * np.abs(r_ref).max())
 
 
-@pytest.mark.parametrize("objective,precision", [("compliance", "fp64"),
-                                                 ("averaged_temp", "fp64"),
-                                                 ("compliance", "mixed")])
-def test_robin_objectives_match_oracle(gpu, objective, precision):
+@pytest.mark.parametrize("objective,precision,w,values",
+                         [("compliance", "fp64", 0.0, 600.0),
+                          ("averaged_temp", "fp64", 0.0, 600.0),
+                          ("compliance", "mixed", 0.0, 600.0),
+                          ("heat_exchange", "fp64", 0.0, 600.0),
+                          ("heat_exchange", "fp64", 0.3, 600.0),
+                          ("heat_exchange", "fp64", 0.3, [600.0, 450.0])])
+def test_robin_objectives_match_oracle(gpu, objective, precision, w, values):
     from sktopt_amd import fea
-    t = _task(objective=objective)
+    t = _task(objective=objective, w=w, values=values)
     _, ofem = _oracle_fem(t, objective)
     cfg = fea.LinearSolverConfig(rtol=1e-12 if precision == "fp64" else 1e-10,
                                  precision=precision)
     fem = fea.FEM_SimpLinearHeatConduction(t, 1e-3, solver_config=cfg)
     rho = np.random.default_rng(5).uniform(0.1, 1.0, t.mesh.nelements)
-    U = np.zeros((t.ndof, 1))
-    Uo = np.zeros((t.ndof, 1))
+    k = len(t.dirichlet_values)
+    U = np.zeros((t.ndof, k))
+    Uo = np.zeros((t.ndof, k))
     J = fem.objectives_multi_load(rho, 3.0, U)
     Jo = ofem.objectives_multi_load(rho, 3.0, Uo)
     tol = 1e-9 if precision == "fp64" else 1e-7
@@ -107,7 +127,8 @@ def test_robin_objectives_match_oracle(gpu, objective, precision):
 
 
 @pytest.mark.parametrize("method,objective", [("OC", "compliance"), ("LogMOC", "compliance"),
-                                              ("OC", "averaged_temp")])
+                                              ("OC", "averaged_temp"),
+                                              ("OC", "heat_exchange")])
 def test_robin_heat_trajectory_matches_oracle(gpu, method, objective, tmp_path):
     from sktopt_amd import core
     from sktopt_amd.tools import SchedulerConfig as SC

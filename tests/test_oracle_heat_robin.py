@@ -107,3 +107,74 @@ def test_averaged_temp_adjoint():
     np.testing.assert_allclose(J[0], fem.lam[free, 0] @ task.forces[0][free], rtol=1e-10)
     e = fem.energy_multi_load(rho, 3.0, U)
     assert e.shape == (task.n_elements, 1)
+
+
+def test_heat_exchange_terms_closed_forms():
+    """Linear nodal fields pin the heat-exchange functionals: a trilinear
+    interpolant reproduces a linear field, so |grad rho_h| = |g| everywhere,
+    J_den = |g| V and J_num = -T_env h |g| (V T(centroid) - T_env V) for a
+    linear T; the adjoint load sums (partition of unity) to
+    -T_env |g| sum_e V_e mean_corners(h_eff), a trilinear field's element
+    mean being its corner mean.  All integrands are polynomial, so the 2-
+    and 4-point rules agree."""
+    m = om.structured_hex(4, 3, 2, 0.5, 0.7, 1.1)
+    X = m.p
+    g = np.array([0.05, -0.03, 0.02])
+    rn = 0.4 + g @ X
+    c = np.array([[3.0, 1.0, -2.0], [-1.0, 0.5, 4.0]])
+    T = np.column_stack([310.0 + c[0] @ X, 280.0 + c[1] @ X])
+    V = (4 * 0.5) * (3 * 0.7) * (2 * 1.1)
+    ctr = np.array([1.0, 1.05, 1.1])
+    gn = np.linalg.norm(g)
+    h, Te, p, q = 0.04, 300.0, 3.0, 4.0
+    heff = h * rn ** p * (1.0 - rn) ** q
+    rhs_sum = -Te * gn * (0.5 * 0.7 * 1.1) * heff[m.t].mean(axis=0).sum()
+    for npts in (2, 4):
+        den, num, rhs = hr.heat_exchange_terms(m, rn, T, p, q, h, Te, npts)
+        np.testing.assert_allclose(den, gn * V, rtol=1e-13)
+        Tc = np.array([310.0 + c[0] @ ctr, 280.0 + c[1] @ ctr])
+        np.testing.assert_allclose(num, -Te * h * gn * (V * Tc - Te * V), rtol=1e-12)
+        np.testing.assert_allclose(rhs.sum(), rhs_sum, rtol=1e-12)
+        assert np.all(rhs <= 0.0)
+
+
+def _hx_fem(task, objective, w=0.0, values=(0.0,)):
+    return hr.HeatRobinFEM(task.mesh, task.dirichlet_dofs, list(values), task.E0, 1e-3, [],
+                           0.04, 300.0, emit=task.forces[0], objective=objective,
+                           avg_temp_weight=w)
+
+
+def test_heat_exchange_blend_and_flat_density():
+    """``avg_temp_weight`` (solver_heat.py:845-869): one load case adds w sum(T)
+    to J and w times the averaged-temperature adjoint to lam; several load
+    cases first divide each part by its mean magnitude.  A density without
+    interface (J_den = 0) gives J = 0 and lam = 0."""
+    task = oprob.heat_patch(5, patch=3, source=1e-2)
+    rho = np.random.default_rng(4).uniform(0.1, 1.0, task.n_elements)
+    for values in ((0.0,), (0.0, 5.0)):
+        k = len(values)
+        res = {}
+        for name, obj, w in (("hx", "heat_exchange", 0.0), ("mix", "heat_exchange", 0.3),
+                             ("avg", "averaged_temp", 0.0)):
+            fem = _hx_fem(task, obj, w, values)
+            U = np.zeros((task.ndof, k))
+            res[name] = (fem.objectives_multi_load(rho, 3.0, U), fem.lam.copy(), U)
+        J0, l0, U = res["hx"]
+        Ja, la, _ = res["avg"]
+        Jw, lw, _ = res["mix"]
+        np.testing.assert_allclose(Ja, U.sum(axis=0), rtol=1e-12)
+        if k == 1:
+            np.testing.assert_allclose(Jw, J0 + 0.3 * Ja, rtol=1e-12)
+            np.testing.assert_allclose(lw, l0 + 0.3 * la, rtol=1e-10, atol=1e-12)
+        else:
+            hs, avs = np.abs(J0).mean(), np.abs(Ja).mean()
+            np.testing.assert_allclose(Jw, J0 / hs + 0.3 * Ja / avs, rtol=1e-12)
+            np.testing.assert_allclose(lw, l0 / hs + (0.3 / avs) * la, rtol=1e-10, atol=1e-12)
+        assert np.all(J0 != 0.0)
+    fem = _hx_fem(task, "heat_exchange", 0.3)
+    U = np.zeros((task.ndof, 1))
+    # a zero field: its gradient is zero to the last bit (a flat non-zero
+    # field leaves rounding-level gradients, J_den ~ 1e-15, as in the
+    # reference's arithmetic)
+    J = fem.objectives_multi_load(np.zeros(task.n_elements), 3.0, U)
+    assert J[0] == 0.0 and not fem.lam.any()
