@@ -118,7 +118,10 @@ int skt_ewise(int op, int64_t n, double a, const double* x, double b,
               const double* y, double* out, hipStream_t stream);
 
 /* ------------------------------------------------------------------------
- * Reductions (float64; results are written to device memory)
+ * Reductions (float64; results are written to device memory).  The
+ * reductions, statistics and percentile share one per-device partials
+ * buffer: calls on one stream are ordered, calls that overlap on different
+ * streams of one device race on it.
  * ---------------------------------------------------------------------- */
 #define SKT_RED_SUM 0        /* sum x                                     */
 #define SKT_RED_DOT 1        /* sum x*y                                   */
