@@ -24,6 +24,7 @@
 // not restarted.  Convergence is only ever declared on a true float64
 // residual.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -66,6 +67,17 @@ constexpr int kSc = 16;
 static_assert(kSc == kStopStride, "the column stop flags are the solver's scalar blocks");
 constexpr int kCheckJacobi = 32;
 constexpr int kCheckMG = 4;
+
+// Iterations between convergence polls with multigrid; SKT_SOLVER_POLL=n
+// (1..32) overrides the default.
+static int mg_poll_interval() {
+  const char* v = std::getenv("SKT_SOLVER_POLL");
+  if (v) {
+    const int n = std::atoi(v);
+    if (n >= 1 && n <= 32) return n;
+  }
+  return kCheckMG;
+}
 // scalar slots
 enum {
   S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_CONV, S_TOL2, S_BAD,
@@ -632,7 +644,7 @@ static int solve_t(skt_solver* s, int mode, const Blk& b, const double* E64, con
   const int64_t total = b.n * b.k;
   const unsigned ge = grid_for(total);
   double* area1 = s->part + (size_t)2 * kParts * s->max_cols;
-  const int check = mg ? kCheckMG : kCheckJacobi;
+  const int check = mg ? mg_poll_interval() : kCheckJacobi;
   // true residual of the initial guess (float64 operator)
   int stt = op_apply(s->op, SKT_F64, b.k, b.kc, E64, s->x64, s->q64, st);
   if (stt) return stt;
