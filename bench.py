@@ -13,8 +13,13 @@ and the compliance error.
 
 A step is one OC iteration (filter, projection, 8-load solve, energies,
 sensitivities, filter adjoint, device-side lambda bisection).  W warm-up
-iterations run untimed, then K iterations are timed between device
-synchronisations.  Prints one JSON line.
+iterations run untimed, then K iterations are timed one by one, each
+between device synchronisations; ``value`` = 1 / median per-iteration time
+(``ms_per_step`` is that median; the mean is reported beside it).  The
+timed iterations carry no per-kernel instrumentation: the operator-apply
+time behind ``roofline`` comes from two further, untimed iterations with
+HIP events around every fp32 apply on the solve stream.  Prints one JSON
+line.
 """
 from __future__ import annotations
 
@@ -123,42 +128,97 @@ def jacobi_iterations(opt):
     return its
 
 
-def timed_run(opt, warmup, steps, timing=False):
+def timed_run(opt, warmup, steps):
+    """W untimed iterations, then K iterations each bracketed by device
+    synchronisations; returns the per-iteration wall times (s)."""
     import torch
-    from sktopt_amd import _lib
-    import ctypes as C
     if warmup:
         opt.optimize_steps(warmup)
     torch.cuda.synchronize()
-    if timing:
-        _lib.call("skt_solver_set_timing", opt.fem._solver, 1)
-        ms = (C.c_double * 2)()
-        n = (C.c_int64 * 2)()
-        _lib.call("skt_solver_get_timing", opt.fem._solver, ms, n, 1)
-    t0 = time.perf_counter()
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        opt.optimize_steps(1)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    return times
+
+
+def apply_timing_pass(opt, steps=2):
+    """Untimed OC iterations with HIP events around every operator apply
+    of the solve (on the solve stream): average fp32 / fp64 apply times."""
+    import ctypes as C
+    import torch
+    from sktopt_amd import _lib
+    _lib.call("skt_solver_set_timing", opt.fem._solver, 1)
+    ms = (C.c_double * 2)()
+    n = (C.c_int64 * 2)()
+    _lib.call("skt_solver_get_timing", opt.fem._solver, ms, n, 1)     # reset
     opt.optimize_steps(steps)
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    apply = None
-    if timing:
-        ms = (C.c_double * 2)()
-        n = (C.c_int64 * 2)()
-        _lib.call("skt_solver_get_timing", opt.fem._solver, ms, n, 1)
-        _lib.call("skt_solver_set_timing", opt.fem._solver, 0)
-        apply = dict(f64_ms=ms[0], f64_n=n[0], f32_ms=ms[1], f32_n=n[1])
-    return t1 - t0, apply
+    _lib.call("skt_solver_get_timing", opt.fem._solver, ms, n, 1)
+    _lib.call("skt_solver_set_timing", opt.fem._solver, 0)
+    return dict(f64_ms=ms[0], f64_n=n[0], f32_ms=ms[1], f32_n=n[1])
+
+
+def median(xs):
+    ys = sorted(xs)
+    m = len(ys) // 2
+    return ys[m] if len(ys) % 2 else 0.5 * (ys[m - 1] + ys[m])
+
+
+KERNEL_TRACE_FILE = os.path.join(ROOT, "profiles",
+                                 "oc_loop_hex8_120x60x60_8load_mixed_mg_kernel_trace.txt")
+# Algorithmic bytes per launch of the fine-level modal variants at c2 (fp32,
+# 8 columns, n = ndof, e = elements): every variant reads x (and writes y)
+# once per column; mode 1 also reads b; mode 2 reads b, the previous iterate
+# and D^-1 (one value per dof, shared by the columns); mode 3 reads b and
+# D^-1 and writes the second iterate.
+MODAL_BYTES = {
+    "skt::k_hex_apply_modal<float, 3, 2, 0, 0, true>": lambda n, e, k: k * n * 8 + e * 4,
+    "skt::k_hex_apply_modal<float, 3, 2, 1, 0, false>": lambda n, e, k: k * n * 12 + e * 4,
+    "skt::k_hex_apply_modal<float, 3, 2, 2, 0, true>": lambda n, e, k: k * n * 16 + n * 4 + e * 4,
+    "skt::k_hex_apply_modal<float, 3, 2, 2, 0, false>": lambda n, e, k: k * n * 16 + n * 4 + e * 4,
+    "skt::k_hex_apply_modal<float, 3, 2, 3, 0, false>": lambda n, e, k: k * n * 8 + n * 4 + e * 4,
+}
+
+
+def modal_variants_from_profile(ndof, n_el, k):
+    """Average launch time of the fine-level modal variants from the
+    committed kernel-trace summary (rocprofv3 --kernel-trace --stats of this
+    benchmark), with their algorithmic bytes and HBM fraction."""
+    out = []
+    try:
+        with open(KERNEL_TRACE_FILE) as f:
+            lines = f.readlines()
+    except OSError:
+        return out
+    for line in lines:
+        for name, fn in MODAL_BYTES.items():
+            if line.startswith(name + " "):
+                parts = line[len(name):].split()
+                calls, avg_us = int(parts[0]), float(parts[1])
+                b = fn(ndof, n_el, k)
+                gbs = b / (avg_us * 1e-6) / 1e9
+                out.append(dict(kernel=name, calls=calls, avg_us=avg_us, bytes=b,
+                                achieved_gbs=gbs, frac=gbs / HBM_PEAK_GBS,
+                                per_iter_ms=float(parts[4])))
+    out.sort(key=lambda d: -d["per_iter_ms"])
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="hex8-120x60x60-8load", choices=sorted(WORKLOADS))
     ap.add_argument("--fp64-steps", type=int, default=None,
                     help="timed float64 iterations for the ratio (default: --steps)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-c0", action="store_true",
+                    help="skip the fully timed CPU iteration of the 40x20x20 case (~100 s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -175,21 +235,29 @@ def main():
     K, W = args.steps, args.warmup
     copy = hbm_copy()
     log(f"hbm copy: {copy['gbs']:.0f} GB/s ({copy['us']:.1f} us for {copy['bytes']} B)")
-    iters = W + K + 2
+    iters = W + K + 4
     t_setup = time.perf_counter()
     opt = make_optimizer(wl, "mixed", 1e-8, iters, "/tmp/skt_bench_mixed")
     log(f"setup {time.perf_counter() - t_setup:.1f}s  ndof={opt.fem.ndof} "
         f"n_el={opt.fem.n_el} loads={opt.fem.n_tasks}")
-    dt, apply = timed_run(opt, W, K, timing=True)
-    value = K / dt
+    times = timed_run(opt, W, K)
+    t_med = median(times)
+    t_mean = sum(times) / len(times)
+    value = 1.0 / t_med
     comp_mixed = opt.recorder.as_object().compliance
     solves = opt.fem.solve_history[-K:]
-    log(f"mixed: {value:.3f} OC it/s  ms/step={1e3 * dt / K:.1f}  solves={solves}")
-    # roofline of the dominant kernel: the float32 operator apply
+    evals = opt.recorder.as_object().bisection_evals[-K:]
+    log(f"mixed: {value:.3f} OC it/s (median {1e3 * t_med:.2f} ms, mean {1e3 * t_mean:.2f} ms)"
+        f"  solves={[s['iters'] for s in solves]}")
+    # roofline of the dominant kernel, the float32 operator apply, from an
+    # untimed pass with HIP events around each apply
+    apply = apply_timing_pass(opt, 2)
     ndof, n_el, k = opt.fem.ndof, opt.fem.n_el, opt.fem.n_tasks
     bytes_apply = k * ndof * (4 + 4) + n_el * 4
     avg_ms = apply["f32_ms"] / max(apply["f32_n"], 1)
     achieved = bytes_apply / (avg_ms * 1e-3) / 1e9
+    log(f"fp32 apply: {1e3 * avg_ms:.2f} us avg over {apply['f32_n']} launches, "
+        f"{achieved:.0f} GB/s")
     jac_its = None if args.no_cpu else jacobi_iterations(opt)
     fp64 = None
     err = None
@@ -197,34 +265,53 @@ def main():
         del opt
         torch.cuda.empty_cache()
         K64 = args.fp64_steps or K
-        opt64 = make_optimizer(wl, "fp64", 1e-8, W + K64 + 2, "/tmp/skt_bench_fp64")
-        dt64, apply64 = timed_run(opt64, W, K64, timing=True)
+        opt64 = make_optimizer(wl, "fp64", 1e-8, W + K64 + 4, "/tmp/skt_bench_fp64")
+        times64 = timed_run(opt64, W, K64)
         comp64 = opt64.recorder.as_object().compliance
         n = min(len(comp64), len(comp_mixed))
         err = max(abs(a - b) / abs(b) for a, b in zip(comp_mixed[:n], comp64[:n]))
-        fp64 = dict(value=K64 / dt64, ms_per_step=1e3 * dt64 / K64,
-                    apply_avg_us=1e3 * apply64["f64_ms"] / max(apply64["f64_n"], 1))
+        apply64 = apply_timing_pass(opt64, 2)
+        t64 = median(times64)
+        fp64 = dict(value=1.0 / t64, ms_per_step=1e3 * t64,
+                    ms_per_step_mean=1e3 * sum(times64) / len(times64),
+                    apply_avg_us=1e3 * apply64["f64_ms"] / max(apply64["f64_n"], 1),
+                    mg_pcg_iterations=[s["iters"] for s in opt64.fem.solve_history[-K64 - 2:-2]])
         log(f"fp64: {fp64}  max compliance rel err mixed vs fp64 = {err:.2e}")
+        del opt64
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
         from oracle import cpu_baseline
-        cpu = cpu_baseline.measure(wl, [{"iters": jac_its}])
+        n_bis = sum(evals) / len(evals)
+        cpu = cpu_baseline.measure(wl, jac_its, n_bis)
+        log(f"cpu (extrapolated c2): {cpu['value']:.4f} OC it/s")
+        if not args.no_cpu_c0:
+            cpu["c0_full_iteration"] = cpu_baseline.measure_c0_iteration(log)
+            log(f"cpu c0 iteration: {cpu['c0_full_iteration']['seconds']:.1f} s")
+    variants = modal_variants_from_profile(ndof, n_el, k)
     line = {
         "metric": "OC iters/s, 1.36M-DOF hex8, GPU FP32-mixed vs FP64 at compliance err≤1e-5",
         "value": value, "unit": "OC iterations/s", "n_gpus": 1, "steps": K, "warmup": W,
-        "ms_per_step": 1e3 * dt / K, "higher_is_better": True, "scaling": "weak",
+        "ms_per_step": 1e3 * t_med, "ms_per_step_mean": 1e3 * t_mean,
+        "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32-operator/f64-reductions",
         "data": "synthetic (seeded random point loads)",
         "config": {"workload": args.workload, "ndof": ndof, "n_elements": n_el,
                    "load_cases": k,
                    "solver": ("batched CG, geometric-multigrid preconditioner (Galerkin, "
                               "Chebyshev-Jacobi smoothing), fp32 with fp64 reliable updates"),
-                   "rtol": 1e-8, "mg_pcg_iterations": [s["iters"] for s in solves]},
+                   "rtol": 1e-8, "mg_pcg_iterations": [s["iters"] for s in solves],
+                   "bisection_evals": evals, "timing": "median of per-iteration wall times"},
         "roofline": {"bound": "hbm", "kernel": APPLY_KERNEL,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
                      "bytes_per_launch": bytes_apply, "avg_launch_us": 1e3 * avg_ms,
-                     "launches": apply["f32_n"], "hbm_copy_gbs": copy["gbs"]},
+                     "launches": apply["f32_n"], "hbm_copy_gbs": copy["gbs"],
+                     "measured": "HIP events per launch on the solve stream, 2 untimed OC "
+                                 "iterations after the timed ones"},
+        "operator_apply_dof_per_s": k * ndof / (avg_ms * 1e-3),
+        "modal_variants": {"source": os.path.relpath(KERNEL_TRACE_FILE, ROOT),
+                           "kernels": variants},
         "fp64": fp64, "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
         "compliance_rel_err_vs_fp64": err,
         "cpu_baseline": cpu,

@@ -119,9 +119,10 @@ int skt_ewise(int op, int64_t n, double a, const double* x, double b,
 
 /* ------------------------------------------------------------------------
  * Reductions (float64; results are written to device memory).  The
- * reductions, statistics and percentile share one per-device partials
- * buffer: calls on one stream are ordered, calls that overlap on different
- * streams of one device race on it.
+ * reductions, statistics and percentile keep their partials in a buffer of
+ * the calling (device, stream) pair, allocated on the stream's first use:
+ * calls on one stream are ordered, and calls on different streams use
+ * different buffers.
  * ---------------------------------------------------------------------- */
 #define SKT_RED_SUM 0        /* sum x                                     */
 #define SKT_RED_DOT 1        /* sum x*y                                   */
@@ -346,6 +347,14 @@ int skt_solver_info(const skt_solver* s, int* precond_used, int* levels);
 int skt_solver_set_timing(skt_solver* s, int enable);
 int skt_solver_get_timing(skt_solver* s, double* apply_ms, int64_t* n_apply,
                           int reset);
+
+/* The PCG iterations between two convergence polls, and the multigrid
+ * set-up once it is warm, run as captured hipGraphs replayed on the solve
+ * stream (no reference counterpart: the reference's solves are host library
+ * calls).  graph_stats returns the replays and captures so far.  SKT_GRAPH=0 in the environment when the
+ * solver is created makes it launch the same kernels one by one instead;
+ * timing (above) also does. */
+int skt_solver_graph_stats(const skt_solver* s, int64_t* launches, int64_t* captures);
 
 /* ------------------------------------------------------------------------
  * Gaussian density filter on element centres (filters/spacial.py)

@@ -49,6 +49,17 @@ def bisection_with_physical_volume(dC, rho_e, rho_full, design_elements,
     return lmid, vol_error, cand, n_evals
 
 
+def kkt_residual(dC_raw, rho_new, lmid, running_scale, dV, rho_min, rho_max):
+    """``oc.py:230-240``: max |dC_raw + lmid * running_scale * dV| over the
+    design elements whose updated density is strictly inside
+    (rho_min + 1e-6, rho_max - 1e-6); 0 when there is none."""
+    mask = (rho_new > rho_min + 1e-6) & (rho_new < rho_max - 1e-6)
+    if not np.any(mask):
+        return 0.0
+    dL = dC_raw[mask] + (lmid * running_scale) * dV[mask]
+    return float(np.linalg.norm(dL, ord=np.inf))
+
+
 class OCUpdate:
     """State of ``OC_Optimizer.rho_update`` (``oc.py:149-250``)."""
 
@@ -79,12 +90,8 @@ class OCUpdate:
             vol_design, vol_design_sum, cfg.scaling_rate_min,
             cfg.scaling_rate_max, max_iter=1000, tolerance=1e-5,
             l1=cfg.lambda_lower, l2=cfg.lambda_upper)
-        mask = (cand > cfg.rho_min + 1e-6) & (cand < cfg.rho_max - 1e-6)
-        if np.any(mask):
-            dL = dC_raw[mask] + (lmid * self.running_scale) * dV_drho_design[mask]
-            self.kkt_residual = float(np.linalg.norm(dL, ord=np.inf))
-        else:
-            self.kkt_residual = 0.0
+        self.kkt_residual = kkt_residual(dC_raw, cand, lmid, self.running_scale,
+                                         dV_drho_design, cfg.rho_min, cfg.rho_max)
         self.last = dict(lmid=lmid, vol_error=vol_error, n_evals=n_evals,
                          running_scale=self.running_scale)
         rho_design[:] = cand

@@ -41,6 +41,14 @@ const char* get_error();
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
+// Largest grid of the reductions, statistics and percentile (partial slots).
+constexpr int kMaxBlocks = 1024;
+
+// np.percentile(|x|, q) ('linear') by radix select, on caller-chosen scratch:
+// part >= 2 * min(ceil(n / kBlock), kMaxBlocks) doubles, hist 256 words,
+// sel 16 words (design.hip).
+int launch_percentile(int64_t n, const double* x, double q, double* out, double* part,
+                      uint32_t* hist, uint64_t* sel, hipStream_t stream);
 
 inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
   int64_t g = (n + block - 1) / block;
@@ -57,12 +65,13 @@ __device__ inline T wave_sum(T v) {
   return v;
 }
 
+// Wave-level maximum; a NaN on any lane propagates (as numpy's max).
 template <typename T>
 __device__ inline T wave_max(T v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
     T o = __shfl_xor(v, off, kWave);
-    v = v > o ? v : o;
+    v = (v > o || v != v) ? v : o;
   }
   return v;
 }
@@ -83,6 +92,7 @@ __device__ inline T block_sum(T v, T* sh) {
   return s;  // valid in thread 0 only
 }
 
+// Deterministic block maximum (NaN propagates).
 template <typename T>
 __device__ inline T block_max(T v, T* sh) {
   v = wave_max(v);
@@ -93,7 +103,8 @@ __device__ inline T block_max(T v, T* sh) {
   __syncthreads();
   T s = sh[0];
   if (threadIdx.x == 0) {
-    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) s = s > sh[i] ? s : sh[i];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i)
+      s = (s > sh[i] || s != s) ? s : sh[i];
   }
   return s;  // valid in thread 0 only
 }

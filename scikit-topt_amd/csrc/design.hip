@@ -1,7 +1,9 @@
 // Design-variable kernels (float64): projection, interpolation, compliance
 // sensitivity, LogMOC update, reductions and the exact percentile.
 #include <cmath>
+#include <map>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -204,22 +206,25 @@ __global__ void k_ewise(int op, int64_t n, double a, const double* __restrict__ 
 
 // ---------------------------------------------------------------- reductions
 
-// Per-device scratch for reduction partials and selection state.
+// Scratch for reduction partials and selection state, one set per
+// (device, stream).  Calls on one stream run in stream order, so they may
+// share a set; calls on different streams get different sets and cannot
+// overwrite each other's partials.  A set is allocated the first time its
+// stream is seen and kept until the process ends (a destroyed stream's
+// handle may be reused by a new stream, which then inherits the set).
 struct Scratch {
   double* part = nullptr;      // kMaxBlocks * 64 doubles
   uint32_t* hist = nullptr;    // 256 bins
   uint64_t* sel = nullptr;     // selection state
 };
-constexpr int kMaxBlocks = 1024;
 static std::mutex g_scratch_mu;
-static std::vector<Scratch> g_scratch;
+static std::map<std::pair<int, uintptr_t>, Scratch> g_scratch;
 
-int get_scratch(Scratch** out) {
+int get_scratch(hipStream_t stream, Scratch** out) {
   int dev = 0;
   SKT_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_scratch_mu);
-  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
-  Scratch& s = g_scratch[dev];
+  Scratch& s = g_scratch[std::make_pair(dev, (uintptr_t)stream)];
   if (!s.part) {
     SKT_HIP(hipMalloc(&s.part, sizeof(double) * kMaxBlocks * 64));
     SKT_HIP(hipMalloc(&s.hist, sizeof(uint32_t) * 256));
@@ -329,8 +334,8 @@ __global__ void k_stats_part(int64_t n, int64_t chunk, const double* __restrict_
   for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
     const double v = x[i];
     s += v;
-    hi = v > hi ? v : hi;
-    nlo = -v > nlo ? -v : nlo;
+    hi = (v > hi || v != v) ? v : hi;
+    nlo = (-v > nlo || v != v) ? -v : nlo;
   }
   const double sum = block_sum(s, sh);
   if (threadIdx.x == 0) shift_sh = sum / (double)(b - a);
@@ -366,8 +371,8 @@ __global__ void k_stats_final(int nb, int64_t n, const double* __restrict__ part
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
     const double* p = part + kStatsPart * i;
     s += p[1];
-    hi = p[6] > hi ? p[6] : hi;
-    nlo = -p[5] > nlo ? -p[5] : nlo;
+    hi = (p[6] > hi || p[6] != p[6]) ? p[6] : hi;
+    nlo = (-p[5] > nlo || p[5] != p[5]) ? -p[5] : nlo;
   }
   const double sum = block_sum(s, sh);
   if (threadIdx.x == 0) mean_sh = sum / (double)n;
@@ -386,7 +391,7 @@ __global__ void k_stats_final(int nb, int64_t n, const double* __restrict__ part
     out[0] = -nlo;
     out[1] = m;
     out[2] = hi;
-    out[3] = sqrt(fmax(M2, 0.0) / (double)n);
+    out[3] = sqrt((M2 < 0.0 ? 0.0 : M2) / (double)n);  // NaN stays NaN
   }
 }
 
@@ -529,6 +534,33 @@ __global__ void k_oc_kkt(int64_t n, const double* __restrict__ dC,
   }
   double m = block_max(v, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// Percentile of |x| (numpy 'linear') by the radix select above, on caller-
+// chosen scratch: part >= 2 * grid partials, hist 256 bins, sel 16 words.
+// Every scratch word is written by a kernel before it is read (k_sel_init
+// clears the histogram), so the scratch needs no clearing by the host.
+int launch_percentile(int64_t n, const double* x, double q, double* out, double* part,
+                      uint32_t* hist, uint64_t* sel, hipStream_t stream) {
+  // numpy 'linear': virtual index (n-1)*q/100, floor, gamma, bounds.
+  const double qq = q / 100.0;
+  const double vidx = (double)(n - 1) * qq;
+  int take_max = vidx >= (double)(n - 1);
+  double prev = std::floor(vidx);
+  double gamma = vidx - prev;
+  int64_t k = take_max ? (n - 1) : (int64_t)prev;
+  if (vidx < 0) { k = 0; gamma = 0.0; }
+  hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(256), 0, stream, sel, hist, k);
+  unsigned nb = grid_for(n, kBlock, kMaxBlocks);
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(k_sel_hist, dim3(nb), dim3(kBlock), 0, stream, n, x, shift, sel, hist);
+    hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(kBlock), 0, stream, shift, sel, hist);
+  }
+  hipLaunchKernelGGL(k_sel_next, dim3(nb), dim3(kBlock), 0, stream, n, x, sel, part);
+  hipLaunchKernelGGL(k_sel_final, dim3(1), dim3(64), 0, stream, (int)nb, k, gamma, take_max,
+                     sel, part, out);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
 }
 
 }  // namespace skt
@@ -712,7 +744,7 @@ int skt_reduce(int op, int64_t n, const double* x, const double* y,
   SKT_CHECK(n >= 0 && x && out, "bad argument");
   SKT_CHECK(!(op == SKT_RED_DOT || op == SKT_RED_MAXABSDIFF) || y, "y required");
   Scratch* s;
-  int st = get_scratch(&s);
+  int st = get_scratch(stream, &s);
   if (st) return st;
   unsigned nb = grid_for(n, kBlock, kMaxBlocks);
   hipLaunchKernelGGL(k_reduce, dim3(nb), dim3(kBlock), 0, stream, op, n, x, y,
@@ -730,7 +762,7 @@ int skt_reduce_cols(int op, int64_t n, int ncols, const double* x,
   SKT_CHECK(ncols >= 1 && ncols <= 64 && n >= 0 && x && out, "bad argument");
   SKT_CHECK(op != SKT_RED_DOT || y, "y required");
   Scratch* s;
-  int st = get_scratch(&s);
+  int st = get_scratch(stream, &s);
   if (st) return st;
   int cp = 1;
   while (cp < ncols) cp <<= 1;
@@ -747,7 +779,7 @@ int skt_reduce_cols(int op, int64_t n, int ncols, const double* x,
 int skt_stats4(int64_t n, const double* x, double* out, hipStream_t stream) {
   SKT_CHECK(n >= 1 && x && out, "statistics of an empty array");
   Scratch* s;
-  int st = get_scratch(&s);
+  int st = get_scratch(stream, &s);
   if (st) return st;
   // at most kMaxBlocks chunks of kStatsPart partials fit the scratch
   int64_t chunk = (n + kMaxBlocks - 1) / kMaxBlocks;
@@ -767,30 +799,9 @@ int skt_percentile_abs(int64_t n, const double* x, double q, double* out,
   SKT_CHECK(n >= 1 && x && out, "percentile of an empty array");
   SKT_CHECK(q >= 0.0 && q <= 100.0, "Percentiles must be in the range [0, 100]");
   Scratch* s;
-  int st = get_scratch(&s);
+  int st = get_scratch(stream, &s);
   if (st) return st;
-  // numpy 'linear': virtual index (n-1)*q/100, floor, gamma, bounds.
-  const double qq = q / 100.0;
-  const double vidx = (double)(n - 1) * qq;
-  int take_max = vidx >= (double)(n - 1);
-  double prev = std::floor(vidx);
-  double gamma = vidx - prev;
-  int64_t k = take_max ? (n - 1) : (int64_t)prev;
-  if (vidx < 0) { k = 0; gamma = 0.0; }
-  hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(256), 0, stream, s->sel, s->hist, k);
-  unsigned nb = grid_for(n, kBlock, kMaxBlocks);
-  for (int shift = 56; shift >= 0; shift -= 8) {
-    hipLaunchKernelGGL(k_sel_hist, dim3(nb), dim3(kBlock), 0, stream, n, x,
-                       shift, s->sel, s->hist);
-    hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(kBlock), 0, stream, shift,
-                       s->sel, s->hist);
-  }
-  hipLaunchKernelGGL(k_sel_next, dim3(nb), dim3(kBlock), 0, stream, n, x,
-                     s->sel, s->part);
-  hipLaunchKernelGGL(k_sel_final, dim3(1), dim3(64), 0, stream, (int)nb, k,
-                     gamma, take_max, s->sel, s->part, out);
-  SKT_LAUNCH_CHECK();
-  return SKT_OK;
+  return launch_percentile(n, x, q, out, s->part, s->hist, s->sel, stream);
 }
 
 int skt_ewise(int op, int64_t n, double a, const double* x, double b, const double* y,
@@ -830,7 +841,7 @@ int skt_oc_kkt(int64_t n, const double* dC_raw, const double* rho, const double*
   SKT_CHECK(n >= 1 && dC_raw && rho && dV && bisect_result && running_scale && out,
             "bad argument");
   Scratch* s;
-  int st = get_scratch(&s);
+  int st = get_scratch(stream, &s);
   if (st) return st;
   unsigned nb = grid_for(n, kBlock, kMaxBlocks);
   hipLaunchKernelGGL(k_oc_kkt, dim3(nb), dim3(kBlock), 0, stream, n, dC_raw, rho, dV,

@@ -1238,6 +1238,8 @@ static int warm_power_steps(const MGData* mg) {
   return mg->power_iters_warm;
 }
 
+int mg_warm_steps(const MGData* mg) { return warm_power_steps(mg); }
+
 int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype, int max_cols) {
   if (mg->dtype == dtype && mg->max_cols == max_cols) return SKT_OK;
   SKT_CHECK((op->nx + 1) * (op->ny + 1) < INT32_MAX &&
@@ -1298,6 +1300,7 @@ int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype,
   SKT_HIP(hipMemcpy(mg->K0b, K0b.data(), sizeof(double) * K0b.size(), hipMemcpyHostToDevice));
   mg->lv = lv;
   mg->dtype = dtype;
+  ++mg->gen;
   mg->max_cols = max_cols;
   mg->degree = std::max(cfg.degree, 1);
   mg->ratio = cfg.ratio;
@@ -1516,10 +1519,12 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   // smoother bounds: power iteration on D^-1 A (one column) per level
   double* inv = mg->lam + (L + 1);  // scratch scalar slot
   int64_t off = 0;
+  bool st_joined = false;
   for (int l = 0; l < L; ++l) {
     if (l == 1) {
       if (int stt = queue_coarsest()) return stt;
       SKT_HIP(hipStreamWaitEvent(s, mg->ev_st, 0));
+      st_joined = true;
     }
     const MGLevel& v = mg->lv[l];
     const int64_t n = v.ndof;
@@ -1565,6 +1570,9 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   hipLaunchKernelGGL(k_cheb_coef, dim3(1), dim3(64), 0, s, L, mg->degree, mg->ratio, mg->lam,
                      mg->coef);
   SKT_LAUNCH_CHECK();
+  // every side-stream branch joins the main stream before the set-up
+  // returns (a graph capture of the set-up requires it)
+  if (!st_joined) SKT_HIP(hipStreamWaitEvent(s, mg->ev_st, 0));
   SKT_HIP(hipStreamWaitEvent(s, mg->ev_join, 0));
   return SKT_OK;
 }

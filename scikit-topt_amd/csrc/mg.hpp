@@ -65,6 +65,7 @@ struct MGData {
   double ratio = 10.0;
   int power_iters = 20;               // steps from the hashed start vector
   int power_iters_warm = 5;           // steps from the previous set-up's vector
+  uint64_t gen = 0;                   // bumped whenever the buffers are (re)allocated
 };
 
 struct MGConfig {
@@ -84,6 +85,8 @@ void mg_free(MGData* mg);
 
 // Numeric setup for the moduli E (fp64) -- Galerkin hierarchy, stencils,
 // coarsest inverse, smoother bounds.  dinv0 is the fine Jacobi inverse in T.
+// Power-iteration steps of a warm set-up (SKT_MG_POWER_WARM overrides).
+int mg_warm_steps(const MGData* mg);
 int mg_setup(MGData* mg, const skt_operator* op, const double* E64, const void* E, const void* dinv0,
              hipStream_t s);
 

@@ -58,6 +58,37 @@ struct skt_solver {
   bool timed = false;  // the last chunk's applies were timed
   double apply_ms[2] = {0, 0};
   int64_t n_apply[2] = {0, 0};
+  // The PCG iterations between two convergence polls, captured once as a
+  // hipGraph per variant (0: plain, 1: reliable update in the first
+  // iteration) on a private stream and replayed on the caller's stream.
+  // A graph is re-captured when anything baked into its kernel arguments
+  // changes: precision, columns, moduli buffers, chunk length, the
+  // preconditioner or its buffers.  SKT_GRAPH=0 launches the kernels one by
+  // one instead (same kernels, same order).
+  struct Chunk {
+    hipGraphExec_t exec = nullptr;
+    int dtype = -1, k = 0, kc = 0, used = -1, degree = 0;
+    int64_t count = 0;
+    const void* E = nullptr;
+    const double* E64 = nullptr;
+    uint64_t mg_gen = 0;
+  };
+  Chunk chunk[2];
+  // The multigrid set-up of a solve (Galerkin products, stencils, coarsest
+  // inverse, warm-started power iterations), captured once it is in its
+  // steady state (warm start, one-off buffers built) and replayed per solve.
+  struct Setup {
+    hipGraphExec_t exec = nullptr;
+    int dtype = -1, degree = 0, steps = 0;
+    const double* E64 = nullptr;
+    const void* E = nullptr;
+    const void* dinv = nullptr;
+    uint64_t mg_gen = 0;
+  };
+  Setup setup;
+  hipStream_t cap = nullptr;
+  int graphs = 1;
+  int64_t graph_launches = 0, graph_captures = 0;
 };
 
 namespace skt {
@@ -596,6 +627,49 @@ static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t
   return SKT_OK;
 }
 
+// Enqueue `count` PCG iterations on `st`: replay of the chunk's captured
+// graph (captured first if missing or stale), or direct launches when
+// graphs are off or the applies are being timed.
+template <typename T>
+static int run_chunk(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t count,
+                     bool replace_first, const double* E64, hipStream_t st) {
+  if (!s->graphs || s->timing)
+    return pcg_iterations<T>(s, b, v, count, replace_first, E64, st);
+  skt_solver::Chunk& g = s->chunk[replace_first ? 1 : 0];
+  const bool fresh = g.exec && g.dtype == v.dtype && g.k == b.k && g.kc == b.kc &&
+                     g.used == s->used && g.degree == s->mg.degree && g.count == count &&
+                     g.E == v.E && g.E64 == E64 && g.mg_gen == s->mg.gen;
+  if (!fresh) {
+    if (g.exec) {
+      // a replay of the old graph may still be queued on the stream
+      SKT_HIP(hipStreamSynchronize(st));
+      SKT_HIP(hipGraphExecDestroy(g.exec));
+      g.exec = nullptr;
+    }
+    if (!s->cap) SKT_HIP(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
+    SKT_HIP(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
+    s->timed = false;
+    int stt = pcg_iterations<T>(s, b, v, count, replace_first, E64, s->cap);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s->cap, &graph);
+    if (stt) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return stt;
+    }
+    SKT_HIP(e);
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SKT_HIP(e);
+    g.dtype = v.dtype; g.k = b.k; g.kc = b.kc; g.used = s->used; g.degree = s->mg.degree;
+    g.count = count; g.E = v.E; g.E64 = E64; g.mg_gen = s->mg.gen;
+    ++s->graph_captures;
+  }
+  s->timed = false;
+  SKT_HIP(hipGraphLaunch(g.exec, st));
+  ++s->graph_launches;
+  return SKT_OK;
+}
+
 // Fold the apply timings of the last chunk into the totals (events of the
 // chunk have completed once the poll has synchronised the stream).
 static int collect_timing(skt_solver* s, int dtype, int64_t count) {
@@ -606,6 +680,47 @@ static int collect_timing(skt_solver* s, int dtype, int64_t count) {
     s->apply_ms[dtype] += ms;
     s->n_apply[dtype] += 1;
   }
+  return SKT_OK;
+}
+
+// The multigrid set-up on `st`: replay of its captured graph once the
+// set-up has reached its steady state, else direct launches.
+template <typename T>
+static int run_setup(skt_solver* s, const double* E64, const Vecs<T>& v, hipStream_t st) {
+  MGData& mg = s->mg;
+  const bool steady = mg.warm && mg.Mch && (s->op->n_fixed == 0 || mg.gflag);
+  if (!s->graphs || s->timing || !steady)
+    return mg_setup(&mg, s->op, E64, v.E, v.dinv, st);
+  skt_solver::Setup& g = s->setup;
+  const int steps = mg_warm_steps(&mg);
+  const bool fresh = g.exec && g.dtype == v.dtype && g.degree == mg.degree &&
+                     g.steps == steps && g.E64 == E64 && g.E == v.E && g.dinv == v.dinv &&
+                     g.mg_gen == mg.gen;
+  if (!fresh) {
+    if (g.exec) {
+      SKT_HIP(hipStreamSynchronize(st));
+      SKT_HIP(hipGraphExecDestroy(g.exec));
+      g.exec = nullptr;
+    }
+    if (!s->cap) SKT_HIP(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
+    SKT_HIP(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
+    int stt = mg_setup(&mg, s->op, E64, v.E, v.dinv, s->cap);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s->cap, &graph);
+    if (stt) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return stt;
+    }
+    SKT_HIP(e);
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SKT_HIP(e);
+    g.dtype = v.dtype; g.degree = mg.degree; g.steps = steps; g.E64 = E64; g.E = v.E;
+    g.dinv = v.dinv; g.mg_gen = mg.gen;
+    ++s->graph_captures;
+  }
+  SKT_HIP(hipGraphLaunch(g.exec, st));
+  ++s->graph_launches;
   return SKT_OK;
 }
 
@@ -630,7 +745,7 @@ static int setup_precond(skt_solver* s, const Blk& b, const double* E64, const V
   }
   int stt = mg_alloc(&s->mg, s->op, s->mgcfg, v.dtype, s->max_cols);
   if (stt) return stt;
-  stt = mg_setup(&s->mg, s->op, E64, v.E, v.dinv, st);
+  stt = run_setup<T>(s, E64, v, st);
   if (stt) return stt;
   s->used = SKT_PC_MG;
   return SKT_OK;
@@ -672,7 +787,7 @@ static int solve_t(skt_solver* s, int mode, const Blk& b, const double* E64, con
   bool conv = all_flag(s, b.k, S_CONV) && !any_flag(s, b.k, S_BAD);
   while (!conv && it < maxiter && !any_flag(s, b.k, S_BAD)) {
     const int64_t chunk = std::min<int64_t>(check, maxiter - it);
-    if ((stt = pcg_iterations<T>(s, b, v, chunk, replace_next, E64, st))) return stt;
+    if ((stt = run_chunk<T>(s, b, v, chunk, replace_next, E64, st))) return stt;
     if (replace_next) ++updates;
     replace_next = false;
     it += chunk;
@@ -758,6 +873,8 @@ int skt_solver_create(skt_operator* op, int max_cols, skt_solver** out) {
   SKT_HIP(hipMalloc(&s->part, sizeof(double) * kParts * 4 * max_cols));
   SKT_HIP(hipMalloc(&s->sc, sizeof(double) * kSc * max_cols));
   SKT_HIP(hipHostMalloc(&s->host, sizeof(double) * kSc * max_cols, hipHostMallocDefault));
+  const char* g = std::getenv("SKT_GRAPH");
+  s->graphs = (g && std::atoi(g) == 0) ? 0 : 1;
   *out = s;
   return SKT_OK;
 }
@@ -804,8 +921,19 @@ int skt_solver_get_timing(skt_solver* s, double* apply_ms, int64_t* n_apply, int
   return SKT_OK;
 }
 
+int skt_solver_graph_stats(const skt_solver* s, int64_t* launches, int64_t* captures) {
+  SKT_CHECK(s && launches && captures, "null pointer");
+  *launches = s->graph_launches;
+  *captures = s->graph_captures;
+  return SKT_OK;
+}
+
 int skt_solver_destroy(skt_solver* s) {
   if (!s) return SKT_OK;
+  for (auto& g : s->chunk)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (s->setup.exec) (void)hipGraphExecDestroy(s->setup.exec);
+  if (s->cap) (void)hipStreamDestroy(s->cap);
   for (int i = 0; i < 2 * 32; ++i)
     if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
   mg_free(&s->mg);

@@ -37,6 +37,14 @@ class ComplianceProblem:
     the warm start of the next solve."""
 
     def __init__(self, fem, filt, beta: float, eta: float = 0.5, p: float = 3.0):
+        # the backward below is the self-adjoint compliance gradient; other
+        # objectives (heat averaged_temp / heat_exchange) and the rho-dependent
+        # Robin terms need an adjoint solve and extra terms it does not have
+        if fem.physics == _lib.HEAT and (getattr(fem, "objective", "compliance") != "compliance"
+                                         or getattr(fem, "robin", False)):
+            raise NotImplementedError(
+                "ComplianceProblem supports elastic compliance and heat compliance without "
+                "Robin terms")
         self.fem = fem
         self.filter = filt
         self.beta = float(beta)
@@ -72,19 +80,21 @@ class Compliance(torch.autograd.Function):
         _lib.call("skt_heaviside_project", n, _dev.ptr(rf), prob.beta, prob.eta, _dev.ptr(rp), S)
         comp = prob.fem.objectives_multi_load(rp, prob.p, prob.U)   # (k,) device
         ctx.prob = prob
-        ctx.save_for_backward(rf, rp)
+        # this forward's displacement: prob.U is the next solve's warm start
+        # and is overwritten by the next forward
+        ctx.save_for_backward(rf, rp, prob.U.clone())
         return comp.mean()
 
     @staticmethod
     def backward(ctx, grad_out):
         prob = ctx.prob
-        rf, rp = ctx.saved_tensors
+        rf, rp, U = ctx.saved_tensors
         fem = prob.fem
         n = rf.numel()
         k = fem.n_tasks
         S = _dev.stream()
         # element energies recomputed from the forward displacement
-        energy = fem.energy(rp, prob.p, prob.U)
+        energy = fem.energy(rp, prob.p, U)
         e_max, e_min = prob.moduli_range()
         g = torch.empty_like(rf)
         _lib.call("skt_sensitivity_chain", prob.sensitivity_kind, n, k, _dev.ptr(rp),

@@ -144,7 +144,7 @@ class DensityMethod:
         self.cfg = cfg
         self.tsk = tsk
         if cfg.scaling:
-            raise NotImplementedError("scaling=True is not built on the GPU path")
+            self.scale()
         os.makedirs(cfg.dst_path, exist_ok=True)
         if cfg.design_dirichlet is False:
             tsk.exlude_dirichlet_from_design()
@@ -210,6 +210,19 @@ class DensityMethod:
         if isinstance(tsk, LinearHeatConduction):
             return getattr(tsk, "objective", "compliance") == "compliance"
         return True
+
+    def scale(self):
+        """``common_density.py:626-636``: lengths divided by the largest
+        bounding-box side, loads by 1e5, before the FEA and filter objects
+        are built from the task."""
+        bbox = np.ptp(self.tsk.mesh.p, axis=1)
+        self.L_scale = float(np.max(bbox))
+        self.F_scale = 10 ** 5
+        self.tsk.scale(1.0 / self.L_scale, 1.0 / self.F_scale)
+
+    def unscale(self):
+        """``common_density.py:638-641``."""
+        self.tsk.scale(self.L_scale, self.F_scale)
 
     def params_latest(self):
         return self.recorder.as_object_latest()
@@ -415,7 +428,7 @@ class DensityMethod:
         rec = torch.cat([sc[:8], comp.reshape(-1), u_max.reshape(-1), sbuf])
         ctx = dict(names=names, k=k, neumann_scale=neumann_scale,
                    solve_iters=self.fem.last_solve.get("iters"),
-                   obj=self._objective_history_name(tsk))
+                   obj=self._objective_history_name(tsk), host=self._host_records())
         if cfg.check_convergence:
             # the stop test needs this iteration's values now
             converged = self._process_records(rec.cpu().numpy(), ctx)
@@ -462,7 +475,7 @@ class DensityMethod:
         convergence flag (common_density.py:1140-1185)."""
         cfg = self.cfg
         k = ctx["k"]
-        upd = self._update_records(rec[:8])
+        upd = self._update_records(rec[:8], ctx["host"])
         rho_change_max = float(rec[0])
         comp_h = rec[8:8 + k]
         umax_h = rec[8 + k:8 + 2 * k]
@@ -496,14 +509,22 @@ class DensityMethod:
         """``common_density.py:944-964``: ``histories.npz`` always; the
         ``cubic_top.vtu`` submesh (rho > 0.5) when checkpoints are written."""
         cfg, tsk = self.cfg, self.tsk
+        if cfg.scaling:
+            self.unscale()
         if cfg.write_checkpoints and self._state is not None:
             visualization.export_submesh(tsk, self.rho_host(), 0.5,
                                          f"{cfg.dst_path}/cubic_top.vtu")
         self.recorder.export_histories(fname="histories.npz")
 
-    def _update_records(self, scal: np.ndarray) -> dict:
+    def _host_records(self) -> dict:
+        """Subclass hook: host scalars of this iteration's update, taken
+        when the iteration runs (its records are processed later)."""
+        return {}
+
+    def _update_records(self, scal: np.ndarray, host: dict) -> dict:
         """Subclass hook: turn the device scalar slots (copied to the host
-        once per iteration) into named records."""
+        once per iteration) and the host snapshot of the same iteration
+        into named records."""
         return {}
 
     def gather_design(self, full, design):

@@ -165,8 +165,14 @@ class LogMOC_Optimizer(common_density.DensityMethod):
         b = self._buf
         return {"-dC": -b["dC_raw"], "dV_chain": b["chain"].abs(), "dL": b["dL"]}
 
-    def _update_records(self, scal: np.ndarray) -> dict:
+    def _host_records(self) -> dict:
+        # the dual variable, constraint coefficient, volume error and scales
+        # of the iteration that just ran (logmoc.py:238-245 records them in
+        # the same iteration's update)
+        return dict(self._last)
+
+    def _update_records(self, scal: np.ndarray, host: dict) -> dict:
         self.kkt_residual = float(scal[5])
-        d = dict(self._last)
+        d = dict(host)
         d["kkt_residual"] = self.kkt_residual
         return d

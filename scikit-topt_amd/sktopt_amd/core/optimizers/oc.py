@@ -153,7 +153,7 @@ class OC_Optimizer(common_density.DensityMethod):
                            cfg.scaling_rate_min, cfg.scaling_rate_max)
         return {"-dC": -dC, "scaling_rate": rate}
 
-    def _update_records(self, scal: np.ndarray) -> dict:
+    def _update_records(self, scal: np.ndarray, host: dict) -> dict:
         self.kkt_residual = float(scal[S_KKT])
         self.running_scale = float(scal[S_RSCALE])
         return {"lmid": float(scal[S_LMID]), "vol_error": float(scal[S_VOLERR]),

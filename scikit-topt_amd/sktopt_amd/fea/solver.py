@@ -170,6 +170,12 @@ class _GpuFEM:
         else:
             _lib.check(st, "skt_solver_solve")
 
+    def graph_stats(self) -> dict:
+        """Replays and captures of the solver's PCG-chunk hipGraphs."""
+        a, b = C.c_int64(), C.c_int64()
+        _lib.call("skt_solver_graph_stats", self._solver, C.byref(a), C.byref(b))
+        return dict(launches=int(a.value), captures=int(b.value))
+
     def _set_force_scale(self, force_scale):
         if force_scale != self._force_scale:
             torch.mul(self.F0, float(force_scale), out=self.F)

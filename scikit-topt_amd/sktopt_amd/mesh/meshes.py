@@ -8,6 +8,8 @@ numbering (connectivity is implicit).
 """
 from __future__ import annotations
 
+import copy
+
 import numpy as np
 
 CORNERS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
@@ -83,6 +85,15 @@ class _Mesh:
         else:                                                # planar quad: diagonals
             c = np.cross((P[:, 2] - P[:, 0]).T, (P[:, 3] - P[:, 1]).T)
         return 0.5 * np.linalg.norm(c, axis=1)
+
+    def scaled(self, factor: float):
+        """A copy with every node coordinate (and box size) multiplied by
+        ``factor``; connectivity and facet numbering are unchanged."""
+        out = copy.copy(self)
+        out.p = self.p * factor
+        if getattr(self, "h", None) is not None:
+            out.h = tuple(float(h) * factor for h in self.h)
+        return out
 
     def elements_touching(self, nodes) -> np.ndarray:
         flag = np.zeros(self.nvertices, dtype=bool)

@@ -52,6 +52,18 @@ class FEMDomain:
 
     exclude_dirichlet_from_design = exlude_dirichlet_from_design
 
+    def scale(self, L_scale: float, F_scale: float):
+        """``mesh/task_common.py:403-420``: node coordinates times
+        ``L_scale`` (a new mesh), every load vector times ``F_scale`` in
+        place.  Element volumes keep their construction-time values, as in
+        the reference.  Only tasks with load vectors can be scaled."""
+        forces = getattr(self, "neumann_linear", None)
+        if forces is None:
+            raise ValueError("should be ndarray or list of ndarray")
+        self.mesh = self.mesh.scaled(L_scale)
+        for f in forces:
+            f *= F_scale
+
     @property
     def ndof(self) -> int:
         return self.mesh.nvertices * self.ncomp

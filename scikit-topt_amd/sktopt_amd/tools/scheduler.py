@@ -15,26 +15,6 @@ import math
 from dataclasses import dataclass
 from typing import Callable, Optional
 
-import numpy as np
-
-
-def schedule_exp_slowdown(it, total, initial_value=1.0, target_value=0.4, rate=10.0):
-    if total <= 0:
-        raise ValueError("total must be positive")
-    t = it / total
-    decay = np.exp(-rate * t)
-    final_decay = np.exp(-rate)
-    if initial_value > target_value:
-        return target_value + (initial_value - target_value) * (decay - final_decay) / (1 - final_decay)
-    return target_value - (target_value - initial_value) * (decay - final_decay) / (1 - final_decay)
-
-
-def schedule_exp_accelerate(it, total, initial_value=1.0, target_value=0.4, rate=10.0):
-    t = it / total
-    if initial_value > target_value:
-        return target_value + (initial_value - target_value) * (1 - np.exp(rate * (t - 1)))
-    return target_value - (target_value - initial_value) * (1 - np.exp(rate * (t - 1)))
-
 
 def schedule_constant(it, total, target_value=0.4, **args):
     return target_value

@@ -132,3 +132,45 @@ def test_backward_matches_central_differences_at_scale(gpu):
     idx = rng.choice(d, 64, replace=False)
     fd = _fd(prob, rho, idx)
     assert np.abs(fd - g[idx]).max() <= 5e-4 * np.abs(g[idx]).max()
+
+
+def test_two_forwards_then_backward_of_the_first(gpu):
+    """Each graph keeps its own displacement: a second forward (which
+    overwrites the problem's warm-start block) leaves the first graph's
+    gradient equal to that of a lone forward + backward."""
+    import torch
+    task = oprob.cantilever(12, 6, 6)
+    rho_a, _, _ = _rho(task, 11)
+    rho_b, _, _ = _rho(task, 12, lo=0.5, hi=0.9)
+    ref = _problem(oprob.cantilever(12, 6, 6), 2.0, 4.0)
+    _, g_ref = _grad(ref, rho_a)
+    prob = _problem(oprob.cantilever(12, 6, 6), 2.0, 4.0)
+    xa = torch.tensor(rho_a, device="cuda", requires_grad=True)
+    xb = torch.tensor(rho_b, device="cuda", requires_grad=True)
+    Ca = prob.compliance(xa)
+    Cb = prob.compliance(xb)
+    Ca.backward()
+    Cb.backward()
+    np.testing.assert_allclose(xa.grad.cpu().numpy(), g_ref, rtol=1e-9,
+                               atol=1e-12 * np.abs(g_ref).max())
+    _, g_b = _grad(_problem(oprob.cantilever(12, 6, 6), 2.0, 4.0), rho_b)
+    np.testing.assert_allclose(xb.grad.cpu().numpy(), g_b, rtol=1e-9,
+                               atol=1e-12 * np.abs(g_b).max())
+
+
+def test_heat_objectives_without_self_adjoint_backward_are_rejected(gpu):
+    """The backward is the self-adjoint compliance gradient: a heat FEM with
+    the averaged-temperature objective has none, so the problem refuses it."""
+    from sktopt_amd import core, fea
+    from sktopt_amd.filters import SpacialFilter
+    from sktopt_amd.mesh import structured_hex, LinearHeatConduction
+    task = oprob.heat_patch(6, patch=2, source=1e-2)
+    pm = task.mesh
+    t = LinearHeatConduction(structured_hex(*pm.grid, *pm.h), task.dirichlet_nodes,
+                             task.dirichlet_values, task.forces[0], None, task.E0,
+                             objective="averaged_temp")
+    fem = fea.FEM_SimpLinearHeatConduction(t, 1e-3)
+    filt = SpacialFilter(None, task.elements_volume, 1.5, task.design_mask,
+                         element_centers=task.centers)
+    with pytest.raises(NotImplementedError):
+        core.ComplianceProblem(fem, filt, beta=1.0)

@@ -168,20 +168,37 @@ def test_full_vs_stepwise_bit_identical(gpu, ftype, tmp_path):
                                   np.array(b.recorder.as_object().compliance))
 
 
-@pytest.mark.parametrize("precision,rtol,ctol,dtol", [("fp64", 1e-12, 1e-9, 1e-7),
-                                                     ("mixed", 1e-9, 1e-6, 1e-4)])
-def test_c0_matches_golden_trajectory(gpu, precision, rtol, ctol, dtol, tmp_path):
-    """c0 (40x20x20 hex8, 54,243 DOF) against the committed FP64 oracle run."""
+@pytest.mark.parametrize("precision,rtol,ctol", [("fp64", 1e-12, 1e-9), ("mixed", 1e-8, 1e-5)])
+def test_c0_50_iterations_match_golden(gpu, precision, rtol, ctol, tmp_path):
+    """c0 (40x20x20 hex8, 54,243 DOF, 50 OC iterations) against the committed
+    FP64 SuperLU run of the oracle (tests/golden/make_golden.py): compliance
+    within 1e-9 relative (fp64 mode) or 1e-5 (mixed mode at the benchmarked
+    solver tolerance, rtol 1e-8) at every iteration, densities within 1e-4
+    max-abs after 50 iterations (also reported after 10 and 25)."""
     import os
-    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "c0_oc_3iter.npz"))
-    opt = _gpu_opt(oprob.cantilever(40, 20, 20), precision=precision, rtol=rtol,
+    path = os.path.join(os.path.dirname(__file__), "golden", "c0_oc_50iter.npz")
+    if not os.path.exists(path):
+        pytest.skip("golden trajectory not written yet (tests/golden/make_golden.py)")
+    g = np.load(path)
+    opt = _gpu_opt(oprob.cantilever(40, 20, 20), iters=50, precision=precision, rtol=rtol,
                    tmp=str(tmp_path))
-    opt.optimize_steps(3)
+    design = g["design_elements"]
+    drho = {}
+    done = 0
+    for upto in (10, 25, 50):
+        opt.optimize_steps(upto - done)
+        done = upto
+        drho[upto] = float(np.abs(opt.rho_host()[design] - g[f"rho_{upto}"]).max())
     c = np.array(opt.recorder.as_object().compliance)
-    np.testing.assert_allclose(c, g["compliance"], rtol=ctol)
-    assert list(opt.recorder.as_object().bisection_evals) == list(g["n_evals"])
-    rho = opt.rho_host()[g["design_elements"]]
-    assert np.abs(rho - g["rho_final"]).max() <= dtol
+    assert c.shape == (50,)
+    rel = np.abs(c - g["compliance"]) / np.abs(g["compliance"])
+    print(f"[c0 {precision}] max rel compliance err {rel.max():.2e} (iter {rel.argmax() + 1}); "
+          f"max|drho| after 10/25/50: {drho[10]:.2e} {drho[25]:.2e} {drho[50]:.2e}", flush=True)
+    assert rel.max() <= ctol, (rel.argmax() + 1, rel.max())
+    assert drho[50] <= 1e-4, drho
+    if precision == "fp64":
+        assert list(opt.recorder.as_object().bisection_evals) == list(g["n_evals"])
+        np.testing.assert_allclose(opt.recorder.as_object().lmid, g["lmid"], rtol=1e-6)
 
 
 def test_output_files_and_restart(gpu, tmp_path):

@@ -142,6 +142,46 @@ def test_stats4_matches_numpy(gpu, n):
         _lib.call("skt_stats4", 0, _p(_t(np.zeros(1))), _p(out), _s())
 
 
+def test_reductions_and_stats_propagate_nan(gpu):
+    """numpy's max / min / mean / std of an array holding a NaN are all NaN;
+    so are the device reductions' (a diverged iteration shows up in every
+    statistic of its record)."""
+    from sktopt_amd import _lib
+    rng = np.random.default_rng(5)
+    for n, pos in ((100001, 77777), (5000, 0), (3, 2)):
+        x = rng.standard_normal(n)
+        x[pos] = np.nan
+        out = _t(np.zeros(4))
+        _lib.call("skt_stats4", n, _p(_t(x)), _p(out), _s())
+        assert np.isnan(out.cpu().numpy()).all(), (n, pos, out)
+        o1 = _t(np.zeros(1))
+        _lib.call("skt_reduce", _lib.RED_MAXABS, n, _p(_t(x)), None, _p(o1), _s())
+        assert np.isnan(o1.item()), (n, pos)
+
+
+def test_reductions_on_two_streams_use_separate_scratch(gpu):
+    """Reductions issued on two streams at once give each stream's own
+    result: the partials buffer belongs to the (device, stream) pair."""
+    import torch
+    from sktopt_amd import _lib
+    rng = np.random.default_rng(9)
+    n = 2_000_003
+    xs = [_t(rng.standard_normal(n)), _t(rng.standard_normal(n) + 3.0)]
+    refs = [float(x.sum().item()) for x in xs]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = [_t(np.zeros(64)), _t(np.zeros(64))]
+    torch.cuda.synchronize()
+    for rep in range(64):
+        for i in (0, 1):
+            _lib.call("skt_reduce", _lib.RED_SUM, n, _p(xs[i]), None,
+                      _p(outs[i][rep:rep + 1]), streams[i].cuda_stream)
+    torch.cuda.synchronize()
+    for i in (0, 1):
+        got = outs[i].cpu().numpy()
+        assert (got == got[0]).all(), i
+        np.testing.assert_allclose(got[0], refs[i], rtol=1e-12)
+
+
 # ---------------------------------------------------------------- filter
 
 def _filter_case(kind, n, r):

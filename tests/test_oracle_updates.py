@@ -238,3 +238,74 @@ def test_default_move_limit_binds_iters_max_3():
     ps = _products_schedulers()
     cfg = ps.SchedulerConfig.sawtooth_decay("move_limit", 0.3, 0.1, 3)
     assert cfg.iters_max == 3 and cfg.num_steps == 6
+
+
+def test_oc_running_scale_hand_values():
+    """``oc.py:187-196``: scale = max|dC| (or the percentile of |dC| when a
+    float is scheduled), floored at 1e-12; running_scale = scale at the
+    first iteration, then 0.6 * previous + 0.4 * scale.  Values worked by
+    hand for a five-iteration sequence of sensitivity fields."""
+    from oracle import driver as odrv
+    cfg = odrv.Config(E_min_coeff=1e-9)
+    upd = ooc.OCUpdate(cfg)
+    n = 3
+    design = np.arange(n)
+    vol = np.ones(n)
+    ident = _StubFilter(1.0, 1.0)
+    steps = [  # (dC, percentile, expected running scale)
+        ([-2.0, -0.5, -1.0], None, 2.0),           # max 2
+        ([-4.0, -1.0, -3.0], None, 2.8),           # 0.6 * 2 + 0.4 * 4
+        ([-1.0, -0.25, -0.5], None, 2.08),         # 0.6 * 2.8 + 0.4 * 1
+        ([-1.0, -3.0, -2.0], 50.0, 2.048),         # median 2: 0.6 * 2.08 + 0.4 * 2
+        ([-1e-14, -2e-14, -3e-14], None, 1.2288 + 4e-13),   # floor 1e-12
+    ]
+    for it, (dC, q, want) in enumerate(steps, start=1):
+        rho = np.full(n, 0.5)
+        upd(it, rho, np.array(dC), 0.2, 0.5, 1.0, q, vol, vol.sum(), 0.5, np.full(n, 0.5),
+            design, ident, vol / vol.sum())
+        np.testing.assert_allclose(upd.running_scale, want, rtol=1e-14, err_msg=f"it {it}")
+
+
+def test_oc_kkt_residual_hand_values():
+    """``oc.py:230-240``: |dC_raw + lmid * running_scale * dV| over the
+    updated densities strictly inside (rho_min + 1e-6, rho_max - 1e-6).
+    With lmid * running_scale = 10: element 0 gives |-3 + 1| = 2, elements 1
+    and 2 sit on the bounds' margins and are left out, element 3 gives
+    |5 + 4| = 9."""
+    dC = np.array([-3.0, -1.0, -2.0, 5.0])
+    dV = np.array([0.1, 0.2, 0.3, 0.4])
+    rho_new = np.array([0.5, 0.01 + 1e-6, 1.0 - 1e-6, 0.3])
+    assert ooc.kkt_residual(dC, rho_new, 2.0, 5.0, dV, 0.01, 1.0) == pytest.approx(9.0, rel=1e-15)
+    # the sign of the multiplier term matters: a dropped or flipped term gives 3 or 1 here
+    assert ooc.kkt_residual(dC[:1], rho_new[:1], 2.0, 5.0, dV[:1], 0.01, 1.0) == \
+        pytest.approx(2.0, rel=1e-15)
+    assert ooc.kkt_residual(dC, np.array([0.01, 0.01, 1.0, 1.0]), 2.0, 5.0, dV, 0.01, 1.0) == 0.0
+
+
+def test_logmoc_running_scale_and_kkt_hand_values():
+    """``logmoc.py:217-240``: with the volume exactly on target the dual
+    variable stays 0, so dL = dC; scale = percentile(|dL|, 95 or the
+    scheduled float), floored at 1e-8; running_scale = scale at the first
+    iteration, then 0.2 * previous + 0.8 * scale; the KKT residual is
+    max |dL / running_scale| over densities inside the bounds' margins
+    (before the update).  Values worked by hand."""
+    from oracle import driver as odrv
+    cfg = odrv.Config(method="LogMOC", lambda_lower=-1e7, lambda_upper=1e7)
+    upd = olm.LogMOCUpdate(cfg)
+    design = np.arange(2)
+    vol = np.ones(2)
+    proj = np.array([0.25, 0.75])                  # volume 0.5 = target: no dual term
+    ident = _StubFilter(1.0, 1.0)
+    steps = [  # (dC, rho before the update, percentile, running scale, kkt)
+        ([-1.0, -3.0], [0.5, 0.5], None, 2.9, 3.0 / 2.9),        # q95 of [1,3]: 1 + 0.95*2
+        ([-2.0, -6.0], [0.5, 0.5], None, 5.22, 6.0 / 5.22),      # 0.2*2.9 + 0.8*5.8
+        ([-1.0, -4.0], [0.5, 1.0], 50.0, 3.044, 1.0 / 3.044),    # 0.2*5.22 + 0.8*2.5; rho 1 out
+        ([-1e-9, -1e-9], [0.01, 0.5], None, 0.6088 + 8e-9, 1e-9 / (0.6088 + 8e-9)),  # floor
+    ]
+    for it, (dC, rho, q, rs, kkt) in enumerate(steps, start=1):
+        rho = np.array(rho)
+        upd(it, rho, np.array(dC), proj, np.full(2, 0.5), 0.2, 0.5, 1.0, q, vol, vol.sum(),
+            0.5, design, ident)
+        assert upd.last["vol_error"] == 0.0 and upd.last["lambda_v"] == 0.0
+        np.testing.assert_allclose(upd.running_scale, rs, rtol=1e-14, err_msg=f"it {it}")
+        np.testing.assert_allclose(upd.kkt_residual, kkt, rtol=1e-14, err_msg=f"it {it}")
