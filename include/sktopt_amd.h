@@ -150,6 +150,15 @@ int skt_stats4(int64_t n, const double* x, double* out, hipStream_t stream);
 int skt_percentile_abs(int64_t n, const double* x, double q, double* out,
                        hipStream_t stream);
 
+/* skt_percentile_abs without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops).  Its scratch is a static 17.5 KB
+ * device array per GPU, held until the process ends; n <= 2^53.  Every call
+ * on one GPU shares that array, so calls that overlap on one GPU (two
+ * streams or two host threads) race. */
+int skt_percentile_abs_noalloc(int64_t n, const double* x, double q, double* out,
+                               hipStream_t stream);
+
 /* ------------------------------------------------------------------------
  * Matrix-free SIMP/RAMP-weighted operators
  *   K(E) = sum_e E_e K_e^0, Dirichlet rows replaced by identity

@@ -44,6 +44,7 @@ SIGNATURES = {
     "skt_reduce": [_i32, _i64, _vp, _vp, _vp, _vp],
     "skt_reduce_cols": [_i32, _i64, _i32, _vp, _vp, _vp, _vp],
     "skt_percentile_abs": [_i64, _vp, _d, _vp, _vp],
+    "skt_percentile_abs_noalloc": [_i64, _vp, _d, _vp, _vp],
     "skt_stats4": [_i64, _vp, _vp, _vp],
     "skt_hex8_create": [_i32, _i64, _i64, _i64, _d, _d, _d, _d, _i32, _vp, _pp],
     "skt_tet4_create": [_i32, _i64, _i64, _vp, _vp, _d, _vp, _pp],
