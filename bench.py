@@ -161,6 +161,75 @@ def apply_timing_pass(opt, steps=2):
     return dict(f64_ms=ms[0], f64_n=n[0], f32_ms=ms[1], f32_n=n[1])
 
 
+def time_launches(fn, reps=20, warm=3):
+    """Average duration (us) of fn()'s launches, HIP events on the current
+    stream around `reps` back-to-back calls."""
+    import torch
+    for _ in range(warm):
+        fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return 1e3 * e0.elapsed_time(e1) / reps
+
+
+def apply_suite():
+    """Rooflines of the operator apply on the other configurations and of
+    the CSR filter gather on the tet mesh, each at uniform density 0.3, fp32
+    operator (float64 filter), HIP events around back-to-back launches.
+    Algorithmic bytes: apply k N (4 + 4) + n_el s_E [+ tet4 connectivity
+    16 + gradients 12 * 4 + volume 4 per element]; CSR gather nnz (4 + 8)
+    (column index, float64 weight) + n_design 8 (row pointer) + n_el 16
+    (the source read once, the output written once)."""
+    import torch
+    from sktopt_amd import fea, filters
+    from sktopt_amd.mesh import toy_problem
+    out = []
+    cases = [
+        ("tet4-100x50x50x6-1load (c1)", lambda: toy_problem.cantilever(100, 50, 50, kind="tet4",
+                                                                    jitter=0.2, seed=0)),
+        ("hex8-200x100x100-1load (c3)", lambda: toy_problem.cantilever(200, 100, 100)),
+        ("heat-hex8-160^3 (c4)", lambda: toy_problem.heat_patch(160, patch=16, source=1e-2)),
+    ]
+    for name, mk in cases:
+        task = mk()
+        cls = fea.FEM_SimpLinearHeatConduction if name.startswith("heat") else \
+            fea.FEM_SimpLinearElasticity
+        fem = cls(task, 1e-3 if name.startswith("heat") else 1e-9,
+                  solver_config=fea.LinearSolverConfig(precision="mixed"))
+        rho = torch.full((fem.n_el,), 0.3, dtype=torch.float64, device="cuda")
+        fem._moduli(rho, 3.0)
+        X = torch.randn((fem.ndof, fem.n_tasks), dtype=torch.float32, device="cuda")
+        us = time_launches(lambda: fem.apply(X, dtype=torch.float32))
+        b = fem.n_tasks * fem.ndof * 8 + fem.n_el * 4
+        if task.mesh.kind == "tet4":
+            b += fem.n_el * (16 + 12 * 4 + 4)
+        gbs = b / (us * 1e-6) / 1e9
+        out.append(dict(config=name, kernel="operator apply (fp32)", ndof=fem.ndof,
+                        n_elements=fem.n_el, bytes=b, avg_us=us, achieved_gbs=gbs,
+                        frac=gbs / HBM_PEAK_GBS, dof_per_s=fem.n_tasks * fem.ndof / (us * 1e-6)))
+        log(f"apply {name}: {us:.1f} us, {gbs:.0f} GB/s")
+        if task.mesh.kind == "tet4":
+            f = filters.SpacialFilter.from_defaults(task.mesh, task.elements_volume, 1.5,
+                                                    design_mask=task.design_mask)
+            x = torch.rand(fem.n_el, dtype=torch.float64, device="cuda")
+            y = torch.empty_like(x)
+            us = time_launches(lambda: f.forward_into(x, y))
+            nd, nnz = f.n_design, f.nnz
+            b = nnz * (4 + 8) + nd * 8 + fem.n_el * 16
+            gbs = b / (us * 1e-6) / 1e9
+            out.append(dict(config=name, kernel="CSR filter gather (fp64, r=1.5)", nnz=nnz,
+                            bytes=b, avg_us=us, achieved_gbs=gbs, frac=gbs / HBM_PEAK_GBS))
+            log(f"filter {name}: nnz={nnz} {us:.1f} us, {gbs:.0f} GB/s")
+            del f
+        del fem
+        torch.cuda.empty_cache()
+    return out
+
+
 def median(xs):
     ys = sorted(xs)
     m = len(ys) // 2
@@ -219,6 +288,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cpu-c0", action="store_true",
                     help="skip the fully timed CPU iteration of the 40x20x20 case (~100 s)")
+    ap.add_argument("--apply-suite", action="store_true",
+                    help="also time the operator apply on the tet, 200x100x100 and heat "
+                         "configurations and the tet mesh's CSR filter gather")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -289,6 +361,7 @@ def main():
             cpu["c0_full_iteration"] = cpu_baseline.measure_c0_iteration(log)
             log(f"cpu c0 iteration: {cpu['c0_full_iteration']['seconds']:.1f} s")
     variants = modal_variants_from_profile(ndof, n_el, k)
+    suite = apply_suite() if args.apply_suite else None
     line = {
         "metric": "OC iters/s, 1.36M-DOF hex8, GPU FP32-mixed vs FP64 at compliance err≤1e-5",
         "value": value, "unit": "OC iterations/s", "n_gpus": 1, "steps": K, "warmup": W,
@@ -312,6 +385,7 @@ def main():
         "operator_apply_dof_per_s": k * ndof / (avg_ms * 1e-3),
         "modal_variants": {"source": os.path.relpath(KERNEL_TRACE_FILE, ROOT),
                            "kernels": variants},
+        "other_configs": suite,
         "fp64": fp64, "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
         "compliance_rel_err_vs_fp64": err,
         "cpu_baseline": cpu,

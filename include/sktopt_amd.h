@@ -139,6 +139,11 @@ int skt_reduce(int op, int64_t n, const double* x, const double* y,
 int skt_reduce_cols(int op, int64_t n, int ncols, const double* x,
                     const double* y, double* out, hipStream_t stream);
 
+/* out[i] = mean of row i of a (n, k) row-major block, k <= 128, summed in
+ * numpy's pairwise order for short rows: energy.mean(axis=1) of
+ * core/optimizers/common_density.py:1065, recorded as "energy" (:1148). */
+int skt_row_mean(int64_t n, int k, const double* x, double* out, hipStream_t stream);
+
 /* out[0..3] = [min, mean, max, std] of x (population std, numpy's np.std),
  * the statistics a "min-max-mean-std" history keeps per iteration
  * (tools/history.py:145-151).  n >= 1. */
@@ -207,6 +212,28 @@ int skt_hex8_node_to_elem(int64_t nx, int64_t ny, int64_t nz, int ncols,
                           const double* rho_node, int clamp_nonpositive,
                           int mean_cols, double* rho_elem, hipStream_t stream);
 
+/* Tet4 scalar operator a M + b K (M: P1 mass V/20 (1 + delta_ab), K: unit-
+ * conductivity Laplacian), the nodal Helmholtz filter's bilinear form on
+ * tetrahedral meshes (filters/helmholtz_filter_nodal.py:140-146 with
+ * ElementTetP1, whose default quadrature integrates both exactly).
+ * node_fixed: (n_nodes,) uint8, identity rows.  Energies of this operator
+ * are those of its conduction part. */
+int skt_tet4_create_scalar(int64_t n_nodes, int64_t n_el, const double* coords,
+                           const int32_t* conn, double mass_coef, double stiff_coef,
+                           const uint8_t* node_fixed, skt_operator** out);
+
+/* The two transfers above on a tet4 mesh, taking the node -> element lists
+ * and connectivity of a tet4 operator built on it (any physics): volume-
+ * weighted element -> node averaging in ascending element order
+ * (helmholtz_filter_nodal.py:31-59) and the 4-node mean (:27-28), with the
+ * same clamp_nonpositive / mean_cols options. */
+int skt_tet4_elem_to_node(const skt_operator* op, int ncols, const double* vol,
+                          const uint8_t* design, double fixed_value,
+                          const double* rho_elem, double* rho_node, hipStream_t stream);
+int skt_tet4_node_to_elem(const skt_operator* op, int ncols, const double* rho_node,
+                          int clamp_nonpositive, int mean_cols, double* rho_elem,
+                          hipStream_t stream);
+
 /* Robin terms of the heat path on structured hex8 grids
  * (fea/solver_heat.py:495-624,705-980).
  *
@@ -274,6 +301,15 @@ int skt_stencil27_apply(int64_t nx, int64_t ny, int64_t nz, int ncols,
 int skt_tet4_create(int physics, int64_t n_nodes, int64_t n_el,
                     const double* coords, const int32_t* conn, double nu,
                     const uint8_t* dof_fixed, skt_operator** out);
+
+/* Declare that a tet4 operator's mesh is a grid of nx*ny*nz cells numbered
+ * x-fastest, nodes at the (nx+1)(ny+1)(nz+1) cell corners numbered the same
+ * way, and element 6 c + s one of the six tets of cell c (checked).  The
+ * solver's multigrid preconditioner then coarsens the grid: its first
+ * coarse level is the Galerkin product of the tets' element matrices with
+ * trilinear interpolation on the cells (the counterpart of the reference's
+ * cg_pyamg option, fea/solver_elastic.py:93-104, for such meshes). */
+int skt_tet4_set_grid(skt_operator* op, int64_t nx, int64_t ny, int64_t nz);
 
 int skt_operator_destroy(skt_operator* op);
 

@@ -313,6 +313,33 @@ __global__ void k_reduce_cols_final(int op, int nb, int ncols,
   }
 }
 
+// Row means of a (n, k) row-major block, k <= 128, summed as numpy's
+// pairwise sum does for short rows (sequential below 8 values; else eight
+// partial sums combined pairwise, then the tail), divided by k.
+__global__ void k_row_mean(int64_t n, int k, const double* __restrict__ x,
+                           double* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double* a = x + i * k;
+    double res;
+    if (k < 8) {
+      res = 0.0;
+      for (int j = 0; j < k; ++j) res += a[j];
+    } else {
+      double r[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) r[j] = a[j];
+      const int body = k - (k % 8);
+      for (int b = 8; b < body; b += 8)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r[j] += a[b + j];
+      res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+      for (int j = body; j < k; ++j) res += a[j];
+    }
+    out[i] = res / (double)k;
+  }
+}
+
 // ------------------------------------------------------------------- stats
 // [min, mean, max, population std] of x.  Block b takes the contiguous
 // chunk [b*chunk, min(n, (b+1)*chunk)), whose count is c_b.  A first pass
@@ -772,6 +799,14 @@ int skt_reduce_cols(int op, int64_t n, int ncols, const double* x,
                      ncols, cp, x, y, s->part);
   hipLaunchKernelGGL(k_reduce_cols_final, dim3(1), dim3(kBlock), 0, stream, op,
                      (int)nb, ncols, s->part, out);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_row_mean(int64_t n, int k, const double* x, double* out, hipStream_t stream) {
+  SKT_CHECK(n >= 0 && k >= 1 && k <= 128 && x && out, "bad argument");
+  if (n == 0) return SKT_OK;
+  hipLaunchKernelGGL(k_row_mean, dim3(grid_for(n)), dim3(kBlock), 0, stream, n, k, x, out);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }

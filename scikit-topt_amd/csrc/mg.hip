@@ -28,7 +28,7 @@ static int axis_factor(int64_t n) {
 
 bool mg_plan(const skt_operator* op, const MGConfig& cfg, std::vector<MGLevel>* out) {
   out->clear();
-  if (op->mesh != SKT_MESH_HEX8) return false;
+  if (op->mesh != SKT_MESH_HEX8 && !(op->mesh == SKT_MESH_TET4 && op->tet_grid)) return false;
   const int NC = op->ncomp;
   MGLevel l0;
   l0.nx = op->nx; l0.ny = op->ny; l0.nz = op->nz;
@@ -161,6 +161,87 @@ k_galerkin(int64_t ncx, int64_t ncy, int64_t ncz, int fx, int fy, int fz,
             sK[i] = v;
           }
         }
+        galerkin_child<NC>(sK, sT, sP, a, b, c, fx, fy, fz, acc);
+      }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = tid + 64 * k;
+    if (i < NN) Kc[ce * NN + i] = acc[k];
+  }
+}
+
+// First coarse level of a tet4 operator whose mesh is a grid of nx*ny*nz
+// cells, each split into six tets (element 6 c + s lies in cell c, nodes
+// numbered as the cell corners of the grid).  One workgroup of 64 threads
+// per coarse element: every child cell's 8-corner matrix
+//   K_cell = sum_{s < 6} E_t Z_t K_t Z_t   (t = 6 cell + s, in order s)
+// is gathered in LDS (each tet's K_t = V B^T D B from its barycentric
+// gradients, Z removing Dirichlet DOFs) and projected as the hex children.
+template <int NC>
+__global__ void __launch_bounds__(64)
+k_galerkin_tet(int64_t ncx, int64_t ncy, int64_t ncz, int fx, int fy, int fz,
+               const int32_t* __restrict__ conn, const double* __restrict__ g,
+               const double* __restrict__ vol, double lam0, double mu0, int64_t n_el,
+               const double* __restrict__ E, const uint8_t* __restrict__ fixed,
+               double* __restrict__ Kc) {
+  constexpr int ND = 8 * NC;
+  constexpr int NN = ND * ND;
+  constexpr int PER = (NN + 63) / 64;
+  __shared__ double sK[NN];
+  __shared__ double sT[NN];
+  __shared__ double sP[64];
+  __shared__ double sg[12];
+  __shared__ int sloc[8];        // cell corner bits -> tet local node, or -1
+  __shared__ int64_t snode[8];   // cell corner bits -> global node
+  __shared__ double sw;
+  const int64_t ce = blockIdx.x;
+  const int64_t CX = ce % ncx, CY = (ce / ncx) % ncy, CZ = ce / (ncx * ncy);
+  const int64_t nfx = ncx * fx, nfy = ncy * fy;
+  const int64_t nfnx = nfx + 1, nfny = nfy + 1;
+  const int tid = threadIdx.x;
+  double acc[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) acc[i] = 0.0;
+  for (int c = 0; c < fz; ++c)
+    for (int b = 0; b < fy; ++b)
+      for (int a = 0; a < fx; ++a) {
+        const int64_t ex = CX * fx + a, ey = CY * fy + b, ez = CZ * fz + c;
+        const int64_t cell = ex + nfx * (ey + nfy * ez);
+        for (int i = tid; i < NN; i += 64) sK[i] = 0.0;
+        if (tid < 8)
+          snode[tid] = (ex + (tid & 1)) + nfnx * ((ey + ((tid >> 1) & 1)) + nfny * (ez + (tid >> 2)));
+        for (int sidx = 0; sidx < 6; ++sidx) {
+          const int64_t t = 6 * cell + sidx;
+          __syncthreads();
+          if (tid < 12) sg[tid] = g[(int64_t)tid * n_el + t];
+          if (tid < 8) {
+            int loc = -1;
+            for (int q = 0; q < 4; ++q)
+              if ((int64_t)conn[(int64_t)q * n_el + t] == snode[tid]) loc = q;
+            sloc[tid] = loc;
+          }
+          if (tid == 0) sw = E[t] * vol[t];
+          __syncthreads();
+          const double w = sw;
+          for (int i = tid; i < NN; i += 64) {
+            const int r = i / ND, col = i % ND;
+            const int qa = r / NC, ca = r % NC, qb = col / NC, cb = col % NC;
+            const int la = sloc[qa], lb = sloc[qb];
+            if (la < 0 || lb < 0) continue;
+            if (fixed && (fixed[snode[qa] * NC + ca] || fixed[snode[qb] * NC + cb])) continue;
+            const double* ga = sg + 3 * la;
+            const double* gb = sg + 3 * lb;
+            double v;
+            if (NC == 1) {
+              v = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+            } else {
+              const double gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+              v = lam0 * ga[ca] * gb[cb] + mu0 * ((ca == cb ? gg : 0.0) + ga[cb] * gb[ca]);
+            }
+            sK[i] += w * v;
+          }
+        }
+        __syncthreads();
         galerkin_child<NC>(sK, sT, sP, a, b, c, fx, fy, fz, acc);
       }
 #pragma unroll
@@ -899,6 +980,21 @@ __global__ void k_cheb(int64_t total, int64_t ndof, int kc, const double* __rest
   }
 }
 
+// w = b - w (the fine level's residual after a generic operator apply)
+template <typename T>
+__global__ void k_rsub(int64_t total, const T* __restrict__ b, T* __restrict__ w) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x)
+    w[t] = b[t] - w[t];
+}
+
+template <typename T>
+__global__ void k_copy_vec(int64_t total, const T* __restrict__ a, T* __restrict__ b) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x)
+    b[t] = a[t];
+}
+
 // Transfers run on row grids: blockIdx.x and threadIdx.x give the node along
 // x, blockIdx.y (strided) the row (y, z, column group), so the index
 // arithmetic is a few 32-bit divisions per row rather than 64-bit ones per
@@ -1399,7 +1495,20 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   // Galerkin hierarchy.  The first coarse level takes E_j M_j for its
   // interior elements (M_j and the boundary flags depend on the operator
   // only, so they are built once) and the masked product for the rest.
-  {
+  if (op->mesh == SKT_MESH_TET4) {
+    const MGLevel& c = mg->lv[1];
+    const int64_t nce = c.nx * c.ny * c.nz;
+    const uint8_t* fx0 = op->n_fixed > 0 ? op->fixed : nullptr;
+    if (NC == 3)
+      hipLaunchKernelGGL((k_galerkin_tet<3>), dim3((unsigned)nce), dim3(64), 0, g, c.nx, c.ny,
+                         c.nz, c.f[0], c.f[1], c.f[2], op->conn, op->g64, op->vol64, op->lam0,
+                         op->mu0, op->n_el, E64, fx0, c.Kel);
+    else
+      hipLaunchKernelGGL((k_galerkin_tet<1>), dim3((unsigned)nce), dim3(64), 0, g, c.nx, c.ny,
+                         c.nz, c.f[0], c.f[1], c.f[2], op->conn, op->g64, op->vol64, op->lam0,
+                         op->mu0, op->n_el, E64, fx0, c.Kel);
+    SKT_LAUNCH_CHECK();
+  } else {
     const MGLevel& c = mg->lv[1];
     const int nch = c.f[0] * c.f[1] * c.f[2];
     const int64_t nce = c.nx * c.ny * c.nz;
@@ -1632,8 +1741,34 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
   int stt;
   T* cur = (T*)mg->lv[0].x;  // fine iterate (ping-pong with z)
   T* oth = z;
-  // fine level, pre-smoothing and restricted residual
-  {
+  // Operators without the fused modal kernels (tet4 on a grid): the fine
+  // level's Chebyshev steps as operator applies plus vector updates, with
+  // the direction kept in z; the iterate lives in lv[0].x and is copied to z
+  // at the end.
+  const bool generic = !(op->mesh == SKT_MESH_HEX8 && op->modal_ok);
+  if (generic) {
+    MGLevel& v = mg->lv[0];
+    const int64_t total = v.ndof * k;
+    const unsigned ge = grid_for(total);
+    const double* cf = mg->coef;
+    T* x = (T*)v.x;
+    T* d = z;
+    for (int st = 0; st < deg; ++st) {
+      T* w = st == 0 ? nullptr : w0;
+      if (st > 0 && (stt = op_apply(const_cast<skt_operator*>(op), mg->dtype, k, KC, E, x, w0,
+                                    s)))
+        return stt;
+      hipLaunchKernelGGL(k_cheb<T>, dim3(ge), dim3(kBlock), 0, s, total, v.ndof, KC,
+                         cf + 2 * st, r, (const T*)w, (const T*)dinv0, d, x, st == 0 ? 1 : 0);
+    }
+    if ((stt = op_apply(const_cast<skt_operator*>(op), mg->dtype, k, KC, E, x, w0, s)))
+      return stt;
+    hipLaunchKernelGGL(k_rsub<T>, dim3(ge), dim3(kBlock), 0, s, total, r, w0);
+    launch_restrict<T, NC, KC>(v, mg->lv[1], groups, (const T*)w0, (const T*)nullptr, fixed0,
+                               (T*)mg->lv[1].b, s);
+    SKT_LAUNCH_CHECK();
+    cur = x;
+  } else {
     MGLevel& v = mg->lv[0];
     const int64_t total = v.ndof * k;
     const double* cf = mg->coef;
@@ -1712,6 +1847,26 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     }
     it[l] = A;
     SKT_LAUNCH_CHECK();
+  }
+  if (generic) {
+    MGLevel& v = mg->lv[0];
+    const int64_t total = v.ndof * k;
+    const unsigned ge = grid_for(total);
+    const double* cf = mg->coef;
+    T* x = (T*)v.x;
+    launch_prolong<T, NC, KC>(v, mg->lv[1], groups, (const T*)it[1], fixed0, x, s);
+    // post-smoothing from the corrected iterate (step 0 has c1 = 0, so the
+    // stale direction in z is not used)
+    for (int st = 0; st < deg; ++st) {
+      if ((stt = op_apply(const_cast<skt_operator*>(op), mg->dtype, k, KC, E, x, w0, s)))
+        return stt;
+      hipLaunchKernelGGL(k_cheb<T>, dim3(ge), dim3(kBlock), 0, s, total, v.ndof, KC,
+                         cf + 2 * st, r, (const T*)w0, (const T*)dinv0, z, x, 0);
+    }
+    hipLaunchKernelGGL(k_copy_vec<T>, dim3(ge), dim3(kBlock), 0, s, total, (const T*)x, z);
+    if (slots) *slots = 0;
+    SKT_LAUNCH_CHECK();
+    return SKT_OK;
   }
   // fine level, correction and post-smoothing
   {

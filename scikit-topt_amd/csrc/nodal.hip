@@ -1,5 +1,5 @@
 // Element <-> node transfers of the nodal Helmholtz filter on structured hex8
-// grids (filters/helmholtz_filter_nodal.py:27-59).
+// grids and tet4 meshes (filters/helmholtz_filter_nodal.py:27-59).
 //
 // Element -> node: volume-weighted average over the (up to 8) elements of a
 // node, non-design elements counting with a fixed value.  The reference
@@ -12,7 +12,7 @@
 // Both transfers work on ncols columns at once ((n, ncols) row-major); the
 // node -> element one can also fold the columns into their mean, summed in
 // column order from 0.0 like the reference's per-load-case accumulation.
-#include "common.hpp"
+#include "operator.hpp"
 
 namespace skt {
 
@@ -73,6 +73,55 @@ __global__ void k_node_to_elem(int64_t nx, int64_t ny, int64_t nz, int ncols,
   }
 }
 
+// Tet4 element -> node: the node's elements come from the operator's
+// node -> element lists, which are sorted by element index, so the
+// accumulation visits them in the reference's (ascending) order.
+__global__ void k_tet_elem_to_node(int64_t n_nodes, int ncols, const int64_t* __restrict__ nptr,
+                                   const int32_t* __restrict__ nel,
+                                   const double* __restrict__ vol,
+                                   const uint8_t* __restrict__ design, double fixed_value,
+                                   const double* __restrict__ re, double* __restrict__ rn) {
+  const int64_t total = n_nodes * ncols;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t node = t / ncols;
+    const int col = (int)(t - node * ncols);
+    double acc = 0.0, ws = 0.0;
+    for (int64_t q = nptr[node]; q < nptr[node + 1]; ++q) {
+      const int64_t e = nel[q];
+      const double w = vol[e];
+      const double v = (!design || design[e]) ? re[e * ncols + col] : fixed_value;
+      acc += w * v;
+      ws += w;
+    }
+    rn[t] = acc / (ws == 0.0 ? 1.0 : ws);
+  }
+}
+
+// Tet4 node -> element: mean of the 4 vertex values in the mesh's local
+// order, divided by 4 (numpy.mean over the node axis).
+__global__ void k_tet_node_to_elem(int64_t n_el, int ncols, const int32_t* __restrict__ conn,
+                                   const double* __restrict__ rn, int clamp_nonpositive,
+                                   int mean_cols, double* __restrict__ re) {
+  const int64_t total = mean_cols ? n_el : n_el * ncols;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = mean_cols ? t : t / ncols;
+    const int c0 = mean_cols ? 0 : (int)(t - e * ncols);
+    const int c1 = mean_cols ? ncols : c0 + 1;
+    double sum = 0.0, v = 0.0;
+    for (int c = c0; c < c1; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc += rn[(int64_t)conn[a * n_el + e] * ncols + c];
+      v = acc / 4.0;
+      if (clamp_nonpositive && v > 0.0) v = 0.0;
+      sum += v;
+    }
+    re[t] = mean_cols ? sum / (double)ncols : v;
+  }
+}
+
 }  // namespace skt
 
 using namespace skt;
@@ -102,6 +151,34 @@ int skt_hex8_node_to_elem(int64_t nx, int64_t ny, int64_t nz, int ncols, const d
   const int64_t total = mean_cols ? ne : ne * ncols;
   hipLaunchKernelGGL(k_node_to_elem, dim3(grid_for(total)), dim3(kBlock), 0, stream, nx, ny, nz,
                      ncols, rho_node, clamp_nonpositive, mean_cols, rho_elem);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_tet4_elem_to_node(const skt_operator* op, int ncols, const double* vol,
+                          const uint8_t* design, double fixed_value, const double* rho_elem,
+                          double* rho_node, hipStream_t stream) {
+  SKT_CHECK(op && op->mesh == SKT_MESH_TET4, "a tet4 operator is required");
+  SKT_CHECK(ncols >= 1, "ncols must be >= 1");
+  SKT_CHECK(vol && rho_elem && rho_node, "null pointer");
+  const int64_t total = op->n_nodes * ncols;
+  hipLaunchKernelGGL(k_tet_elem_to_node, dim3(grid_for(total)), dim3(kBlock), 0, stream,
+                     op->n_nodes, ncols, op->nptr, op->nel, vol, design, fixed_value, rho_elem,
+                     rho_node);
+  SKT_LAUNCH_CHECK();
+  return SKT_OK;
+}
+
+int skt_tet4_node_to_elem(const skt_operator* op, int ncols, const double* rho_node,
+                          int clamp_nonpositive, int mean_cols, double* rho_elem,
+                          hipStream_t stream) {
+  SKT_CHECK(op && op->mesh == SKT_MESH_TET4, "a tet4 operator is required");
+  SKT_CHECK(ncols >= 1, "ncols must be >= 1");
+  SKT_CHECK(rho_node && rho_elem, "null pointer");
+  const int64_t total = mean_cols ? op->n_el : op->n_el * ncols;
+  hipLaunchKernelGGL(k_tet_node_to_elem, dim3(grid_for(total)), dim3(kBlock), 0, stream,
+                     op->n_el, ncols, op->conn, rho_node, clamp_nonpositive, mean_cols,
+                     rho_elem);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }

@@ -37,6 +37,12 @@ struct skt_operator {
   int32_t* nel = nullptr;
   uint8_t* nloc = nullptr;
   double lam0 = 0.0, mu0 = 0.0;  // unit-modulus Lame parameters
+  // scalar tet4 operators: a M + b K (mass and conduction), a = 0 for plain
+  // conduction (skt_tet4_create_scalar sets them)
+  double mass_a = 0.0, cond_b = 1.0;
+  // 1: the tet4 mesh is a grid of nx*ny*nz cells, six tets per cell
+  // (skt_tet4_set_grid), which lets the multigrid coarsen it
+  int tet_grid = 0;
   // extra scalar 27-point stencil added on free rows (Robin terms), or null
   double* extra64 = nullptr;   // (27, n_nodes)
   float* extra32 = nullptr;

@@ -412,7 +412,7 @@ __global__ void k_tet_sort(int64_t n_nodes, const int64_t* __restrict__ ptr,
 // of the node (elastic) or E_e V_e g_a . sum_b g_b x_b (conduction).
 template <typename T, int NC>
 __global__ void __launch_bounds__(256)
-k_tet_apply(int64_t n_nodes, int64_t n_el, int ncols, T lam0, T mu0,
+k_tet_apply(int64_t n_nodes, int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
             const int32_t* __restrict__ conn, const T* __restrict__ g,
             const T* __restrict__ vol, const int64_t* __restrict__ nptr,
             const int32_t* __restrict__ nel, const uint8_t* __restrict__ nloc,
@@ -439,14 +439,18 @@ k_tet_apply(int64_t n_nodes, int64_t n_el, int ncols, T lam0, T mu0,
       const T ga1 = a == 0 ? gg[0][1] : (a == 1 ? gg[1][1] : (a == 2 ? gg[2][1] : gg[3][1]));
       const T ga2 = a == 0 ? gg[0][2] : (a == 1 ? gg[1][2] : (a == 2 ? gg[2][2] : gg[3][2]));
       if (NC == 1) {
-        T s = T(0);
+        T s = T(0), xs = T(0), xa = T(0);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
           const int64_t nb = conn[b * n_el + e];
           const T xb = X[nb * ncols + col];
           s += (ga0 * gg[b][0] + ga1 * gg[b][1] + ga2 * gg[b][2]) * xb;
+          xs += xb;
+          xa = b == a ? xb : xa;
         }
-        acc[0] += w * s;
+        // scalar operator a M + b K: P1 mass V / 20 (1 + delta_ab)
+        if (ma == T(0) && cb == T(1)) acc[0] += w * s;
+        else acc[0] += w * (cb * s + ma * ((xs + xa) * T(0.05)));
       } else {
         T exx = 0, eyy = 0, ezz = 0, gyz = 0, gxz = 0, gxy = 0;
 #pragma unroll
@@ -479,7 +483,7 @@ k_tet_apply(int64_t n_nodes, int64_t n_el, int ncols, T lam0, T mu0,
 }
 
 template <typename T, int NC>
-__global__ void k_tet_diag(int64_t n_nodes, int64_t n_el, T lam0, T mu0,
+__global__ void k_tet_diag(int64_t n_nodes, int64_t n_el, T lam0, T mu0, T ma, T cb,
                            const T* __restrict__ g, const T* __restrict__ vol,
                            const int64_t* __restrict__ nptr, const int32_t* __restrict__ nel,
                            const uint8_t* __restrict__ nloc, const T* __restrict__ E,
@@ -497,7 +501,8 @@ __global__ void k_tet_diag(int64_t n_nodes, int64_t n_el, T lam0, T mu0,
       const T w = E[e] * vol[e];
       const T g2 = gx * gx + gy * gy + gz * gz;
       if (NC == 1) {
-        acc[0] += w * g2;
+        if (ma == T(0) && cb == T(1)) acc[0] += w * g2;
+        else acc[0] += w * (cb * g2 + ma * T(0.1));
       } else {
         acc[0] += w * (mu0 * g2 + (lam0 + mu0) * gx * gx);
         acc[NC > 1 ? 1 : 0] += w * (mu0 * g2 + (lam0 + mu0) * gy * gy);
@@ -575,11 +580,13 @@ static int tet_apply_t(skt_operator* op, int ncols, const T* g, const T* vol, co
   const int64_t total = op->n_nodes * ncols;
   if (op->ncomp == 3)
     hipLaunchKernelGGL((k_tet_apply<T, 3>), dim3(grid_for(total)), dim3(kBlock), 0, s,
-                       op->n_nodes, op->n_el, ncols, (T)op->lam0, (T)op->mu0, op->conn, g,
+                       op->n_nodes, op->n_el, ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
+                       (T)op->cond_b, op->conn, g,
                        vol, op->nptr, op->nel, op->nloc, E, X, op->fixed, Y);
   else
     hipLaunchKernelGGL((k_tet_apply<T, 1>), dim3(grid_for(total)), dim3(kBlock), 0, s,
-                       op->n_nodes, op->n_el, ncols, (T)op->lam0, (T)op->mu0, op->conn, g,
+                       op->n_nodes, op->n_el, ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
+                       (T)op->cond_b, op->conn, g,
                        vol, op->nptr, op->nel, op->nloc, E, X, op->fixed, Y);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
@@ -657,11 +664,13 @@ static int diag_t(skt_operator* op, const T* K0, const T* g, const T* vol, const
   } else {
     if (op->ncomp == 3)
       hipLaunchKernelGGL((k_tet_diag<T, 3>), dim3(grid_for(op->n_nodes)), dim3(kBlock), 0, s,
-                         op->n_nodes, op->n_el, (T)op->lam0, (T)op->mu0, g, vol, op->nptr,
+                         op->n_nodes, op->n_el, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
+                         (T)op->cond_b, g, vol, op->nptr,
                          op->nel, op->nloc, E, op->fixed, d);
     else
       hipLaunchKernelGGL((k_tet_diag<T, 1>), dim3(grid_for(op->n_nodes)), dim3(kBlock), 0, s,
-                         op->n_nodes, op->n_el, (T)op->lam0, (T)op->mu0, g, vol, op->nptr,
+                         op->n_nodes, op->n_el, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
+                         (T)op->cond_b, g, vol, op->nptr,
                          op->nel, op->nloc, E, op->fixed, d);
   }
   SKT_LAUNCH_CHECK();
@@ -871,7 +880,55 @@ int skt_tet4_create(int physics, int64_t n_nodes, int64_t n_el, const double* co
   (void)hipFree(cnt);
   SKT_HIP(hipMalloc(&op->fixed, op->ndof));
   SKT_HIP(hipMemcpy(op->fixed, dof_fixed, op->ndof, hipMemcpyDeviceToDevice));
+  {
+    std::vector<uint8_t> hf(op->ndof);
+    SKT_HIP(hipMemcpy(hf.data(), dof_fixed, op->ndof, hipMemcpyDeviceToHost));
+    int64_t nf = 0;
+    for (int64_t i = 0; i < op->ndof; ++i) nf += hf[i] ? 1 : 0;
+    op->n_fixed = nf;
+  }
   *out = op;
+  return SKT_OK;
+}
+
+int skt_tet4_set_grid(skt_operator* op, int64_t nx, int64_t ny, int64_t nz) {
+  SKT_CHECK(op && op->mesh == SKT_MESH_TET4, "a tet4 operator is required");
+  SKT_CHECK(nx >= 1 && ny >= 1 && nz >= 1, "grid dimensions must be >= 1");
+  const int64_t nnx = nx + 1, nny = ny + 1;
+  SKT_CHECK(op->n_nodes == nnx * nny * (nz + 1) && op->n_el == 6 * nx * ny * nz,
+            "the mesh is not a grid of nx*ny*nz cells split into six tets each");
+  // every tet 6 c + s has its four nodes among the corners of cell c
+  std::vector<int32_t> hc(4 * op->n_el);
+  SKT_HIP(hipMemcpy(hc.data(), op->conn, sizeof(int32_t) * 4 * op->n_el,
+                    hipMemcpyDeviceToHost));
+  for (int64_t e = 0; e < op->n_el; ++e) {
+    const int64_t c = e / 6;
+    const int64_t cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    for (int q = 0; q < 4; ++q) {
+      const int64_t n = hc[(size_t)q * op->n_el + e];
+      const int64_t i = n % nnx, j = (n / nnx) % nny, k = n / (nnx * nny);
+      SKT_CHECK(i - cx >= 0 && i - cx <= 1 && j - cy >= 0 && j - cy <= 1 && k - cz >= 0 &&
+                    k - cz <= 1,
+                "a tet's nodes are not corners of its grid cell (element 6 c + s in cell c)");
+    }
+  }
+  op->nx = nx;
+  op->ny = ny;
+  op->nz = nz;
+  op->tet_grid = 1;
+  return SKT_OK;
+}
+
+int skt_tet4_create_scalar(int64_t n_nodes, int64_t n_el, const double* coords,
+                           const int32_t* conn, double a_mass, double b_cond,
+                           const uint8_t* node_fixed, skt_operator** out) {
+  SKT_CHECK(out, "null output handle");
+  SKT_CHECK(a_mass >= 0.0 && b_cond >= 0.0 && a_mass + b_cond > 0.0,
+            "a_mass and b_cond must be >= 0 and not both 0");
+  int st = skt_tet4_create(SKT_HEAT, n_nodes, n_el, coords, conn, 0.0, node_fixed, out);
+  if (st) return st;
+  (*out)->mass_a = a_mass;
+  (*out)->cond_b = b_cond;
   return SKT_OK;
 }
 

@@ -688,7 +688,8 @@ static int collect_timing(skt_solver* s, int dtype, int64_t count) {
 template <typename T>
 static int run_setup(skt_solver* s, const double* E64, const Vecs<T>& v, hipStream_t st) {
   MGData& mg = s->mg;
-  const bool steady = mg.warm && mg.Mch && (s->op->n_fixed == 0 || mg.gflag);
+  const bool steady = mg.warm && (s->op->mesh == SKT_MESH_TET4 ||
+                                  (mg.Mch && (s->op->n_fixed == 0 || mg.gflag)));
   if (!s->graphs || s->timing || !steady)
     return mg_setup(&mg, s->op, E64, v.E, v.dinv, st);
   skt_solver::Setup& g = s->setup;
@@ -732,8 +733,9 @@ static int setup_precond(skt_solver* s, const Blk& b, const double* E64, const V
   if (s->precond == SKT_PC_JACOBI) return SKT_OK;
   if (s->mg_possible < 0) {
     std::vector<MGLevel> lv;
-    s->mg_possible = (s->op->mesh == SKT_MESH_HEX8 && s->op->modal_ok &&
-                      mg_plan(s->op, s->mgcfg, &lv)) ? 1 : 0;
+    const bool grid = (s->op->mesh == SKT_MESH_HEX8 && s->op->modal_ok) ||
+                      (s->op->mesh == SKT_MESH_TET4 && s->op->tet_grid);
+    s->mg_possible = (grid && mg_plan(s->op, s->mgcfg, &lv)) ? 1 : 0;
   }
   const bool kc_ok = b.kc <= (s->op->ncomp == 3 ? 2 : 4);
   if (!s->mg_possible || !kc_ok) {

@@ -46,8 +46,13 @@ SIGNATURES = {
     "skt_percentile_abs": [_i64, _vp, _d, _vp, _vp],
     "skt_percentile_abs_noalloc": [_i64, _vp, _d, _vp, _vp],
     "skt_stats4": [_i64, _vp, _vp, _vp],
+    "skt_row_mean": [_i64, _i32, _vp, _vp, _vp],
     "skt_hex8_create": [_i32, _i64, _i64, _i64, _d, _d, _d, _d, _i32, _vp, _pp],
     "skt_tet4_create": [_i32, _i64, _i64, _vp, _vp, _d, _vp, _pp],
+    "skt_tet4_create_scalar": [_i64, _i64, _vp, _vp, _d, _d, _vp, _pp],
+    "skt_tet4_set_grid": [_vp, _i64, _i64, _i64],
+    "skt_tet4_elem_to_node": [_vp, _i32, _vp, _vp, _d, _vp, _vp, _vp],
+    "skt_tet4_node_to_elem": [_vp, _i32, _vp, _i32, _i32, _vp, _vp],
     "skt_operator_destroy": [_vp],
     "skt_operator_info": [_vp, _i64p, _i64p, C.POINTER(C.c_int)],
     "skt_operator_element_matrix": [_vp, _dp],

@@ -182,6 +182,11 @@ class DensityMethod:
         self._completed = False
         self.filter = None
         self.timings = []
+        # device phases of an iteration captured as CUDA graphs (OC with the
+        # sparse-matrix filter); SKT_STEP_GRAPH=0 runs them eagerly
+        self._step_graphs = os.environ.get("SKT_STEP_GRAPH", "1") != "0"
+        self._phases = {}
+        self._cap_stream = None
 
     # --- setup ------------------------------------------------------------
     def add_recorder(self, tsk) -> HistoryCollection:
@@ -351,20 +356,83 @@ class DensityMethod:
                        "percentile", "filter_radius"])
         if filter_radius != self.filter.radius:
             self.filter.update_radius(filter_radius)
+            self._phases.clear()
+        k = tsk.n_tasks
+        # filter -> projection (a replayed graph once captured)
+        self._phase("pre", (float(beta),), lambda: self._step_pre(beta))
+        comp = self.fem.objectives_multi_load(st["rho_projected"], p, st["u_dofs"],
+                                              force_scale=neumann_scale)
+        if "comp" not in st:
+            st["comp"] = torch.empty_like(comp)
+        st["comp"].copy_(comp)
+        # energies -> sensitivities -> filter adjoint -> update -> records
+        # (a replayed graph once captured; the key holds every host scalar
+        # its kernels take)
+        key = (float(p), float(beta), float(move_limit), float(eta), float(vol_frac),
+               percentile if percentile is None else float(percentile), iter_num <= 1)
+        out = self._phase("post", key, lambda: self._step_post(iter_num, p, beta, move_limit,
+                                                                eta, percentile, vol_frac))
+        st["energy"] = out["energy"]
+        names, rec = out["names"], out["rec"]
+        # the previous iteration's records, while the GPU runs this one's
+        # update
+        self._flush_records()
+        ctx = dict(names=names, k=k, neumann_scale=neumann_scale,
+                   solve_iters=self.fem.last_solve.get("iters"),
+                   obj=self._objective_history_name(tsk), host=self._host_records())
+        if cfg.check_convergence:
+            # the stop test needs this iteration's values now
+            converged = self._process_records(rec.cpu().numpy(), ctx)
+        else:
+            # copied asynchronously and fed to the recorder during the next
+            # iteration (or when the run returns), so the host never waits here
+            buf = self._rec_host
+            if buf is None or buf.numel() != rec.numel():
+                buf = self._rec_host = torch.empty(rec.numel(), dtype=rec.dtype,
+                                                   pin_memory=True)
+            buf.copy_(rec, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            self._rec_pending = (buf, ev, ctx)
+            converged = False
+        rt = max(cfg.max_iters // max(cfg.record_times, 1), 1)
+        # save points of common_density.py:1186-1193 (the last iteration of
+        # this call, as the reference does for partial runs)
+        last = getattr(self, "_iter_limit", self._iter_end) - 1
+        if cfg.write_checkpoints and (iter_num % rt == 0 or iter_num == 1 or converged or
+                                      iter_num == last):
+            os.makedirs(f"{cfg.dst_path}/data", exist_ok=True)
+            visualization.export_mesh_with_info(
+                tsk.mesh, cell_data_names=["rho_projected", "energy"],
+                cell_data_values=[st["rho_projected"].cpu().numpy(),
+                                  out["energy_mean"].cpu().numpy()],
+                filepath=cfg.vtu_path(iter_num))
+            np.savez_compressed(f"{cfg.dst_path}/data/{iter_num:06d}-rho.npz",
+                                rho_design_elements=self.rho_host()[tsk.design_elements])
+        return converged
+
+    # --- the iteration's device phases --------------------------------------
+    def _step_pre(self, beta):
+        """rho_prev for the change record, rho_filtered = F(rho), projection."""
+        cfg, st = self.cfg, self._state
+        rho, rf, rp = st["rho"], st["rho_filtered"], st["rho_projected"]
+        self.gather_design(rho, st["rho_design_prev"])
+        self.filter.forward_into(rho, rf)
+        _lib.call("skt_heaviside_project", rho.numel(), _dev.ptr(rf), float(beta),
+                  float(cfg.beta_eta), _dev.ptr(rp), _dev.stream())
+
+    def _step_post(self, iter_num, p, beta, move_limit, eta, percentile, vol_frac) -> dict:
+        """Energies, the sensitivity chain, the filter adjoint, the design
+        update and the record values of one iteration (device work only;
+        ``comp`` is the state's persistent copy of the objectives)."""
+        cfg, tsk, st = self.cfg, self.tsk, self._state
         S = _dev.stream()
         filt = self.filter
         rho, rf, rp = st["rho"], st["rho_filtered"], st["rho_projected"]
         n_el = rho.numel()
         nd = st["rho_design"].numel()
         k = tsk.n_tasks
-        # rho_prev for the change record (design entries only ever change)
-        self.gather_design(rho, st["rho_design_prev"])
-        filt.forward_into(rho, rf)
-        _lib.call("skt_heaviside_project", n_el, _dev.ptr(rf), float(beta), float(cfg.beta_eta),
-                  _dev.ptr(rp), S)
-        comp = self.fem.objectives_multi_load(rp, p, st["u_dofs"], force_scale=neumann_scale)
         energy = self.fem.energy_multi_load(rp, p, st["u_dofs"])
-        st["energy"] = energy
         u_max = self.fem.col_reduce(_lib.RED_MAXABS, st["u_dofs"])
         # sensitivity: the heat FEM always uses the SIMP form (solver_heat.py:945-947)
         kind = _lib.SIMP if isinstance(tsk, LinearHeatConduction) else self.kind
@@ -393,7 +461,9 @@ class DensityMethod:
                       _dev.ptr(G), S)
             filt.gradient_mean_into(G, st["dC_full"])
         if cfg.sensitivity_filter:
-            tmp = torch.empty_like(st["dC_full"])
+            tmp = st.get("dC_tmp")
+            if tmp is None:
+                tmp = st["dC_tmp"] = torch.empty_like(st["dC_full"])
             filt.forward_into(st["dC_full"], tmp)
             st["dC_full"].copy_(tmp)
         self.gather_design(st["dC_full"], st["dC_design"])
@@ -402,19 +472,17 @@ class DensityMethod:
                         None, move_limit, eta, beta, None, None, percentile, self._vol_design,
                         self._vol_design_sum, vol_frac)
         self.scatter_design(st["rho_design"], rho)
-        # the previous iteration's records, while the GPU runs this one's
-        # bisection
-        self._flush_records()
-        # records (one small device->host copy per iteration)
+        # record values, reduced on the device
         sc = st["scal"]
         _lib.call("skt_reduce", _lib.RED_MAXABSDIFF, nd, _dev.ptr(st["rho_design"]),
                   _dev.ptr(st["rho_design_prev"]), _dev.ptr(sc[0:1]), S)
-        # array records as [min, mean, max, std] taken on the device
         rpd = st.get("rp_design")
         if rpd is None:
             rpd = st["rp_design"] = torch.empty_like(st["rho_design"])
         self.gather_design(rp, rpd)
-        arrays = {"rho_projected": rpd, "energy": energy.mean(dim=1)}
+        e_mean = torch.empty(n_el, dtype=torch.float64, device=self.dev)
+        _lib.call("skt_row_mean", n_el, k, _dev.ptr(energy), _dev.ptr(e_mean), S)
+        arrays = {"rho_projected": rpd, "energy": e_mean}
         arrays.update(self._record_arrays(st))
         names = list(arrays)
         # [min, mean, max, std] (population std, as numpy's) per array
@@ -425,39 +493,52 @@ class DensityMethod:
         for i, n in enumerate(names):
             x = arrays[n].reshape(-1).to(torch.float64).contiguous()
             _lib.call("skt_stats4", x.numel(), _dev.ptr(x), _dev.ptr(sbuf[4 * i:4 * i + 4]), S)
-        rec = torch.cat([sc[:8], comp.reshape(-1), u_max.reshape(-1), sbuf])
-        ctx = dict(names=names, k=k, neumann_scale=neumann_scale,
-                   solve_iters=self.fem.last_solve.get("iters"),
-                   obj=self._objective_history_name(tsk), host=self._host_records())
-        if cfg.check_convergence:
-            # the stop test needs this iteration's values now
-            converged = self._process_records(rec.cpu().numpy(), ctx)
-        else:
-            # copied asynchronously and fed to the recorder during the next
-            # iteration (or when the run returns), so the host never waits here
-            buf = self._rec_host
-            if buf is None or buf.numel() != rec.numel():
-                buf = self._rec_host = torch.empty(rec.numel(), dtype=rec.dtype,
-                                                   pin_memory=True)
-            buf.copy_(rec, non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record()
-            self._rec_pending = (buf, ev, ctx)
-            converged = False
-        rt = max(cfg.max_iters // max(cfg.record_times, 1), 1)
-        # save points of common_density.py:1186-1193 (the last iteration of
-        # this call, as the reference does for partial runs)
-        last = getattr(self, "_iter_limit", self._iter_end) - 1
-        if cfg.write_checkpoints and (iter_num % rt == 0 or iter_num == 1 or converged or
-                                      iter_num == last):
-            os.makedirs(f"{cfg.dst_path}/data", exist_ok=True)
-            visualization.export_mesh_with_info(
-                tsk.mesh, cell_data_names=["rho_projected", "energy"],
-                cell_data_values=[rp.cpu().numpy(), arrays["energy"].cpu().numpy()],
-                filepath=cfg.vtu_path(iter_num))
-            np.savez_compressed(f"{cfg.dst_path}/data/{iter_num:06d}-rho.npz",
-                                rho_design_elements=self.rho_host()[tsk.design_elements])
-        return converged
+        rec = torch.cat([sc[:8], st["comp"].reshape(-1), u_max.reshape(-1), sbuf])
+        return dict(energy=energy, energy_mean=e_mean, names=names, rec=rec)
+
+    def _graphable(self) -> bool:
+        """The device phases can be captured when every decision inside them
+        is made on the device: the OC update with the sparse-matrix filter
+        on the elastic FEM (the Helmholtz filter's bisection keeps its
+        bracket on the host; LogMOC reads scalars back)."""
+        from .oc import OC_Optimizer
+        return (self._step_graphs and isinstance(self, OC_Optimizer) and
+                hasattr(self.filter, "handle") and
+                isinstance(self.fem, fea.FEM_SimpLinearElasticity))
+
+    def _phase(self, name, key, fn):
+        """Run a device phase: eagerly the first time a key is seen (and
+        capture it as a CUDA graph for the next iterations), then by graph
+        replay while the key stays the same.  The graph's kernels take the
+        same pointers and host scalars as the eager run, so both give the
+        same bits."""
+        if not self._graphable():
+            return fn()
+        ph = self._phases.get(name)
+        if ph is not None and ph["key"] == key:
+            ph["graph"].replay()
+            return ph["out"]
+        out = fn()
+        self._capture(name, key, fn)
+        return out
+
+    def _capture(self, name, key, fn):
+        s = self._cap_stream
+        if s is None:
+            s = self._cap_stream = torch.cuda.Stream()
+            # the reductions keep per-stream partials, allocated on a
+            # stream's first use: create them before any capture on s
+            one = torch.ones(1, dtype=torch.float64, device=self.dev)
+            res = torch.empty(1, dtype=torch.float64, device=self.dev)
+            s.wait_stream(torch.cuda.current_stream())
+            _lib.call("skt_reduce", _lib.RED_SUM, 1, _dev.ptr(one), None, _dev.ptr(res),
+                      s.cuda_stream)
+            torch.cuda.current_stream().wait_stream(s)
+            torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            out = fn()
+        self._phases[name] = dict(key=key, graph=g, out=out)
 
     def _flush_records(self):
         """Feed the recorder with a pending iteration's records (waits for

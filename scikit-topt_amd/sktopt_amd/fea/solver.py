@@ -102,6 +102,10 @@ class _GpuFEM:
             conn = _dev.to_dev(mesh.t, torch.int32)
             _lib.call("skt_tet4_create", self.physics, mesh.nvertices, mesh.nelements,
                       _dev.ptr(coords), _dev.ptr(conn), nu, _dev.ptr(fixed), C.byref(h))
+            grid = getattr(mesh, "cell_grid", None)
+            if grid is not None:
+                # a grid of cells, six tets each: the multigrid can coarsen it
+                _lib.call("skt_tet4_set_grid", h, *grid)
         self._op = h
         ndof = C.c_int64()
         nel = C.c_int64()

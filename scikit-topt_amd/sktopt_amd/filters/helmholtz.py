@@ -16,6 +16,12 @@ On structured hex8 grids ``M + r^2 K`` and ``M`` are matrix-free scalar
 operators (diagonal in the Walsh-Hadamard corner basis, the same modal
 kernel as the state operator) and the solves use the batched CG with the
 geometric-multigrid preconditioner, warm-started from the previous call.
+On tet4 meshes (``ElementTetP1``) they are the matrix-free P1 operator with
+the exact element mass ``V/20 (1 + delta_ab)`` and conduction
+``V grad N_a . grad N_b``, gathered node by node over the node's elements;
+``M + r^2 K`` is well conditioned (its spectrum is the mass's, shifted by
+r^2 times the Laplacian's), so the Jacobi-preconditioned CG solves it in a
+few dozen iterations.
 """
 from __future__ import annotations
 
@@ -30,9 +36,11 @@ from .. import _dev, _lib
 class HelmholtzFilterNodal:
     def __init__(self, mesh, elements_volume, radius, design_mask=None, rtol=1e-12,
                  precision="fp64"):
-        if getattr(mesh, "kind", None) != "hex8" or not hasattr(mesh, "grid"):
+        kind = getattr(mesh, "kind", None)
+        if not ((kind == "hex8" and hasattr(mesh, "grid")) or kind == "tet4"):
             raise NotImplementedError("the nodal Helmholtz filter is built for structured "
-                                      "hex8 meshes")
+                                      "hex8 grids and tet4 meshes")
+        self.kind = kind
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.mesh = mesh
@@ -41,12 +49,18 @@ class HelmholtzFilterNodal:
         self.design_mask = None if design_mask is None else np.asarray(design_mask, dtype=bool)
         self.rtol = float(rtol)
         self.precision = precision
-        self.grid = tuple(int(v) for v in mesh.grid)
-        self.h = tuple(float(v) for v in mesh.h)
-        nx, ny, nz = self.grid
-        self.n_el = nx * ny * nz
-        self.n_nodes = (nx + 1) * (ny + 1) * (nz + 1)
         dev = _dev.device()
+        if kind == "hex8":
+            self.grid = tuple(int(v) for v in mesh.grid)
+            self.h = tuple(float(v) for v in mesh.h)
+            nx, ny, nz = self.grid
+            self.n_el = nx * ny * nz
+            self.n_nodes = (nx + 1) * (ny + 1) * (nz + 1)
+        else:
+            self.n_el = int(mesh.nelements)
+            self.n_nodes = int(mesh.nvertices)
+            self._coords = _dev.to_dev(mesh.p)
+            self._conn = _dev.to_dev(mesh.t, torch.int32)
         self._vol = _dev.to_dev(np.asarray(elements_volume, dtype=np.float64))
         self._design = None if self.design_mask is None else \
             torch.as_tensor(self.design_mask.astype(np.uint8), device=dev)
@@ -76,10 +90,33 @@ class HelmholtzFilterNodal:
     # --- device objects -----------------------------------------------
     def _scalar_op(self, a, b, fixed):
         h = C.c_void_p()
-        nx, ny, nz = self.grid
-        _lib.call("skt_hex8_create_scalar", nx, ny, nz, *self.h, float(a), float(b),
-                  _dev.ptr(fixed), C.byref(h))
+        if self.kind == "hex8":
+            nx, ny, nz = self.grid
+            _lib.call("skt_hex8_create_scalar", nx, ny, nz, *self.h, float(a), float(b),
+                      _dev.ptr(fixed), C.byref(h))
+        else:
+            _lib.call("skt_tet4_create_scalar", self.n_nodes, self.n_el, _dev.ptr(self._coords),
+                      _dev.ptr(self._conn), float(a), float(b), _dev.ptr(fixed), C.byref(h))
         return h
+
+    def _to_nodes(self, k, fixed_value, x, rn, S):
+        design = _dev.ptr(self._design) if self._design is not None else None
+        if self.kind == "hex8":
+            nx, ny, nz = self.grid
+            _lib.call("skt_hex8_elem_to_node", nx, ny, nz, k, _dev.ptr(self._vol), design,
+                      fixed_value, _dev.ptr(x), _dev.ptr(rn), S)
+        else:
+            _lib.call("skt_tet4_elem_to_node", self._mass, k, _dev.ptr(self._vol), design,
+                      fixed_value, _dev.ptr(x), _dev.ptr(rn), S)
+
+    def _to_elements(self, k, U, clamp, mean_cols, out, S):
+        if self.kind == "hex8":
+            nx, ny, nz = self.grid
+            _lib.call("skt_hex8_node_to_elem", nx, ny, nz, k, _dev.ptr(U), clamp, mean_cols,
+                      _dev.ptr(out), S)
+        else:
+            _lib.call("skt_tet4_node_to_elem", self._mass, k, _dev.ptr(U), clamp, mean_cols,
+                      _dev.ptr(out), S)
 
     def _ensure(self, case, ncols):
         if self._built_radius != self.radius:
@@ -122,12 +159,9 @@ class HelmholtzFilterNodal:
         k = 1 if x.dim() == 1 else int(x.shape[1])
         op, solver, _ = self._ensure(case, k)
         S = _dev.stream()
-        nx, ny, nz = self.grid
         dev = x.device
         rn = torch.empty((self.n_nodes, k), dtype=torch.float64, device=dev)
-        _lib.call("skt_hex8_elem_to_node", nx, ny, nz, k, _dev.ptr(self._vol),
-                  _dev.ptr(self._design) if self._design is not None else None,
-                  1.0 if case == "forward" else 0.0, _dev.ptr(x), _dev.ptr(rn), S)
+        self._to_nodes(k, 1.0 if case == "forward" else 0.0, x, rn, S)
         b = torch.empty((self.n_nodes, k), dtype=torch.float64, device=dev)
         _lib.call("skt_operator_apply", self._mass, _lib.F64, k, _dev.ptr(self._ones64),
                   _dev.ptr(rn), _dev.ptr(b), S)
@@ -146,9 +180,8 @@ class HelmholtzFilterNodal:
             self.rtol, 100000, 0.0, iters, relres, S)
         self.last_solve = dict(case=case, iters=int(iters[0]), relres=max(relres))
         _lib.check(st, "skt_solver_solve")
-        _lib.call("skt_hex8_node_to_elem", nx, ny, nz, k, _dev.ptr(U),
-                  1 if case == "gradient" else 0, 1 if (mean_cols or k == 1) else 0,
-                  _dev.ptr(out), S)
+        self._to_elements(k, U, 1 if case == "gradient" else 0,
+                          1 if (mean_cols or k == 1) else 0, out, S)
         return out
 
     def _run(self, case, v):

@@ -148,7 +148,15 @@ class MeshHex(_Mesh):
 
 
 class MeshTet(_Mesh):
+    """Tetrahedral mesh.  ``cell_grid=(nx, ny, nz)`` records that the nodes
+    are the (nx+1)(ny+1)(nz+1) corners of a grid of cells numbered x-fastest
+    and that element 6 c + s is one of the six tets of cell c (the layout of
+    ``jittered_tet``); the geometric multigrid uses that grid."""
     kind = "tet4"
+
+    def __init__(self, p, t, cell_grid=None):
+        super().__init__(p, t)
+        self.cell_grid = None if cell_grid is None else tuple(int(v) for v in cell_grid)
 
     def elements_volume(self) -> np.ndarray:
         """Signed volumes; a negative one raises (``fea/composer.py:164-180``)."""
@@ -183,4 +191,4 @@ def jittered_tet(nx, ny, nz, jitter=0.2, seed=0, h=1.0) -> MeshTet:
         a = t[1, neg].copy()
         t[1, neg] = t[2, neg]
         t[2, neg] = a
-    return MeshTet(p, t)
+    return MeshTet(p, t, cell_grid=(nx, ny, nz))
