@@ -176,6 +176,33 @@ def time_launches(fn, reps=20, warm=3):
     return 1e3 * e0.elapsed_time(e1) / reps
 
 
+def tet_oc_iterations(n=4):
+    """OC iterations on the tet configuration (100x50x50 cells x 6 jittered
+    tets, 1.5M tets, r = 1.5 Gaussian filter, Emin = 1e-9, vol 0.3, SIMP
+    p = 3, mixed precision, multigrid solve): median of n iterations after
+    one warm-up, each between device synchronisations."""
+    import torch
+    from sktopt_amd import core
+    from sktopt_amd.mesh import toy_problem
+    from sktopt_amd.tools import SchedulerConfig as SC
+    task = toy_problem.cantilever(100, 50, 50, kind="tet4", jitter=0.2, seed=0)
+    cfg = core.OC_Config(dst_path="/tmp/skt_bench_tet", max_iters=n + 3,
+                         p=SC.constant(target_value=3.0), vol_frac=SC.constant(target_value=0.3),
+                         beta=SC.constant(target_value=1.0), filter_type="spacial",
+                         filter_radius=SC.constant(target_value=1.5),
+                         move_limit=SC.constant(target_value=0.2), E_min_coeff=1e-9,
+                         precision="mixed", solver_rtol=1e-8)
+    opt = core.OC_Optimizer(cfg, task)
+    opt.parameterize()
+    times = timed_run(opt, 1, n)
+    solves = [s["iters"] for s in opt.fem.solve_history[-n:]]
+    t = median(times)
+    log(f"tet OC: {1.0 / t:.2f} it/s (median {1e3 * t:.1f} ms), solves {solves}")
+    return dict(config="tet4-100x50x50x6-1load (c1)", kernel="OC iteration (mixed, multigrid)",
+                value=1.0 / t, unit="OC iterations/s", ms_per_step=1e3 * t,
+                mg_pcg_iterations=solves, steps=n, warmup=1)
+
+
 def apply_suite():
     """Rooflines of the operator apply on the other configurations and of
     the CSR filter gather on the tet mesh, each at uniform density 0.3, fp32
@@ -225,6 +252,7 @@ def apply_suite():
                             bytes=b, avg_us=us, achieved_gbs=gbs, frac=gbs / HBM_PEAK_GBS))
             log(f"filter {name}: nnz={nnz} {us:.1f} us, {gbs:.0f} GB/s")
             del f
+            out.append(tet_oc_iterations())
         del fem
         torch.cuda.empty_cache()
     return out

@@ -43,6 +43,9 @@ struct skt_operator {
   // 1: the tet4 mesh is a grid of nx*ny*nz cells, six tets per cell
   // (skt_tet4_set_grid), which lets the multigrid coarsen it
   int tet_grid = 0;
+  // tet4 apply staging: 4 x ncomp nodal forces per element and column
+  void* tet_f = nullptr;
+  size_t tet_f_bytes = 0;
   // extra scalar 27-point stencil added on free rows (Robin terms), or null
   double* extra64 = nullptr;   // (27, n_nodes)
   float* extra32 = nullptr;

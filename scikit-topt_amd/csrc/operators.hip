@@ -482,6 +482,108 @@ k_tet_apply(int64_t n_nodes, int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
   }
 }
 
+// Element-centric tet apply, phase 1: one thread per (element, column)
+// reads the element's connectivity, gradients, volume and modulus once
+// (structure-of-arrays, coalesced), gathers its 4 x NC values and writes its
+// 4 x NC nodal forces F[((a NC + c) ncols + col) n_el + e], each the same
+// expression the node-centric kernel forms for that element.
+template <typename T, int NC>
+__global__ void __launch_bounds__(256)
+k_tet_elem_forces(int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
+                  const int32_t* __restrict__ conn, const T* __restrict__ g,
+                  const T* __restrict__ vol, const T* __restrict__ E,
+                  const T* __restrict__ X, T* __restrict__ F) {
+  const int64_t total = n_el * ncols;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int col = (int)(t / n_el);
+    const int64_t e = t - (int64_t)col * n_el;
+    T gg[4][3];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) gg[b][d] = g[(3 * b + d) * n_el + e];
+    int64_t nb[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) nb[b] = conn[b * n_el + e];
+    const T w = E[e] * vol[e];
+    if (NC == 1) {
+      T x[4];
+      T xs = T(0);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        x[b] = X[nb[b] * ncols + col];
+        xs += x[b];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        T s = T(0);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          s += (gg[a][0] * gg[b][0] + gg[a][1] * gg[b][1] + gg[a][2] * gg[b][2]) * x[b];
+        T f;
+        if (ma == T(0) && cb == T(1)) f = w * s;
+        else f = w * (cb * s + ma * ((xs + x[a]) * T(0.05)));
+        F[((int64_t)a * ncols + col) * n_el + e] = f;
+      }
+    } else {
+      T exx = 0, eyy = 0, ezz = 0, gyz = 0, gxz = 0, gxy = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const T ux = X[(nb[b] * 3 + 0) * ncols + col];
+        const T uy = X[(nb[b] * 3 + 1) * ncols + col];
+        const T uz = X[(nb[b] * 3 + 2) * ncols + col];
+        exx += gg[b][0] * ux;
+        eyy += gg[b][1] * uy;
+        ezz += gg[b][2] * uz;
+        gyz += gg[b][2] * uy + gg[b][1] * uz;
+        gxz += gg[b][2] * ux + gg[b][0] * uz;
+        gxy += gg[b][1] * ux + gg[b][0] * uy;
+      }
+      const T tr = lam0 * (exx + eyy + ezz);
+      const T sxx = tr + 2 * mu0 * exx, syy = tr + 2 * mu0 * eyy, szz = tr + 2 * mu0 * ezz;
+      const T syz = mu0 * gyz, sxz = mu0 * gxz, sxy = mu0 * gxy;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const T ga0 = gg[a][0], ga1 = gg[a][1], ga2 = gg[a][2];
+        F[((int64_t)(a * 3 + 0) * ncols + col) * n_el + e] = w * (ga0 * sxx + ga1 * sxy + ga2 * sxz);
+        F[((int64_t)(a * 3 + 1) * ncols + col) * n_el + e] = w * (ga1 * syy + ga0 * sxy + ga2 * syz);
+        F[((int64_t)(a * 3 + 2) * ncols + col) * n_el + e] = w * (ga2 * szz + ga0 * sxz + ga1 * syz);
+      }
+    }
+  }
+}
+
+// Phase 2: one thread per (node, column) sums its elements' forces in the
+// node -> element list's (ascending element) order; Dirichlet rows copy x.
+template <typename T, int NC>
+__global__ void __launch_bounds__(256)
+k_tet_gather(int64_t n_nodes, int64_t n_el, int ncols, const int64_t* __restrict__ nptr,
+             const int32_t* __restrict__ nel, const uint8_t* __restrict__ nloc,
+             const T* __restrict__ F, const T* __restrict__ X,
+             const uint8_t* __restrict__ fixed, T* __restrict__ Y) {
+  const int64_t total = n_nodes * ncols;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int col = (int)(t % ncols);
+    const int64_t node = t / ncols;
+    T acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = T(0);
+    for (int64_t q = nptr[node]; q < nptr[node + 1]; ++q) {
+      const int64_t e = nel[q];
+      const int a = nloc[q];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] += F[((int64_t)(a * NC + c) * ncols + col) * n_el + e];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int64_t dof = node * NC + c;
+      Y[dof * ncols + col] = fixed[dof] ? X[dof * ncols + col] : acc[c];
+    }
+  }
+}
+
 template <typename T, int NC>
 __global__ void k_tet_diag(int64_t n_nodes, int64_t n_el, T lam0, T mu0, T ma, T cb,
                            const T* __restrict__ g, const T* __restrict__ vol,
@@ -578,6 +680,29 @@ template <typename T>
 static int tet_apply_t(skt_operator* op, int ncols, const T* g, const T* vol, const T* E,
                        const T* X, T* Y, hipStream_t s) {
   const int64_t total = op->n_nodes * ncols;
+  const size_t fbytes = (size_t)op->n_el * 4 * op->ncomp * ncols * sizeof(T);
+  if (op->tet_f && fbytes <= op->tet_f_bytes) {
+    // element forces, then the node gather
+    T* F = (T*)op->tet_f;
+    const int64_t te = op->n_el * ncols;
+    if (op->ncomp == 3) {
+      hipLaunchKernelGGL((k_tet_elem_forces<T, 3>), dim3(grid_for(te)), dim3(kBlock), 0, s,
+                         op->n_el, ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
+                         (T)op->cond_b, op->conn, g, vol, E, X, F);
+      hipLaunchKernelGGL((k_tet_gather<T, 3>), dim3(grid_for(total)), dim3(kBlock), 0, s,
+                         op->n_nodes, op->n_el, ncols, op->nptr, op->nel, op->nloc,
+                         (const T*)F, X, op->fixed, Y);
+    } else {
+      hipLaunchKernelGGL((k_tet_elem_forces<T, 1>), dim3(grid_for(te)), dim3(kBlock), 0, s,
+                         op->n_el, ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
+                         (T)op->cond_b, op->conn, g, vol, E, X, F);
+      hipLaunchKernelGGL((k_tet_gather<T, 1>), dim3(grid_for(total)), dim3(kBlock), 0, s,
+                         op->n_nodes, op->n_el, ncols, op->nptr, op->nel, op->nloc,
+                         (const T*)F, X, op->fixed, Y);
+    }
+    SKT_LAUNCH_CHECK();
+    return SKT_OK;
+  }
   if (op->ncomp == 3)
     hipLaunchKernelGGL((k_tet_apply<T, 3>), dim3(grid_for(total)), dim3(kBlock), 0, s,
                        op->n_nodes, op->n_el, ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
@@ -887,6 +1012,9 @@ int skt_tet4_create(int physics, int64_t n_nodes, int64_t n_el, const double* co
     for (int64_t i = 0; i < op->ndof; ++i) nf += hf[i] ? 1 : 0;
     op->n_fixed = nf;
   }
+  // element-force staging of the apply: one float64 column (two float32)
+  op->tet_f_bytes = (size_t)n_el * 4 * op->ncomp * sizeof(double);
+  SKT_HIP(hipMalloc(&op->tet_f, op->tet_f_bytes));
   *out = op;
   return SKT_OK;
 }
@@ -948,6 +1076,7 @@ int skt_operator_destroy(skt_operator* op) {
   (void)hipFree(op->fixed_list);
   (void)hipFree(op->extra64);
   (void)hipFree(op->extra32);
+  (void)hipFree(op->tet_f);
   delete op;
   return SKT_OK;
 }
