@@ -1772,8 +1772,10 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     MGLevel& v = mg->lv[0];
     const int64_t total = v.ndof * k;
     const double* cf = mg->coef;
-    if (deg >= 2) {
-      // steps 0 and 1 in one pass (step 0's iterate is formed on the fly)
+    if (deg == 2) {
+      // steps 0 and 1 in one pass (step 0's iterate is formed on the fly
+      // and never stored, so this form serves degree 2 only: a third step
+      // would need it as its previous iterate)
       if ((stt = hex_cheb0_modal<T>(op, k, KC, coef0, E0, r, oth, (const T*)dinv0, cf, cf + 2,
                                     s, stop)))
         return stt;
@@ -1781,12 +1783,14 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     } else {
       hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
                          v.ndof, KC, cf, r, (const T*)dinv0, cur);
-    }
-    for (int st = 2; st < deg; ++st) {
-      if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, (const T*)dinv0,
-                                   cf + 2 * st, 0, s, nullptr, stop)))
-        return stt;
-      std::swap(cur, oth);
+      // step st reads its previous iterate from the buffer it overwrites:
+      // zero for step 1, the iterate of step st - 2 after that
+      for (int st = 1; st < deg; ++st) {
+        if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, (const T*)dinv0,
+                                     cf + 2 * st, st == 1 ? 1 : 0, s, nullptr, stop)))
+          return stt;
+        std::swap(cur, oth);
+      }
     }
     if ((stt = hex_resid_modal<T>(op, k, KC, coef0, E0, cur, r, w0, s, stop))) return stt;
     launch_restrict<T, NC, KC>(v, mg->lv[1], groups, (const T*)w0, (const T*)nullptr, fixed0,

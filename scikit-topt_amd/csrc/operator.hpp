@@ -43,9 +43,12 @@ struct skt_operator {
   // 1: the tet4 mesh is a grid of nx*ny*nz cells, six tets per cell
   // (skt_tet4_set_grid), which lets the multigrid coarsen it
   int tet_grid = 0;
-  // tet4 apply staging: 4 x ncomp nodal forces per element and column
+  // tet4 apply staging: ncomp nodal forces per (element, corner) pair and
+  // column, stored in the node -> element lists' order; tet_slot[a n_el + e]
+  // is the position of (e, a) in those lists
   void* tet_f = nullptr;
   size_t tet_f_bytes = 0;
+  int32_t* tet_slot = nullptr;
   // extra scalar 27-point stencil added on free rows (Robin terms), or null
   double* extra64 = nullptr;   // (27, n_nodes)
   float* extra32 = nullptr;

@@ -408,6 +408,16 @@ __global__ void k_tet_sort(int64_t n_nodes, const int64_t* __restrict__ ptr,
   }
 }
 
+// position of (element, corner) in the sorted node -> element lists
+__global__ void k_tet_slots(int64_t n_nodes, int64_t n_el, const int64_t* __restrict__ ptr,
+                            const int32_t* __restrict__ nel, const uint8_t* __restrict__ nloc,
+                            int32_t* __restrict__ slot) {
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < n_nodes;
+       n += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t q = ptr[n]; q < ptr[n + 1]; ++q)
+      slot[(int64_t)nloc[q] * n_el + nel[q]] = (int32_t)q;
+}
+
 // Node-centric tet gather: y_a += E_e V_e B_a^T D B_e x_e over the elements
 // of the node (elastic) or E_e V_e g_a . sum_b g_b x_b (conduction).
 template <typename T, int NC>
@@ -490,9 +500,10 @@ k_tet_apply(int64_t n_nodes, int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
 template <typename T, int NC>
 __global__ void __launch_bounds__(256)
 k_tet_elem_forces(int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
-                  const int32_t* __restrict__ conn, const T* __restrict__ g,
-                  const T* __restrict__ vol, const T* __restrict__ E,
-                  const T* __restrict__ X, T* __restrict__ F) {
+                  const int32_t* __restrict__ conn, const int32_t* __restrict__ slot,
+                  const T* __restrict__ g, const T* __restrict__ vol,
+                  const T* __restrict__ E, const T* __restrict__ X, T* __restrict__ F) {
+  const int64_t ns = 4 * n_el;
   const int64_t total = n_el * ncols;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -524,7 +535,7 @@ k_tet_elem_forces(int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
         T f;
         if (ma == T(0) && cb == T(1)) f = w * s;
         else f = w * (cb * s + ma * ((xs + x[a]) * T(0.05)));
-        F[((int64_t)a * ncols + col) * n_el + e] = f;
+        F[(int64_t)col * ns + slot[a * n_el + e]] = f;
       }
     } else {
       T exx = 0, eyy = 0, ezz = 0, gyz = 0, gxz = 0, gxy = 0;
@@ -546,40 +557,53 @@ k_tet_elem_forces(int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         const T ga0 = gg[a][0], ga1 = gg[a][1], ga2 = gg[a][2];
-        F[((int64_t)(a * 3 + 0) * ncols + col) * n_el + e] = w * (ga0 * sxx + ga1 * sxy + ga2 * sxz);
-        F[((int64_t)(a * 3 + 1) * ncols + col) * n_el + e] = w * (ga1 * syy + ga0 * sxy + ga2 * syz);
-        F[((int64_t)(a * 3 + 2) * ncols + col) * n_el + e] = w * (ga2 * szz + ga0 * sxz + ga1 * syz);
+        const int64_t q = slot[a * n_el + e];
+        F[(int64_t)(0 * ncols + col) * ns + q] = w * (ga0 * sxx + ga1 * sxy + ga2 * sxz);
+        F[(int64_t)(1 * ncols + col) * ns + q] = w * (ga1 * syy + ga0 * sxy + ga2 * syz);
+        F[(int64_t)(2 * ncols + col) * ns + q] = w * (ga2 * szz + ga0 * sxz + ga1 * syz);
       }
     }
   }
 }
 
-// Phase 2: one thread per (node, column) sums its elements' forces in the
-// node -> element list's (ascending element) order; Dirichlet rows copy x.
-template <typename T, int NC>
+// Phase 2: LPN lanes per (node, column) read the node's contiguous run of
+// staged forces (lane l takes entries l, l + LPN, ...) and combine their
+// partial sums by a fixed butterfly, so the result does not depend on
+// timing; Dirichlet rows copy x.
+template <typename T, int NC, int LPN>
 __global__ void __launch_bounds__(256)
-k_tet_gather(int64_t n_nodes, int64_t n_el, int ncols, const int64_t* __restrict__ nptr,
-             const int32_t* __restrict__ nel, const uint8_t* __restrict__ nloc,
+k_tet_gather(int64_t n_nodes, int64_t ns, int ncols, const int64_t* __restrict__ nptr,
              const T* __restrict__ F, const T* __restrict__ X,
              const uint8_t* __restrict__ fixed, T* __restrict__ Y) {
-  const int64_t total = n_nodes * ncols;
+  static_assert(LPN >= 1 && LPN <= 64 && (LPN & (LPN - 1)) == 0, "LPN: a power of two");
+  // the grid stride is a multiple of the block size, itself a multiple of
+  // LPN, so the lanes of one group stay together in every pass
+  const int64_t total = n_nodes * ncols * LPN;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    const int col = (int)(t % ncols);
-    const int64_t node = t / ncols;
+    const int lane = (int)(t & (LPN - 1));
+    const int64_t grp = t / LPN;
+    const int col = (int)(grp % ncols);
+    const int64_t node = grp / ncols;
     T acc[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) acc[c] = T(0);
-    for (int64_t q = nptr[node]; q < nptr[node + 1]; ++q) {
-      const int64_t e = nel[q];
-      const int a = nloc[q];
+    const int64_t q1 = nptr[node + 1];
+    for (int64_t q = nptr[node] + lane; q < q1; q += LPN) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) acc[c] += F[((int64_t)(a * NC + c) * ncols + col) * n_el + e];
+      for (int c = 0; c < NC; ++c) acc[c] += F[(int64_t)(c * ncols + col) * ns + q];
     }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int64_t dof = node * NC + c;
-      Y[dof * ncols + col] = fixed[dof] ? X[dof * ncols + col] : acc[c];
+    for (int off = LPN / 2; off >= 1; off >>= 1) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] += __shfl_xor(acc[c], off, LPN);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int64_t dof = node * NC + c;
+        Y[dof * ncols + col] = fixed[dof] ? X[dof * ncols + col] : acc[c];
+      }
     }
   }
 }
@@ -682,23 +706,24 @@ static int tet_apply_t(skt_operator* op, int ncols, const T* g, const T* vol, co
   const int64_t total = op->n_nodes * ncols;
   const size_t fbytes = (size_t)op->n_el * 4 * op->ncomp * ncols * sizeof(T);
   if (op->tet_f && fbytes <= op->tet_f_bytes) {
-    // element forces, then the node gather
+    // element forces, then the node gather (8 lanes per node and column)
+    constexpr int kLpn = 8;
     T* F = (T*)op->tet_f;
     const int64_t te = op->n_el * ncols;
+    const int64_t tg = total * kLpn;
+    const int64_t ns = 4 * op->n_el;
     if (op->ncomp == 3) {
       hipLaunchKernelGGL((k_tet_elem_forces<T, 3>), dim3(grid_for(te)), dim3(kBlock), 0, s,
                          op->n_el, ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
-                         (T)op->cond_b, op->conn, g, vol, E, X, F);
-      hipLaunchKernelGGL((k_tet_gather<T, 3>), dim3(grid_for(total)), dim3(kBlock), 0, s,
-                         op->n_nodes, op->n_el, ncols, op->nptr, op->nel, op->nloc,
-                         (const T*)F, X, op->fixed, Y);
+                         (T)op->cond_b, op->conn, op->tet_slot, g, vol, E, X, F);
+      hipLaunchKernelGGL((k_tet_gather<T, 3, kLpn>), dim3(grid_for(tg)), dim3(kBlock), 0, s,
+                         op->n_nodes, ns, ncols, op->nptr, (const T*)F, X, op->fixed, Y);
     } else {
       hipLaunchKernelGGL((k_tet_elem_forces<T, 1>), dim3(grid_for(te)), dim3(kBlock), 0, s,
                          op->n_el, ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
-                         (T)op->cond_b, op->conn, g, vol, E, X, F);
-      hipLaunchKernelGGL((k_tet_gather<T, 1>), dim3(grid_for(total)), dim3(kBlock), 0, s,
-                         op->n_nodes, op->n_el, ncols, op->nptr, op->nel, op->nloc,
-                         (const T*)F, X, op->fixed, Y);
+                         (T)op->cond_b, op->conn, op->tet_slot, g, vol, E, X, F);
+      hipLaunchKernelGGL((k_tet_gather<T, 1, kLpn>), dim3(grid_for(tg)), dim3(kBlock), 0, s,
+                         op->n_nodes, ns, ncols, op->nptr, (const T*)F, X, op->fixed, Y);
     }
     SKT_LAUNCH_CHECK();
     return SKT_OK;
@@ -1012,9 +1037,18 @@ int skt_tet4_create(int physics, int64_t n_nodes, int64_t n_el, const double* co
     for (int64_t i = 0; i < op->ndof; ++i) nf += hf[i] ? 1 : 0;
     op->n_fixed = nf;
   }
-  // element-force staging of the apply: one float64 column (two float32)
-  op->tet_f_bytes = (size_t)n_el * 4 * op->ncomp * sizeof(double);
-  SKT_HIP(hipMalloc(&op->tet_f, op->tet_f_bytes));
+  // element-force staging of the apply: one float64 column (two float32),
+  // in slot order (slots are int32: meshes with 4 n_el >= 2^31 use the
+  // node-centric apply)
+  if (4 * n_el < (int64_t)INT32_MAX) {
+    SKT_HIP(hipMalloc(&op->tet_slot, sizeof(int32_t) * 4 * n_el));
+    hipLaunchKernelGGL(k_tet_slots, dim3(grid_for(n_nodes)), dim3(kBlock), 0, 0, n_nodes, n_el,
+                       op->nptr, op->nel, op->nloc, op->tet_slot);
+    SKT_LAUNCH_CHECK();
+    SKT_HIP(hipDeviceSynchronize());
+    op->tet_f_bytes = (size_t)n_el * 4 * op->ncomp * sizeof(double);
+    SKT_HIP(hipMalloc(&op->tet_f, op->tet_f_bytes));
+  }
   *out = op;
   return SKT_OK;
 }
@@ -1077,6 +1111,7 @@ int skt_operator_destroy(skt_operator* op) {
   (void)hipFree(op->extra64);
   (void)hipFree(op->extra32);
   (void)hipFree(op->tet_f);
+  (void)hipFree(op->tet_slot);
   delete op;
   return SKT_OK;
 }

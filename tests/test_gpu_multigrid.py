@@ -71,6 +71,31 @@ def test_mg_cuts_iterations(gpu, precision):
     assert its["mg"] * 8 <= its["jacobi"], its
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_mg_smoothing_degrees(gpu, precision):
+    """Chebyshev degrees 1..4 (the fine level's first steps take a different
+    kernel path for degree 1, 2 and >= 3): every degree solves to the direct
+    solution, and more smoothing never needs more PCG iterations."""
+    task = oprob.cantilever(24, 12, 12)
+    rng = np.random.default_rng(11)
+    rho = np.where(rng.uniform(size=task.n_elements) < 0.5, 1.0, 0.02)
+    ofe = ofem.ElasticFEM(task, 1e-9)
+    Uo = np.zeros((task.ndof, 1))
+    co = ofe.objectives_multi_load(rho, 3.0, Uo)
+    its = {}
+    for degree in (1, 2, 3, 4):
+        fem = _fem(task, "elastic", precision, 1e-10, "mg", degree=degree)
+        U = np.zeros((task.ndof, 1))
+        c = fem.objectives_multi_load(rho, 3.0, U)
+        assert fem.last_solve["preconditioner"] == "mg" and fem.last_solve["converged"]
+        np.testing.assert_allclose(c, co, rtol=1e-8, err_msg=f"degree {degree}")
+        assert np.abs(U - Uo).max() <= 1e-6 * np.abs(Uo).max(), degree
+        its[degree] = fem.last_solve["iters"]
+    print(f"[mg degrees {precision}] iterations {its}", flush=True)
+    for d in (2, 3, 4):
+        assert its[d] <= its[d - 1], its
+
+
 def test_mg_multiload_columns_independent(gpu):
     """Each column of a blocked multi-load MG solve equals its own solve."""
     task = oprob.multiload(16, 8, 8, n_loads=4, seed=3)
