@@ -1275,9 +1275,19 @@ __global__ void k_cheb_coef(int nlev, int degree, double ratio, const double* __
 static size_t tsize(int dtype) { return dtype == SKT_F64 ? sizeof(double) : sizeof(float); }
 
 void mg_free(MGData* mg) {
-  // the set-up's side work is done
+  // the set-up's side work and any background inverse are done
   if (mg->side) (void)hipStreamSynchronize(mg->side);
   if (mg->side2) (void)hipStreamSynchronize(mg->side2);
+  if (mg->gjs) {
+    (void)hipStreamSynchronize(mg->gjs);
+    (void)hipStreamDestroy(mg->gjs);
+    (void)hipEventDestroy(mg->ev_asm);
+    (void)hipEventDestroy(mg->ev_gj);
+    mg->gjs = nullptr;
+    mg->ev_asm = mg->ev_gj = nullptr;
+  }
+  mg->have_inv = 0;
+  mg->inv_pending = 0;
   for (MGLevel& l : mg->lv) {
     (void)hipFree(l.Kel); (void)hipFree(l.st); (void)hipFree(l.dinv);
     (void)hipFree(l.b); (void)hipFree(l.x); (void)hipFree(l.w); (void)hipFree(l.d);
@@ -1335,6 +1345,11 @@ static int warm_power_steps(const MGData* mg) {
 }
 
 int mg_warm_steps(const MGData* mg) { return warm_power_steps(mg); }
+
+static int lag_enabled() {
+  const char* v = std::getenv("SKT_MG_LAG");
+  return (v && v[0] == '1' && v[1] == 0) ? 1 : 0;
+}
 
 int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype, int max_cols) {
   if (mg->dtype == dtype && mg->max_cols == max_cols) return SKT_OK;
@@ -1401,6 +1416,7 @@ int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype,
   mg->degree = std::max(cfg.degree, 1);
   mg->ratio = cfg.ratio;
   mg->power_iters = cfg.power_iters;
+  mg->lag = lag_enabled();
   return SKT_OK;
 }
 
@@ -1465,6 +1481,60 @@ static int level_cheb(MGData* mg, const skt_operator* op, int l, int k, int kc, 
   oa.prev_zero = prev_zero;
   oa.stop = stop;
   return level_apply<T, ST_CHEB>(mg, op, nullptr, l, k, kc, x, xout, s, oa);
+}
+
+// In-place blocked Gauss-Jordan inverse of the coarsest matrix in Ad, on
+// stream g (pivot block inverses alternate between the two halves of Ad2).
+static void queue_gj(MGData* mg, hipStream_t g) {
+  const int64_t n = mg->lv.back().ndof;
+  const unsigned tiles = (unsigned)((n + kTile - 1) / kTile);
+  const unsigned strips = (unsigned)((n + kTileRC - 1) / kTileRC);
+  hipLaunchKernelGGL(k_gj_block_inv, dim3(1), dim3(256), 0, g, n, (int64_t)0,
+                     (int)std::min<int64_t>(kGJ, n), mg->Ad, mg->Ad2);
+  int par = 0;
+  for (int64_t b0 = 0; b0 < n; b0 += kGJ, par ^= 1) {
+    const int nb = (int)std::min<int64_t>(kGJ, n - b0);
+    const int64_t b1 = b0 + kGJ;
+    const int nb1 = b1 < n ? (int)std::min<int64_t>(kGJ, n - b1) : 0;
+    double* bi = mg->Ad2 + par * kGJ * kGJ;
+    double* bi1 = mg->Ad2 + (par ^ 1) * kGJ * kGJ;
+    hipLaunchKernelGGL(k_gj_rows, dim3(strips), dim3(256), 0, g, n, b0, nb, bi, mg->Ad);
+    hipLaunchKernelGGL(k_gj_update, dim3(tiles, tiles), dim3(256), 0, g, n, b0, nb, mg->Ad);
+    hipLaunchKernelGGL(k_gj_cols_next, dim3(strips + 1), dim3(256), 0, g, n, b0, nb,
+                       (const double*)bi, b1, nb1, bi1, mg->Ad);
+  }
+}
+
+bool mg_lagged(const MGData* mg) { return mg->lag && mg->have_inv; }
+
+int mg_lag_pre(MGData* mg, hipStream_t s) {
+  if (!mg->inv_pending) return SKT_OK;
+  const int64_t n = mg->lv.back().ndof;
+  SKT_HIP(hipStreamWaitEvent(s, mg->ev_gj, 0));
+  if (mg->dtype == SKT_F64)
+    hipLaunchKernelGGL(k_cast<double>, dim3(grid_for(n * n)), dim3(kBlock), 0, s, n * n,
+                       (const double*)mg->Ad, (double*)mg->Ainv);
+  else
+    hipLaunchKernelGGL(k_cast<float>, dim3(grid_for(n * n)), dim3(kBlock), 0, s, n * n,
+                       (const double*)mg->Ad, (float*)mg->Ainv);
+  SKT_LAUNCH_CHECK();
+  mg->inv_pending = 0;
+  return SKT_OK;
+}
+
+int mg_lag_post(MGData* mg, hipStream_t s) {
+  if (!mg->gjs) {
+    SKT_HIP(hipStreamCreateWithFlags(&mg->gjs, hipStreamNonBlocking));
+    SKT_HIP(hipEventCreateWithFlags(&mg->ev_asm, hipEventDisableTiming));
+    SKT_HIP(hipEventCreateWithFlags(&mg->ev_gj, hipEventDisableTiming));
+  }
+  SKT_HIP(hipEventRecord(mg->ev_asm, s));
+  SKT_HIP(hipStreamWaitEvent(mg->gjs, mg->ev_asm, 0));
+  queue_gj(mg, mg->gjs);
+  SKT_LAUNCH_CHECK();
+  SKT_HIP(hipEventRecord(mg->ev_gj, mg->gjs));
+  mg->inv_pending = 1;
+  return SKT_OK;
 }
 
 template <typename T>
@@ -1590,6 +1660,9 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   // coarsest: dense matrix, blocked Gauss-Jordan inverse, cast to T (side
   // stream).  Enqueued after the fine level's power iteration, so that the
   // host's many small launches for it do not hold back the main stream.
+  // With a lagged inverse (mg_lagged) only the matrix is assembled here;
+  // mg_lag_post inverts it in the background after the set-up.
+  const bool lagged = mg_lagged(mg);
   bool coarsest_queued = false;
   auto queue_coarsest = [&]() -> int {
     coarsest_queued = true;
@@ -1602,25 +1675,11 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
       hipLaunchKernelGGL((k_dense_asm<1>), dim3(grid_for(n * n)), dim3(kBlock), 0, g, c.nx, c.ny,
                          c.nz, c.Kel, mg->Ad);
     hipLaunchKernelGGL(k_dense_fix_diag, dim3(grid_for(n)), dim3(kBlock), 0, g, n, mg->Ad);
-    const unsigned tiles = (unsigned)((n + kTile - 1) / kTile);
-    const unsigned strips = (unsigned)((n + kTileRC - 1) / kTileRC);
-    // pivot block inverses alternate between the two halves of Ad2
-    hipLaunchKernelGGL(k_gj_block_inv, dim3(1), dim3(256), 0, g, n, (int64_t)0,
-                       (int)std::min<int64_t>(kGJ, n), mg->Ad, mg->Ad2);
-    int par = 0;
-    for (int64_t b0 = 0; b0 < n; b0 += kGJ, par ^= 1) {
-      const int nb = (int)std::min<int64_t>(kGJ, n - b0);
-      const int64_t b1 = b0 + kGJ;
-      const int nb1 = b1 < n ? (int)std::min<int64_t>(kGJ, n - b1) : 0;
-      double* bi = mg->Ad2 + par * kGJ * kGJ;
-      double* bi1 = mg->Ad2 + (par ^ 1) * kGJ * kGJ;
-      hipLaunchKernelGGL(k_gj_rows, dim3(strips), dim3(256), 0, g, n, b0, nb, bi, mg->Ad);
-      hipLaunchKernelGGL(k_gj_update, dim3(tiles, tiles), dim3(256), 0, g, n, b0, nb, mg->Ad);
-      hipLaunchKernelGGL(k_gj_cols_next, dim3(strips + 1), dim3(256), 0, g, n, b0, nb,
-                         (const double*)bi, b1, nb1, bi1, mg->Ad);
+    if (!lagged) {
+      queue_gj(mg, g);
+      hipLaunchKernelGGL(k_cast<T>, dim3(grid_for(n * n)), dim3(kBlock), 0, g, n * n, mg->Ad,
+                         (T*)mg->Ainv);
     }
-    hipLaunchKernelGGL(k_cast<T>, dim3(grid_for(n * n)), dim3(kBlock), 0, g, n * n, mg->Ad,
-                       (T*)mg->Ainv);
     SKT_LAUNCH_CHECK();
     SKT_HIP(hipEventRecord(mg->ev_join, g));
     return SKT_OK;
@@ -1683,6 +1742,7 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   // returns (a graph capture of the set-up requires it)
   if (!st_joined) SKT_HIP(hipStreamWaitEvent(s, mg->ev_st, 0));
   SKT_HIP(hipStreamWaitEvent(s, mg->ev_join, 0));
+  if (!lagged) mg->have_inv = 1;
   return SKT_OK;
 }
 

@@ -66,6 +66,16 @@ struct MGData {
   int power_iters = 20;               // steps from the hashed start vector
   int power_iters_warm = 5;           // steps from the previous set-up's vector
   uint64_t gen = 0;                   // bumped whenever the buffers are (re)allocated
+  // Lagged coarsest inverse: once an inverse exists, a set-up only assembles
+  // the coarsest matrix; the solve that follows uses the inverse of the
+  // previous set-up's matrix, and the new matrix is inverted on `gjs` during
+  // that solve (mg_lag_post), to be cast into Ainv by the next set-up
+  // (mg_lag_pre).  On with SKT_MG_LAG=1.
+  int lag = 0;
+  int have_inv = 0;                   // Ainv holds an inverse
+  int inv_pending = 0;                // Ad holds, or is becoming, a newer inverse
+  hipStream_t gjs = nullptr;
+  hipEvent_t ev_asm = nullptr, ev_gj = nullptr;
 };
 
 struct MGConfig {
@@ -89,6 +99,15 @@ void mg_free(MGData* mg);
 int mg_warm_steps(const MGData* mg);
 int mg_setup(MGData* mg, const skt_operator* op, const double* E64, const void* E, const void* dinv0,
              hipStream_t s);
+
+// Lagged coarsest inverse (see MGData::lag).  mg_lagged: the next set-up
+// assembles the coarsest matrix only.  mg_lag_pre, before that set-up and
+// outside any stream capture: `s` waits for the background inverse and casts
+// it into Ainv.  mg_lag_post, after it: the assembled matrix is inverted on
+// the multigrid's own stream once `s` reaches this point.
+bool mg_lagged(const MGData* mg);
+int mg_lag_pre(MGData* mg, hipStream_t s);
+int mg_lag_post(MGData* mg, hipStream_t s);
 
 // z = V(r) for ncols columns in the blocked layout [ncols/kc][ndof][kc];
 // w0 is fine-level scratch of the same size.  With dot, the last smoothing

@@ -84,6 +84,7 @@ struct skt_solver {
     const void* E = nullptr;
     const void* dinv = nullptr;
     uint64_t mg_gen = 0;
+    int lagged = -1;
   };
   Setup setup;
   hipStream_t cap = nullptr;
@@ -686,7 +687,8 @@ static int collect_timing(skt_solver* s, int dtype, int64_t count) {
 // The multigrid set-up on `st`: replay of its captured graph once the
 // set-up has reached its steady state, else direct launches.
 template <typename T>
-static int run_setup(skt_solver* s, const double* E64, const Vecs<T>& v, hipStream_t st) {
+static int run_setup_body(skt_solver* s, const double* E64, const Vecs<T>& v, hipStream_t st,
+                          bool lagged) {
   MGData& mg = s->mg;
   const bool steady = mg.warm && (s->op->mesh == SKT_MESH_TET4 ||
                                   (mg.Mch && (s->op->n_fixed == 0 || mg.gflag)));
@@ -696,7 +698,7 @@ static int run_setup(skt_solver* s, const double* E64, const Vecs<T>& v, hipStre
   const int steps = mg_warm_steps(&mg);
   const bool fresh = g.exec && g.dtype == v.dtype && g.degree == mg.degree &&
                      g.steps == steps && g.E64 == E64 && g.E == v.E && g.dinv == v.dinv &&
-                     g.mg_gen == mg.gen;
+                     g.mg_gen == mg.gen && g.lagged == (int)lagged;
   if (!fresh) {
     if (g.exec) {
       SKT_HIP(hipStreamSynchronize(st));
@@ -717,12 +719,25 @@ static int run_setup(skt_solver* s, const double* E64, const Vecs<T>& v, hipStre
     (void)hipGraphDestroy(graph);
     SKT_HIP(e);
     g.dtype = v.dtype; g.degree = mg.degree; g.steps = steps; g.E64 = E64; g.E = v.E;
-    g.dinv = v.dinv; g.mg_gen = mg.gen;
+    g.dinv = v.dinv; g.mg_gen = mg.gen; g.lagged = (int)lagged;
     ++s->graph_captures;
   }
   SKT_HIP(hipGraphLaunch(g.exec, st));
   ++s->graph_launches;
   return SKT_OK;
+}
+
+// The set-up with the lagged coarsest inverse around it: the previous
+// background inverse is cast into place first, and the matrix this set-up
+// assembled is inverted in the background during the solve.
+template <typename T>
+static int run_setup(skt_solver* s, const double* E64, const Vecs<T>& v, hipStream_t st) {
+  MGData& mg = s->mg;
+  const bool lagged = mg_lagged(&mg);
+  int stt = mg_lag_pre(&mg, st);
+  if (stt) return stt;
+  if ((stt = run_setup_body<T>(s, E64, v, st, lagged))) return stt;
+  return lagged ? mg_lag_post(&mg, st) : SKT_OK;
 }
 
 // Pick the preconditioner and, for multigrid, build its hierarchy for E.

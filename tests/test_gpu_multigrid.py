@@ -96,6 +96,42 @@ def test_mg_smoothing_degrees(gpu, precision):
         assert its[d] <= its[d - 1], its
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_mg_lagged_coarsest_inverse(gpu, precision, monkeypatch):
+    """Solves for rho1, rho2, rho2 on one solver.  With the lagged coarsest
+    inverse the second solve preconditions with the inverse of rho1's coarse
+    matrix (computed while nothing waits for it) and still reaches the direct
+    solution; the third has rho2's own inverse again and is bit-identical to
+    the same sequence without the lag."""
+    task = oprob.cantilever(24, 12, 12)
+    rng = np.random.default_rng(21)
+    rho1 = np.where(rng.uniform(size=task.n_elements) < 0.5, 1.0, 0.05)
+    rho2 = np.clip(rho1 + rng.uniform(-0.2, 0.2, task.n_elements), 0.05, 1.0)
+    ofe = ofem.ElasticFEM(task, 1e-9)
+    ref = {}
+    for name, rho in (("1", rho1), ("2", rho2)):
+        Uo = np.zeros((task.ndof, 1))
+        ref[name] = (ofe.objectives_multi_load(rho, 3.0, Uo), Uo)
+    runs = {}
+    for lag in ("1", "0"):
+        monkeypatch.setenv("SKT_MG_LAG", lag)
+        fem = _fem(task, "elastic", precision, 1e-10, "mg")
+        out = []
+        for name, rho in (("1", rho1), ("2", rho2), ("2", rho2)):
+            U = np.zeros((task.ndof, 1))
+            c = fem.objectives_multi_load(rho, 3.0, U)
+            assert fem.last_solve["preconditioner"] == "mg" and fem.last_solve["converged"]
+            np.testing.assert_allclose(c, ref[name][0], rtol=1e-8)
+            assert np.abs(U - ref[name][1]).max() <= 1e-6 * np.abs(ref[name][1]).max()
+            out.append((U, fem.last_solve["iters"]))
+        runs[lag] = out
+    print(f"[mg lag {precision}] iterations lagged {[o[1] for o in runs['1']]}, "
+          f"exact {[o[1] for o in runs['0']]}", flush=True)
+    for i in (0, 2):
+        np.testing.assert_array_equal(runs["1"][i][0], runs["0"][i][0], err_msg=f"solve {i}")
+        assert runs["1"][i][1] == runs["0"][i][1]
+
+
 def test_mg_multiload_columns_independent(gpu):
     """Each column of a blocked multi-load MG solve equals its own solve."""
     task = oprob.multiload(16, 8, 8, n_loads=4, seed=3)
