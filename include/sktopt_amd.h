@@ -235,7 +235,7 @@ int skt_tet4_node_to_elem(const skt_operator* op, int ncols, const double* rho_n
                           hipStream_t stream);
 
 /* Robin terms of the heat path on structured hex8 grids
- * (fea/solver_heat.py:495-624,705-980).
+ * (fea/solver_heat.py:552-627,705-980).
  *
  * Nodal densities as objectives_multi_load (:738-745) and
  * _element_to_nodal_average (:600-613) compute them: np.add.at(rho_nodal,
@@ -246,7 +246,7 @@ int skt_hex8_nodal_average_pairing(int64_t nx, int64_t ny, int64_t nz,
                                    const double* rho_elem, double* rho_node,
                                    double* count, hipStream_t stream);
 
-/* get_robin_virtual (:495-517): the 27-point nodal stencil (27, n_nodes) of
+/* get_robin_virtual (:552-573): the 27-point nodal stencil (27, n_nodes) of
  * int h a |grad rho| u v and the load int h a |grad rho| T_env v, a =
  * rho^p (1 - rho)^q, rho the trilinear interpolant of rho_node, Gauss rule
  * of npts (1..4) points per axis. */
@@ -272,9 +272,9 @@ int skt_hex8_heat_exchange(int64_t nx, int64_t ny, int64_t nz, double hx,
                            double* den, double* num, double* rhs,
                            hipStream_t stream);
 
-/* _robin_compliance_explicit_grad_ (:520-546) per load column of T
+/* _robin_compliance_explicit_grad_ (:575-597) per load column of T
  * (n_nodes, ncols), mapped to elements by _nodal_gradient_to_element_gradient
- * (:616-624): out (n_el, ncols); work: n_nodes doubles. */
+ * (:616-626): out (n_el, ncols); work: n_nodes doubles. */
 int skt_hex8_robin_grad(int64_t nx, int64_t ny, int64_t nz, double hx,
                         double hy, double hz, int npts, const double* rho_node,
                         const double* count, int ncols, const double* T,
@@ -505,6 +505,52 @@ int skt_oc_candidate(int64_t n_design, const int32_t* design_ids,
                      double eta, double rho_min, double rho_max,
                      double rate_min, double rate_max, double move_limit,
                      double* rho_full, double* rho_out, hipStream_t stream);
+
+/* Robin terms of the heat path on tet4 meshes (fea/solver_heat.py:306-624,
+ * 705-980) with P1 fields, for a scalar tet4 operator (its node -> element
+ * lists, gradients and volumes).  order selects the tetrahedral rule of the
+ * volume forms: 1 the centroid, 2 the symmetric 4-point rule of degree 2
+ * (the rule of a P1 basis' default integration order 2).
+ *
+ * Nodal densities of objectives_multi_load (:738-745): np.add.at(rho_nodal,
+ * t.ravel(), np.repeat(rho, 4)) / count, i.e. node n adds rho[(a n_el + e) /
+ * 4] for its corner slots a = 0..3 in that order; count (nullable). */
+int skt_tet4_nodal_average_pairing(const skt_operator* op, const double* rho_elem,
+                                   double* rho_node, double* count, hipStream_t stream);
+
+/* get_robin_virtual (:552-573): elem_mat (10, n_el), the element matrices of
+ * int h a |grad rho| u v packed 00 11 22 33 01 02 03 12 13 23, and load
+ * (n_nodes) of int h a |grad rho| T_env v, a = rho^p (1 - rho)^q; work:
+ * 4 n_el doubles. */
+int skt_tet4_robin_virtual(const skt_operator* op, int order, const double* rho_node,
+                           double p, double q, double h, double T_env, double* elem_mat,
+                           double* load, double* work, hipStream_t stream);
+
+/* Heat-exchange terms (:306-322,380-436,791-832) as skt_hex8_heat_exchange,
+ * exact for P1 fields; work: 4 n_el doubles. */
+int skt_tet4_heat_exchange(const skt_operator* op, const double* rho_node, int ncols,
+                           const double* T, double p, double q, double h, double T_env,
+                           double* den, double* num, double* rhs, double* work,
+                           hipStream_t stream);
+
+/* _robin_compliance_explicit_grad_ (:575-597) per load column of T
+ * (n_nodes, ncols), mapped to elements by _nodal_gradient_to_element_gradient
+ * (:616-626): out (n_el, ncols); work: 4 n_el + n_nodes doubles. */
+int skt_tet4_robin_grad(const skt_operator* op, int order, const double* rho_node,
+                        const double* count, int ncols, const double* T, double p, double q,
+                        double h, double T_env, double* work, double* out, hipStream_t stream);
+
+/* Attach (elem_mat != NULL, copied; (10, n_el) packed as above) or drop
+ * (NULL) extra element matrices on a scalar tet4 operator: applies add R_e x_e
+ * on free rows and the diagonal includes R_aa -- the conduction plus Robin
+ * matrix of solve_multi_load (:198-253); a multigrid hierarchy keeps the
+ * conduction part. */
+int skt_tet4_set_extra(skt_operator* op, const double* elem_mat, hipStream_t stream);
+
+/* out (n_el, ncols) = x_e^T R_e x_e of the attached extra element matrices,
+ * X (n_nodes, ncols): the Robin part of T^T K T (:785-789). */
+int skt_tet4_extra_quadform(const skt_operator* op, int ncols, const double* X, double* out,
+                            hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -122,17 +122,25 @@ def multiload(nx, ny, nz, n_loads=8, seed=0, E0=1.0, nu=0.3):
     return Task(mesh, "elastic", E0, nu, clamp, dofs, forces, np.sort(nodes))
 
 
-def heat_patch(n, patch=16, source=1e-2, k0=1.0, quad_order=2):
-    """Cube of n^3 unit hexes, uniform volumetric source, T = 0 on a
-    ``patch x patch`` node patch centred on the z = 0 face."""
-    mesh = m.structured_hex(n, n, n)
+def heat_patch(n, patch=16, source=1e-2, k0=1.0, quad_order=2, kind="hex8", jitter=0.0,
+               seed=0):
+    """Cube of n^3 unit hexes (or the six-tet split of each, interior nodes
+    jittered), uniform volumetric source, T = 0 on a ``patch x patch`` node
+    patch centred on the z = 0 face."""
+    mesh = m.structured_hex(n, n, n) if kind == "hex8" else m.jittered_tet(n, n, n, jitter, seed)
     i, j, k = _grid_index(n, n, n)
     lo = (n + 1 - patch) // 2
     sel = (k == 0) & (i >= lo) & (i < lo + patch) & (j >= lo) & (j < lo + patch)
     nodes = np.nonzero(sel)[0]
     f = np.zeros(mesh.nnodes)
-    w = el.hex8_load_weights(mesh.h) * source
-    for a in range(8):
-        np.add.at(f, mesh.t[a], w[a])
+    if kind == "hex8":
+        w = el.hex8_load_weights(mesh.h) * source
+        for a in range(8):
+            np.add.at(f, mesh.t[a], w[a])
+    else:
+        # int s N_a over a tet is s V / 4 for each of its nodes
+        vol = m.tet_signed_volumes(mesh.p, mesh.t)
+        for a in range(4):
+            np.add.at(f, mesh.t[a], source * vol / 4.0)
     return Task(mesh, "heat", k0, 0.0, nodes, nodes.copy(), [f], None,
                 dirichlet_values=[0.0], quad_order=quad_order)

@@ -30,7 +30,7 @@ FLAGS = [
 # Elementwise design-variable kernels restate numpy formulas operation by
 # operation; keeping the compiler from fusing a*b+c into one rounding keeps
 # them within the last bit of the float64 host arithmetic.
-NO_CONTRACT = {"design.hip", "filter.hip", "nodal.hip", "robin.hip"}
+NO_CONTRACT = {"design.hip", "filter.hip", "nodal.hip", "robin.hip", "robin_tet.hip"}
 
 
 def sources():

@@ -49,6 +49,11 @@ struct skt_operator {
   void* tet_f = nullptr;
   size_t tet_f_bytes = 0;
   int32_t* tet_slot = nullptr;
+  // scalar tet4 operators: extra symmetric 4 x 4 element matrices (Robin
+  // terms) added on free rows, packed (10, n_el) in the order 00 11 22 33 01
+  // 02 03 12 13 23 (skt_tet4_set_extra), or null
+  double* tet_x64 = nullptr;
+  float* tet_x32 = nullptr;
   // extra scalar 27-point stencil added on free rows (Robin terms), or null
   double* extra64 = nullptr;   // (27, n_nodes)
   float* extra32 = nullptr;

@@ -418,15 +418,24 @@ __global__ void k_tet_slots(int64_t n_nodes, int64_t n_el, const int64_t* __rest
       slot[(int64_t)nloc[q] * n_el + nel[q]] = (int32_t)q;
 }
 
+// Packed index of entry (a, b) of a symmetric 4 x 4 element matrix stored
+// as 00 11 22 33 01 02 03 12 13 23.
+__device__ __forceinline__ int sym10(int a, int b) {
+  if (a == b) return a;
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  return lo == 0 ? 3 + hi : (lo == 1 ? 5 + hi : 9);
+}
+
 // Node-centric tet gather: y_a += E_e V_e B_a^T D B_e x_e over the elements
-// of the node (elastic) or E_e V_e g_a . sum_b g_b x_b (conduction).
+// of the node (elastic) or E_e V_e g_a . sum_b g_b x_b (conduction), plus
+// the extra element matrix R_e (scalar operators with Robin terms).
 template <typename T, int NC>
 __global__ void __launch_bounds__(256)
 k_tet_apply(int64_t n_nodes, int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
             const int32_t* __restrict__ conn, const T* __restrict__ g,
             const T* __restrict__ vol, const int64_t* __restrict__ nptr,
             const int32_t* __restrict__ nel, const uint8_t* __restrict__ nloc,
-            const T* __restrict__ E, const T* __restrict__ X,
+            const T* __restrict__ E, const T* __restrict__ R, const T* __restrict__ X,
             const uint8_t* __restrict__ fixed, T* __restrict__ Y) {
   const int64_t total = n_nodes * ncols;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
@@ -449,7 +458,7 @@ k_tet_apply(int64_t n_nodes, int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
       const T ga1 = a == 0 ? gg[0][1] : (a == 1 ? gg[1][1] : (a == 2 ? gg[2][1] : gg[3][1]));
       const T ga2 = a == 0 ? gg[0][2] : (a == 1 ? gg[1][2] : (a == 2 ? gg[2][2] : gg[3][2]));
       if (NC == 1) {
-        T s = T(0), xs = T(0), xa = T(0);
+        T s = T(0), xs = T(0), xa = T(0), rx = T(0);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
           const int64_t nb = conn[b * n_el + e];
@@ -457,10 +466,12 @@ k_tet_apply(int64_t n_nodes, int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
           s += (ga0 * gg[b][0] + ga1 * gg[b][1] + ga2 * gg[b][2]) * xb;
           xs += xb;
           xa = b == a ? xb : xa;
+          if (R) rx += R[(int64_t)sym10(a, b) * n_el + e] * xb;
         }
         // scalar operator a M + b K: P1 mass V / 20 (1 + delta_ab)
         if (ma == T(0) && cb == T(1)) acc[0] += w * s;
         else acc[0] += w * (cb * s + ma * ((xs + xa) * T(0.05)));
+        if (R) acc[0] += rx;
       } else {
         T exx = 0, eyy = 0, ezz = 0, gyz = 0, gxz = 0, gxy = 0;
 #pragma unroll
@@ -495,14 +506,15 @@ k_tet_apply(int64_t n_nodes, int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
 // Element-centric tet apply, phase 1: one thread per (element, column)
 // reads the element's connectivity, gradients, volume and modulus once
 // (structure-of-arrays, coalesced), gathers its 4 x NC values and writes its
-// 4 x NC nodal forces F[((a NC + c) ncols + col) n_el + e], each the same
-// expression the node-centric kernel forms for that element.
+// 4 x NC nodal forces to F[(c ncols + col) 4 n_el + slot(a, e)], each the
+// same expression the node-centric kernel forms for that element.
 template <typename T, int NC>
 __global__ void __launch_bounds__(256)
 k_tet_elem_forces(int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
                   const int32_t* __restrict__ conn, const int32_t* __restrict__ slot,
                   const T* __restrict__ g, const T* __restrict__ vol,
-                  const T* __restrict__ E, const T* __restrict__ X, T* __restrict__ F) {
+                  const T* __restrict__ E, const T* __restrict__ R,
+                  const T* __restrict__ X, T* __restrict__ F) {
   const int64_t ns = 4 * n_el;
   const int64_t total = n_el * ncols;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
@@ -535,6 +547,12 @@ k_tet_elem_forces(int64_t n_el, int ncols, T lam0, T mu0, T ma, T cb,
         T f;
         if (ma == T(0) && cb == T(1)) f = w * s;
         else f = w * (cb * s + ma * ((xs + x[a]) * T(0.05)));
+        if (R) {
+          T rx = T(0);
+#pragma unroll
+          for (int b = 0; b < 4; ++b) rx += R[(int64_t)sym10(a, b) * n_el + e] * x[b];
+          f += rx;
+        }
         F[(int64_t)col * ns + slot[a * n_el + e]] = f;
       }
     } else {
@@ -613,7 +631,8 @@ __global__ void k_tet_diag(int64_t n_nodes, int64_t n_el, T lam0, T mu0, T ma, T
                            const T* __restrict__ g, const T* __restrict__ vol,
                            const int64_t* __restrict__ nptr, const int32_t* __restrict__ nel,
                            const uint8_t* __restrict__ nloc, const T* __restrict__ E,
-                           const uint8_t* __restrict__ fixed, T* __restrict__ diag) {
+                           const T* __restrict__ R, const uint8_t* __restrict__ fixed,
+                           T* __restrict__ diag) {
   for (int64_t node = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; node < n_nodes;
        node += (int64_t)gridDim.x * blockDim.x) {
     T acc[NC];
@@ -629,6 +648,7 @@ __global__ void k_tet_diag(int64_t n_nodes, int64_t n_el, T lam0, T mu0, T ma, T
       if (NC == 1) {
         if (ma == T(0) && cb == T(1)) acc[0] += w * g2;
         else acc[0] += w * (cb * g2 + ma * T(0.1));
+        if (R) acc[0] += R[(int64_t)a * n_el + e];
       } else {
         acc[0] += w * (mu0 * g2 + (lam0 + mu0) * gx * gx);
         acc[NC > 1 ? 1 : 0] += w * (mu0 * g2 + (lam0 + mu0) * gy * gy);
@@ -701,9 +721,17 @@ static int hex_apply_t(skt_operator* op, int ncols, const T* K0, const T* E, con
 }
 
 template <typename T>
+static const T* tet_extra(const skt_operator* op);
+template <>
+const double* tet_extra<double>(const skt_operator* op) { return op->tet_x64; }
+template <>
+const float* tet_extra<float>(const skt_operator* op) { return op->tet_x32; }
+
+template <typename T>
 static int tet_apply_t(skt_operator* op, int ncols, const T* g, const T* vol, const T* E,
                        const T* X, T* Y, hipStream_t s) {
   const int64_t total = op->n_nodes * ncols;
+  const T* R = tet_extra<T>(op);
   const size_t fbytes = (size_t)op->n_el * 4 * op->ncomp * ncols * sizeof(T);
   if (op->tet_f && fbytes <= op->tet_f_bytes) {
     // element forces, then the node gather (8 lanes per node and column)
@@ -715,13 +743,13 @@ static int tet_apply_t(skt_operator* op, int ncols, const T* g, const T* vol, co
     if (op->ncomp == 3) {
       hipLaunchKernelGGL((k_tet_elem_forces<T, 3>), dim3(grid_for(te)), dim3(kBlock), 0, s,
                          op->n_el, ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
-                         (T)op->cond_b, op->conn, op->tet_slot, g, vol, E, X, F);
+                         (T)op->cond_b, op->conn, op->tet_slot, g, vol, E, R, X, F);
       hipLaunchKernelGGL((k_tet_gather<T, 3, kLpn>), dim3(grid_for(tg)), dim3(kBlock), 0, s,
                          op->n_nodes, ns, ncols, op->nptr, (const T*)F, X, op->fixed, Y);
     } else {
       hipLaunchKernelGGL((k_tet_elem_forces<T, 1>), dim3(grid_for(te)), dim3(kBlock), 0, s,
                          op->n_el, ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
-                         (T)op->cond_b, op->conn, op->tet_slot, g, vol, E, X, F);
+                         (T)op->cond_b, op->conn, op->tet_slot, g, vol, E, R, X, F);
       hipLaunchKernelGGL((k_tet_gather<T, 1, kLpn>), dim3(grid_for(tg)), dim3(kBlock), 0, s,
                          op->n_nodes, ns, ncols, op->nptr, (const T*)F, X, op->fixed, Y);
     }
@@ -732,12 +760,12 @@ static int tet_apply_t(skt_operator* op, int ncols, const T* g, const T* vol, co
     hipLaunchKernelGGL((k_tet_apply<T, 3>), dim3(grid_for(total)), dim3(kBlock), 0, s,
                        op->n_nodes, op->n_el, ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
                        (T)op->cond_b, op->conn, g,
-                       vol, op->nptr, op->nel, op->nloc, E, X, op->fixed, Y);
+                       vol, op->nptr, op->nel, op->nloc, E, R, X, op->fixed, Y);
   else
     hipLaunchKernelGGL((k_tet_apply<T, 1>), dim3(grid_for(total)), dim3(kBlock), 0, s,
                        op->n_nodes, op->n_el, ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
                        (T)op->cond_b, op->conn, g,
-                       vol, op->nptr, op->nel, op->nloc, E, X, op->fixed, Y);
+                       vol, op->nptr, op->nel, op->nloc, E, R, X, op->fixed, Y);
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }
@@ -816,12 +844,12 @@ static int diag_t(skt_operator* op, const T* K0, const T* g, const T* vol, const
       hipLaunchKernelGGL((k_tet_diag<T, 3>), dim3(grid_for(op->n_nodes)), dim3(kBlock), 0, s,
                          op->n_nodes, op->n_el, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
                          (T)op->cond_b, g, vol, op->nptr,
-                         op->nel, op->nloc, E, op->fixed, d);
+                         op->nel, op->nloc, E, tet_extra<T>(op), op->fixed, d);
     else
       hipLaunchKernelGGL((k_tet_diag<T, 1>), dim3(grid_for(op->n_nodes)), dim3(kBlock), 0, s,
                          op->n_nodes, op->n_el, (T)op->lam0, (T)op->mu0, (T)op->mass_a,
                          (T)op->cond_b, g, vol, op->nptr,
-                         op->nel, op->nloc, E, op->fixed, d);
+                         op->nel, op->nloc, E, tet_extra<T>(op), op->fixed, d);
   }
   SKT_LAUNCH_CHECK();
   return SKT_OK;
@@ -1112,6 +1140,8 @@ int skt_operator_destroy(skt_operator* op) {
   (void)hipFree(op->extra32);
   (void)hipFree(op->tet_f);
   (void)hipFree(op->tet_slot);
+  (void)hipFree(op->tet_x64);
+  (void)hipFree(op->tet_x32);
   delete op;
   return SKT_OK;
 }

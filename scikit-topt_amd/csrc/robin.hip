@@ -1,5 +1,5 @@
 // Robin terms of the heat conduction path on structured hex8 grids
-// (fea/solver_heat.py:495-624,705-980; mesh/task_heat.py:60-139).
+// (fea/solver_heat.py:552-627,705-980; mesh/task_heat.py:60-139).
 //
 // * Nodal densities with the reference's pairing: np.add.at(rho_nodal,
 //   t.ravel(), np.repeat(rho, 8)) adds rho[k / 8] at node t.ravel()[k]; for

@@ -89,6 +89,12 @@ SIGNATURES = {
                             _d, _d, _vp, _vp, _vp],
     "skt_operator_set_extra": [_vp, _vp, _vp],
     "skt_stencil27_apply": [_i64, _i64, _i64, _i32, _vp, _vp, _vp, _vp],
+    "skt_tet4_nodal_average_pairing": [_vp, _vp, _vp, _vp, _vp],
+    "skt_tet4_robin_virtual": [_vp, _i32, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _vp],
+    "skt_tet4_heat_exchange": [_vp, _vp, _i32, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _vp],
+    "skt_tet4_robin_grad": [_vp, _i32, _vp, _vp, _i32, _vp, _d, _d, _d, _d, _vp, _vp, _vp],
+    "skt_tet4_set_extra": [_vp, _vp, _vp],
+    "skt_tet4_extra_quadform": [_vp, _i32, _vp, _vp, _vp],
     "skt_design_scatter": [_i64, _vp, _vp, _vp, _vp],
     "skt_oc_candidate": [_i64, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d, _d, _vp, _vp, _vp],
     "skt_oc_bisect": [_vp, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d,

@@ -264,17 +264,22 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
     With Robin data (``task.robin_coefficient`` set) every solve uses
     ``K = K_cond(rho) + B_facets + B_virtual(rho)`` and the load
     ``emit + L_facets + L_virtual(rho)`` (``:724-760``, ``get_robin_virtual``
-    :495-517); the Robin matrices are a 27-point stencil attached to the
-    operator.  The volume integrals use ``npts`` Gauss points per axis
-    (``npts = 2`` integrates the basis products exactly on the reference's
-    ``intorder`` 2 and 3).  The heat-exchange functionals use ``npts_cell``
-    points per axis: the reference evaluates them on ``skfem.CellBasis`` with
-    its default rule, 4 Gauss points per axis for trilinear hexahedra."""
+    :552-573).  On structured hex grids the Robin matrices are a 27-point
+    stencil attached to the operator, and the volume integrals use ``npts``
+    Gauss points per axis (``npts = 2`` integrates the basis products exactly
+    on the reference's ``intorder`` 2 and 3); the heat-exchange functionals
+    use ``npts_cell`` points per axis: the reference evaluates them on
+    ``skfem.CellBasis`` with its default rule, 4 Gauss points per axis for
+    trilinear hexahedra.  On tet4 meshes they are 4 x 4 element matrices
+    attached to the operator, the volume forms use the tetrahedral rule of
+    order ``tet_order`` (1: centroid, 2: the 4-point rule of degree 2, which a
+    P1 basis of ``intorder`` 2 uses), and the heat-exchange functionals are
+    exact."""
     physics = _lib.HEAT
 
     def __init__(self, task, E_min_coeff, density_interpolation=simp_interpolation,
                  solver_config=None, solver_option="spsolve", q=4, quad_order=2, npts=2,
-                 npts_cell=4):
+                 npts_cell=4, tet_order=2):
         if getattr(task, "objective", "compliance") not in ("compliance", "averaged_temp",
                                                              "heat_exchange"):
             raise ValueError(f"Unknown objective: {task.objective}")
@@ -286,8 +291,13 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
         self.npts = int(npts)
         self.objective = getattr(task, "objective", "compliance")
         self.robin = getattr(task, "robin_coefficient", None) is not None
-        if self.robin and (task.mesh.kind != "hex8" or getattr(task.mesh, "grid", None) is None):
-            raise NotImplementedError("Robin boundaries are built for structured hex8 grids")
+        self.tet = task.mesh.kind == "tet4"
+        if tet_order not in (1, 2):
+            raise ValueError("tet_order must be 1 or 2")
+        self.tet_order = int(tet_order)
+        if self.robin and not self.tet and getattr(task.mesh, "grid", None) is None:
+            raise NotImplementedError("Robin boundaries are built for structured hex8 grids "
+                                      "and tet4 meshes")
         super().__init__(task, E_min_coeff, density_interpolation, solver_config,
                          solver_option, quad_order)
         self.lam = None
@@ -297,12 +307,19 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
         if self.robin:
             dev, nn = self.device, task.mesh.nvertices
             z = lambda *sh: torch.zeros(sh, dtype=torch.float64, device=dev)  # noqa: E731
-            self._st_facet = _dev.to_dev(task.robin_stencil)
             self._L_facet = _dev.to_dev(task.robin_load)
             self._emit = _dev.to_dev(task.emit)
-            self._rn, self._cnt, self._work = z(nn), z(nn), z(nn)
-            self._st_virt, self._st_tot = z(27, nn), z(27, nn)
+            self._rn, self._cnt = z(nn), z(nn)
             self._L_virt, self._emit_tot = z(nn), z(nn)
+            if self.tet:
+                ne = task.mesh.nelements
+                self._st_facet = _dev.to_dev(task.robin_elem_mat)
+                self._st_virt, self._st_tot = z(10, ne), z(10, ne)
+                self._work = z(4 * ne + nn)
+            else:
+                self._st_facet = _dev.to_dev(task.robin_stencil)
+                self._st_virt, self._st_tot = z(27, nn), z(27, nn)
+                self._work = z(nn)
 
     def _build_rhs(self):
         cols = []
@@ -324,22 +341,40 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
         (``solver_heat.py:724-760``), attached to the operator; the enforced
         right-hand side is rebuilt from the total load."""
         t = self.task
-        nx, ny, nz = t.mesh.grid
         S = _dev.stream()
-        _lib.call("skt_hex8_nodal_average_pairing", nx, ny, nz, _dev.ptr(rho), _dev.ptr(self._rn),
-                  _dev.ptr(self._cnt), S)
-        _lib.call("skt_hex8_robin_virtual", nx, ny, nz, *t.mesh.h, self.npts, _dev.ptr(self._rn),
-                  float(p), float(self.q), float(t.robin_coefficient), float(t.robin_bc_value),
-                  _dev.ptr(self._st_virt), _dev.ptr(self._L_virt), S)
+        self._nodal_pairing(rho)
+        h, T_env = float(t.robin_coefficient), float(t.robin_bc_value)
+        if self.tet:
+            _lib.call("skt_tet4_robin_virtual", self._op, self.tet_order, _dev.ptr(self._rn),
+                      float(p), float(self.q), h, T_env, _dev.ptr(self._st_virt),
+                      _dev.ptr(self._L_virt), _dev.ptr(self._work), S)
+        else:
+            nx, ny, nz = t.mesh.grid
+            _lib.call("skt_hex8_robin_virtual", nx, ny, nz, *t.mesh.h, self.npts,
+                      _dev.ptr(self._rn), float(p), float(self.q), h, T_env,
+                      _dev.ptr(self._st_virt), _dev.ptr(self._L_virt), S)
         _lib.call("skt_ewise", 0, self._st_tot.numel(), 1.0, _dev.ptr(self._st_facet), 1.0,
                   _dev.ptr(self._st_virt), _dev.ptr(self._st_tot), S)
-        _lib.call("skt_operator_set_extra", self._op, _dev.ptr(self._st_tot), S)
+        _lib.call("skt_tet4_set_extra" if self.tet else "skt_operator_set_extra", self._op,
+                  _dev.ptr(self._st_tot), S)
         # emit + facet loads + virtual load, in the reference's list order
         _lib.call("skt_ewise", 0, self._emit.numel(), 1.0, _dev.ptr(self._emit), 1.0,
                   _dev.ptr(self._L_facet), _dev.ptr(self._emit_tot), S)
         _lib.call("skt_ewise", 0, self._emit.numel(), 1.0, _dev.ptr(self._emit_tot), 1.0,
                   _dev.ptr(self._L_virt), _dev.ptr(self._emit_tot), S)
         self.F = self._enforced(self._emit_tot)
+
+    def _nodal_pairing(self, rho):
+        """Nodal densities (and element counts) with the reference's add.at
+        pairing (``solver_heat.py:738-745``) into ``self._rn``, ``self._cnt``."""
+        S = _dev.stream()
+        if self.tet:
+            _lib.call("skt_tet4_nodal_average_pairing", self._op, _dev.ptr(rho),
+                      _dev.ptr(self._rn), _dev.ptr(self._cnt), S)
+        else:
+            nx, ny, nz = self.task.mesh.grid
+            _lib.call("skt_hex8_nodal_average_pairing", nx, ny, nz, _dev.ptr(rho),
+                      _dev.ptr(self._rn), _dev.ptr(self._cnt), S)
 
     def _solve_rhs(self, F, U):
         saved = self.F
@@ -370,7 +405,12 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
                   _dev.ptr(U), _dev.ptr(en), _dev.stream())
         J = 2.0 * self.col_reduce(_lib.RED_SUM, en)
         self.lam = None
-        if self.robin:
+        if self.robin and self.tet:
+            q = torch.empty((self.n_el, self.n_tasks), dtype=torch.float64, device=self.device)
+            _lib.call("skt_tet4_extra_quadform", self._op, self.n_tasks, _dev.ptr(U),
+                      _dev.ptr(q), _dev.stream())
+            J = J + self.col_reduce(_lib.RED_SUM, q)
+        elif self.robin:
             nx, ny, nz = self.task.mesh.grid
             SU = torch.empty_like(U)
             _lib.call("skt_stencil27_apply", nx, ny, nz, self.n_tasks, _dev.ptr(self._st_tot),
@@ -397,20 +437,24 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
         t = self.task
         if not self.robin:
             raise RuntimeError("heat_exchange objective requires Robin boundary data.")
-        nx, ny, nz = t.mesh.grid
         S = _dev.stream()
         k = self.n_tasks
         dev = self.device
-        _lib.call("skt_hex8_nodal_average_pairing", nx, ny, nz, _dev.ptr(rho),
-                  _dev.ptr(self._rn), _dev.ptr(self._cnt), S)
+        self._nodal_pairing(rho)
         den = torch.empty(self.n_el, dtype=torch.float64, device=dev)
         num = torch.empty((self.n_el, k), dtype=torch.float64, device=dev)
         rhs = torch.empty(self.ndof, dtype=torch.float64, device=dev)
         h, T_env = float(t.robin_coefficient), float(t.robin_bc_value)
         Uc = U.contiguous()
-        _lib.call("skt_hex8_heat_exchange", nx, ny, nz, *t.mesh.h, self.npts_cell,
-                  _dev.ptr(self._rn), k, _dev.ptr(Uc), float(p), float(self.q), h, T_env,
-                  _dev.ptr(den), _dev.ptr(num), _dev.ptr(rhs), S)
+        if self.tet:
+            _lib.call("skt_tet4_heat_exchange", self._op, _dev.ptr(self._rn), k, _dev.ptr(Uc),
+                      float(p), float(self.q), h, T_env, _dev.ptr(den), _dev.ptr(num),
+                      _dev.ptr(rhs), _dev.ptr(self._work), S)
+        else:
+            nx, ny, nz = t.mesh.grid
+            _lib.call("skt_hex8_heat_exchange", nx, ny, nz, *t.mesh.h, self.npts_cell,
+                      _dev.ptr(self._rn), k, _dev.ptr(Uc), float(p), float(self.q), h, T_env,
+                      _dev.ptr(den), _dev.ptr(num), _dev.ptr(rhs), S)
         s = torch.empty(1, dtype=torch.float64, device=dev)
         _lib.call("skt_reduce", _lib.RED_SUM, self.n_el, _dev.ptr(den), None, _dev.ptr(s), S)
         J_den = float(s.item())
@@ -466,15 +510,20 @@ class FEM_SimpLinearHeatConduction(_GpuFEM):
         if not (self.robin and self.objective == "compliance"):
             return None
         t = self.task
-        nx, ny, nz = t.mesh.grid
         S = _dev.stream()
-        _lib.call("skt_hex8_nodal_average_pairing", nx, ny, nz, _dev.ptr(rho), _dev.ptr(self._rn),
-                  _dev.ptr(self._cnt), S)
+        self._nodal_pairing(rho)
         out = torch.empty((self.n_el, self.n_tasks), dtype=torch.float64, device=self.device)
-        _lib.call("skt_hex8_robin_grad", nx, ny, nz, *t.mesh.h, self.npts, _dev.ptr(self._rn),
-                  _dev.ptr(self._cnt), self.n_tasks, _dev.ptr(U.contiguous()), float(p),
-                  float(self.q), float(t.robin_coefficient), float(t.robin_bc_value),
-                  _dev.ptr(self._work), _dev.ptr(out), S)
+        h, T_env = float(t.robin_coefficient), float(t.robin_bc_value)
+        if self.tet:
+            _lib.call("skt_tet4_robin_grad", self._op, self.tet_order, _dev.ptr(self._rn),
+                      _dev.ptr(self._cnt), self.n_tasks, _dev.ptr(U.contiguous()), float(p),
+                      float(self.q), h, T_env, _dev.ptr(self._work), _dev.ptr(out), S)
+        else:
+            nx, ny, nz = t.mesh.grid
+            _lib.call("skt_hex8_robin_grad", nx, ny, nz, *t.mesh.h, self.npts,
+                      _dev.ptr(self._rn), _dev.ptr(self._cnt), self.n_tasks,
+                      _dev.ptr(U.contiguous()), float(p), float(self.q), h, T_env,
+                      _dev.ptr(self._work), _dev.ptr(out), S)
         return out
 
     def compliance_sensitivity_multi_load(self, rho, p, u_dofs):

@@ -195,15 +195,49 @@ _QUAD_MASS = np.array([[4, 2, 1, 2], [2, 4, 2, 1], [1, 2, 4, 2], [2, 1, 2, 4]],
                       dtype=np.float64) / 36.0
 
 
+# packed index of entry (a, b) of a symmetric 4 x 4 element matrix, order
+# 00 11 22 33 01 02 03 12 13 23 (the layout of skt_tet4_set_extra)
+_SYM10 = np.array([[0, 4, 5, 6], [4, 1, 7, 8], [5, 7, 2, 9], [6, 8, 9, 3]], dtype=np.int64)
+
+
+def _tet_facet_owners(mesh, ids):
+    """(element (nf,), local corners (3, nf)) of boundary triangle facets:
+    the tet each facet bounds and the corner slot of each facet node in it."""
+    from .meshes import TET_FACES
+    f = mesh.facets[:, ids]                                   # (3, nf)
+    nn = np.int64(mesh.nvertices)
+    code = lambda k: (k[0] * nn + k[1]) * nn + k[2]           # noqa: E731
+    allf = np.concatenate([mesh.t[face] for face in TET_FACES], axis=1)   # (3, 4 n_el)
+    ac = code(np.sort(allf, axis=0))
+    order = np.argsort(ac, kind="stable")
+    fc = code(np.sort(f, axis=0))
+    pos = np.searchsorted(ac[order], fc)
+    if np.any(pos >= ac.size) or np.any(ac[order][np.minimum(pos, ac.size - 1)] != fc):
+        raise ValueError("a Robin facet is not a face of the mesh")
+    elem = order[pos] % mesh.nelements
+    loc = np.empty_like(f)
+    for i in range(3):
+        hit = mesh.t[:, elem] == f[i][None, :]                # (4, nf)
+        loc[i] = np.argmax(hit, axis=0)
+    return elem, loc
+
+
 def assemble_surface_robin(mesh, robin_facets_ids, robin_coefficient, robin_bc_value):
-    """``mesh/task_heat.py:60-139`` on a structured hex grid: the facet forms
-    ``int_G h u v`` and ``int_G h T_env v`` of every Robin region, summed,
-    as a 27-point nodal stencil (27, n_nodes) and a load vector.  Facets are
-    planar parallelograms: mass ``h A / 36 [[4 2 1 2] ...]`` in cyclic node
-    order and load ``h T_env A / 4`` per node.  (The reference's density
-    argument never reaches these forms, so they do not change with rho.)"""
-    if getattr(mesh, "kind", None) != "hex8" or getattr(mesh, "grid", None) is None:
-        raise NotImplementedError("Robin boundaries are built for structured hex8 grids")
+    """``mesh/task_heat.py:60-139``: the facet forms ``int_G h u v`` and
+    ``int_G h T_env v`` of every Robin region, summed, with a load vector.
+    (The reference's density argument never reaches these forms, so they do
+    not change with rho.)
+
+    * Structured hex grids: a 27-point nodal stencil (27, n_nodes).  Facets
+      are planar parallelograms: mass ``h A / 36 [[4 2 1 2] ...]`` in cyclic
+      node order and load ``h T_env A / 4`` per node.
+    * Tet4 meshes: element matrices (10, n_el) packed 00 11 22 33 01 02 03 12
+      13 23, each triangle's mass ``h A / 12 (1 + d_ab)`` added to the tet it
+      bounds; load ``h T_env A / 3`` per node."""
+    kind = getattr(mesh, "kind", None)
+    if not (kind == "tet4" or (kind == "hex8" and getattr(mesh, "grid", None) is not None)):
+        raise NotImplementedError("Robin boundaries are built for structured hex8 grids and "
+                                  "tet4 meshes")
     facets_list = _as_list(robin_facets_ids)
     hs = [float(robin_coefficient)] * len(facets_list) if not isinstance(
         robin_coefficient, list) else [float(v) for v in robin_coefficient]
@@ -211,6 +245,22 @@ def assemble_surface_robin(mesh, robin_facets_ids, robin_coefficient, robin_bc_v
         robin_bc_value, list) else [float(v) for v in robin_bc_value]
     if not (len(facets_list) == len(hs) == len(ts)):
         raise ValueError("Lengths of robin_facets_ids and robin_value must match when lists.")
+    if kind == "tet4":
+        R = np.zeros((10, mesh.nelements))
+        load = np.zeros(mesh.nvertices)
+        for ids, h, te in zip(facets_list, hs, ts):
+            ids = np.asarray(ids, dtype=np.int64)
+            f = mesh.facets[:, ids]
+            area = mesh.facet_areas(ids)
+            elem, loc = _tet_facet_owners(mesh, ids)
+            for i in range(3):
+                np.add.at(load, f[i], h * te * area / 3.0)
+                for j in range(3):
+                    w = (2.0 if i == j else 1.0) / 12.0
+                    k = _SYM10[loc[i], loc[j]]
+                    # each off-diagonal pair is visited twice (i, j) and (j, i)
+                    np.add.at(R, (k, elem), h * area * w * (1.0 if i == j else 0.5))
+        return R, load
     nx, ny, nz = mesh.grid
     nn = mesh.nvertices
     i, j, k = mesh.node_ijk(nx, ny, nz)
@@ -267,8 +317,12 @@ class LinearHeatConduction(FEMDomain):
             if isinstance(robin_coefficient, list) or isinstance(robin_bc_value, list):
                 raise NotImplementedError("one scalar Robin coefficient and ambient value "
                                           "(the virtual Robin terms take scalars)")
-            self.robin_stencil, self.robin_load = assemble_surface_robin(
+            mats, self.robin_load = assemble_surface_robin(
                 mesh, robin_facets_ids, robin_coefficient, robin_bc_value)
+            if mesh.kind == "tet4":
+                self.robin_elem_mat = mats
+            else:
+                self.robin_stencil = mats
         else:
             self.robin_nodes = None
             self.robin_elements = None

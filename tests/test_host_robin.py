@@ -50,3 +50,42 @@ def test_heat_task_from_facets():
     t2 = LinearHeatConduction.from_facets(m, d, 600.0, r, 4e-2, 300.0, True,
                                           np.arange(m.nelements), 10.0)
     assert np.any(np.isin(t2.design_elements, t2.robin_elements))
+
+
+def _elem_matrix_assemble(mesh, R10):
+    """Global sparse matrix of packed symmetric 4 x 4 element matrices."""
+    import scipy.sparse as sp
+    from sktopt_amd.mesh.task import _SYM10
+    Ke = R10[_SYM10]                                     # (4, 4, n_el)
+    rows = np.repeat(mesh.t.T, 4, axis=1).ravel()
+    cols = np.tile(mesh.t.T, (1, 4)).ravel()
+    vals = np.transpose(Ke, (2, 0, 1)).ravel()
+    n = mesh.nvertices
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def test_tet_facet_robin_matches_oracle():
+    """Triangle facets on a jittered tet mesh: each facet's mass is folded
+    into the tet it bounds; assembled, it equals the oracle's edge-midpoint
+    facet integrals, and so do the loads."""
+    from oracle import mesh as om
+    from sktopt_amd.mesh.meshes import MeshTet
+    o = om.jittered_tet(4, 3, 2, 0.2, 3)
+    m = MeshTet(o.p, o.t)
+    y0 = m.facets_satisfying(tp.get_points_in_range((0, 4), (0, 0), (0, 2)), True)
+    x0 = m.facets_satisfying(tp.get_points_in_range((0, 0), (0, 3), (0, 2)), True)
+    assert len(y0) == 2 * 4 * 2 and len(x0) == 2 * 3 * 2
+    R, L = assemble_surface_robin(m, [y0, x0], [0.25, 0.5], [310.0, 290.0])
+    assert R.shape == (10, m.nelements)
+    B = None
+    Lr = np.zeros(m.nvertices)
+    for ids, h, te in ((y0, 0.25, 310.0), (x0, 0.5, 290.0)):
+        b, l = hr.facet_robin(m.p, m.facets[:, ids], h, te, m.nvertices)
+        B = b if B is None else B + b
+        Lr += l
+    np.testing.assert_allclose(_elem_matrix_assemble(m, R).toarray(), B.toarray(), rtol=1e-13,
+                               atol=1e-15)
+    np.testing.assert_allclose(L, Lr, rtol=1e-13)
+    # only tets with a face on those planes carry facet terms
+    owners = np.nonzero(np.any(R != 0.0, axis=0))[0]
+    assert len(owners) <= len(y0) + len(x0)
