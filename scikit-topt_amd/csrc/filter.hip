@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "common.hpp"
@@ -39,6 +40,8 @@ struct skt_filter {
   double* u_off = nullptr;         // (n_off) unnormalised Gaussian weight
   // bisection workspace
   double* cand_full = nullptr;     // (n_el)
+  double* cand_tree = nullptr;     // (n_el, kTreeStride): candidates at the tree's midpoints
+  double* part_tree = nullptr;     // (kTreeK, kPartMax)
   double* part = nullptr;          // (kPartMax)
   double* state = nullptr;         // bisection state
 };
@@ -46,6 +49,12 @@ struct skt_filter {
 namespace skt {
 
 constexpr int kPartMax = 4096;
+// CSR bisection: each pass evaluates the midpoints of the next kTreeLevels
+// levels of the bisection tree (kTreeK of them) in one read of the matrix
+constexpr int kTreeLevels = 3;
+constexpr int kTreeK = (1 << kTreeLevels) - 1;
+constexpr int kTreeStride = 8;
+static_assert(kTreeK <= kTreeStride, "tree candidates per element");
 constexpr int kSeg = 8;  // lanes per CSR row in the gathers
 
 __device__ __forceinline__ double dist2(const double* __restrict__ c, int64_t n,
@@ -514,6 +523,156 @@ __global__ void k_oc_bracket(int nb, const double* __restrict__ part,
   }
 }
 
+// The bisection tree below bracket (l1, l2), heap order: node n's midpoint
+// 0.5 (lo + hi), its children 2n + 1 (volume <= 0: l2 = mid) and 2n + 2
+// (volume > 0: l1 = mid) -- the values the sequential updates compute.
+__device__ __forceinline__ void oc_tree(double l1, double l2, double* mid) {
+  double lo[kTreeK], hi[kTreeK];
+  lo[0] = l1;
+  hi[0] = l2;
+#pragma unroll
+  for (int n = 0; n < kTreeK; ++n) {
+    mid[n] = 0.5 * (lo[n] + hi[n]);
+    const int a = 2 * n + 1, b = 2 * n + 2;
+    if (a < kTreeK) {
+      lo[a] = lo[n];
+      hi[a] = mid[n];
+    }
+    if (b < kTreeK) {
+      lo[b] = mid[n];
+      hi[b] = hi[n];
+    }
+  }
+}
+
+__global__ void k_oc_candidate_tree(int64_t n_design, const int32_t* __restrict__ ids,
+                                    const double* __restrict__ rho_e,
+                                    const double* __restrict__ dC,
+                                    const double* __restrict__ st, double eps, double eta,
+                                    double rate_min, double rate_max, double move_limit,
+                                    double rho_min, double rho_max,
+                                    double* __restrict__ cand_tree) {
+  if (st[5] != 0.0) return;
+  double mid[kTreeK];
+  oc_tree(st[0], st[1], mid);
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    double* out = cand_tree + (int64_t)ids[r] * kTreeStride;
+#pragma unroll
+    for (int k = 0; k < kTreeK; ++k)
+      out[k] = oc_candidate(dC[r], rho_e[r], mid[k], eps, eta, rate_min, rate_max, move_limit,
+                            rho_min, rho_max);
+  }
+}
+
+// k_oc_volume at every midpoint of the tree: one read of the matrix, the
+// same per-row and per-block summation order for each midpoint.
+__global__ void k_oc_volume_tree(int64_t n_design, const int64_t* __restrict__ row_ptr,
+                                 const int32_t* __restrict__ gcol,
+                                 const double* __restrict__ w,
+                                 const double* __restrict__ cand_tree,
+                                 const double* __restrict__ vol, double beta, double beta_eta,
+                                 double tbe, double denom, const double* __restrict__ st,
+                                 double* __restrict__ part) {
+  __shared__ double sh[kBlock / kWave];
+  if (st[5] != 0.0) return;
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int sub = threadIdx.x & (kSeg - 1);
+  double vsum[kTreeK];
+#pragma unroll
+  for (int k = 0; k < kTreeK; ++k) vsum[k] = 0.0;
+  for (int64_t r = tid / kSeg; r < n_design; r += ((int64_t)gridDim.x * blockDim.x) / kSeg) {
+    const int64_t s0 = row_ptr[r], s1 = row_ptr[r + 1];
+    double acc[kTreeK];
+#pragma unroll
+    for (int k = 0; k < kTreeK; ++k) acc[k] = 0.0;
+    for (int64_t q = s0 + sub; q < s1; q += kSeg) {
+      const double wq = w[q];
+      const double* c = cand_tree + (int64_t)gcol[q] * kTreeStride;
+#pragma unroll
+      for (int k = 0; k < kTreeK; ++k) acc[k] += wq * c[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kTreeK; ++k)
+#pragma unroll
+      for (int off = kSeg / 2; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off, kSeg);
+    if (sub == 0) {
+#pragma unroll
+      for (int k = 0; k < kTreeK; ++k) {
+        double t = (acc[k] - beta_eta) * beta;
+        t = tanh(t);
+        double proj = (tbe + t) / denom;
+        vsum[k] += proj * vol[r];
+      }
+    }
+  }
+  for (int k = 0; k < kTreeK; ++k) {
+    const double b = block_sum(vsum[k], sh);
+    if (threadIdx.x == 0) part[(int64_t)k * gridDim.x + blockIdx.x] = b;
+    __syncthreads();
+  }
+}
+
+// Walk the tree's path as the sequential bisection would (k_oc_bracket per
+// level), stopping when it terminates or after max_evals evaluations in all;
+// st[7] keeps the last evaluated midpoint.
+__global__ void k_oc_bracket_tree(int nb, const double* __restrict__ part, double vol_sum,
+                                  double vol_frac, double vol_tol, double tolerance,
+                                  int max_iter, int max_evals, double* __restrict__ st) {
+  __shared__ double sh[kBlock / kWave];
+  __shared__ double tot[kTreeK];
+  if (st[5] != 0.0) return;
+  for (int k = 0; k < kTreeK; ++k) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) v += part[(int64_t)k * nb + i];
+    const double t = block_sum(v, sh);
+    if (threadIdx.x == 0) tot[k] = t;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double mid[kTreeK];
+    oc_tree(st[0], st[1], mid);
+    int node = 0;
+    for (int lev = 0; lev < kTreeLevels; ++lev) {
+      if (st[6] >= (double)max_evals) break;
+      const double vol_error = tot[node] / vol_sum - vol_frac;
+      st[3] = vol_error;
+      st[6] += 1.0;
+      st[7] = mid[node];
+      double l1 = st[0], l2 = st[1];
+      const double it = st[4];
+      if (fabs(vol_error) < vol_tol || it >= (double)max_iter || fabs(l2 - l1) <= tolerance) {
+        st[5] = 1.0;
+        break;
+      }
+      if (vol_error > 0) {
+        l1 = mid[node];
+        node = 2 * node + 2;
+      } else {
+        l2 = mid[node];
+        node = 2 * node + 1;
+      }
+      st[0] = l1;
+      st[1] = l2;
+      st[4] = it + 1.0;
+      st[2] = 0.5 * (l1 + l2);
+    }
+  }
+}
+
+// the design rows' candidate at the last evaluated midpoint (st[7])
+__global__ void k_oc_candidate_at(int64_t n_design, const double* __restrict__ rho_e,
+                                  const double* __restrict__ dC,
+                                  const double* __restrict__ st, double eps, double eta,
+                                  double rate_min, double rate_max, double move_limit,
+                                  double rho_min, double rho_max, double* __restrict__ rho_out) {
+  const double lmid = st[7];
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_design;
+       r += (int64_t)gridDim.x * blockDim.x)
+    rho_out[r] = oc_candidate(dC[r], rho_e[r], lmid, eps, eta, rate_min, rate_max, move_limit,
+                              rho_min, rho_max);
+}
+
 __global__ void k_oc_result(const double* __restrict__ st, double* __restrict__ res) {
   res[0] = st[2];
   res[1] = st[3];
@@ -649,6 +808,8 @@ int skt_filter_create(int64_t n_el, const double* centers,
                      f->design_ids, f->colmap, f->row_ptr, f->gcol, f->w, f->wt);
   SKT_LAUNCH_CHECK();
   SKT_HIP(hipMalloc(&f->cand_full, sizeof(double) * n_el));
+  SKT_HIP(hipMalloc(&f->cand_tree, sizeof(double) * kTreeStride * n_el));
+  SKT_HIP(hipMalloc(&f->part_tree, sizeof(double) * kTreeK * kPartMax));
   SKT_HIP(hipMalloc(&f->part, sizeof(double) * kPartMax));
   SKT_HIP(hipMalloc(&f->state, sizeof(double) * 16));
   SKT_HIP(hipStreamSynchronize(stream));
@@ -730,6 +891,11 @@ int skt_filter_set_grid(skt_filter* f, int64_t nx, int64_t ny, int64_t nz, doubl
   f->gny = ny;
   f->gnz = nz;
   f->stencil = 1;
+  // the stencil form's bisection evaluates one midpoint at a time
+  (void)hipFree(f->cand_tree);
+  (void)hipFree(f->part_tree);
+  f->cand_tree = nullptr;
+  f->part_tree = nullptr;
   *active = 1;
   return SKT_OK;
 }
@@ -747,6 +913,8 @@ int skt_filter_destroy(skt_filter* f) {
   (void)hipFree(f->del);
   (void)hipFree(f->u_off);
   (void)hipFree(f->cand_full);
+  (void)hipFree(f->cand_tree);
+  (void)hipFree(f->part_tree);
   (void)hipFree(f->part);
   (void)hipFree(f->state);
   delete f;
@@ -833,6 +1001,13 @@ int skt_filter_scatter_design(skt_filter* f, const double* design, double* full,
   return SKT_OK;
 }
 
+// SKT_OC_TREE=0 evaluates the CSR form's bisection one midpoint at a time
+// (same decisions and results; read at every call)
+static bool oc_tree_enabled() {
+  const char* v = std::getenv("SKT_OC_TREE");
+  return !(v && v[0] == '0' && v[1] == 0);
+}
+
 int skt_oc_bisect(skt_filter* f, const double* rho_full, const double* rho_e,
                   const double* dC, const double* vol, double vol_sum,
                   double vol_frac, double move_limit, double eta,
@@ -855,9 +1030,32 @@ int skt_oc_bisect(skt_filter* f, const double* rho_full, const double* rho_e,
   unsigned nb = grid_for(nd * kSeg, kBlock, kPartMax);
   // candidates live on the design rows only; the other entries stay zero,
   // which the stencil volume sum relies on (its design-masked row form)
+  hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(1), 0, stream, f->state, l1, l2);
+  if (!f->stencil && f->cand_tree && oc_tree_enabled()) {
+    // CSR form: kTreeLevels bisection steps per read of the matrix; the
+    // decisions, evaluation count and result equal the step-by-step loop's
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(kTreeStride * f->n_el)), dim3(kBlock), 0, stream,
+                       kTreeStride * f->n_el, 0.0, f->cand_tree);
+    for (int s = 0; s < steps; s += kTreeLevels) {
+      hipLaunchKernelGGL(k_oc_candidate_tree, dim3(grid_for(nd)), dim3(kBlock), 0, stream, nd,
+                         f->design_ids, rho_e, dC, f->state, 1e-12, eta, rate_min, rate_max,
+                         move_limit, rho_min, rho_max, f->cand_tree);
+      hipLaunchKernelGGL(k_oc_volume_tree, dim3(nb), dim3(kBlock), 0, stream, nd, f->row_ptr,
+                         f->gcol, f->w, f->cand_tree, vol, beta, beta_eta, tbe, denom, f->state,
+                         f->part_tree);
+      hipLaunchKernelGGL(k_oc_bracket_tree, dim3(1), dim3(kBlock), 0, stream, (int)nb,
+                         f->part_tree, vol_sum, vol_frac, vol_tol, tolerance, max_iter, steps,
+                         f->state);
+    }
+    hipLaunchKernelGGL(k_oc_candidate_at, dim3(grid_for(nd)), dim3(kBlock), 0, stream, nd, rho_e,
+                       dC, f->state, 1e-12, eta, rate_min, rate_max, move_limit, rho_min,
+                       rho_max, rho_out);
+    hipLaunchKernelGGL(k_oc_result, dim3(1), dim3(1), 0, stream, f->state, result);
+    SKT_LAUNCH_CHECK();
+    return SKT_OK;
+  }
   hipLaunchKernelGGL(k_fill, dim3(grid_for(f->n_el)), dim3(kBlock), 0, stream, f->n_el, 0.0,
                      f->cand_full);
-  hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(1), 0, stream, f->state, l1, l2);
   for (int s = 0; s < steps; ++s) {
     hipLaunchKernelGGL(k_oc_candidate, dim3(grid_for(nd)), dim3(kBlock), 0, stream, nd,
                        f->design_ids, rho_e, dC, f->state, 1e-12, eta, rate_min, rate_max,

@@ -318,12 +318,16 @@ def test_solver_reports_non_convergence(gpu):
 
 # ---------------------------------------------------------------- OC bisection
 
-@pytest.mark.parametrize("it_seed", [0, 1, 2])
-def test_oc_bisection_matches_oracle(gpu, it_seed):
+@pytest.mark.parametrize("it_seed,kind", [(0, "hex8"), (1, "hex8"), (2, "hex8"), (3, "tet4")])
+def test_oc_bisection_matches_oracle(gpu, it_seed, kind):
+    """The device bisection on the CSR form (three tree levels per read of
+    the matrix) against the oracle's loop: evaluation count, multiplier,
+    volume error and densities."""
     import torch
     from sktopt_amd import _lib
     from sktopt_amd.filters import SpacialFilter
-    task = oprob.cantilever(12, 6, 6)
+    task = oprob.cantilever(12, 6, 6) if kind == "hex8" else \
+        oprob.cantilever(8, 4, 4, kind="tet4", jitter=0.2, seed=it_seed)
     mask = task.design_mask.copy()
     mask[task.dirichlet_elements] = False
     design = np.nonzero(mask)[0]
@@ -442,6 +446,45 @@ def test_filter_stencil_form_matches_csr_and_oracle(gpu, n, r, h):
     assert int(rr[2]) == ref[3]
     np.testing.assert_allclose(rr[0], ref[0], rtol=1e-12)
     np.testing.assert_allclose(out.cpu().numpy(), ref[2], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("kind,seed", [("hex8", 0), ("tet4", 1), ("tet4", 2)])
+def test_oc_bisection_tree_equals_one_midpoint_at_a_time(gpu, kind, seed, monkeypatch):
+    """The CSR bisection evaluated three tree levels per pass
+    (k_oc_volume_tree) and one midpoint at a time (SKT_OC_TREE=0) take the
+    same decisions: evaluation count, multiplier, volume error, bracket and
+    densities bit for bit, over several volume targets (early and late
+    terminations inside a pass)."""
+    import torch
+    from sktopt_amd import _lib
+    from sktopt_amd.filters import SpacialFilter
+    task = oprob.cantilever(14, 7, 7) if kind == "hex8" else \
+        oprob.cantilever(10, 5, 5, kind="tet4", jitter=0.2, seed=seed)
+    mask = task.design_mask.copy()
+    mask[task.dirichlet_elements] = False
+    design = np.nonzero(mask)[0]
+    rng = np.random.default_rng(seed)
+    rho = np.ones(task.n_elements)
+    rho[design] = rng.uniform(0.05, 0.9, design.size)
+    dC = -rng.uniform(0.0, 1.0, design.size) ** 3
+    vol = task.elements_volume[design]
+    f = SpacialFilter(None, task.elements_volume, 1.5, mask, element_centers=task.centers)
+    counts = set()
+    for vf, tol in ((0.3, 1e-5), (0.45, 1e-5), (0.2, 1e-3), (0.3, 1e-9), (0.25, 1e-5),
+                    (0.35, 1e-4), (0.4, 1e-6), (0.5, 1e-7), (0.15, 1e-2)):
+        out = {}
+        for tree in ("1", "0"):
+            monkeypatch.setenv("SKT_OC_TREE", tree)
+            o = torch.empty(design.size, dtype=torch.float64, device="cuda")
+            res = torch.empty(4, dtype=torch.float64, device="cuda")
+            _lib.call("skt_oc_bisect", f.handle, _p(_t(rho)), _p(_t(rho[design])), _p(_t(dC)),
+                      _p(_t(vol)), float(vol.sum()), vf, 0.2, 0.5, 1e-2, 1.0, 0.7, 1.3, 2.0,
+                      0.5, 1e-7, 1e7, tol, 1e-4, 1000, _p(o), _p(res), _s())
+            out[tree] = (res.cpu().numpy(), o.cpu().numpy())
+        np.testing.assert_array_equal(out["1"][0], out["0"][0], err_msg=f"vf={vf} tol={tol}")
+        np.testing.assert_array_equal(out["1"][1], out["0"][1], err_msg=f"vf={vf} tol={tol}")
+        counts.add(int(out["1"][0][2]) % 3)
+    print(f"[oc tree {kind}] evaluation counts mod 3: {sorted(counts)}", flush=True)
 
 
 def test_filter_stencil_form_declines_other_grids(gpu):
