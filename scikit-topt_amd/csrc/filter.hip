@@ -1033,7 +1033,10 @@ int skt_oc_bisect(skt_filter* f, const double* rho_full, const double* rho_e,
   hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(1), 0, stream, f->state, l1, l2);
   if (!f->stencil && f->cand_tree && oc_tree_enabled()) {
     // CSR form: kTreeLevels bisection steps per read of the matrix; the
-    // decisions, evaluation count and result equal the step-by-step loop's
+    // decisions, evaluation count and result equal the step-by-step loop's.
+    // (The stencil form stays one midpoint per pass: its volume kernel reads
+    // no matrix, and seven candidates per offset cost more than the launches
+    // they save.)
     hipLaunchKernelGGL(k_fill, dim3(grid_for(kTreeStride * f->n_el)), dim3(kBlock), 0, stream,
                        kTreeStride * f->n_el, 0.0, f->cand_tree);
     for (int s = 0; s < steps; s += kTreeLevels) {

@@ -453,8 +453,8 @@ def test_oc_bisection_tree_equals_one_midpoint_at_a_time(gpu, kind, seed, monkey
     """The CSR bisection evaluated three tree levels per pass
     (k_oc_volume_tree) and one midpoint at a time (SKT_OC_TREE=0) take the
     same decisions: evaluation count, multiplier, volume error, bracket and
-    densities bit for bit, over several volume targets (early and late
-    terminations inside a pass)."""
+    densities bit for bit, over several volume targets (terminations at
+    every level of a pass)."""
     import torch
     from sktopt_amd import _lib
     from sktopt_amd.filters import SpacialFilter
@@ -485,6 +485,7 @@ def test_oc_bisection_tree_equals_one_midpoint_at_a_time(gpu, kind, seed, monkey
         np.testing.assert_array_equal(out["1"][1], out["0"][1], err_msg=f"vf={vf} tol={tol}")
         counts.add(int(out["1"][0][2]) % 3)
     print(f"[oc tree {kind}] evaluation counts mod 3: {sorted(counts)}", flush=True)
+    assert counts == {0, 1, 2}
 
 
 def test_filter_stencil_form_declines_other_grids(gpu):
