@@ -16,10 +16,12 @@ sensitivities, filter adjoint, device-side lambda bisection).  W warm-up
 iterations run untimed, then K iterations are timed one by one, each
 between device synchronisations; ``value`` = 1 / median per-iteration time
 (``ms_per_step`` is that median; the mean is reported beside it).  The
-timed iterations carry no per-kernel instrumentation: the operator-apply
-time behind ``roofline`` comes from two further, untimed iterations with
-HIP events around every fp32 apply on the solve stream.  Prints one JSON
-line.
+timed iterations carry no per-kernel instrumentation: the kernel times
+behind ``roofline`` come from two further, untimed iterations with HIP
+events on the solve stream around every fp32 operator apply of the CG and
+every fine-level launch of the V-cycle; ``roofline`` is the fine-level
+modal kernel with the largest total time (``fine_level_kernels`` lists all
+five).  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -65,6 +67,13 @@ def make_optimizer(wl, precision, rtol, iters, dst):
 
 
 APPLY_KERNEL = "skt::k_hex_apply_modal<float, 3, 2, 0, 0, true>"  # plain apply + fused p.Kp
+# the V-cycle's fine-level variants in the order skt_solver_get_vcycle_timing
+# reports them: fused first two Chebyshev steps, residual, Chebyshev step,
+# last Chebyshev step with the fused r . z
+VCYCLE_KERNELS = ["skt::k_hex_apply_modal<float, 3, 2, 3, 0, false>",
+                  "skt::k_hex_apply_modal<float, 3, 2, 1, 0, false>",
+                  "skt::k_hex_apply_modal<float, 3, 2, 2, 0, false>",
+                  "skt::k_hex_apply_modal<float, 3, 2, 2, 0, true>"]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hex8_modal_apply_f32_120x60x60_8col_pmc.txt")
 COPY_DOUBLES = 1 << 25            # 256 MiB read + 256 MiB written per copy
 
@@ -90,14 +99,16 @@ def hbm_copy(reps=5):
     return dict(bytes=2 * 8 * COPY_DOUBLES, us=us, gbs=2 * 8 * COPY_DOUBLES / us / 1e3)
 
 
-def traffic_from_profile():
-    """HBM bytes per apply launch from the committed pmc summary (FETCH_SIZE
-    and WRITE_SIZE passes, scaled by the copy calibration), if present."""
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` from the committed pmc summary
+    (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy calibration), if
+    present."""
+    key = f"traffic_bytes_per_launch {kernel}:"
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                if line.startswith(key):
+                    return float(line[len(key):])
     except OSError:
         pass
     return None
@@ -145,20 +156,28 @@ def timed_run(opt, warmup, steps):
 
 
 def apply_timing_pass(opt, steps=2):
-    """Untimed OC iterations with HIP events around every operator apply
-    of the solve (on the solve stream): average fp32 / fp64 apply times."""
+    """Untimed OC iterations with HIP events (on the solve stream) around
+    every operator apply of the CG loop and every fine-level launch of the
+    multigrid V-cycle: totals and counts per fp32 / fp64 apply and per
+    V-cycle variant."""
     import ctypes as C
     import torch
     from sktopt_amd import _lib
-    _lib.call("skt_solver_set_timing", opt.fem._solver, 1)
+    s = opt.fem._solver
+    _lib.call("skt_solver_set_timing", s, 1)
     ms = (C.c_double * 2)()
     n = (C.c_int64 * 2)()
-    _lib.call("skt_solver_get_timing", opt.fem._solver, ms, n, 1)     # reset
+    vms = (C.c_double * 4)()
+    vn = (C.c_int64 * 4)()
+    _lib.call("skt_solver_get_timing", s, ms, n, 1)     # reset
+    _lib.call("skt_solver_get_vcycle_timing", s, vms, vn, 1)
     opt.optimize_steps(steps)
     torch.cuda.synchronize()
-    _lib.call("skt_solver_get_timing", opt.fem._solver, ms, n, 1)
-    _lib.call("skt_solver_set_timing", opt.fem._solver, 0)
-    return dict(f64_ms=ms[0], f64_n=n[0], f32_ms=ms[1], f32_n=n[1])
+    _lib.call("skt_solver_get_timing", s, ms, n, 1)
+    _lib.call("skt_solver_get_vcycle_timing", s, vms, vn, 1)
+    _lib.call("skt_solver_set_timing", s, 0)
+    return dict(f64_ms=ms[0], f64_n=n[0], f32_ms=ms[1], f32_n=n[1],
+                vcycle_ms=list(vms), vcycle_n=list(vn))
 
 
 def time_launches(fn, reps=20, warm=3):
@@ -349,8 +368,9 @@ def main():
     evals = opt.recorder.as_object().bisection_evals[-K:]
     log(f"mixed: {value:.3f} OC it/s (median {1e3 * t_med:.2f} ms, mean {1e3 * t_mean:.2f} ms)"
         f"  solves={[s['iters'] for s in solves]}")
-    # roofline of the dominant kernel, the float32 operator apply, from an
-    # untimed pass with HIP events around each apply
+    # the fine-level modal kernels (the CG's apply and the V-cycle's four
+    # variants) from an untimed pass with HIP events around each launch; the
+    # roofline is the one with the largest total time
     apply = apply_timing_pass(opt, 2)
     ndof, n_el, k = opt.fem.ndof, opt.fem.n_el, opt.fem.n_tasks
     bytes_apply = k * ndof * (4 + 4) + n_el * 4
@@ -358,6 +378,19 @@ def main():
     achieved = bytes_apply / (avg_ms * 1e-3) / 1e9
     log(f"fp32 apply: {1e3 * avg_ms:.2f} us avg over {apply['f32_n']} launches, "
         f"{achieved:.0f} GB/s")
+    trace_us = {d["kernel"]: d["avg_us"] for d in modal_variants_from_profile(ndof, n_el, k)}
+    fine = [dict(kernel=APPLY_KERNEL, total_ms=apply["f32_ms"], launches=apply["f32_n"])]
+    for name, tms, tn in zip(VCYCLE_KERNELS, apply["vcycle_ms"], apply["vcycle_n"]):
+        fine.append(dict(kernel=name, total_ms=tms, launches=tn))
+    for d in fine:
+        d["avg_us"] = 1e3 * d["total_ms"] / max(d["launches"], 1)
+        d["bytes"] = MODAL_BYTES[d["kernel"]](ndof, n_el, k)
+        d["achieved_gbs"] = d["bytes"] / (d["avg_us"] * 1e-6) / 1e9 if d["launches"] else None
+        d["frac"] = d["achieved_gbs"] / HBM_PEAK_GBS if d["launches"] else None
+        d["trace_avg_us"] = trace_us.get(d["kernel"])
+        log(f"  {d['kernel']}: {d['avg_us']:.2f} us x {d['launches']}, "
+            f"{d['total_ms']:.2f} ms in all")
+    dom = max(fine, key=lambda d: d["total_ms"])
     jac_its = None if args.no_cpu else jacobi_iterations(opt)
     fp64 = None
     err = None
@@ -403,14 +436,20 @@ def main():
                               "Chebyshev-Jacobi smoothing), fp32 with fp64 reliable updates"),
                    "rtol": 1e-8, "mg_pcg_iterations": [s["iters"] for s in solves],
                    "bisection_evals": evals, "timing": "median of per-iteration wall times"},
-        "roofline": {"bound": "hbm", "kernel": APPLY_KERNEL,
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
-                     "bytes_per_launch": bytes_apply, "avg_launch_us": 1e3 * avg_ms,
-                     "launches": apply["f32_n"], "hbm_copy_gbs": copy["gbs"],
+        "roofline": {"bound": "hbm", "kernel": dom["kernel"],
+                     "achieved": dom["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["frac"], "traffic": traffic_from_profile(dom["kernel"]),
+                     "bytes_per_launch": dom["bytes"], "avg_launch_us": dom["avg_us"],
+                     "launches": dom["launches"], "hbm_copy_gbs": copy["gbs"],
+                     "dominant": "largest total time among the fine-level modal kernels",
                      "measured": "HIP events per launch on the solve stream, 2 untimed OC "
                                  "iterations after the timed ones"},
+        "operator_apply": {"kernel": APPLY_KERNEL, "avg_launch_us": 1e3 * avg_ms,
+                           "achieved": achieved, "frac": achieved / HBM_PEAK_GBS,
+                           "bytes_per_launch": bytes_apply,
+                           "traffic": traffic_from_profile(APPLY_KERNEL)},
         "operator_apply_dof_per_s": k * ndof / (avg_ms * 1e-3),
+        "fine_level_kernels": fine,
         "modal_variants": {"source": os.path.relpath(KERNEL_TRACE_FILE, ROOT),
                            "kernels": variants},
         "other_configs": suite,

@@ -392,6 +392,11 @@ int skt_solver_info(const skt_solver* s, int* precond_used, int* levels);
 int skt_solver_set_timing(skt_solver* s, int enable);
 int skt_solver_get_timing(skt_solver* s, double* apply_ms, int64_t* n_apply,
                           int reset);
+/* With timing enabled, the multigrid V-cycle's fine-level launches on a
+ * structured hex8 grid are bracketed too: ms / n ([4] each) per variant --
+ * [0] the fused first two Chebyshev steps, [1] the residual, [2] a
+ * Chebyshev step, [3] the last Chebyshev step with its fused r . z. */
+int skt_solver_get_vcycle_timing(skt_solver* s, double* ms, int64_t* n, int reset);
 
 /* The PCG iterations between two convergence polls, and the multigrid
  * set-up once it is warm, run as captured hipGraphs replayed on the solve

@@ -29,6 +29,8 @@
 // the blocked layout its loads and stores are contiguous.
 #include <cstdlib>
 
+#include <hip/hip_ext.h>
+
 #include "operator.hpp"
 
 namespace skt {
@@ -611,6 +613,21 @@ int hex_modal_dot_slots(const skt_operator* op, int k, int kc) {
   return g.tiles_x * g.tiles_y * g.nzc;
 }
 
+// Events for the next modal launch on this host thread (modal_time_next):
+// the launch records them itself through hipExtLaunchKernelGGL.
+static thread_local hipEvent_t g_time_start = nullptr, g_time_stop = nullptr;
+
+void modal_time_next(hipEvent_t start, hipEvent_t stop) {
+  g_time_start = start;
+  g_time_stop = stop;
+}
+
+bool modal_time_clear() {
+  const bool pending = g_time_start != nullptr;
+  g_time_start = g_time_stop = nullptr;
+  return pending;
+}
+
 // Launch helper for structured grids whose element matrix has the modal
 // pattern.  k columns in the layout described above.
 template <typename T, int NC, int KC, int MODE, int SF, bool DOT>
@@ -637,10 +654,21 @@ static int launch_modal(const skt_operator* op, int k, int lkc, int64_t gstride,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_hex_apply_modal<T, NC, KC, MODE, SF, DOT>), dim3((unsigned)gr.nblocks),
-                     dim3(256), shmem, s, op->nx, op->ny, op->nz, lkc, gstride, dstride,
-                     (int)gr.lz, gr.tiles_x, gr.tiles_y, gr.nzc, a, E,
-                     op->n_fixed > 0 ? op->fixed : nullptr, X, Y, oa);
+  const uint8_t* fx = op->n_fixed > 0 ? op->fixed : nullptr;
+  if (g_time_start) {
+    // timed launch: the events take the dispatch's own start and end
+    hipEvent_t t0 = g_time_start, t1 = g_time_stop;
+    g_time_start = g_time_stop = nullptr;
+    hipExtLaunchKernelGGL((k_hex_apply_modal<T, NC, KC, MODE, SF, DOT>),
+                          dim3((unsigned)gr.nblocks), dim3(256), (uint32_t)shmem, s, t0, t1, 0u,
+                          op->nx, op->ny, op->nz, lkc, gstride, dstride, (int)gr.lz, gr.tiles_x,
+                          gr.tiles_y, gr.nzc, a, E, fx, X, Y, oa);
+  } else {
+    hipLaunchKernelGGL((k_hex_apply_modal<T, NC, KC, MODE, SF, DOT>),
+                       dim3((unsigned)gr.nblocks), dim3(256), shmem, s, op->nx, op->ny, op->nz,
+                       lkc, gstride, dstride, (int)gr.lz, gr.tiles_x, gr.tiles_y, gr.nzc, a, E,
+                       fx, X, Y, oa);
+  }
   SKT_LAUNCH_CHECK();
   return SKT_OK;
 }

@@ -1288,6 +1288,13 @@ void mg_free(MGData* mg) {
   }
   mg->have_inv = 0;
   mg->inv_pending = 0;
+  for (hipEvent_t& e : mg->tev)
+    if (e) {
+      (void)hipEventDestroy(e);
+      e = nullptr;
+    }
+  mg->tcount = 0;
+  mg->time_fine = 0;
   for (MGLevel& l : mg->lv) {
     (void)hipFree(l.Kel); (void)hipFree(l.st); (void)hipFree(l.dinv);
     (void)hipFree(l.b); (void)hipFree(l.x); (void)hipFree(l.w); (void)hipFree(l.d);
@@ -1506,6 +1513,40 @@ static void queue_gj(MGData* mg, hipStream_t g) {
 }
 
 bool mg_lagged(const MGData* mg) { return mg->lag && mg->have_inv; }
+
+int mg_set_timing(MGData* mg, int enable) {
+  if (enable && !mg->tev[0])
+    for (hipEvent_t& e : mg->tev) SKT_HIP(hipEventCreate(&e));
+  mg->time_fine = enable ? 1 : 0;
+  return SKT_OK;
+}
+
+int mg_collect_timing(MGData* mg) {
+  for (int i = 0; i < mg->tcount; ++i) {
+    if (mg->ttag[i] < 0) continue;
+    float ms = 0.f;
+    SKT_HIP(hipEventElapsedTime(&ms, mg->tev[2 * i], mg->tev[2 * i + 1]));
+    mg->tms[mg->ttag[i]] += ms;
+    mg->tn[mg->ttag[i]] += 1;
+  }
+  mg->tcount = 0;
+  return SKT_OK;
+}
+
+// event pair around one fine-level launch when timing (at most kTimedMax
+// between two collections; later launches go untimed)
+static int fine_timer_begin(MGData* mg, int tag, hipStream_t) {
+  if (!mg->time_fine || mg->tcount >= MGData::kTimedMax) return -1;
+  const int i = mg->tcount++;
+  mg->ttag[i] = tag;
+  modal_time_next(mg->tev[2 * i], mg->tev[2 * i + 1]);
+  return i;
+}
+
+// a request that no launch took leaves its events unrecorded: drop the slot
+static void fine_timer_end(MGData* mg, int i, hipStream_t) {
+  if (i >= 0 && modal_time_clear()) mg->ttag[i] = -1;
+}
 
 int mg_lag_pre(MGData* mg, hipStream_t s) {
   if (!mg->inv_pending) return SKT_OK;
@@ -1836,9 +1877,11 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
       // steps 0 and 1 in one pass (step 0's iterate is formed on the fly
       // and never stored, so this form serves degree 2 only: a third step
       // would need it as its previous iterate)
+      const int ti = fine_timer_begin(mg, MG_T_CHEB0, s);
       if ((stt = hex_cheb0_modal<T>(op, k, KC, coef0, E0, r, oth, (const T*)dinv0, cf, cf + 2,
                                     s, stop)))
         return stt;
+      fine_timer_end(mg, ti, s);
       std::swap(cur, oth);
     } else {
       hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
@@ -1846,13 +1889,17 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
       // step st reads its previous iterate from the buffer it overwrites:
       // zero for step 1, the iterate of step st - 2 after that
       for (int st = 1; st < deg; ++st) {
+        const int ti = fine_timer_begin(mg, MG_T_CHEB, s);
         if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, (const T*)dinv0,
                                      cf + 2 * st, st == 1 ? 1 : 0, s, nullptr, stop)))
           return stt;
+        fine_timer_end(mg, ti, s);
         std::swap(cur, oth);
       }
     }
+    const int tr = fine_timer_begin(mg, MG_T_RESID, s);
     if ((stt = hex_resid_modal<T>(op, k, KC, coef0, E0, cur, r, w0, s, stop))) return stt;
+    fine_timer_end(mg, tr, s);
     launch_restrict<T, NC, KC>(v, mg->lv[1], groups, (const T*)w0, (const T*)nullptr, fixed0,
                                (T*)mg->lv[1].b, s);
     SKT_LAUNCH_CHECK();
@@ -1942,10 +1989,12 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     double* last_dot = (dot && slots && ns <= max_slots) ? dot : nullptr;
     for (int st = 0; st < deg; ++st) {
       // step 0 has c1 = 0 and needs no previous iterate
+      double* sd = st == deg - 1 ? last_dot : nullptr;
+      const int ti = fine_timer_begin(mg, sd ? MG_T_CHEB_DOT : MG_T_CHEB, s);
       if ((stt = hex_cheb_modal<T>(op, k, KC, coef0, E0, cur, oth, r, (const T*)dinv0,
-                                   cf + 2 * st, 0, s, st == deg - 1 ? last_dot : nullptr,
-                                   stop)))
+                                   cf + 2 * st, 0, s, sd, stop)))
         return stt;
+      fine_timer_end(mg, ti, s);
       std::swap(cur, oth);
     }
     if (slots) *slots = (last_dot && deg >= 1) ? ns : 0;

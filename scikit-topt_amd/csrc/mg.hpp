@@ -76,7 +76,20 @@ struct MGData {
   int inv_pending = 0;                // Ad holds, or is becoming, a newer inverse
   hipStream_t gjs = nullptr;
   hipEvent_t ev_asm = nullptr, ev_gj = nullptr;
+  // Optional HIP-event timing of the fine level's fused modal launches
+  // (the solver's timing mode, direct launches): event pairs tagged with
+  // the variant (MG_T_*), folded into per-variant totals by mg_collect_timing.
+  static constexpr int kTimedMax = 256;
+  hipEvent_t tev[2 * kTimedMax] = {};
+  int ttag[kTimedMax] = {};
+  int tcount = 0;
+  int time_fine = 0;
+  double tms[4] = {0, 0, 0, 0};
+  int64_t tn[4] = {0, 0, 0, 0};
 };
+
+// fine-level modal variants timed by the V-cycle
+enum { MG_T_CHEB0 = 0, MG_T_RESID = 1, MG_T_CHEB = 2, MG_T_CHEB_DOT = 3 };
 
 struct MGConfig {
   int degree = 2;          // Chebyshev steps per smoothing
@@ -106,6 +119,11 @@ int mg_setup(MGData* mg, const skt_operator* op, const double* E64, const void* 
 // it into Ainv.  mg_lag_post, after it: the assembled matrix is inverted on
 // the multigrid's own stream once `s` reaches this point.
 bool mg_lagged(const MGData* mg);
+
+// Timing of the fine-level V-cycle launches (creates the events on first
+// use); mg_collect_timing folds completed event pairs into tms / tn.
+int mg_set_timing(MGData* mg, int enable);
+int mg_collect_timing(MGData* mg);
 int mg_lag_pre(MGData* mg, hipStream_t s);
 int mg_lag_post(MGData* mg, hipStream_t s);
 

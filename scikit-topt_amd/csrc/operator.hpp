@@ -109,6 +109,12 @@ int hex_cheb_modal(const skt_operator* op, int k, int kc, const T* coef, const T
 // Y = K(E) X for ncols columns.  kc == ncols: (ndof, ncols) row-major;
 // otherwise the column-blocked layout [ncols/kc][ndof][kc] (structured hex8
 // only).
+// Timing of the next modal-kernel launch on this host thread: the launch
+// records `start` and `stop` itself (hipExtLaunchKernelGGL).  modal_time_clear
+// drops a request no launch took and returns whether one was pending.
+void modal_time_next(hipEvent_t start, hipEvent_t stop);
+bool modal_time_clear();
+
 int op_apply(skt_operator* op, int dtype, int ncols, int kc, const void* E,
              const void* X, void* Y, hipStream_t stream);
 // op_apply that also leaves the per-column partials of X . Y in dot (see

@@ -570,14 +570,28 @@ static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t
   // initialisation and the reliable updates
   double* area1 = s->part + (size_t)2 * kParts * s->max_cols;
   s->timed = s->timing && !any_flag(s, b.k, S_CONV);
+  // a modal operator's apply is timed by its own dispatch (the events
+  // bracket the kernel, not the queue around it); others by event records
+  const bool modal = s->op->mesh == SKT_MESH_HEX8 && s->op->modal_ok;
   for (int64_t it = 0; it < count; ++it) {
-    if (s->timed) SKT_HIP(hipEventRecord(s->ev[2 * it], st));
+    if (s->timed) {
+      if (modal) modal_time_next(s->ev[2 * it], s->ev[2 * it + 1]);
+      else SKT_HIP(hipEventRecord(s->ev[2 * it], st));
+    }
     int pq_nb = 0;
     // stopped columns' q is not used (alpha = 0), so their work may be skipped
     int stt = op_apply_dot(s->op, v.dtype, b.k, b.kc, v.E, v.p, v.q, s->part, 2 * kParts,
                            &pq_nb, st, s->sc + S_CONV);
     if (stt) return stt;
-    if (s->timed) SKT_HIP(hipEventRecord(s->ev[2 * it + 1], st));
+    if (s->timed) {
+      if (!modal) {
+        SKT_HIP(hipEventRecord(s->ev[2 * it + 1], st));
+      } else if (modal_time_clear()) {
+        // no modal launch took the events: record them here instead
+        SKT_HIP(hipEventRecord(s->ev[2 * it], st));
+        SKT_HIP(hipEventRecord(s->ev[2 * it + 1], st));
+      }
+    }
     if (pq_nb == 0) {
       hipLaunchKernelGGL(k_dot<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
                          b.cp, (const T*)v.p, (const T*)v.q, s->part);
@@ -674,6 +688,8 @@ static int run_chunk(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t coun
 // Fold the apply timings of the last chunk into the totals (events of the
 // chunk have completed once the poll has synchronised the stream).
 static int collect_timing(skt_solver* s, int dtype, int64_t count) {
+  if (s->mg.time_fine)
+    if (int st = mg_collect_timing(&s->mg)) return st;
   if (!s->timed) return SKT_OK;
   for (int64_t k = 0; k < count; ++k) {
     float ms = 0.f;
@@ -765,7 +781,8 @@ static int setup_precond(skt_solver* s, const Blk& b, const double* E64, const V
   stt = run_setup<T>(s, E64, v, st);
   if (stt) return stt;
   s->used = SKT_PC_MG;
-  return SKT_OK;
+  // the V-cycle's fine-level launches are timed with the applies
+  return mg_set_timing(&s->mg, s->timing);
 }
 
 template <typename T>
@@ -925,6 +942,19 @@ int skt_solver_set_timing(skt_solver* s, int enable) {
     for (int i = 0; i < 2 * 32; ++i) SKT_HIP(hipEventCreate(&s->ev[i]));
   }
   s->timing = enable ? 1 : 0;
+  return SKT_OK;
+}
+
+int skt_solver_get_vcycle_timing(skt_solver* s, double* ms, int64_t* n, int reset) {
+  SKT_CHECK(s && ms && n, "null pointer");
+  for (int i = 0; i < 4; ++i) {
+    ms[i] = s->mg.tms[i];
+    n[i] = s->mg.tn[i];
+    if (reset) {
+      s->mg.tms[i] = 0.0;
+      s->mg.tn[i] = 0;
+    }
+  }
   return SKT_OK;
 }
 

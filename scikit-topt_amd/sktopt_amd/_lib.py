@@ -68,6 +68,7 @@ SIGNATURES = {
     "skt_solver_info": [_vp, _i32p, _i32p],
     "skt_solver_set_timing": [_vp, _i32],
     "skt_solver_get_timing": [_vp, _dp, _i64p, _i32],
+    "skt_solver_get_vcycle_timing": [_vp, _dp, _i64p, _i32],
     "skt_solver_graph_stats": [_vp, _i64p, _i64p],
     "skt_solver_solve": [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _d, _i32, _d, _i64p, _dp, _vp],
     "skt_filter_create": [_i64, _vp, _vp, _d, _d, _pp],
