@@ -96,7 +96,9 @@ class DensityLoop:
             self.fem = fem_mod.HeatFEM(task, cfg.E_min_coeff,
                                        cfg.interpolation, cfg.solver,
                                        cfg.solver_rtol)
-        self.design = task.design_elements
+        self.design = task.design_elements_with_dirichlet if cfg.design_dirichlet \
+            else task.design_elements
+        design_mask = np.isin(np.arange(task.n_elements), self.design)
         vol = task.elements_volume[self.design]
         self.vol_design = vol
         self.vol_design_sum = np.sum(vol)
@@ -106,10 +108,10 @@ class DensityLoop:
         if cfg.filter_type == "helmholtz":
             self.filter = helmholtz.HelmholtzFilterNodal(task.mesh, task.elements_volume,
                                                          cfg.filter_radius.init,
-                                                         task.design_mask)
+                                                         design_mask)
         else:
             self.filter = filt_mod.SpacialFilter(task.centers, cfg.filter_radius.init,
-                                                 task.design_mask)
+                                                 design_mask)
         self.filter.update_radius(r0)
         self.update = oc_mod.OCUpdate(cfg) if cfg.method == "OC" \
             else logmoc_mod.LogMOCUpdate(cfg)

@@ -30,6 +30,7 @@ class Task:
     dirichlet_values: list = field(default_factory=lambda: [0.0])
     quad_order: int = 2
     design_elements: np.ndarray = None
+    design_elements_with_dirichlet: np.ndarray = None
     dirichlet_elements: np.ndarray = None
     neumann_elements: np.ndarray = None
     fixed_elements: np.ndarray = None
@@ -47,6 +48,10 @@ class Task:
             self.neumann_elements = np.array([], dtype=np.int64)
         design = all_el[~np.isin(all_el, self.neumann_elements)]
         self.fixed_elements = all_el[~np.isin(all_el, design)]
+        # the task's design set (mesh/task_common.py:211-227) keeps the
+        # elements touching Dirichlet nodes; DensityMethod drops them unless
+        # design_dirichlet=True (common_density.py:546-547, task_common.py:398-401)
+        self.design_elements_with_dirichlet = np.ascontiguousarray(design)
         design = design[~np.isin(design, self.dirichlet_elements)]
         self.design_elements = np.ascontiguousarray(design)
         valid = [a for a in (self.dirichlet_elements, self.neumann_elements) if len(a) > 0]

@@ -204,3 +204,43 @@ def test_scaling_option_matches_scaled_oracle_problem(gpu, tmp_path):
     # the run ended: the task is back at its own lengths and loads
     np.testing.assert_array_equal(task.mesh.p, p0)
     np.testing.assert_allclose(task.neumann_linear[0], f0, rtol=1e-15)
+
+
+@pytest.mark.parametrize("method", ["OC", "LogMOC"])
+def test_design_dirichlet_true_matches_oracle(gpu, method, tmp_path):
+    """``design_dirichlet=True`` (``common_density.py:736-744``): the elements
+    touching Dirichlet nodes stay design variables (only the Neumann ones
+    are pinned to 1), six iterations against the oracle's loop."""
+    from sktopt_amd import core
+    from sktopt_amd.mesh import toy_problem
+    from sktopt_amd.tools import SchedulerConfig as SC
+    n = 6
+    S = odrv.Sched
+    kw = dict(lambda_lower=-1e7, lambda_upper=1e7) if method == "LogMOC" else {}
+    ocfg = odrv.Config(method=method, max_iters=n, p=S("Constant", 3.0, 3.0),
+                       vol_frac=S("Constant", 0.3, 0.3), beta=S("Constant", 1.0, 1.0),
+                       filter_radius=S("Constant", 1.5, 1.5), move_limit=S("Constant", 0.2, 0.2),
+                       eta=S("Constant", 0.5, 0.5), E_min_coeff=1e-9, solver="spsolve",
+                       design_dirichlet=True, **kw)
+    loop = odrv.DensityLoop(ocfg, oprob.cantilever(16, 8, 8))
+    common = dict(dst_path=str(tmp_path), max_iters=n, p=SC.constant(target_value=3.0),
+                  vol_frac=SC.constant(target_value=0.3), beta=SC.constant(target_value=1.0),
+                  filter_type="spacial", filter_radius=SC.constant(target_value=1.5),
+                  move_limit=SC.constant(target_value=0.2), E_min_coeff=1e-9,
+                  eta=SC.constant(target_value=0.5), precision="fp64", solver_rtol=1e-13,
+                  design_dirichlet=True)
+    task = toy_problem.cantilever(16, 8, 8)
+    if method == "OC":
+        opt = core.OC_Optimizer(core.OC_Config(**common), task)
+    else:
+        opt = core.LogMOC_Optimizer(core.LogMOC_Config(**common, **kw), task)
+    opt.parameterize()
+    # the design set keeps the clamped face's elements
+    assert np.intersect1d(task.design_elements, task.dirichlet_elements).size > 0
+    np.testing.assert_array_equal(np.sort(task.design_elements), np.sort(loop.design))
+    for it in range(1, n + 1):
+        rec = loop.step(it)
+        opt.optimize_steps(1)
+        c = opt.recorder.latest("compliance")
+        assert abs(c - rec["compliance"]) <= 1e-9 * abs(rec["compliance"]), (it, c, rec)
+        assert np.abs(opt.rho_host() - loop.rho).max() <= 1e-8, it

@@ -92,3 +92,28 @@ def test_golden_c0_trajectory_reproduces():
     assert np.all(np.diff(g["compliance"][:10]) < 0)
     for k in (10, 25, 50):
         assert g[f"rho_{k}"].min() >= 0.01 - 1e-15 and g[f"rho_{k}"].max() <= 1.0
+
+
+def test_design_set_follows_design_dirichlet():
+    """mesh/task_common.py:211-227 keeps the elements touching Dirichlet nodes
+    in the task's design set; DensityMethod removes them unless
+    design_dirichlet=True (common_density.py:546-547), and pins the Neumann
+    elements (True) or the Dirichlet and Neumann ones (False) to 1
+    (:736-744, :1131-1134)."""
+    import numpy as np
+    from oracle import driver as odrv
+    from oracle import problems as oprob
+    S = odrv.Sched
+    task = oprob.cantilever(8, 4, 4)
+    all_el = np.arange(task.n_elements)
+    for dd in (False, True):
+        cfg = odrv.Config(max_iters=1, p=S("Constant", 3.0, 3.0), vol_frac=S("Constant", 0.3, 0.3),
+                          beta=S("Constant", 1.0, 1.0), filter_radius=S("Constant", 1.5, 1.5),
+                          move_limit=S("Constant", 0.2, 0.2), design_dirichlet=dd)
+        loop = odrv.DensityLoop(cfg, oprob.cantilever(8, 4, 4))
+        excluded = task.neumann_elements if dd else \
+            np.union1d(task.neumann_elements, task.dirichlet_elements)
+        np.testing.assert_array_equal(loop.design, np.setdiff1d(all_el, excluded))
+        pinned = np.nonzero(loop.rho == 1.0)[0]
+        np.testing.assert_array_equal(pinned, np.sort(excluded))
+        assert np.all(loop.rho[loop.design] == 0.3)
