@@ -360,6 +360,9 @@ class DensityMethod:
         k = tsk.n_tasks
         # filter -> projection (a replayed graph once captured)
         self._phase("pre", (float(beta),), lambda: self._step_pre(beta))
+        predict = getattr(self.fem, "predict_warm_start", None)
+        if predict is not None:
+            predict(st["u_dofs"], st.setdefault("u_hist", {}))
         comp = self.fem.objectives_multi_load(st["rho_projected"], p, st["u_dofs"],
                                               force_scale=neumann_scale)
         if "comp" not in st:

@@ -49,6 +49,9 @@ class LinearSolverConfig:
     inner_rtol: float = 0.0           # fp32 residual drop that triggers a reliable update
                                       # (<= 0: 1e-2 with multigrid, 1e-1 with Jacobi)
     warm_start: bool = True           # start from the previous u_dofs
+    # the design loop's state solves start from U + theta (U - U_prev), U and
+    # U_prev the last two solutions (0: the last solution as it is)
+    warm_start_extrapolation: float = 0.5
     on_noconv: str = "raise"          # "raise" or "warn"
     preconditioner: str = "auto"      # "auto" (multigrid where possible), "mg", "jacobi"
     mg_smooth_degree: int = 2         # Chebyshev-Jacobi steps per smoothing
@@ -173,6 +176,35 @@ class _GpuFEM:
             logger.warning("state solve did not converge: %s", _lib.last_error())
         else:
             _lib.check(st, "skt_solver_solve")
+
+    def predict_warm_start(self, U: torch.Tensor, hist: dict):
+        """Before a design loop's state solve: U holds the last solution;
+        replace it by ``U + theta (U - U_prev)`` (theta =
+        ``solver_config.warm_start_extrapolation``), U_prev the solution
+        before it, kept in ``hist`` (the caller's, one per state).  The first
+        two solves keep U as it is (the solution before the first one is only
+        the initial guess).  The solve's tolerance fixes the result; a better
+        start saves iterations."""
+        cfg = self.solver_config
+        theta = float(cfg.warm_start_extrapolation)
+        if not cfg.warm_start or theta == 0.0:
+            return
+        S = _dev.stream()
+        n = U.numel()
+        prev = hist.get("prev")
+        if prev is None or prev.shape != U.shape:
+            hist["prev"] = U.clone()
+            hist["spare"] = torch.empty_like(U)
+            hist["count"] = 1
+            return
+        spare = hist["spare"]
+        # spare <- U (the last solution), then U <- (1 + theta) U - theta U_prev
+        _lib.call("skt_ewise", 0, n, 1.0, _dev.ptr(U), 0.0, None, _dev.ptr(spare), S)
+        if hist["count"] >= 2:
+            _lib.call("skt_ewise", 0, n, 1.0 + theta, _dev.ptr(spare), -theta, _dev.ptr(prev),
+                      _dev.ptr(U), S)
+        hist["prev"], hist["spare"] = spare, prev
+        hist["count"] += 1
 
     def graph_stats(self) -> dict:
         """Replays and captures of the solver's PCG-chunk hipGraphs."""

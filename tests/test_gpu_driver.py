@@ -271,3 +271,40 @@ def test_reference_toy_problem_matches_oracle(gpu, tmp_path):
         c = opt.recorder.latest("compliance")
         assert abs(c - rec["compliance"]) <= 1e-9 * abs(rec["compliance"]), (it, c, rec)
         assert np.abs(opt.rho_host() - loop.rho).max() <= 1e-8
+
+
+def test_warm_start_extrapolation(gpu, tmp_path):
+    """The state solve's start U + theta (U - U_prev): the first two solves
+    keep U, the third starts from (1 + theta) U_2 - theta U_1; and the design
+    loop's trajectory is the one without extrapolation, to the solve's
+    tolerance."""
+    import dataclasses
+    import torch
+    opt = _gpu_opt(oprob.cantilever(16, 8, 8), precision="mixed", rtol=1e-10,
+                   tmp=str(tmp_path / "a"))
+    fem = opt.fem
+    assert fem.solver_config.warm_start_extrapolation == 0.5
+    g = torch.Generator().manual_seed(3)
+    seq = [torch.randn(fem.ndof, 1, generator=g, dtype=torch.float64).cuda() for _ in range(4)]
+    hist, U = {}, seq[0].clone()
+    fem.predict_warm_start(U, hist)                 # initial guess
+    assert torch.equal(U, seq[0])
+    U.copy_(seq[1])
+    fem.predict_warm_start(U, hist)                 # after the first solve
+    assert torch.equal(U, seq[1])
+    for k in (2, 3):
+        U.copy_(seq[k])
+        fem.predict_warm_start(U, hist)
+        torch.cuda.synchronize()
+        ref = 1.5 * seq[k] - 0.5 * seq[k - 1]
+        assert torch.allclose(U, ref, rtol=1e-15, atol=1e-14), k
+    runs = {}
+    for theta in (0.0, 0.5):
+        o = _gpu_opt(oprob.cantilever(16, 8, 8), precision="mixed", rtol=1e-10,
+                     tmp=str(tmp_path / str(theta)))
+        o.fem.solver_config = dataclasses.replace(o.fem.solver_config,
+                                                  warm_start_extrapolation=theta)
+        o.optimize_steps(8)
+        runs[theta] = (np.array(o.recorder.as_object().compliance), o.rho_host())
+    np.testing.assert_allclose(runs[0.5][0], runs[0.0][0], rtol=1e-7)
+    assert np.abs(runs[0.5][1] - runs[0.0][1]).max() <= 1e-5
