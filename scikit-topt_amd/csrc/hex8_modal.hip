@@ -220,7 +220,11 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   X += cbase;
   Y += cbase;
   constexpr bool kCheb = MODE == OUT_CHEB || MODE == OUT_CHEB0;
-  constexpr bool kLate = sizeof(T) == 4 && MODE == OUT_CHEB && DOT;
+  constexpr bool kC0 = MODE == OUT_CHEB0;
+  // dinv (and for OUT_CHEB0 b) of the owned nodes read in the epilogue rather
+  // than at the start of the layer, which keeps these variants within 128
+  // VGPRs (four waves per SIMD) without spills
+  constexpr bool kLate = (sizeof(T) == 4 && MODE == OUT_CHEB && DOT) || kC0;
   const T* Bv = MODE != OUT_APPLY ? oa.B + cbase : nullptr;
   T c1 = T(0), c2 = T(0), s0 = T(0);
   if (kCheb) {
@@ -251,39 +255,55 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   // the owned node's values: ((own_node * NC + d) * dstride + col) in a plane
   const int64_t own_off = own_node * NC * (int64_t)dstride;
 
-  int f_off[kPre], f_lds[kPre];
-  int f_dof[MODE == OUT_CHEB0 ? kPre : 1];
+  // staged item r of this thread: LDS slot, and its in-plane dof (node * NC +
+  // comp; -1 outside the grid); the column of every item is tid % KC (256 is a
+  // multiple of KC), so the value offset is dof * dstride + tid % KC.
+  // OUT_CHEB0 keeps only the dof and forms the offset at each fetch.
+  int f_off[kC0 ? 1 : kPre], f_lds[kPre];
+  int f_dof[kC0 ? kPre : 1];
+  const int f_col = tid % KC;
 #pragma unroll
   for (int r = 0; r < kPre; ++r) {
     const int i = tid + 256 * r;
-    f_off[r] = -1;
+    f_off[kC0 ? 0 : r] = -1;
     f_lds[r] = -1;
-    if constexpr (MODE == OUT_CHEB0) f_dof[r] = 0;
+    if constexpr (kC0) f_dof[r] = -1;
     if (i < kItems) {
       const int col = i % KC, comp = (i / KC) % NC, nl = i / (KC * NC);
       const int64_t gx = x0 - 1 + nl % kPW, gy = y0 - 1 + nl / kPW;
       f_lds[r] = (comp * KC + col) * kPS + node_slot(nl % kPW, nl / kPW);
       if (gx >= 0 && gx < nnx && gy >= 0 && gy < nny) {
-        f_off[r] = (int)(((gy * nnx + gx) * NC + comp) * dstride + col);
-        if constexpr (MODE == OUT_CHEB0) f_dof[r] = (int)((gy * nnx + gx) * NC + comp);
+        if constexpr (kC0)
+          f_dof[r] = (int)((gy * nnx + gx) * NC + comp);
+        else
+          f_off[r] = (int)(((gy * nnx + gx) * NC + comp) * dstride + col);
       }
     }
   }
 
   // prefetched plane values stay in flight until stash (the compute of the
-  // current layer runs meanwhile); OUT_CHEB0 also fetches dinv and forms the
-  // implied first iterate s0 dinv b only at stash time
+  // current layer runs meanwhile); OUT_CHEB0 fetches dinv separately, after
+  // the layer's compute (fetch_dinv), and forms the implied first iterate
+  // s0 dinv b at stash time
   T pre[kPre];
-  T pred[MODE == OUT_CHEB0 ? kPre : 1];
+  T pred[kC0 ? kPre : 1];
   auto fetch = [&](int64_t gz) {
     const bool zin = gz >= 0 && gz < nnz;
     const T* Xp = X + (zin ? gz : 0) * plane_stride;
 #pragma unroll
-    for (int r = 0; r < kPre; ++r) pre[r] = (zin && f_off[r] >= 0) ? Xp[f_off[r]] : T(0);
-    if constexpr (MODE == OUT_CHEB0) {
+    for (int r = 0; r < kPre; ++r) {
+      if constexpr (kC0)
+        pre[r] = (zin && f_dof[r] >= 0) ? Xp[f_dof[r] * dstride + f_col] : T(0);
+      else
+        pre[r] = (zin && f_off[r] >= 0) ? Xp[f_off[r]] : T(0);
+    }
+  };
+  auto fetch_dinv = [&](int64_t gz) {
+    if constexpr (kC0) {
+      const bool zin = gz >= 0 && gz < nnz;
       const T* Dp = oa.dinv + (zin ? gz : 0) * nnx * nny * NC;
 #pragma unroll
-      for (int r = 0; r < kPre; ++r) pred[r] = (zin && f_off[r] >= 0) ? Dp[f_dof[r]] : T(0);
+      for (int r = 0; r < kPre; ++r) pred[r] = (zin && f_dof[r] >= 0) ? Dp[f_dof[r]] : T(0);
     }
   };
   auto stash = [&](T* dst) {
@@ -308,8 +328,10 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
 
   int bot = 0, top = 1;
   fetch(z0 - 1);
+  fetch_dinv(z0 - 1);
   stash(planes + bot * kPlane);
   fetch(z0);
+  fetch_dinv(z0);
   stash(planes + top * kPlane);
   // SF carries: face butterfly of the element's footprint on the bottom
   // plane, and the top-face forces of the layer below
@@ -356,7 +378,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
         oi[d] = out ? oa.dinv[(ez * nnx * nny + own_node) * NC + d] : T(0);
 #pragma unroll
       for (int c = 0; c < KC; ++c) {
-        if constexpr (MODE != OUT_APPLY) ob[c][d] = out ? Bv[po + d * dstride + c] : T(0);
+        if constexpr (MODE != OUT_APPLY && !kC0) ob[c][d] = out ? Bv[po + d * dstride + c] : T(0);
         if constexpr (MODE == OUT_CHEB && !kLate)
           op[c][d] = (out && use_prev) ? Y[po + d * dstride + c] : T(0);
       }
@@ -442,6 +464,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
       }
     }
     __syncthreads();
+    fetch_dinv(ez + 2);  // for the stash below
     if (is_node) {
       T* Yp = Y + po;
 #pragma unroll
@@ -473,13 +496,14 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
             } else if constexpr (MODE == OUT_RESID) {
               Yp[o] = ob[col][d] - y;
             } else {
+              if constexpr (kC0) ob[col][d] = Bv[po + o];
               // Chebyshev step with the direction recovered from the two
               // iterates: x_out = x + c1 (x - x_prev) + c2 dinv (b - K x)
               // (x_prev = 0 for the implied first step of OUT_CHEB0)
               T xp = MODE == OUT_CHEB ? op[col][d] : T(0);
               T di = oi[d];
               if constexpr (kLate) {
-                xp = use_prev ? Yp[o] : T(0);
+                if constexpr (MODE == OUT_CHEB) xp = use_prev ? Yp[o] : T(0);
                 di = oa.dinv[(ez * nnx * nny + own_node) * NC + d];
               }
               const T dn = c1 * (xo[col][d] - xp) + c2 * di * (ob[col][d] - y);

@@ -1353,6 +1353,14 @@ static int warm_power_steps(const MGData* mg) {
 
 int mg_warm_steps(const MGData* mg) { return warm_power_steps(mg); }
 
+// SKT_MG_CHEB0=0 runs a degree-2 fine-level smoothing as the separate first
+// step plus one fused step instead of the single two-step pass (the same
+// arithmetic; a test checks that the solves agree bit for bit).
+static bool cheb0_enabled() {
+  const char* v = std::getenv("SKT_MG_CHEB0");
+  return !(v && v[0] == '0' && v[1] == 0);
+}
+
 static int lag_enabled() {
   const char* v = std::getenv("SKT_MG_LAG");
   return (v && v[0] == '1' && v[1] == 0) ? 1 : 0;
@@ -1873,7 +1881,7 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     MGLevel& v = mg->lv[0];
     const int64_t total = v.ndof * k;
     const double* cf = mg->coef;
-    if (deg == 2) {
+    if (deg == 2 && cheb0_enabled()) {
       // steps 0 and 1 in one pass (step 0's iterate is formed on the fly
       // and never stored, so this form serves degree 2 only: a third step
       // would need it as its previous iterate)

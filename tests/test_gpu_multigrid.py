@@ -201,3 +201,30 @@ def test_mg_stencil_layouts_match(gpu, precision, monkeypatch):
     for mode in ("big", "small"):
         assert out[mode][1] == out["node"][1], mode
         np.testing.assert_array_equal(out[mode][0], out["node"][0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("physics", ["elastic", "heat"])
+def test_mg_two_step_smoothing_pass_matches_separate_steps(gpu, precision, physics,
+                                                            monkeypatch):
+    """At degree 2 the fine level's two Chebyshev steps from the zero iterate
+    run as one operator pass, the first iterate s0 D^-1 b formed while the
+    node planes are staged.  The separate form (first step, then one fused
+    step, forced by SKT_MG_CHEB0=0) does the same arithmetic, so the solves
+    agree bit for bit, iteration count included."""
+    if physics == "elastic":
+        task, max_coarse = oprob.multiload(32, 16, 16, n_loads=2, seed=7), 100
+    else:
+        task, max_coarse = oprob.heat_patch(16, patch=4), 40
+    rng = np.random.default_rng(5)
+    rho = rng.uniform(0.05, 1.0, task.n_elements)
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("SKT_MG_CHEB0", flag)
+        fem = _fem(task, physics, precision, 1e-10, "mg", max_coarse=max_coarse)
+        U = np.zeros((task.ndof, task.n_tasks))
+        fem.objectives_multi_load(rho, 3.0, U)
+        assert fem.last_solve["levels"] >= 2
+        out[flag] = (U, fem.last_solve["iters"])
+    assert out["0"][1] == out["1"][1]
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
