@@ -221,10 +221,10 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   Y += cbase;
   constexpr bool kCheb = MODE == OUT_CHEB || MODE == OUT_CHEB0;
   constexpr bool kC0 = MODE == OUT_CHEB0;
-  // dinv (and for OUT_CHEB0 b) of the owned nodes read in the epilogue rather
-  // than at the start of the layer, which keeps these variants within 128
-  // VGPRs (four waves per SIMD) without spills
-  constexpr bool kLate = (sizeof(T) == 4 && MODE == OUT_CHEB && DOT) || kC0;
+  // OUT_CHEB0 reads b and dinv of the owned nodes in the epilogue rather than
+  // at the start of the layer, which keeps it within 128 VGPRs (four waves
+  // per SIMD) without spills
+  constexpr bool kLate = kC0;
   const T* Bv = MODE != OUT_APPLY ? oa.B + cbase : nullptr;
   T c1 = T(0), c2 = T(0), s0 = T(0);
   if (kCheb) {
@@ -367,9 +367,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
     const bool out = is_node && ez >= z0;
     const int64_t po = ez * plane_stride + own_off;
     bool fix[NC];
-    // kLate (the float32 smoothing step with the fused r.z): the previous
-    // iterate and dinv are loaded in the epilogue instead, which keeps the
-    // kernel within 128 VGPRs (four waves per SIMD) without spills
+    // (kLate: loaded in the epilogue instead)
     T ob[KC][NC], op[KC][NC], oi[NC];
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
@@ -386,13 +384,6 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
     __syncthreads();
     const T* pb = planes + bot * kPlane;
     const T* pt = planes + top * kPlane;
-    // the owned node's x (node plane ez = bottom LDS plane), for Dirichlet
-    // rows (y = x) and the Chebyshev update
-    T xo[KC][NC];
-#pragma unroll
-    for (int c = 0; c < KC; ++c)
-#pragma unroll
-      for (int d = 0; d < NC; ++d) xo[c][d] = pb[(d * KC + c) * kPS + own_slot];
 #pragma unroll
     for (int col = 0; col < KC; ++col) {
       T m[NC][8];
@@ -463,6 +454,15 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
         }
       }
     }
+    // the owned node's x (node plane ez, the bottom LDS plane), for Dirichlet
+    // rows (y = x) and the Chebyshev update: read after the compute, so that
+    // it holds no registers there, and before the barrier, after which other
+    // waves' stash overwrites that plane
+    T xo[KC][NC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+      for (int d = 0; d < NC; ++d) xo[c][d] = pb[(d * KC + c) * kPS + own_slot];
     __syncthreads();
     fetch_dinv(ez + 2);  // for the stash below
     if (is_node) {
