@@ -221,10 +221,10 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   Y += cbase;
   constexpr bool kCheb = MODE == OUT_CHEB || MODE == OUT_CHEB0;
   constexpr bool kC0 = MODE == OUT_CHEB0;
-  // OUT_CHEB0 reads b and dinv of the owned nodes in the epilogue rather than
-  // at the start of the layer, which keeps it within 128 VGPRs (four waves
-  // per SIMD) without spills
-  constexpr bool kLate = kC0;
+  // float64 OUT_CHEB0 reads b and dinv of the owned nodes in the epilogue
+  // rather than at the start of the layer (158 instead of 178 VGPRs, three
+  // waves per SIMD instead of two); in float32 the early loads fit in 128
+  constexpr bool kLate = kC0 && sizeof(T) == 8;
   const T* Bv = MODE != OUT_APPLY ? oa.B + cbase : nullptr;
   T c1 = T(0), c2 = T(0), s0 = T(0);
   if (kCheb) {
@@ -376,7 +376,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
         oi[d] = out ? oa.dinv[(ez * nnx * nny + own_node) * NC + d] : T(0);
 #pragma unroll
       for (int c = 0; c < KC; ++c) {
-        if constexpr (MODE != OUT_APPLY && !kC0) ob[c][d] = out ? Bv[po + d * dstride + c] : T(0);
+        if constexpr (MODE != OUT_APPLY && !kLate) ob[c][d] = out ? Bv[po + d * dstride + c] : T(0);
         if constexpr (MODE == OUT_CHEB && !kLate)
           op[c][d] = (out && use_prev) ? Y[po + d * dstride + c] : T(0);
       }
@@ -496,7 +496,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
             } else if constexpr (MODE == OUT_RESID) {
               Yp[o] = ob[col][d] - y;
             } else {
-              if constexpr (kC0) ob[col][d] = Bv[po + o];
+              if constexpr (kLate) ob[col][d] = Bv[po + o];
               // Chebyshev step with the direction recovered from the two
               // iterates: x_out = x + c1 (x - x_prev) + c2 dinv (b - K x)
               // (x_prev = 0 for the implied first step of OUT_CHEB0)
