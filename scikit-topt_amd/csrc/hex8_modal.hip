@@ -494,7 +494,8 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
               Yp[o] = y;
               if constexpr (DOT) dacc[col] += (double)xo[col][d] * (double)y;
             } else if constexpr (MODE == OUT_RESID) {
-              Yp[o] = ob[col][d] - y;
+              // zero at Dirichlet DOFs, which the restriction drops
+              Yp[o] = fix[d] ? T(0) : ob[col][d] - y;
             } else {
               if constexpr (kLate) ob[col][d] = Bv[po + o];
               // Chebyshev step with the direction recovered from the two

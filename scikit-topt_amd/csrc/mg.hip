@@ -1105,6 +1105,164 @@ __global__ void k_restrict_z(int cnnx, int cnny, int fnz, int fz, int groups,
   }
 }
 
+// Restriction for factor 2 on every axis in one pass, b_c = P^T (b - w)
+// (HASW: w given).  A 256-thread workgroup owns kR2I x kR2J coarse nodes of
+// one column group and a chunk of zlen coarse planes (one where that leaves
+// too few workgroups to hide the planes' latency), and streams the fine planes
+// 2 K0 - 1 .. 2 K1 - 1 through LDS: per plane, the tile's (2 kR2I + 1) x
+// (2 kR2J + 1) fine nodes, each fine row a contiguous run of the blocked
+// layout (coalesced), prefetched one plane ahead in registers.  Each thread
+// sums its coarse node's 3 x 3 fine neighbourhood on a plane with the
+// weights and in the order of k_restrict_xy, and the planes with those of
+// k_restrict_z, so the result is the two-pass one bit for bit.  No Dirichlet
+// mask is applied: b must be zero at Dirichlet DOFs (the fine residual is) or
+// there are none (coarse levels).  With x0 the kernel also writes the coarse
+// level's first smoothing iterate from zero, x0 = c2 dinv_c b_c (k_cheb_first
+// with c2 = cf[1]).
+constexpr int kR2I = 32, kR2J = 8;
+
+template <typename T, int NC, int KC, bool HASW>
+__global__ void __launch_bounds__(256)
+k_restrict222(int fnx, int fny, int fnz, int tiles_x, int tiles_y, int zchunks, int zlen,
+              const T* __restrict__ b, const T* __restrict__ w, T* __restrict__ bc,
+              const T* __restrict__ dinv_c, const double* __restrict__ cf, T* __restrict__ x0) {
+  static_assert(kR2I * kR2J == 256, "one thread per coarse node of a tile");
+  constexpr int NV = NC * KC;
+  constexpr int kW = 2 * kR2I + 1, kH = 2 * kR2J + 1;  // staged fine nodes per row, rows
+  constexpr int kRow = kW * NV;                          // values of one staged row
+  constexpr int kM = (kRow + 63) / 64;                   // values per lane per row
+  constexpr int kQ = (kH + 3) / 4;                       // rows per wavefront
+  __shared__ T sv[NV * kH * kW];                         // [value][row][node]
+  const int fnnx = fnx + 1, fnny = fny + 1, fnnz = fnz + 1;
+  const int cnnx = fnx / 2 + 1, cnny = fny / 2 + 1, cnnz = fnz / 2 + 1;
+  const int64_t fnn = (int64_t)fnnx * fnny * fnnz, cnn = (int64_t)cnnx * cnny * cnnz;
+  int blk = blockIdx.x;
+  const int tx = blk % tiles_x;
+  blk /= tiles_x;
+  const int ty = blk % tiles_y;
+  blk /= tiles_y;
+  const int zc = blk % zchunks;
+  const int g = blk / zchunks;
+  const int K0 = zc * zlen, K1 = K0 + zlen < cnnz ? K0 + zlen : cnnz;
+  const int gx0 = 2 * kR2I * tx - 1, gy0 = 2 * kR2J * ty - 1;
+  const T* bg = b + g * fnn * NV;
+  const T* wg = HASW ? w + g * fnn * NV : nullptr;
+  const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+  // staging: wavefront wv takes rows wv, wv + 4, ...; lane l takes values
+  // l, l + 64, ... of a row (value e: fine node gx0 + e / NV, component and
+  // column e % NV)
+  int s_lds[kM];
+  unsigned s_ok = 0;
+#pragma unroll
+  for (int m = 0; m < kM; ++m) {
+    const int e = lane + 64 * m;
+    const int xn = e / NV, v = e % NV;
+    s_lds[m] = (v * kH) * kW + xn;
+    if (e < kRow && gx0 + xn >= 0 && gx0 + xn < fnnx) s_ok |= 1u << m;
+  }
+  T pb[kQ][kM], pw[HASW ? kQ : 1][HASW ? kM : 1];
+  auto fetch = [&](int gz) {
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const int r = wv + 4 * q, gy = gy0 + r;
+      const bool row_ok = r < kH && gy >= 0 && gy < fnny && gz >= 0 && gz < fnnz;
+      const int64_t rowoff = (((int64_t)gz * fnny + gy) * fnnx + gx0) * NV;
+#pragma unroll
+      for (int m = 0; m < kM; ++m) {
+        const bool ok = row_ok && ((s_ok >> m) & 1u);
+        const int64_t o = rowoff + lane + 64 * m;
+        pb[q][m] = ok ? bg[o] : T(0);
+        if constexpr (HASW) pw[q][m] = ok ? wg[o] : T(0);
+      }
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const int r = wv + 4 * q;
+      if (r >= kH) continue;
+#pragma unroll
+      for (int m = 0; m < kM; ++m) {
+        if (lane + 64 * m >= kRow) continue;
+        if constexpr (HASW)
+          sv[s_lds[m] + r * kW] = pb[q][m] - pw[q][m];
+        else
+          sv[s_lds[m] + r * kW] = pb[q][m];
+      }
+    }
+  };
+  // compute: coarse node (I, J) of the tile, fine node (2 Il + 1, 2 Jl + 1)
+  // of the staged plane at its centre
+  const int Il = tid % kR2I, Jl = tid / kR2I;
+  const int I = tx * kR2I + Il, J = ty * kR2J + Jl;
+  const bool own = I < cnnx && J < cnny;
+  T acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = T(0);
+  const T c2 = x0 ? (T)cf[1] : T(0);
+  auto write = [&](int K) {
+    if (!own) return;
+    const int64_t cn = I + cnnx * (J + (int64_t)cnny * K);
+    T* out = bc + (g * cnn + cn) * NV;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[v] = acc[v];
+    if (x0) {
+      T* xo = x0 + (g * cnn + cn) * NV;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) xo[v] = c2 * dinv_c[cn * NC + v / KC] * acc[v];
+    }
+  };
+  // planes 2 K0 - 1 .. 2 K1 - 1 that exist
+  const int gz_lo = K0 > 0 ? 2 * K0 - 1 : 0, gz_hi = 2 * K1 - 1 < fnz ? 2 * K1 - 1 : fnz;
+  fetch(gz_lo);
+  for (int gz = gz_lo; gz <= gz_hi; ++gz) {
+    stash();
+    __syncthreads();
+    if (gz < gz_hi) fetch(gz + 1);  // in flight while this plane is summed
+    const bool have = gz >= 0 && gz < fnnz;
+    T t[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t[v] = T(0);
+    if (have) {
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy) {
+        const T wy = T(1) - (T)(dy < 0 ? -dy : dy) / T(2);
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          const T wgt = (T(1) - (T)(dx < 0 ? -dx : dx) / T(2)) * wy;
+          const int node = (2 * Jl + 1 + dy) * kW + 2 * Il + 1 + dx;
+#pragma unroll
+          for (int v = 0; v < NV; ++v) t[v] += wgt * sv[v * kH * kW + node];
+        }
+      }
+    }
+    __syncthreads();  // the next stash overwrites the plane
+    // plane gz = 2K + dz contributes wz t to coarse plane K (dz = -1, 0, 1 in
+    // this order, as k_restrict_z sums them)
+    if (gz & 1) {
+      const int K = (gz - 1) >> 1;  // dz = +1 for K, -1 for K + 1
+      if (K >= K0) {
+        if (have) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) acc[v] += T(0.5) * t[v];
+        }
+        write(K);
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[v] = T(0);
+      if (have && K + 1 < K1) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += T(0.5) * t[v];
+      }
+    } else {
+      const int K = gz >> 1;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[v] += T(1) * t[v];
+      if (gz + 1 >= fnnz) write(K);  // the last plane: no dz = +1 term
+    }
+  }
+}
+
 // x_f += Z P x_c.  Thread: fine i; row: (j, k, group).
 template <typename T, int NC, int KC>
 __global__ void k_prolong_add(int fnx, int fny, int fnz, int fx, int fy, int fz, int groups,
@@ -1814,6 +1972,42 @@ static void launch_restrict(const MGLevel& f, const MGLevel& c, int groups, cons
                      (int)(c.ny + 1), (int)f.nz, c.f[2], groups, (const T*)tbuf, bc);
 }
 
+// SKT_MG_RESTRICT=2pass forces the two-pass restriction on every level (the
+// fused one gives the same sums; a test checks that the solves agree).
+static bool fused_restrict_enabled() {
+  const char* v = std::getenv("SKT_MG_RESTRICT");
+  return !(v && std::strcmp(v, "2pass") == 0);
+}
+
+// Fused restriction into level c when every factor is 2 (k_restrict222; b
+// zero at Dirichlet DOFs); with x0, also the first smoothing iterate of c
+// (coefficients cf).  Returns false, launching nothing, otherwise.
+template <typename T, int NC, int KC>
+static bool launch_restrict_fused(const MGLevel& f, const MGLevel& c, int groups, const T* b,
+                                  const T* w, T* bc, const double* cf, T* x0, hipStream_t s) {
+  if (c.f[0] != 2 || c.f[1] != 2 || c.f[2] != 2 || !fused_restrict_enabled()) return false;
+  const int tiles_x = (int)((c.nx + 1 + kR2I - 1) / kR2I);
+  const int tiles_y = (int)((c.ny + 1 + kR2J - 1) / kR2J);
+  const int cnnz = (int)(c.nz + 1);
+  // planes per chunk: halved until there are >= 1024 workgroups (each chunk
+  // streams its 2 zlen + 1 fine planes one after the other)
+  int zlen = 4;
+  const int64_t per = (int64_t)tiles_x * tiles_y * groups;
+  while (zlen > 1 && per * ((cnnz + zlen - 1) / zlen) < 1024) zlen /= 2;
+  const int zch = (cnnz + zlen - 1) / zlen;
+  const int64_t nblk = per * zch;
+  const T* dc = x0 ? (const T*)c.dinv : nullptr;
+  if (w)
+    hipLaunchKernelGGL((k_restrict222<T, NC, KC, true>), dim3((unsigned)nblk), dim3(256), 0, s,
+                       (int)f.nx, (int)f.ny, (int)f.nz, tiles_x, tiles_y, zch, zlen, b, w, bc,
+                       dc, cf, x0);
+  else
+    hipLaunchKernelGGL((k_restrict222<T, NC, KC, false>), dim3((unsigned)nblk), dim3(256), 0, s,
+                       (int)f.nx, (int)f.ny, (int)f.nz, tiles_x, tiles_y, zch, zlen, b, w, bc,
+                       dc, cf, x0);
+  return true;
+}
+
 template <typename T, int NC, int KC>
 static void launch_prolong(const MGLevel& f, const MGLevel& c, int groups, const T* xc,
                            const uint8_t* fixed, T* xf, hipStream_t s) {
@@ -1848,6 +2042,12 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
   const T* coef0 = modal_coef<T>(op);
   const T* E0 = (const T*)E;
   int stt;
+  if (L >= 32) {
+    set_error("internal: too many multigrid levels");
+    return SKT_EINVAL;
+  }
+  // levels whose first smoothing iterate the restriction into them formed
+  bool x0_done[32] = {};
   T* cur = (T*)mg->lv[0].x;  // fine iterate (ping-pong with z)
   T* oth = z;
   // Operators without the fused modal kernels (tet4 on a grid): the fine
@@ -1908,17 +2108,21 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     const int tr = fine_timer_begin(mg, MG_T_RESID, s);
     if ((stt = hex_resid_modal<T>(op, k, KC, coef0, E0, cur, r, w0, s, stop))) return stt;
     fine_timer_end(mg, tr, s);
-    launch_restrict<T, NC, KC>(v, mg->lv[1], groups, (const T*)w0, (const T*)nullptr, fixed0,
-                               (T*)mg->lv[1].b, s);
+    // the residual is zero at Dirichlet DOFs, as the fused restriction needs;
+    // it also forms level 1's first smoothing iterate unless level 1 is the
+    // coarsest
+    T* x01 = L > 1 ? (T*)mg->lv[1].x : nullptr;
+    if (launch_restrict_fused<T, NC, KC>(v, mg->lv[1], groups, (const T*)w0, (const T*)nullptr,
+                                         (T*)mg->lv[1].b, mg->coef + (size_t)2 * deg, x01, s))
+      x0_done[1] = x01 != nullptr;
+    else
+      launch_restrict<T, NC, KC>(v, mg->lv[1], groups, (const T*)w0, (const T*)nullptr, fixed0,
+                                 (T*)mg->lv[1].b, s);
     SKT_LAUNCH_CHECK();
   }
   // coarse levels: each level's iterate ping-pongs between its x and d
   // buffers through fused Chebyshev steps; it[l] holds the current one
   T* it[32];
-  if (L >= 32) {
-    set_error("internal: too many multigrid levels");
-    return SKT_EINVAL;
-  }
   for (int l = 1; l < L; ++l) {
     MGLevel& v = mg->lv[l];
     const T* b = (const T*)v.b;
@@ -1927,10 +2131,11 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     T* w = (T*)v.w;
     const int64_t total = v.ndof * k;
     const double* cf = mg->coef + (size_t)l * 2 * deg;
-    // step 0 from a zero iterate, then fused steps (the iterate before
-    // step 0 is zero)
-    hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
-                       v.ndof, KC, cf, b, (const T*)v.dinv, A);
+    // step 0 from a zero iterate (formed by the fused restriction into this
+    // level, or here), then fused steps (the iterate before step 0 is zero)
+    if (!x0_done[l])
+      hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
+                         v.ndof, KC, cf, b, (const T*)v.dinv, A);
     for (int st = 1; st < deg; ++st) {
       if ((stt = level_cheb<T>(mg, op, l, k, KC, A, B, cf + 2 * st, st == 1 ? 1 : 0, s, stop)))
         return stt;
@@ -1938,7 +2143,13 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     }
     it[l] = A;
     if ((stt = level_apply<T>(mg, op, E, l, k, KC, A, w, s, ost))) return stt;
-    launch_restrict<T, NC, KC>(v, mg->lv[l + 1], groups, b, w, nullptr, (T*)mg->lv[l + 1].b, s);
+    T* x0n = l + 1 < L ? (T*)mg->lv[l + 1].x : nullptr;
+    if (launch_restrict_fused<T, NC, KC>(v, mg->lv[l + 1], groups, b, w, (T*)mg->lv[l + 1].b,
+                                         mg->coef + (size_t)(l + 1) * 2 * deg, x0n, s))
+      x0_done[l + 1] = x0n != nullptr;
+    else
+      launch_restrict<T, NC, KC>(v, mg->lv[l + 1], groups, b, w, nullptr, (T*)mg->lv[l + 1].b,
+                                 s);
     SKT_LAUNCH_CHECK();
   }
   // coarsest

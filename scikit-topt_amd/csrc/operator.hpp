@@ -88,7 +88,7 @@ int hex_modal_dot_slots(const skt_operator* op, int k, int kc);
 template <typename T>
 int hex_apply_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E, const T* X,
                     T* Y, hipStream_t s, double* dot = nullptr, const double* stop = nullptr);
-// R = B - K X
+// R = B - K X, zero at Dirichlet DOFs
 template <typename T>
 int hex_resid_modal(const skt_operator* op, int k, int kc, const T* coef, const T* E,
                     const T* X, const T* B, T* R, hipStream_t s, const double* stop = nullptr);
