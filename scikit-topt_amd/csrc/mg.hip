@@ -1972,11 +1972,12 @@ static void launch_restrict(const MGLevel& f, const MGLevel& c, int groups, cons
                      (int)(c.ny + 1), (int)f.nz, c.f[2], groups, (const T*)tbuf, bc);
 }
 
-// SKT_MG_RESTRICT=2pass forces the two-pass restriction on every level (the
-// fused one gives the same sums; a test checks that the solves agree).
-static bool fused_restrict_enabled() {
-  const char* v = std::getenv("SKT_MG_RESTRICT");
-  return !(v && std::strcmp(v, "2pass") == 0);
+// SKT_MG_TRANSFERS=rowgrid forces the row-grid restriction on every level
+// (the factor-2 kernel gives the same sums; a test checks that the solves
+// agree bit for bit).
+static bool fused_transfers_enabled() {
+  const char* v = std::getenv("SKT_MG_TRANSFERS");
+  return !(v && std::strcmp(v, "rowgrid") == 0);
 }
 
 // Fused restriction into level c when every factor is 2 (k_restrict222; b
@@ -1985,7 +1986,7 @@ static bool fused_restrict_enabled() {
 template <typename T, int NC, int KC>
 static bool launch_restrict_fused(const MGLevel& f, const MGLevel& c, int groups, const T* b,
                                   const T* w, T* bc, const double* cf, T* x0, hipStream_t s) {
-  if (c.f[0] != 2 || c.f[1] != 2 || c.f[2] != 2 || !fused_restrict_enabled()) return false;
+  if (c.f[0] != 2 || c.f[1] != 2 || c.f[2] != 2 || !fused_transfers_enabled()) return false;
   const int tiles_x = (int)((c.nx + 1 + kR2I - 1) / kR2I);
   const int tiles_y = (int)((c.ny + 1 + kR2J - 1) / kR2J);
   const int cnnz = (int)(c.nz + 1);

@@ -232,11 +232,12 @@ def test_mg_two_step_smoothing_pass_matches_separate_steps(gpu, precision, physi
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("physics", ["elastic", "heat"])
-def test_mg_fused_restriction_matches_two_passes(gpu, precision, physics, monkeypatch):
+def test_mg_factor2_restriction_matches_row_grid_kernels(gpu, precision, physics, monkeypatch):
     """Restriction into a level coarsened by 2 on every axis runs as one
-    LDS-staged pass that also forms the level's first smoothing iterate; it
-    sums in the order of the two separable passes (SKT_MG_RESTRICT=2pass,
-    followed by the separate first step), so the solves agree bit for bit."""
+    LDS-staged pass that also forms the coarse level's first smoothing
+    iterate; it sums in the order of the row-grid kernels
+    (SKT_MG_TRANSFERS=rowgrid: two separable passes, then the separate first
+    step), so the solves agree bit for bit."""
     if physics == "elastic":
         task, max_coarse = oprob.multiload(32, 16, 16, n_loads=2, seed=7), 100
     else:
@@ -244,12 +245,12 @@ def test_mg_fused_restriction_matches_two_passes(gpu, precision, physics, monkey
     rng = np.random.default_rng(9)
     rho = rng.uniform(0.05, 1.0, task.n_elements)
     out = {}
-    for mode in ("fused", "2pass"):
-        monkeypatch.setenv("SKT_MG_RESTRICT", mode)
+    for mode in ("fused", "rowgrid"):
+        monkeypatch.setenv("SKT_MG_TRANSFERS", mode)
         fem = _fem(task, physics, precision, 1e-10, "mg", max_coarse=max_coarse)
         U = np.zeros((task.ndof, task.n_tasks))
         fem.objectives_multi_load(rho, 3.0, U)
         assert fem.last_solve["levels"] >= 3
         out[mode] = (U, fem.last_solve["iters"])
-    assert out["fused"][1] == out["2pass"][1]
-    np.testing.assert_array_equal(out["fused"][0], out["2pass"][0])
+    assert out["fused"][1] == out["rowgrid"][1]
+    np.testing.assert_array_equal(out["fused"][0], out["rowgrid"][0])
