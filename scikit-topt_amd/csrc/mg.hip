@@ -1519,6 +1519,16 @@ static bool cheb0_enabled() {
   return !(v && v[0] == '0' && v[1] == 0);
 }
 
+// SKT_MG_COARSE_REFRESH=n (1..64): set-ups one coarsest inverse may serve.
+int mg_coarse_refresh(const MGData* mg) {
+  const char* v = std::getenv("SKT_MG_COARSE_REFRESH");
+  if (v) {
+    const int n = std::atoi(v);
+    if (n >= 1 && n <= 64) return n;
+  }
+  return mg->coarse_refresh;
+}
+
 static int lag_enabled() {
   const char* v = std::getenv("SKT_MG_LAG");
   return (v && v[0] == '1' && v[1] == 0) ? 1 : 0;
@@ -1870,11 +1880,16 @@ static int setup_t(MGData* mg, const skt_operator* op, const double* E64, const 
   // With a lagged inverse (mg_lagged) only the matrix is assembled here;
   // mg_lag_post inverts it in the background after the set-up.
   const bool lagged = mg_lagged(mg);
+  const bool reuse = mg->reuse_inv && mg->have_inv && !lagged;
   bool coarsest_queued = false;
   auto queue_coarsest = [&]() -> int {
     coarsest_queued = true;
     const MGLevel& c = mg->lv[L];
     const int64_t n = c.ndof;
+    if (reuse) {  // the previous inverse stays in Ainv
+      SKT_HIP(hipEventRecord(mg->ev_join, g));
+      return SKT_OK;
+    }
     if (NC == 3)
       hipLaunchKernelGGL((k_dense_asm<3>), dim3(grid_for(n * n)), dim3(kBlock), 0, g, c.nx, c.ny,
                          c.nz, c.Kel, mg->Ad);

@@ -73,6 +73,13 @@ struct MGData {
   // (mg_lag_pre).  On with SKT_MG_LAG=1.
   int lag = 0;
   int have_inv = 0;                   // Ainv holds an inverse
+  // A set-up may keep the previous coarsest inverse (reuse_inv, decided per
+  // set-up by the solver); it stays an SPD coarse solve, so the V-cycle is a
+  // fixed symmetric operator all the same.  An inverse serves at most
+  // `coarse_refresh` set-ups (1: a fresh one every time;
+  // SKT_MG_COARSE_REFRESH=n overrides).
+  int coarse_refresh = 2;
+  int reuse_inv = 0;
   int inv_pending = 0;                // Ad holds, or is becoming, a newer inverse
   hipStream_t gjs = nullptr;
   hipEvent_t ev_asm = nullptr, ev_gj = nullptr;
@@ -110,6 +117,8 @@ void mg_free(MGData* mg);
 // coarsest inverse, smoother bounds.  dinv0 is the fine Jacobi inverse in T.
 // Power-iteration steps of a warm set-up (SKT_MG_POWER_WARM overrides).
 int mg_warm_steps(const MGData* mg);
+// Set-ups one coarsest inverse may serve (SKT_MG_COARSE_REFRESH overrides).
+int mg_coarse_refresh(const MGData* mg);
 int mg_setup(MGData* mg, const skt_operator* op, const double* E64, const void* E, const void* dinv0,
              hipStream_t s);
 

@@ -43,6 +43,10 @@ struct skt_solver {
   double* part = nullptr;   // 4 * kParts * max_cols: area0 | area1
   double* sc = nullptr;     // max_cols * kSc
   double* host = nullptr;   // pinned copy of sc
+  // iterations of the last two solves, and set-ups the current coarsest
+  // inverse has served beyond its own
+  int64_t it_last = 0, it_prev = 0;
+  int inv_age = 0;
   // preconditioner
   int precond = SKT_PC_AUTO;
   int used = SKT_PC_JACOBI;  // the one the last solve ran with
@@ -86,7 +90,7 @@ struct skt_solver {
     uint64_t mg_gen = 0;
     int lagged = -1;
   };
-  Setup setup;
+  Setup setup[2];  // [reuse of the previous coarsest inverse]
   hipStream_t cap = nullptr;
   int graphs = 1;
   int64_t graph_launches = 0, graph_captures = 0;
@@ -704,13 +708,13 @@ static int collect_timing(skt_solver* s, int dtype, int64_t count) {
 // set-up has reached its steady state, else direct launches.
 template <typename T>
 static int run_setup_body(skt_solver* s, const double* E64, const Vecs<T>& v, hipStream_t st,
-                          bool lagged) {
+                          bool lagged, bool reuse) {
   MGData& mg = s->mg;
   const bool steady = mg.warm && (s->op->mesh == SKT_MESH_TET4 ||
                                   (mg.Mch && (s->op->n_fixed == 0 || mg.gflag)));
   if (!s->graphs || s->timing || !steady)
     return mg_setup(&mg, s->op, E64, v.E, v.dinv, st);
-  skt_solver::Setup& g = s->setup;
+  skt_solver::Setup& g = s->setup[reuse ? 1 : 0];
   const int steps = mg_warm_steps(&mg);
   const bool fresh = g.exec && g.dtype == v.dtype && g.degree == mg.degree &&
                      g.steps == steps && g.E64 == E64 && g.E == v.E && g.dinv == v.dinv &&
@@ -750,9 +754,18 @@ template <typename T>
 static int run_setup(skt_solver* s, const double* E64, const Vecs<T>& v, hipStream_t st) {
   MGData& mg = s->mg;
   const bool lagged = mg_lagged(&mg);
+  // keep the previous coarsest inverse while the solves are steady (the
+  // last one took no more iterations than the one before it), for at most
+  // mg_coarse_refresh set-ups per inverse: while the design moves fast an
+  // outdated inverse costs more iterations than its refresh
+  const int max_use = mg_coarse_refresh(&mg);
+  const bool steady = s->it_last > 0 && s->it_prev > 0 && s->it_last <= s->it_prev;
+  const bool reuse = !lagged && mg.have_inv && s->inv_age + 1 < max_use && steady;
+  s->inv_age = reuse ? s->inv_age + 1 : 0;
+  mg.reuse_inv = reuse ? 1 : 0;
   int stt = mg_lag_pre(&mg, st);
   if (stt) return stt;
-  if ((stt = run_setup_body<T>(s, E64, v, st, lagged))) return stt;
+  if ((stt = run_setup_body<T>(s, E64, v, st, lagged, reuse))) return stt;
   return lagged ? mg_lag_post(&mg, st) : SKT_OK;
 }
 
@@ -979,7 +992,8 @@ int skt_solver_destroy(skt_solver* s) {
   if (!s) return SKT_OK;
   for (auto& g : s->chunk)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
-  if (s->setup.exec) (void)hipGraphExecDestroy(s->setup.exec);
+  for (auto& g : s->setup)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (s->cap) (void)hipStreamDestroy(s->cap);
   for (int i = 0; i < 2 * 32; ++i)
     if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
@@ -1035,6 +1049,10 @@ int skt_solver_solve(skt_solver* s, int mode, int ncols, const double* E64,
     stt = solve_t<float>(s, mode, b, E64, v, rtol, maxiter, inner_rtol, iters, st);
   }
   if (stt != SKT_OK && stt != SKT_ENOCONV) return stt;
+  if (stt == SKT_OK) {
+    s->it_prev = s->it_last;
+    s->it_last = iters[0];
+  }
   // solution back to the caller's layout (stream-ordered; the flags and
   // norms below come from the last poll, so the host does not wait for it)
   hipLaunchKernelGGL(k_from_blocked, dim3(ge), dim3(kBlock), 0, st, n, ncols, b.kc,

@@ -254,3 +254,39 @@ def test_mg_factor2_restriction_matches_row_grid_kernels(gpu, precision, physics
         out[mode] = (U, fem.last_solve["iters"])
     assert out["fused"][1] == out["rowgrid"][1]
     np.testing.assert_array_equal(out["fused"][0], out["rowgrid"][0])
+
+
+@pytest.mark.parametrize("precision,rtol,ctol", [("fp64", 1e-12, 1e-9), ("mixed", 1e-10, 1e-8)])
+def test_mg_reused_coarsest_inverse_reaches_direct_solutions(gpu, precision, rtol, ctol,
+                                                             monkeypatch):
+    """Once the solves are steady (the last one took no more iterations than
+    the one before), a set-up keeps the previous coarsest inverse, for at most
+    two set-ups per inverse.  That is still an SPD coarse solve: every solve
+    of a sequence with slowly changing densities reaches the direct solution,
+    and the sequence differs from one with a fresh inverse at every set-up
+    (SKT_MG_COARSE_REFRESH=1) only within the tolerance."""
+    task = oprob.multiload(32, 16, 16, n_loads=2, seed=7)
+    rng = np.random.default_rng(3)
+    rho0 = rng.uniform(0.05, 1.0, task.n_elements)
+    rhos = [np.clip(rho0 * (1.0 + 0.02 * k), 0.0, 1.0) for k in range(6)]
+    runs = {}
+    for refresh in ("1", "2"):
+        monkeypatch.setenv("SKT_MG_COARSE_REFRESH", refresh)
+        fem = _fem(task, "elastic", precision, rtol, "mg", max_coarse=100)
+        U = np.zeros((task.ndof, task.n_tasks))
+        out = []
+        for rho in rhos:  # each solve starts from the previous solution
+            c = fem.objectives_multi_load(rho, 3.0, U)
+            assert fem.last_solve["levels"] >= 3
+            out.append((np.array(c), U.copy()))
+        runs[refresh] = out
+    ofe = ofem.ElasticFEM(task, 1e-9)
+    for k, rho in enumerate(rhos):
+        Uo = np.zeros((task.ndof, task.n_tasks))
+        co = ofe.objectives_multi_load(rho, 3.0, Uo)
+        for refresh in runs:
+            c, U = runs[refresh][k]
+            np.testing.assert_allclose(c, co, rtol=ctol)
+            assert np.abs(U - Uo).max() <= 1e-6 * np.abs(Uo).max(), (refresh, k)
+    # the reuse took effect (a different coarse solve changes the iterates)
+    assert any(not np.array_equal(runs["1"][k][1], runs["2"][k][1]) for k in range(6))
