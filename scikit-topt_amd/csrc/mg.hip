@@ -980,12 +980,14 @@ __global__ void k_cheb(int64_t total, int64_t ndof, int kc, const double* __rest
   }
 }
 
-// w = b - w (the fine level's residual after a generic operator apply)
+// w = b - w (the fine level's residual after a generic operator apply), zero at
+// Dirichlet DOFs
 template <typename T>
-__global__ void k_rsub(int64_t total, const T* __restrict__ b, T* __restrict__ w) {
+__global__ void k_rsub(int64_t total, int64_t ndof, int kc, const uint8_t* __restrict__ fixed,
+                       const T* __restrict__ b, T* __restrict__ w) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x)
-    w[t] = b[t] - w[t];
+    w[t] = (fixed && fixed[(t / kc) % ndof]) ? T(0) : b[t] - w[t];
 }
 
 template <typename T>
@@ -2088,9 +2090,15 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
     }
     if ((stt = op_apply(const_cast<skt_operator*>(op), mg->dtype, k, KC, E, x, w0, s)))
       return stt;
-    hipLaunchKernelGGL(k_rsub<T>, dim3(ge), dim3(kBlock), 0, s, total, r, w0);
-    launch_restrict<T, NC, KC>(v, mg->lv[1], groups, (const T*)w0, (const T*)nullptr, fixed0,
-                               (T*)mg->lv[1].b, s);
+    // zero at Dirichlet DOFs, as the fused restriction needs
+    hipLaunchKernelGGL(k_rsub<T>, dim3(ge), dim3(kBlock), 0, s, total, v.ndof, KC, fixed0, r, w0);
+    T* x01 = L > 1 ? (T*)mg->lv[1].x : nullptr;
+    if (launch_restrict_fused<T, NC, KC>(v, mg->lv[1], groups, (const T*)w0, (const T*)nullptr,
+                                         (T*)mg->lv[1].b, mg->coef + (size_t)2 * deg, x01, s))
+      x0_done[1] = x01 != nullptr;
+    else
+      launch_restrict<T, NC, KC>(v, mg->lv[1], groups, (const T*)w0, (const T*)nullptr, fixed0,
+                                 (T*)mg->lv[1].b, s);
     SKT_LAUNCH_CHECK();
     cur = x;
   } else {
