@@ -195,7 +195,7 @@ def time_launches(fn, reps=20, warm=3):
     return 1e3 * e0.elapsed_time(e1) / reps
 
 
-def tet_oc_iterations(n=4):
+def tet_oc_iterations(n=8):
     """OC iterations on the tet configuration (100x50x50 cells x 6 jittered
     tets, 1.5M tets, r = 1.5 Gaussian filter, Emin = 1e-9, vol 0.3, SIMP
     p = 3, mixed precision, multigrid solve): median of n iterations after

@@ -190,10 +190,11 @@ k_galerkin_tet(int64_t ncx, int64_t ncy, int64_t ncz, int fx, int fy, int fz,
   __shared__ double sK[NN];
   __shared__ double sT[NN];
   __shared__ double sP[64];
-  __shared__ double sg[12];
-  __shared__ int sloc[8];        // cell corner bits -> tet local node, or -1
-  __shared__ int64_t snode[8];   // cell corner bits -> global node
-  __shared__ double sw;
+  __shared__ double sg[6][12];    // barycentric gradients of the cell's six tets
+  __shared__ int scor[6][4];      // per tet: local node -> cell corner bits, or -1
+  __shared__ double sw[6];        // per tet: E_t V_t
+  __shared__ int64_t snode[8];    // cell corner bits -> global node
+  __shared__ uint8_t sfix[ND];    // the cell corners' Dirichlet flags
   const int64_t ce = blockIdx.x;
   const int64_t CX = ce % ncx, CY = (ce / ncx) % ncy, CZ = ce / (ncx * ncy);
   const int64_t nfx = ncx * fx, nfy = ncy * fy;
@@ -207,30 +208,38 @@ k_galerkin_tet(int64_t ncx, int64_t ncy, int64_t ncz, int fx, int fy, int fz,
       for (int a = 0; a < fx; ++a) {
         const int64_t ex = CX * fx + a, ey = CY * fy + b, ez = CZ * fz + c;
         const int64_t cell = ex + nfx * (ey + nfy * ez);
-        for (int i = tid; i < NN; i += 64) sK[i] = 0.0;
+        // (the previous child's projection ended with a barrier)
         if (tid < 8)
           snode[tid] = (ex + (tid & 1)) + nfnx * ((ey + ((tid >> 1) & 1)) + nfny * (ez + (tid >> 2)));
+        for (int idx = tid; idx < 6 * 12; idx += 64)
+          sg[idx / 12][idx % 12] = g[(int64_t)(idx % 12) * n_el + 6 * cell + idx / 12];
+        if (tid < 6) sw[tid] = E[6 * cell + tid] * vol[6 * cell + tid];
+        __syncthreads();
+        if (tid < 24) {
+          const int sidx = tid / 4, q = tid % 4;
+          const int64_t node = conn[(int64_t)q * n_el + 6 * cell + sidx];
+          int corner = -1;
+          for (int c8 = 0; c8 < 8; ++c8)
+            if (snode[c8] == node) corner = c8;
+          scor[sidx][q] = corner;
+        }
+        if (tid < ND) sfix[tid] = fixed ? fixed[snode[tid / NC] * NC + tid % NC] : 0;
+        for (int i = tid; i < NN; i += 64) sK[i] = 0.0;
+        __syncthreads();
+        // K_cell = sum_{s < 6} E_t V_t Z_t B^T D B Z_t: tet by tet (in order
+        // s), each thread adding some of the tet's (4 NC)^2 entries; distinct
+        // entries of a tet land on distinct cell entries
+        constexpr int NL = 4 * NC;
         for (int sidx = 0; sidx < 6; ++sidx) {
-          const int64_t t = 6 * cell + sidx;
-          __syncthreads();
-          if (tid < 12) sg[tid] = g[(int64_t)tid * n_el + t];
-          if (tid < 8) {
-            int loc = -1;
-            for (int q = 0; q < 4; ++q)
-              if ((int64_t)conn[(int64_t)q * n_el + t] == snode[tid]) loc = q;
-            sloc[tid] = loc;
-          }
-          if (tid == 0) sw = E[t] * vol[t];
-          __syncthreads();
-          const double w = sw;
-          for (int i = tid; i < NN; i += 64) {
-            const int r = i / ND, col = i % ND;
-            const int qa = r / NC, ca = r % NC, qb = col / NC, cb = col % NC;
-            const int la = sloc[qa], lb = sloc[qb];
-            if (la < 0 || lb < 0) continue;
-            if (fixed && (fixed[snode[qa] * NC + ca] || fixed[snode[qb] * NC + cb])) continue;
-            const double* ga = sg + 3 * la;
-            const double* gb = sg + 3 * lb;
+          for (int e = tid; e < NL * NL; e += 64) {
+            const int r = e / NL, col = e % NL;
+            const int la = r / NC, ca = r % NC, lb = col / NC, cb = col % NC;
+            const int qa = scor[sidx][la], qb = scor[sidx][lb];
+            if (qa < 0 || qb < 0) continue;
+            const int ia = qa * NC + ca, ib = qb * NC + cb;
+            if (sfix[ia] || sfix[ib]) continue;
+            const double* ga = sg[sidx] + 3 * la;
+            const double* gb = sg[sidx] + 3 * lb;
             double v;
             if (NC == 1) {
               v = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
@@ -238,10 +247,10 @@ k_galerkin_tet(int64_t ncx, int64_t ncy, int64_t ncz, int fx, int fy, int fz,
               const double gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
               v = lam0 * ga[ca] * gb[cb] + mu0 * ((ca == cb ? gg : 0.0) + ga[cb] * gb[ca]);
             }
-            sK[i] += w * v;
+            sK[ia * ND + ib] += sw[sidx] * v;
           }
+          __syncthreads();
         }
-        __syncthreads();
         galerkin_child<NC>(sK, sT, sP, a, b, c, fx, fy, fz, acc);
       }
 #pragma unroll
