@@ -424,8 +424,15 @@ __global__ void k_stencil_asm(int64_t nx, int64_t ny, int64_t nz, int layout,
   const TileShape ts = tile_shape(layout);
   const int tsn = ts.x * ts.y * ts.z;
   const int64_t bx = (nx + ts.x) / ts.x, by = (ny + ts.y) / ts.y;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
+  // Consecutive blocks of entries (consecutive components of a tile) read
+  // the same element-matrix lines; workgroups go to the 8 XCDs (each with
+  // its own L2) in turn, so XCD x takes the contiguous range of logical
+  // blocks starting at x q + min(x, r) (q, r: quotient and remainder of the
+  // grid by 8) and those reads meet in one L2.
+  const int64_t nb = gridDim.x, q8 = nb / 8, r8 = nb % 8;
+  const int64_t xcd = blockIdx.x % 8;
+  const int64_t lb = xcd * q8 + (xcd < r8 ? xcd : r8) + blockIdx.x / 8;
+  for (int64_t t = lb * blockDim.x + threadIdx.x; t < total; t += nb * blockDim.x) {
     int64_t i, j, k;
     int c;
     if (layout != ST_NODE) {
