@@ -43,6 +43,9 @@ struct skt_operator {
   // 1: the tet4 mesh is a grid of nx*ny*nz cells, six tets per cell
   // (skt_tet4_set_grid), which lets the multigrid coarsen it
   int tet_grid = 0;
+  // with tet_grid: per tet, the cell corner (bits x, y, z) of each local node
+  // q in bits 3q..3q+2
+  int32_t* tet_code = nullptr;
   // tet4 apply staging: ncomp nodal forces per (element, corner) pair and
   // column, stored in the node -> element lists' order; tet_slot[a n_el + e]
   // is the position of (e, a) in those lists

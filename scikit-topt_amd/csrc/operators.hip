@@ -8,6 +8,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -626,6 +627,161 @@ k_tet_gather(int64_t n_nodes, int64_t ns, int ncols, const int64_t* __restrict__
   }
 }
 
+// Tet4 meshes on a grid of cells (six tets per cell, skt_tet4_set_grid):
+// the apply without per-tet staging.  Phase 1, one thread per tet (the tet
+// arrays are read with consecutive lanes on consecutive tets), computes the
+// tet's four corner forces as k_tet_elem_forces does; a workgroup holds 32
+// whole cells, whose six tets add their forces per cell corner in LDS (tets
+// in order s, local nodes in order), and the cells' 8 corner forces are
+// written corner-major, Fc[((col * 8 + corner) * NC + d) * n_cells + cell],
+// so that consecutive cells are consecutive addresses.
+constexpr int kTgCells = 32;
+
+template <typename T, int NC>
+__global__ void __launch_bounds__(6 * kTgCells)
+k_tetg_cell_forces(int64_t n_cells, int ncols, T lam0, T mu0, T ma, T cb,
+                   const int32_t* __restrict__ code, const int32_t* __restrict__ conn,
+                   const T* __restrict__ g, const T* __restrict__ vol,
+                   const T* __restrict__ E, const T* __restrict__ R,
+                   const T* __restrict__ X, T* __restrict__ Fc) {
+  constexpr int kT = 6 * kTgCells;
+  __shared__ T sf[kT][4 * NC + 1];
+  __shared__ int scode[kT];
+  __shared__ T sacc[kTgCells][8 * NC];
+  const int col = blockIdx.y;
+  const int64_t n_el = 6 * n_cells;
+  const int tid = threadIdx.x;
+  const int64_t e = (int64_t)blockIdx.x * kT + tid;
+  T f[4][NC];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < NC; ++d) f[a][d] = T(0);
+  int cd = 0;
+  if (e < n_el) {
+    cd = code[e];
+    T gg[4][3];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) gg[b][d] = g[(3 * b + d) * n_el + e];
+    int64_t nb[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) nb[b] = conn[b * n_el + e];
+    const T w = E[e] * vol[e];
+    if (NC == 1) {
+      T x[4];
+      T xs = T(0);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        x[b] = X[nb[b] * ncols + col];
+        xs += x[b];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        T s = T(0);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          s += (gg[a][0] * gg[b][0] + gg[a][1] * gg[b][1] + gg[a][2] * gg[b][2]) * x[b];
+        T fa;
+        if (ma == T(0) && cb == T(1)) fa = w * s;
+        else fa = w * (cb * s + ma * ((xs + x[a]) * T(0.05)));
+        if (R) {
+          T rx = T(0);
+#pragma unroll
+          for (int b = 0; b < 4; ++b) rx += R[(int64_t)sym10(a, b) * n_el + e] * x[b];
+          fa += rx;
+        }
+        f[a][0] = fa;
+      }
+    } else {
+      T exx = 0, eyy = 0, ezz = 0, gyz = 0, gxz = 0, gxy = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const T ux = X[(nb[b] * 3 + 0) * ncols + col];
+        const T uy = X[(nb[b] * 3 + 1) * ncols + col];
+        const T uz = X[(nb[b] * 3 + 2) * ncols + col];
+        exx += gg[b][0] * ux;
+        eyy += gg[b][1] * uy;
+        ezz += gg[b][2] * uz;
+        gyz += gg[b][2] * uy + gg[b][1] * uz;
+        gxz += gg[b][2] * ux + gg[b][0] * uz;
+        gxy += gg[b][1] * ux + gg[b][0] * uy;
+      }
+      const T tr = lam0 * (exx + eyy + ezz);
+      const T sxx = tr + 2 * mu0 * exx, syy = tr + 2 * mu0 * eyy, szz = tr + 2 * mu0 * ezz;
+      const T syz = mu0 * gyz, sxz = mu0 * gxz, sxy = mu0 * gxy;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const T ga0 = gg[a][0], ga1 = gg[a][1], ga2 = gg[a][2];
+        f[a][0] = w * (ga0 * sxx + ga1 * sxy + ga2 * sxz);
+        f[a][1] = w * (ga1 * syy + ga0 * sxy + ga2 * syz);
+        f[a][2] = w * (ga2 * szz + ga0 * sxz + ga1 * syz);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < NC; ++d) sf[tid][a * NC + d] = f[a][d];
+  scode[tid] = cd;
+  for (int i = tid; i < kTgCells * 8 * NC; i += kT) sacc[i / (8 * NC)][i % (8 * NC)] = T(0);
+  __syncthreads();
+  if (tid < kTgCells) {
+    for (int s = 0; s < 6; ++s) {
+      const int t = 6 * tid + s;
+      const int c4 = scode[t];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int corner = (c4 >> (3 * a)) & 7;
+#pragma unroll
+        for (int d = 0; d < NC; ++d) sacc[tid][corner * NC + d] += sf[t][a * NC + d];
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t cell0 = (int64_t)blockIdx.x * kTgCells;
+  for (int i = tid; i < kTgCells * 8 * NC; i += kT) {
+    const int cl = i % kTgCells, v = i / kTgCells;  // v = corner * NC + d
+    if (cell0 + cl < n_cells)
+      Fc[((int64_t)col * 8 * NC + v) * n_cells + cell0 + cl] = sacc[cl][v];
+  }
+}
+
+// Phase 2: one thread per node (consecutive along x) sums the corner forces
+// of its up to 8 cells, in a fixed order; Dirichlet rows copy x.
+template <typename T, int NC>
+__global__ void k_tetg_gather(int64_t nx, int64_t ny, int64_t nz, int ncols,
+                              const T* __restrict__ Fc, const T* __restrict__ X,
+                              const uint8_t* __restrict__ fixed, T* __restrict__ Y) {
+  const int64_t nnx = nx + 1, nny = ny + 1, nn = nnx * nny * (nz + 1);
+  const int64_t n_cells = nx * ny * nz;
+  const int col = blockIdx.y;
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < nn;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = n % nnx, j = (n / nnx) % nny, k = n / (nnx * nny);
+    T acc[NC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d) acc[d] = T(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int sx = q & 1, sy = (q >> 1) & 1, sz = q >> 2;  // cell (i - 1 + sx, ...)
+      const int64_t ci = i - 1 + sx, cj = j - 1 + sy, ck = k - 1 + sz;
+      if (ci < 0 || ci >= nx || cj < 0 || cj >= ny || ck < 0 || ck >= nz) continue;
+      const int corner = (1 - sx) | ((1 - sy) << 1) | ((1 - sz) << 2);
+      const int64_t cell = ci + nx * (cj + ny * ck);
+#pragma unroll
+      for (int d = 0; d < NC; ++d)
+        acc[d] += Fc[((int64_t)col * 8 * NC + corner * NC + d) * n_cells + cell];
+    }
+#pragma unroll
+    for (int d = 0; d < NC; ++d) {
+      const int64_t dof = n * NC + d;
+      Y[dof * ncols + col] = (fixed && fixed[dof]) ? X[dof * ncols + col] : acc[d];
+    }
+  }
+}
+
 template <typename T, int NC>
 __global__ void k_tet_diag(int64_t n_nodes, int64_t n_el, T lam0, T mu0, T ma, T cb,
                            const T* __restrict__ g, const T* __restrict__ vol,
@@ -727,12 +883,44 @@ const double* tet_extra<double>(const skt_operator* op) { return op->tet_x64; }
 template <>
 const float* tet_extra<float>(const skt_operator* op) { return op->tet_x32; }
 
+// SKT_TET_APPLY=slots forces the per-tet staging form on grid meshes (the
+// two forms sum in different orders; tests compare both with assembled
+// matrices).
+static bool tet_grid_apply_enabled() {
+  const char* v = std::getenv("SKT_TET_APPLY");
+  return !(v && std::strcmp(v, "slots") == 0);
+}
+
 template <typename T>
 static int tet_apply_t(skt_operator* op, int ncols, const T* g, const T* vol, const T* E,
                        const T* X, T* Y, hipStream_t s) {
   const int64_t total = op->n_nodes * ncols;
   const T* R = tet_extra<T>(op);
   const size_t fbytes = (size_t)op->n_el * 4 * op->ncomp * ncols * sizeof(T);
+  const int64_t n_cells = op->nx * op->ny * op->nz;
+  const size_t cbytes = (size_t)n_cells * 8 * op->ncomp * ncols * sizeof(T);
+  if (op->tet_grid && op->tet_code && op->tet_f && cbytes <= op->tet_f_bytes &&
+      tet_grid_apply_enabled()) {
+    // grid of cells: per-cell corner forces, then the node gather
+    T* F = (T*)op->tet_f;
+    const dim3 g1((unsigned)((n_cells + kTgCells - 1) / kTgCells), (unsigned)ncols);
+    const dim3 g2(std::min<unsigned>(grid_for(op->n_nodes), 65535u), (unsigned)ncols);
+    if (op->ncomp == 3) {
+      hipLaunchKernelGGL((k_tetg_cell_forces<T, 3>), g1, dim3(6 * kTgCells), 0, s, n_cells,
+                         ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a, (T)op->cond_b,
+                         op->tet_code, op->conn, g, vol, E, R, X, F);
+      hipLaunchKernelGGL((k_tetg_gather<T, 3>), g2, dim3(kBlock), 0, s, op->nx, op->ny, op->nz,
+                         ncols, (const T*)F, X, op->fixed, Y);
+    } else {
+      hipLaunchKernelGGL((k_tetg_cell_forces<T, 1>), g1, dim3(6 * kTgCells), 0, s, n_cells,
+                         ncols, (T)op->lam0, (T)op->mu0, (T)op->mass_a, (T)op->cond_b,
+                         op->tet_code, op->conn, g, vol, E, R, X, F);
+      hipLaunchKernelGGL((k_tetg_gather<T, 1>), g2, dim3(kBlock), 0, s, op->nx, op->ny, op->nz,
+                         ncols, (const T*)F, X, op->fixed, Y);
+    }
+    SKT_LAUNCH_CHECK();
+    return SKT_OK;
+  }
   if (op->tet_f && fbytes <= op->tet_f_bytes) {
     // element forces, then the node gather (8 lanes per node and column)
     constexpr int kLpn = 8;
@@ -1102,6 +1290,22 @@ int skt_tet4_set_grid(skt_operator* op, int64_t nx, int64_t ny, int64_t nz) {
                 "a tet's nodes are not corners of its grid cell (element 6 c + s in cell c)");
     }
   }
+  // each tet's local nodes as corners of its cell
+  std::vector<int32_t> code(op->n_el);
+  for (int64_t e = 0; e < op->n_el; ++e) {
+    const int64_t c = e / 6;
+    const int64_t cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    int32_t v = 0;
+    for (int q = 0; q < 4; ++q) {
+      const int64_t n = hc[(size_t)q * op->n_el + e];
+      const int64_t i = n % nnx, j = (n / nnx) % nny, k = n / (nnx * nny);
+      v |= (int32_t)((i - cx) | ((j - cy) << 1) | ((k - cz) << 2)) << (3 * q);
+    }
+    code[(size_t)e] = v;
+  }
+  if (!op->tet_code) SKT_HIP(hipMalloc(&op->tet_code, sizeof(int32_t) * op->n_el));
+  SKT_HIP(hipMemcpy(op->tet_code, code.data(), sizeof(int32_t) * op->n_el,
+                    hipMemcpyHostToDevice));
   op->nx = nx;
   op->ny = ny;
   op->nz = nz;
@@ -1140,6 +1344,7 @@ int skt_operator_destroy(skt_operator* op) {
   (void)hipFree(op->extra32);
   (void)hipFree(op->tet_f);
   (void)hipFree(op->tet_slot);
+  (void)hipFree(op->tet_code);
   (void)hipFree(op->tet_x64);
   (void)hipFree(op->tet_x32);
   delete op;

@@ -499,3 +499,49 @@ def test_filter_stencil_form_declines_other_grids(gpu):
     x = np.random.default_rng(0).uniform(size=t.mesh.nelements)
     of = ofilt.SpacialFilter(t.mesh.element_centers(), 0.9, t.design_mask)
     np.testing.assert_allclose(f.forward(_t(x)).cpu().numpy(), of.forward(x), rtol=1e-13)
+
+
+@pytest.mark.parametrize("physics", ["elastic", "heat"])
+def test_tet_grid_apply_matches_assembled(gpu, physics, monkeypatch):
+    """Tet4 meshes on a grid of cells apply through per-cell corner forces
+    and a node gather over the 8 cells of a node (no per-tet staging); the
+    result equals the assembled matrix, and the per-tet staging form
+    (SKT_TET_APPLY=slots) to rounding."""
+    import torch
+    from sktopt_amd import _lib, fea, mesh as pmesh
+    if physics == "elastic":
+        task = oprob.cantilever(6, 4, 3, kind="tet4", jitter=0.2, seed=1)
+    else:
+        task = oprob.heat_patch(6, patch=3, kind="tet4", jitter=0.2, seed=1)
+    pm = task.mesh
+    m = pmesh.MeshTet(pm.p, pm.t, cell_grid=pm.grid)
+    cfg = fea.LinearSolverConfig(rtol=1e-12, precision="fp64")
+    if physics == "elastic":
+        t = pmesh.LinearElasticity(m, task.dirichlet_nodes, task.dirichlet_dofs, task.forces,
+                                   task.neumann_nodes, None, task.E0, task.nu)
+        fem = fea.FEM_SimpLinearElasticity(t, 1e-9, solver_config=cfg)
+    else:
+        t = pmesh.LinearHeatConduction(m, task.dirichlet_nodes, task.dirichlet_values,
+                                       task.forces[0], None, task.E0)
+        fem = fea.FEM_SimpLinearHeatConduction(t, 1e-3, solver_config=cfg)
+    op = task.element_operator()
+    rng = np.random.default_rng(13)
+    E = rng.uniform(1e-3, 1.0, task.n_elements)
+    K = op.assemble(E)
+    Ke, _ = ofem.enforce(K, np.zeros(task.ndof), task.dirichlet_dofs)
+    for k in (1, 3, 8):
+        X = rng.standard_normal((task.ndof, k))
+        ref = Ke @ X
+        out = {}
+        for mode in ("grid", "slots"):
+            monkeypatch.setenv("SKT_TET_APPLY", mode)
+            Y = torch.empty((task.ndof, k), dtype=torch.float64, device="cuda")
+            _lib.call("skt_operator_apply", fem._op, _lib.F64, k, _p(_t(E)), _p(_t(X)), _p(Y),
+                      _s())
+            Y32 = torch.empty((task.ndof, k), dtype=torch.float32, device="cuda")
+            _lib.call("skt_operator_apply", fem._op, _lib.F32, k, _p(_t(E, torch.float32)),
+                      _p(_t(X, torch.float32)), _p(Y32), _s())
+            out[mode] = (Y.cpu().numpy(), Y32.cpu().numpy().astype(np.float64))
+            assert np.abs(out[mode][0] - ref).max() <= 1e-12 * np.abs(ref).max(), (mode, k)
+            assert np.abs(out[mode][1] - ref).max() <= 2e-5 * np.abs(ref).max(), (mode, k)
+        assert np.abs(out["grid"][0] - out["slots"][0]).max() <= 1e-13 * np.abs(ref).max()
