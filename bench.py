@@ -195,12 +195,11 @@ def time_launches(fn, reps=20, warm=3):
     return 1e3 * e0.elapsed_time(e1) / reps
 
 
-def tet_oc_iterations(n=8):
+def tet_oc_iterations(n=8, precision="mixed"):
     """OC iterations on the tet configuration (100x50x50 cells x 6 jittered
-    tets, 1.5M tets, r = 1.5 Gaussian filter, Emin = 1e-9, vol 0.3, SIMP
-    p = 3, mixed precision, multigrid solve): median of n iterations after
-    one warm-up, each between device synchronisations."""
-    import torch
+    tets, 1.5M tets, r = 1.5 filter, Emin = 1e-9, vol 0.3, SIMP p = 3,
+    multigrid solve to rtol 1e-8) in the given precision: median of n
+    iterations after one warm-up, each between device synchronisations."""
     from sktopt_amd import core
     from sktopt_amd.mesh import toy_problem
     from sktopt_amd.tools import SchedulerConfig as SC
@@ -210,16 +209,55 @@ def tet_oc_iterations(n=8):
                          beta=SC.constant(target_value=1.0), filter_type="spacial",
                          filter_radius=SC.constant(target_value=1.5),
                          move_limit=SC.constant(target_value=0.2), E_min_coeff=1e-9,
-                         precision="mixed", solver_rtol=1e-8)
+                         precision=precision, solver_rtol=1e-8)
     opt = core.OC_Optimizer(cfg, task)
     opt.parameterize()
+    return _loop_record("tet4-100x50x50x6-1load (c1)", "tet OC", opt, n, precision)
+
+
+def heat_logmoc_iterations(n=6, precision="mixed"):
+    """Log-space modified OC on the heat configuration (160^3 hex8 scalar
+    conduction, 4.2M DOF, k0 = 1, kmin = 1e-3, source 1e-2, T = 0 on a 16x16
+    patch of z = 0, vol 0.3, r = 2.0 filter, SIMP p = 3, multigrid solve to
+    rtol 1e-8) in the given precision: median of n iterations after one
+    warm-up."""
+    from sktopt_amd import core
+    from sktopt_amd.mesh import toy_problem
+    from sktopt_amd.tools import SchedulerConfig as SC
+    task = toy_problem.heat_patch(160, patch=16, source=1e-2)
+    cfg = core.LogMOC_Config(
+        dst_path="/tmp/skt_bench_heat", max_iters=n + 3, p=SC.constant(target_value=3.0),
+        vol_frac=SC.constant(target_value=0.3), beta=SC.constant(target_value=1.0),
+        filter_type="spacial", filter_radius=SC.constant(target_value=2.0),
+        move_limit=SC.constant(target_value=0.2), eta=SC.constant(target_value=0.6),
+        E_min_coeff=1e-3, precision=precision, solver_rtol=1e-8)
+    opt = core.LogMOC_Optimizer(cfg, task)
+    opt.parameterize()
+    return _loop_record("heat-hex8-160^3 (c4)", "heat LogMOC", opt, n, precision)
+
+
+def _loop_record(config, label, opt, n, precision):
     times = timed_run(opt, 1, n)
     solves = [s["iters"] for s in opt.fem.solve_history[-n:]]
     t = median(times)
-    log(f"tet OC: {1.0 / t:.2f} it/s (median {1e3 * t:.1f} ms), solves {solves}")
-    return dict(config="tet4-100x50x50x6-1load (c1)", kernel="OC iteration (mixed, multigrid)",
-                value=1.0 / t, unit="OC iterations/s", ms_per_step=1e3 * t,
-                mg_pcg_iterations=solves, steps=n, warmup=1)
+    comp = [float(c) for c in opt.recorder.as_object().compliance]
+    log(f"{label} ({precision}): {1.0 / t:.2f} it/s (median {1e3 * t:.1f} ms), solves {solves}")
+    return dict(config=config, kernel=f"{label} iteration ({precision}, multigrid)",
+                value=1.0 / t, unit="iterations/s", ms_per_step=1e3 * t,
+                mg_pcg_iterations=solves, steps=n, warmup=1, compliance=comp)
+
+
+def _precision_pair(rec_mixed, rec_fp64):
+    """Mixed and float64 records of one loop, with the rate ratio and the
+    largest relative compliance difference over the iterations both ran."""
+    cm, cf = rec_mixed["compliance"], rec_fp64["compliance"]
+    m = min(len(cm), len(cf))
+    err = max(abs(a - b) / abs(b) for a, b in zip(cm[:m], cf[:m]))
+    log(f"{rec_mixed['config']}: mixed/fp64 {rec_mixed['value'] / rec_fp64['value']:.2f}x, "
+        f"compliance rel err {err:.1e}")
+    return dict(config=rec_mixed["config"], kernel=rec_mixed["kernel"].split(" (")[0],
+                mixed=rec_mixed, fp64=rec_fp64, mixed_over_fp64=rec_mixed["value"] /
+                rec_fp64["value"], compliance_rel_err_vs_fp64=err)
 
 
 def apply_suite():
@@ -271,8 +309,14 @@ def apply_suite():
                             bytes=b, avg_us=us, achieved_gbs=gbs, frac=gbs / HBM_PEAK_GBS))
             log(f"filter {name}: nnz={nnz} {us:.1f} us, {gbs:.0f} GB/s")
             del f
-            out.append(tet_oc_iterations())
         del fem
+        torch.cuda.empty_cache()
+        if task.mesh.kind == "tet4":
+            out.append(_precision_pair(tet_oc_iterations(precision="mixed"),
+                                       tet_oc_iterations(precision="fp64")))
+        elif name.startswith("heat"):
+            out.append(_precision_pair(heat_logmoc_iterations(precision="mixed"),
+                                       heat_logmoc_iterations(precision="fp64")))
         torch.cuda.empty_cache()
     return out
 
