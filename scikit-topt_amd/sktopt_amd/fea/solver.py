@@ -189,6 +189,8 @@ class _GpuFEM:
         theta = float(cfg.warm_start_extrapolation)
         if not cfg.warm_start or theta == 0.0:
             return
+        if U.dtype != torch.float64:
+            raise ValueError("predict_warm_start expects the float64 state U")
         S = _dev.stream()
         n = U.numel()
         prev = hist.get("prev")
