@@ -136,3 +136,71 @@ def test_hex8_120x60x60_8load_50_iterations_mixed_within_bounds(gpu, tmp_path):
           f"max |drho| {derr:.2e}", flush=True)
     assert cerr.max() <= 1e-5
     assert derr <= 1e-4
+
+
+def _run_loop(make_opt, steps):
+    import torch
+    opt = make_opt()
+    opt.parameterize()
+    opt.optimize_steps(steps)
+    out = (np.array(opt.recorder.as_object().compliance), opt.rho_host())
+    del opt
+    torch.cuda.empty_cache()
+    return out
+
+
+def test_tet4_100x50x50_50_iterations_mixed_within_bounds(gpu, tmp_path):
+    """The tet configuration (1.5M jittered tets, r = 1.5, multigrid) over
+    50 OC iterations: mixed within 1e-5 of the float64 run's compliance at
+    every iteration, densities within 1e-4 after the 50th."""
+    from sktopt_amd import core
+    from sktopt_amd.mesh import toy_problem
+    from sktopt_amd.tools import SchedulerConfig as SC
+    res = {}
+    for prec in ("fp64", "mixed"):
+        def make(prec=prec):
+            task = toy_problem.cantilever(100, 50, 50, kind="tet4", jitter=0.2, seed=0)
+            cfg = core.OC_Config(
+                dst_path=str(tmp_path / prec), max_iters=50, p=SC.constant(target_value=3.0),
+                vol_frac=SC.constant(target_value=0.3), beta=SC.constant(target_value=1.0),
+                filter_type="spacial", filter_radius=SC.constant(target_value=1.5),
+                move_limit=SC.constant(target_value=0.2), E_min_coeff=1e-9, precision=prec,
+                solver_rtol=1e-8)
+            return core.OC_Optimizer(cfg, task)
+        res[prec] = _run_loop(make, 50)
+    cerr = np.abs(res["mixed"][0] - res["fp64"][0]) / np.abs(res["fp64"][0])
+    derr = np.abs(res["mixed"][1] - res["fp64"][1]).max()
+    print(f"[c1 50 iterations] max compliance rel err {cerr.max():.2e}, "
+          f"max |drho| {derr:.2e}", flush=True)
+    assert len(res["mixed"][0]) == 50
+    assert cerr.max() <= 1e-5
+    assert derr <= 1e-4
+
+
+def test_heat_160_logmoc_100_iterations_mixed_within_bounds(gpu, tmp_path):
+    """The heat configuration (160^3, 4.2M DOF, r = 2.0) over its 100
+    log-space modified OC iterations: mixed within 1e-5 of the float64
+    run's compliance at every iteration, densities within 1e-4 after the
+    100th."""
+    from sktopt_amd import core
+    from sktopt_amd.mesh import toy_problem
+    from sktopt_amd.tools import SchedulerConfig as SC
+    res = {}
+    for prec in ("fp64", "mixed"):
+        def make(prec=prec):
+            task = toy_problem.heat_patch(160, patch=16, source=1e-2)
+            cfg = core.LogMOC_Config(
+                dst_path=str(tmp_path / prec), max_iters=100, p=SC.constant(target_value=3.0),
+                vol_frac=SC.constant(target_value=0.3), beta=SC.constant(target_value=1.0),
+                filter_type="spacial", filter_radius=SC.constant(target_value=2.0),
+                move_limit=SC.constant(target_value=0.2), eta=SC.constant(target_value=0.6),
+                E_min_coeff=1e-3, precision=prec, solver_rtol=1e-8)
+            return core.LogMOC_Optimizer(cfg, task)
+        res[prec] = _run_loop(make, 100)
+    cerr = np.abs(res["mixed"][0] - res["fp64"][0]) / np.abs(res["fp64"][0])
+    derr = np.abs(res["mixed"][1] - res["fp64"][1]).max()
+    print(f"[c4 100 iterations] max compliance rel err {cerr.max():.2e}, "
+          f"max |drho| {derr:.2e}", flush=True)
+    assert len(res["mixed"][0]) == 100
+    assert cerr.max() <= 1e-5
+    assert derr <= 1e-4
