@@ -1358,6 +1358,148 @@ __global__ void __launch_bounds__(256) k_dense_apply(int64_t n, int groups,
   }
 }
 
+// The same product with every column group in one pass over Ainv: b (all G
+// groups, G * KC columns) is staged in LDS node-major, and each wavefront
+// takes R consecutive rows, so a row of Ainv is read once rather than once
+// per group, and each b value read from LDS feeds R rows.  A lane's NJ values
+// of its rows (n <= 64 NJ) are loaded into registers before the staging, so
+// that every load of the launch is in flight at once.  Per (row, column) the
+// lane sums and the wave reduction run in k_dense_apply's order, so the
+// results are bit-identical to it.
+template <typename T, int KC, int G, int R, int NJ>
+__global__ void __launch_bounds__(256) k_dense_apply_rows(int n, const T* __restrict__ Ainv,
+                                                          const T* __restrict__ b,
+                                                          T* __restrict__ x) {
+  constexpr int GK = G * KC;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* bl = reinterpret_cast<T*>(smem_raw);  // [n][GK]
+  const int tid = threadIdx.x;
+  const int lane = tid % kWave;
+  const int row0 = (blockIdx.x * (256 / kWave) + tid / kWave) * R;
+  const bool live = row0 < n;
+  // unpredicated loads (row and column clamped into the matrix; the clamped
+  // values are never used), so that none waits for the one before
+  T av[R][NJ];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const T* ar = Ainv + (int64_t)(row0 + r < n ? row0 + r : n - 1) * n;
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) {
+      const int j = lane + kWave * t;
+      av[r][t] = ar[j < n ? j : n - 1];
+    }
+  }
+  // b: a thread's NBG values of every group are loaded (again unpredicated,
+  // clamped) before any is stored
+  constexpr int NBG = (kWave * NJ * KC + 255) / 256;
+  const int nk = n * KC;
+  T sb[G][NBG];
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int u = 0; u < NBG; ++u) {
+      const int i = tid + 256 * u;
+      sb[g][u] = b[(int64_t)g * nk + (i < nk ? i : nk - 1)];
+    }
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int u = 0; u < NBG; ++u) {
+      const int i = tid + 256 * u;
+      if (i < nk) {
+        const int j = i / KC, c = i - j * KC;
+        bl[j * GK + g * KC + c] = sb[g][u];
+      }
+    }
+  __syncthreads();
+  if (!live) return;
+  double acc[R][GK];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int c = 0; c < GK; ++c) acc[r][c] = 0.0;
+#pragma unroll
+  for (int t = 0; t < NJ; ++t) {
+    const int j = lane + kWave * t;
+    if (j < n) {
+      double bv[GK];
+#pragma unroll
+      for (int c = 0; c < GK; ++c) bv[c] = (double)bl[j * GK + c];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double a = (double)av[r][t];
+#pragma unroll
+        for (int c = 0; c < GK; ++c) acc[r][c] += a * bv[c];
+      }
+    }
+  }
+  // wave_sum's butterfly, one step for all R * GK sums at a time, so that
+  // their exchanges are in flight together
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < GK; ++c) acc[r][c] += __shfl_xor(acc[r][c], off, kWave);
+  if (lane == 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = row0 + r;
+      if (row >= n) continue;
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int c = 0; c < KC; ++c) x[((int64_t)g * n + row) * KC + c] = (T)acc[r][g * KC + c];
+    }
+  }
+}
+
+template <typename T, int KC, int R, int NJ>
+static bool launch_dense_rows(int n, int groups, const T* Ainv, const T* b, T* x,
+                              hipStream_t s) {
+  const size_t lds = (size_t)n * groups * KC * sizeof(T);
+  const dim3 grid((unsigned)((n + 4 * R - 1) / (4 * R)));
+  switch (groups) {
+    case 1:
+      hipLaunchKernelGGL((k_dense_apply_rows<T, KC, 1, R, NJ>), grid, dim3(256), lds, s, n,
+                         Ainv, b, x);
+      return true;
+    case 2:
+      hipLaunchKernelGGL((k_dense_apply_rows<T, KC, 2, R, NJ>), grid, dim3(256), lds, s, n,
+                         Ainv, b, x);
+      return true;
+    case 4:
+      hipLaunchKernelGGL((k_dense_apply_rows<T, KC, 4, R, NJ>), grid, dim3(256), lds, s, n,
+                         Ainv, b, x);
+      return true;
+    default:
+      return false;
+  }
+}
+
+// SKT_MG_DENSE_APPLY=wave: one wavefront per (row, group) at every size
+static bool dense_rows_enabled() {
+  const char* v = std::getenv("SKT_MG_DENSE_APPLY");
+  return !(v && std::strcmp(v, "wave") == 0);
+}
+
+// x = Ainv b for `groups` column groups of KC: the row-blocked kernel when b
+// fits in 64 KB of LDS, n <= 2048 and the group count is 1, 2 or 4, else one
+// wavefront per (row, group)
+template <typename T, int KC>
+static void launch_dense_apply(int64_t n, int groups, const T* Ainv, const T* b, T* x,
+                               hipStream_t s) {
+  const size_t lds = (size_t)n * groups * KC * sizeof(T);
+  if (lds <= 65536 && n <= 64 * 32 && dense_rows_enabled()) {
+    if (n <= 64 * 8 ? launch_dense_rows<T, KC, 2, 8>((int)n, groups, Ainv, b, x, s)
+                    : launch_dense_rows<T, KC, 2, 32>((int)n, groups, Ainv, b, x, s))
+      return;
+  }
+  const int64_t waves = n * groups;
+  hipLaunchKernelGGL((k_dense_apply<T, KC>), dim3((unsigned)((waves * kWave + 255) / 256)),
+                     dim3(256), 0, s, n, groups, Ainv, b, x);
+}
+
 // ------------------------------------------------- smoother bounds (power)
 
 __device__ __forceinline__ double hash_unit(int64_t i) {
@@ -2195,10 +2337,7 @@ static int vcycle_t(MGData* mg, const skt_operator* op, const void* E, const voi
   // coarsest
   {
     const MGLevel& c = mg->lv[L];
-    const int64_t waves = c.ndof * groups;
-    hipLaunchKernelGGL((k_dense_apply<T, KC>), dim3((unsigned)((waves * kWave + 255) / 256)),
-                       dim3(256), 0, s, c.ndof, groups, (const T*)mg->Ainv, (const T*)c.b,
-                       (T*)c.x);
+    launch_dense_apply<T, KC>(c.ndof, groups, (const T*)mg->Ainv, (const T*)c.b, (T*)c.x, s);
     SKT_LAUNCH_CHECK();
     it[L] = (T*)c.x;
   }

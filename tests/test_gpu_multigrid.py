@@ -256,6 +256,34 @@ def test_mg_factor2_restriction_matches_row_grid_kernels(gpu, precision, physics
     np.testing.assert_array_equal(out["fused"][0], out["rowgrid"][0])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", ["elastic-8", "elastic-4", "elastic-3", "heat-1"])
+def test_mg_row_blocked_coarsest_apply_matches_wave_per_row(gpu, precision, case, monkeypatch):
+    """The coarsest level's dense product reads the inverse once for every
+    column group (b staged in LDS, two rows per wavefront); per row and column
+    it sums in the order of the wavefront-per-row kernel
+    (SKT_MG_DENSE_APPLY=wave), so the solves agree bit for bit, for 1, 2 and 4
+    column groups and the 3-column block that takes the other kernel."""
+    physics, n = case.split("-")
+    n = int(n)
+    if physics == "elastic":
+        task, max_coarse = oprob.multiload(32, 16, 16, n_loads=n, seed=7), 700
+    else:
+        task, max_coarse = oprob.heat_patch(16, patch=4), 800
+    rng = np.random.default_rng(11)
+    rho = rng.uniform(0.05, 1.0, task.n_elements)
+    out = {}
+    for mode in ("rows", "wave"):
+        monkeypatch.setenv("SKT_MG_DENSE_APPLY", mode)
+        fem = _fem(task, physics, precision, 1e-10, "mg", max_coarse=max_coarse)
+        U = np.zeros((task.ndof, task.n_tasks))
+        fem.objectives_multi_load(rho, 3.0, U)
+        assert fem.last_solve["levels"] >= 2
+        out[mode] = (U, fem.last_solve["iters"])
+    assert out["rows"][1] == out["wave"][1]
+    np.testing.assert_array_equal(out["rows"][0], out["wave"][0])
+
+
 @pytest.mark.parametrize("precision,rtol,ctol", [("fp64", 1e-12, 1e-9), ("mixed", 1e-10, 1e-8)])
 def test_mg_reused_coarsest_inverse_reaches_direct_solutions(gpu, precision, rtol, ctol,
                                                              monkeypatch):
