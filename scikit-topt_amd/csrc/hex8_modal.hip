@@ -287,33 +287,37 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
   // s0 dinv b at stash time
   T pre[kPre];
   T pred[kC0 ? kPre : 1];
+  // The loads are unpredicated and unselected: an item outside the grid
+  // reads offset 0 of the plane, and stash writes zero for it.  A predicated
+  // load, or a select right after the load, compiles to a branch with a wait
+  // for that load inside it, which drains every load in flight.
   auto fetch = [&](int64_t gz) {
-    const bool zin = gz >= 0 && gz < nnz;
-    const T* Xp = X + (zin ? gz : 0) * plane_stride;
+    const T* Xp = X + (gz >= 0 && gz < nnz ? gz : 0) * plane_stride;
 #pragma unroll
     for (int r = 0; r < kPre; ++r) {
       if constexpr (kC0)
-        pre[r] = (zin && f_dof[r] >= 0) ? Xp[f_dof[r] * dstride + f_col] : T(0);
+        pre[r] = Xp[(f_dof[r] >= 0 ? f_dof[r] : 0) * dstride + f_col];
       else
-        pre[r] = (zin && f_off[r] >= 0) ? Xp[f_off[r]] : T(0);
+        pre[r] = Xp[f_off[r] >= 0 ? f_off[r] : 0];
     }
   };
   auto fetch_dinv = [&](int64_t gz) {
     if constexpr (kC0) {
-      const bool zin = gz >= 0 && gz < nnz;
-      const T* Dp = oa.dinv + (zin ? gz : 0) * nnx * nny * NC;
+      const T* Dp = oa.dinv + (gz >= 0 && gz < nnz ? gz : 0) * nnx * nny * NC;
 #pragma unroll
-      for (int r = 0; r < kPre; ++r) pred[r] = (zin && f_dof[r] >= 0) ? Dp[f_dof[r]] : T(0);
+      for (int r = 0; r < kPre; ++r) pred[r] = Dp[f_dof[r] >= 0 ? f_dof[r] : 0];
     }
   };
-  auto stash = [&](T* dst) {
+  auto stash = [&](T* dst, int64_t gz) {
+    const bool zin = gz >= 0 && gz < nnz;
 #pragma unroll
     for (int r = 0; r < kPre; ++r) {
       if (f_lds[r] < 0) continue;
+      const bool ok = zin && (kC0 ? f_dof[r] : f_off[kC0 ? 0 : r]) >= 0;
       if constexpr (MODE == OUT_CHEB0)
-        dst[f_lds[r]] = s0 * pred[r] * pre[r];
+        dst[f_lds[r]] = ok ? s0 * pred[r] * pre[r] : T(0);
       else
-        dst[f_lds[r]] = pre[r];
+        dst[f_lds[r]] = ok ? pre[r] : T(0);
     }
   };
 
@@ -326,13 +330,23 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
 #pragma unroll
   for (int c = 0; c < KC; ++c) dacc[c] = 0.0;
 
+  // Dirichlet flags of the owned node on plane ez (zero-offset reads where
+  // the thread writes nothing on that plane)
+  uint8_t fxn[NC];
+  auto load_flags = [&](int64_t ez) {
+    const int64_t pn = (is_node && ez >= z0 && ez < z1) ? (ez * nnx * nny + own_node) * NC : 0;
+#pragma unroll
+    for (int d = 0; d < NC; ++d) fxn[d] = fixed[pn + d];
+  };
+  load_flags(z0 - 1);
+
   int bot = 0, top = 1;
   fetch(z0 - 1);
   fetch_dinv(z0 - 1);
-  stash(planes + bot * kPlane);
+  stash(planes + bot * kPlane, z0 - 1);
   fetch(z0);
   fetch_dinv(z0);
-  stash(planes + top * kPlane);
+  stash(planes + top * kPlane, z0);
   // SF carries: face butterfly of the element's footprint on the bottom
   // plane, and the top-face forces of the layer below
   constexpr bool kSFF = (SF & 1) != 0, kSFB = (SF & 2) != 0;
@@ -359,29 +373,40 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
         for (int i2 = 0; i2 < 4; ++i2) Gc[col][d][i2] = T(0);
   }
   for (int64_t ez = z0 - 1; ez < z1; ++ez) {
-    fetch(ez + 2);  // in flight while this layer is computed
-    const T Ee = (el_in && ez >= 0 && ez < nz) ? E[e_base + nx * ny * ez] : T(0);
+    // this layer's operands are loaded before the prefetch of plane ez + 2, so
+    // that waiting for them leaves the prefetch in flight; all unpredicated
+    // (clamped addresses; a value a thread does not use is never selected)
+    const T Ev = E[e_base + nx * ny * (ez < 0 ? 0 : (ez < nz ? ez : nz - 1))];
     // this step completes node plane ez; its owned values are written by the
     // node threads, whose extra operands are loaded now so that their latency
     // hides behind the layer's compute
     const bool out = is_node && ez >= z0;
     const int64_t po = ez * plane_stride + own_off;
-    bool fix[NC];
+    const int64_t po_c = out ? po : 0;
+    const int64_t pn_c = out ? (ez * nnx * nny + own_node) * NC : 0;
+    // the Dirichlet flags come one layer ahead: loaded in the epilogue, where
+    // they are used, they would be issued after the prefetch and their wait
+    // would drain it
+    uint8_t fxv[NC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d) fxv[d] = fxn[d];
+    load_flags(ez + 1);
     // (kLate: loaded in the epilogue instead)
     T ob[KC][NC], op[KC][NC], oi[NC];
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
-      fix[d] = fixed && out && fixed[(ez * nnx * nny + own_node) * NC + d];
-      if constexpr (kCheb && !kLate)
-        oi[d] = out ? oa.dinv[(ez * nnx * nny + own_node) * NC + d] : T(0);
+      if constexpr (kCheb && !kLate) oi[d] = oa.dinv[pn_c + d];
 #pragma unroll
       for (int c = 0; c < KC; ++c) {
-        if constexpr (MODE != OUT_APPLY && !kLate) ob[c][d] = out ? Bv[po + d * dstride + c] : T(0);
+        if constexpr (MODE != OUT_APPLY && !kLate) ob[c][d] = Bv[po_c + d * dstride + c];
+        // without a previous iterate every thread reads offset 0
         if constexpr (MODE == OUT_CHEB && !kLate)
-          op[c][d] = (out && use_prev) ? Y[po + d * dstride + c] : T(0);
+          op[c][d] = Y[use_prev ? po_c + d * dstride + c : 0];
       }
     }
+    fetch(ez + 2);  // in flight while this layer is computed
     __syncthreads();
+    const T Ee = (el_in && ez >= 0 && ez < nz) ? Ev : T(0);
     const T* pb = planes + bot * kPlane;
     const T* pt = planes + top * kPlane;
 #pragma unroll
@@ -473,11 +498,12 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
         for (int d = 0; d < NC; ++d) {
           // element row ny_l + 1 is outside the tile only for the 16th node
           // row of a closing tile, where it is outside the grid as well
+          const bool fixd = out && fxv[d] != 0;
           T y;
           if constexpr (kSFB) {
             const T tot = pairs[((1 * NC + d) * KC + col) * 256 + e_lo] +
                           (has_up ? pairs[((0 * NC + d) * KC + col) * 256 + e_lo + kTE] : T(0));
-            y = fix[d] ? xo[col][d] : tot;
+            y = fixd ? xo[col][d] : tot;
           } else {
             const T below = pairs[(((0 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
                             (has_up ? pairs[(((0 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE]
@@ -485,7 +511,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
             const T above = pairs[(((1 * 2 + 1) * NC + d) * KC + col) * 256 + e_lo] +
                             (has_up ? pairs[(((1 * 2 + 0) * NC + d) * KC + col) * 256 + e_lo + kTE]
                                     : T(0));
-            y = fix[d] ? xo[col][d] : partial[col][d] + below;
+            y = fixd ? xo[col][d] : partial[col][d] + below;
             partial[col][d] = above;
           }
           if (out) {
@@ -495,13 +521,13 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
               if constexpr (DOT) dacc[col] += (double)xo[col][d] * (double)y;
             } else if constexpr (MODE == OUT_RESID) {
               // zero at Dirichlet DOFs, which the restriction drops
-              Yp[o] = fix[d] ? T(0) : ob[col][d] - y;
+              Yp[o] = fixd ? T(0) : ob[col][d] - y;
             } else {
               if constexpr (kLate) ob[col][d] = Bv[po + o];
               // Chebyshev step with the direction recovered from the two
               // iterates: x_out = x + c1 (x - x_prev) + c2 dinv (b - K x)
               // (x_prev = 0 for the implied first step of OUT_CHEB0)
-              T xp = MODE == OUT_CHEB ? op[col][d] : T(0);
+              T xp = (MODE == OUT_CHEB && use_prev) ? op[col][d] : T(0);
               T di = oi[d];
               if constexpr (kLate) {
                 if constexpr (MODE == OUT_CHEB) xp = use_prev ? Yp[o] : T(0);
@@ -517,7 +543,7 @@ k_hex_apply_modal(int64_t nx, int64_t ny, int64_t nz, int lkc, int64_t gstride, 
     }
     // the bottom plane receives plane ez + 2 and becomes the new top (every
     // read of it in this step happened before the second barrier)
-    stash(planes + bot * kPlane);
+    stash(planes + bot * kPlane, ez + 2);
     const int t = bot;
     bot = top;
     top = t;
@@ -679,7 +705,9 @@ static int launch_modal(const skt_operator* op, int k, int lkc, int64_t gstride,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     attr_set = true;
   }
-  const uint8_t* fx = op->n_fixed > 0 ? op->fixed : nullptr;
+  // the flags (all zero without fixed DOFs) are read unconditionally
+  SKT_CHECK(op->fixed != nullptr, "internal: hex8 operator without Dirichlet flags");
+  const uint8_t* fx = op->fixed;
   if (g_time_start) {
     // timed launch: the events take the dispatch's own start and end
     hipEvent_t t0 = g_time_start, t1 = g_time_stop;
