@@ -906,24 +906,38 @@ k_stencil_apply_tiled(int64_t nx, int64_t ny, int64_t nz, int groups, const T* _
   const int64_t x0 = (int64_t)tbx * SX, y0 = (int64_t)tby * SY, z0 = (int64_t)tbz * SZ;
   // this tile's coefficients (tile-major storage)
   st += (int64_t)(tbx + bx * (tby + by * tbz)) * (27 * NC * NC) * TS;
+  const int nt = threadIdx.x % TS, d = threadIdx.x / TS;
   // stage x (halo nodes outside the grid read as zero; their stencil
-  // entries are zero as well)
-  for (int it = threadIdx.x; it < HZ * HY * HX * GM; it += TS * NC) {
-    const int q = it % GM, hn = it / GM;
-    const int hx = hn % HX, hy = (hn / HX) % HY, hz = hn / (HX * HY);
-    const int64_t gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-    T v[V];
-    if (q < ng && gx >= 0 && gx < nnx && gy >= 0 && gy < nny && gz >= 0 && gz < nnz) {
-      load_node<T, V, KC>(X + ((int64_t)(g0 + q) * nn + gx + nnx * (gy + nny * gz)) * V, v);
-    } else {
+  // entries are zero as well).  A thread's NIT items are all loaded before
+  // any is stored, with unpredicated loads (an item outside the grid reads
+  // the first node of the vector and stores zero): a load in a branch of its
+  // own is waited for inside it, one memory latency per item.
+  constexpr int NH = HZ * HY * HX * GM, NIT = (NH + TS * NC - 1) / (TS * NC);
+  {
+    T v[NIT][V];
+    bool ok[NIT];
 #pragma unroll
-      for (int i = 0; i < V; ++i) v[i] = T(0);
+    for (int u = 0; u < NIT; ++u) {
+      const int it = threadIdx.x + u * TS * NC;
+      const int q = it % GM, hn = it / GM;
+      const int hx = hn % HX, hy = (hn / HX) % HY, hz = hn / (HX * HY);
+      const int64_t gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+      ok[u] = it < NH && q < ng && gx >= 0 && gx < nnx && gy >= 0 && gy < nny && gz >= 0 &&
+              gz < nnz;
+      const int64_t off = ok[u] ? ((int64_t)(g0 + q) * nn + gx + nnx * (gy + nny * gz)) * V : 0;
+      load_node<T, V, KC>(X + off, v[u]);
     }
 #pragma unroll
-    for (int i = 0; i < V; ++i) xs[hn * W + q * V + i] = v[i];
+    for (int u = 0; u < NIT; ++u) {
+      const int it = threadIdx.x + u * TS * NC;
+      if (it < NH) {
+        const int q = it % GM, hn = it / GM;
+#pragma unroll
+        for (int i = 0; i < V; ++i) xs[hn * W + q * V + i] = ok[u] ? v[u][i] : T(0);
+      }
+    }
   }
   __syncthreads();
-  const int nt = threadIdx.x % TS, d = threadIdx.x / TS;
   const int tx = nt % SX, ty = (nt / SX) % SY, tz = nt / (SX * SY);
   const int64_t gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
   if (gx >= nnx || gy >= nny || gz >= nnz) return;
@@ -947,34 +961,45 @@ k_stencil_apply_tiled(int64_t nx, int64_t ny, int64_t nz, int groups, const T* _
 #pragma unroll
         for (int c = 0; c < KC; ++c) acc[q][c] += a[e] * xn[q * V + e * KC + c];
   }
-  T c1 = T(0), c2 = T(0), dv = T(0);
-  if constexpr (MODE == ST_CHEB) {
-    c1 = (T)oa.coef[0];
-    c2 = (T)oa.coef[1];
-    dv = oa.dinv[node * NC + d];
-  }
   const T* xcen = xs + ((tz + 1) * HY * HX + (ty + 1) * HX + (tx + 1)) * W;
-#pragma unroll
-  for (int q = 0; q < GM; ++q) {
-    if (q >= ng) break;
-    const int64_t off = ((int64_t)(g0 + q) * nn + node) * V + d * KC;
+  // The Chebyshev operands of every group are loaded together (unpredicated:
+  // a group past the last, or x_prev when it is zero, reads offset 0).  With
+  // every group present (nq = GM, a constant after inlining) no store is
+  // predicated, so the compiler does not sink a group's loads into a branch
+  // of its own, where each would be waited for in turn.
+  auto finish = [&](const int nq) {
+    T c1 = T(0), c2 = T(0), dv = T(0);
+    T bv[GM][KC], xp[GM][KC];
     if constexpr (MODE == ST_CHEB) {
-      T bv[KC], xp[KC];
-      load_node<T, KC, KC>(oa.b + off, bv);
-      if (oa.prev_zero) {
+      c1 = (T)oa.coef[0];
+      c2 = (T)oa.coef[1];
+      dv = oa.dinv[node * NC + d];
 #pragma unroll
-        for (int c = 0; c < KC; ++c) xp[c] = T(0);
-      } else {
-        load_node<T, KC, KC>(Y + off, xp);
-      }
-#pragma unroll
-      for (int c = 0; c < KC; ++c) {
-        const T xc = xcen[q * V + d * KC + c];
-        acc[q][c] = xc + c1 * (xc - xp[c]) + c2 * dv * (bv[c] - acc[q][c]);
+      for (int q = 0; q < GM; ++q) {
+        const int64_t off = ((int64_t)(g0 + (q < nq ? q : 0)) * nn + node) * V + d * KC;
+        load_node<T, KC, KC>(oa.b + off, bv[q]);
+        load_node<T, KC, KC>(Y + (oa.prev_zero ? 0 : off), xp[q]);
       }
     }
-    store_node<T, KC, KC>(Y + off, acc[q]);
-  }
+#pragma unroll
+    for (int q = 0; q < GM; ++q) {
+      if (q >= nq) break;
+      const int64_t off = ((int64_t)(g0 + q) * nn + node) * V + d * KC;
+      if constexpr (MODE == ST_CHEB) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const T xc = xcen[q * V + d * KC + c];
+          const T xpc = oa.prev_zero ? T(0) : xp[q][c];
+          acc[q][c] = xc + c1 * (xc - xpc) + c2 * dv * (bv[q][c] - acc[q][c]);
+        }
+      }
+      store_node<T, KC, KC>(Y + off, acc[q]);
+    }
+  };
+  if (ng == GM)
+    finish(GM);
+  else
+    finish(ng);
 }
 
 // Chebyshev step on a level (blocked vectors, total = ndof * k):
@@ -1281,8 +1306,14 @@ k_restrict222(int fnx, int fny, int fnz, int tiles_x, int tiles_y, int zchunks, 
   }
 }
 
-// x_f += Z P x_c.  Thread: fine i; row: (j, k, group).
-template <typename T, int NC, int KC>
+// x_f += Z P x_c.  Thread: fine i; row: (j, k, group).  FIX: Dirichlet flags
+// of the fine level (no correction on fixed DOFs).  Every load of a row (the
+// 8 coarse corners, the fine node, its flags) is unpredicated and issued
+// before any is used: a corner outside the interpolation (weight zero) reads
+// a clamped node and is dropped by a select, so the sums are those of the
+// loop that skips it.  Predicated loads compiled to one branch and one wait
+// each.
+template <typename T, int NC, int KC, bool FIX>
 __global__ void k_prolong_add(int fnx, int fny, int fnz, int fx, int fy, int fz, int groups,
                               const T* __restrict__ xc, const uint8_t* __restrict__ fixed,
                               T* __restrict__ xf) {
@@ -1300,6 +1331,22 @@ __global__ void k_prolong_add(int fnx, int fny, int fnz, int fx, int fy, int fz,
     const T sy = (T)ay / (T)fy, sz = (T)az / (T)fz;
     const int64_t fnode = i + fnnx * (j + (int64_t)fnny * k);
     const T* xg = xc + g * cnn * NC * KC;
+    T* xfn = xf + (g * fnn + fnode) * NC * KC;
+    uint8_t fv[NC];
+    if constexpr (FIX) {
+#pragma unroll
+      for (int d = 0; d < NC; ++d) fv[d] = fixed[fnode * NC + d];
+    }
+    T out[NC * KC];
+    load_node<T, NC * KC, KC>(xfn, out);
+    T xv[8][NC * KC];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      // the +1 corner exists wherever its weight is nonzero
+      const int qx = (q & 1) && ax != 0, qy = ((q >> 1) & 1) && ay != 0, qz = (q >> 2) && az != 0;
+      const int64_t cn = (I + qx) + cnnx * ((J + qy) + (int64_t)cnny * (K + qz));
+      load_node<T, NC * KC, KC>(xg + cn * NC * KC, xv[q]);
+    }
     T acc[NC][KC];
 #pragma unroll
     for (int d = 0; d < NC; ++d)
@@ -1308,24 +1355,24 @@ __global__ void k_prolong_add(int fnx, int fny, int fnz, int fx, int fy, int fz,
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
-      if ((qx && ax == 0) || (qy && ay == 0) || (qz && az == 0)) continue;
+      const bool skip = (qx && ax == 0) || (qy && ay == 0) || (qz && az == 0);
       const T wgt = (qx ? sx : T(1) - sx) * (qy ? sy : T(1) - sy) * (qz ? sz : T(1) - sz);
-      const int64_t cn = (I + qx) + cnnx * ((J + qy) + (int64_t)cnny * (K + qz));
-      T xv[NC * KC];
-      load_node<T, NC * KC, KC>(xg + cn * NC * KC, xv);
 #pragma unroll
       for (int d = 0; d < NC; ++d)
 #pragma unroll
-        for (int c = 0; c < KC; ++c) acc[d][c] += wgt * xv[d * KC + c];
+        for (int c = 0; c < KC; ++c) {
+          const T a = acc[d][c] + wgt * xv[q][d * KC + c];
+          acc[d][c] = skip ? acc[d][c] : a;
+        }
     }
-    T* xfn = xf + (g * fnn + fnode) * NC * KC;
-    T out[NC * KC];
-    load_node<T, NC * KC, KC>(xfn, out);
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
-      if (fixed && fixed[fnode * NC + d]) continue;
+      const bool keep = FIX && fv[d] != 0;
 #pragma unroll
-      for (int c = 0; c < KC; ++c) out[d * KC + c] += acc[d][c];
+      for (int c = 0; c < KC; ++c) {
+        const T a = out[d * KC + c] + acc[d][c];
+        out[d * KC + c] = keep ? out[d * KC + c] : a;
+      }
     }
     store_node<T, NC * KC, KC>(xfn, out);
   }
@@ -2188,8 +2235,12 @@ template <typename T, int NC, int KC>
 static void launch_prolong(const MGLevel& f, const MGLevel& c, int groups, const T* xc,
                            const uint8_t* fixed, T* xf, hipStream_t s) {
   const RowGrid gp = row_grid(f.nx + 1, (f.ny + 1) * (f.nz + 1) * groups);
-  hipLaunchKernelGGL((k_prolong_add<T, NC, KC>), gp.grid, gp.block, 0, s, (int)f.nx, (int)f.ny,
-                     (int)f.nz, c.f[0], c.f[1], c.f[2], groups, xc, fixed, xf);
+  if (fixed)
+    hipLaunchKernelGGL((k_prolong_add<T, NC, KC, true>), gp.grid, gp.block, 0, s, (int)f.nx,
+                       (int)f.ny, (int)f.nz, c.f[0], c.f[1], c.f[2], groups, xc, fixed, xf);
+  else
+    hipLaunchKernelGGL((k_prolong_add<T, NC, KC, false>), gp.grid, gp.block, 0, s, (int)f.nx,
+                       (int)f.ny, (int)f.nz, c.f[0], c.f[1], c.f[2], groups, xc, fixed, xf);
 }
 
 template <typename T>
