@@ -335,10 +335,21 @@ __global__ void k_restart_dir(int64_t n, int k, int kc, int cp, const double* __
 }
 
 // Sum of the nb partials of quantity q for column c (one wavefront per
-// column, fixed order).
+// column, fixed order: each lane adds its partials in index order, then the
+// wave reduction).  Eight loads are issued before their adds, so a lane
+// waits for memory once per eight partials rather than once per partial.
 __device__ __forceinline__ double col_sum(const double* part, int k, int nb, int q, int c) {
+  const double* p = part + ((int64_t)q * k + c) * nb;
   double a = 0.0;
-  for (int i = threadIdx.x; i < nb; i += kWave) a += part[((int64_t)q * k + c) * nb + i];
+  int i = threadIdx.x;
+  for (; i + 7 * kWave < nb; i += 8 * kWave) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[i + u * kWave];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a += v[u];
+  }
+  for (; i < nb; i += kWave) a += p[i];
   return wave_sum(a);
 }
 
