@@ -438,6 +438,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cpu-c0", action="store_true",
                     help="skip the fully timed CPU iteration of the 40x20x20 case (~100 s)")
+    ap.add_argument("--no-c0", action="store_true",
+                    help="skip the 40x20x20 GPU iterations (kernel traces of the c2 loop alone)")
     ap.add_argument("--apply-suite", action="store_true",
                     help="also time the operator apply on the tet, 200x100x100 and heat "
                          "configurations and the tet mesh's CSR filter gather")
@@ -515,7 +517,7 @@ def main():
         log(f"fp64: {fp64}  max compliance rel err mixed vs fp64 = {err:.2e}")
         del opt64
         torch.cuda.empty_cache()
-    c0_gpu = gpu_c0_iterations()
+    c0_gpu = None if args.no_c0 else gpu_c0_iterations()
     torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
@@ -523,7 +525,7 @@ def main():
         n_bis = sum(evals) / len(evals)
         cpu = cpu_baseline.measure(wl, jac_its, n_bis)
         log(f"cpu (extrapolated c2): {cpu['value']:.4f} OC it/s")
-        if not args.no_cpu_c0:
+        if not args.no_cpu_c0 and c0_gpu is not None:
             cpu["c0_full_iteration"] = cpu_baseline.measure_c0_iteration(log)
             cs = cpu["c0_full_iteration"]["seconds"]
             c0_gpu["vs_cpu_first_iteration"] = cs / c0_gpu["first_iteration_s"]
