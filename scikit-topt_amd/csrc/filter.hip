@@ -352,7 +352,9 @@ __device__ __forceinline__ double stencil_row(int e, int32_t r, int nx, int ny, 
   // branch-free over all offsets (lanes near a face take the same path as
   // interior ones): an out-of-grid neighbour reads the element itself and a
   // selected zero, so loads never wait on a branch and several offsets are
-  // in flight at once
+  // in flight at once (four per unrolled step; twelve, three waits instead
+  // of nine over the 33 offsets of r = 2, ran slower: 17.5 against 14.7 us
+  // per bisection evaluation at 120x60x60)
 #pragma unroll 4
   for (int o = 0; o < n_off; ++o) {
     const int pk = off[o];

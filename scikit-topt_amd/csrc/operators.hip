@@ -760,24 +760,45 @@ __global__ void k_tetg_gather(int64_t nx, int64_t ny, int64_t nz, int ncols,
   for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < nn;
        n += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = n % nnx, j = (n / nnx) % nny, k = n / (nnx * nny);
-    T acc[NC];
+    // every load of the node is unpredicated and issued before the sums (a
+    // cell outside the grid reads cell 0 and is dropped by a select, which
+    // keeps the skipping loop's sums): a load in a branch of its own was
+    // waited for inside it
+    T xv[NC];
+    uint8_t fv[NC];
 #pragma unroll
-    for (int d = 0; d < NC; ++d) acc[d] = T(0);
+    for (int d = 0; d < NC; ++d) {
+      const int64_t dof = n * NC + d;
+      xv[d] = X[dof * ncols + col];
+      fv[d] = fixed[dof];  // every tet operator holds its flags (all zero if none fixed)
+    }
+    T fc[8][NC];
+    bool in[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int sx = q & 1, sy = (q >> 1) & 1, sz = q >> 2;  // cell (i - 1 + sx, ...)
       const int64_t ci = i - 1 + sx, cj = j - 1 + sy, ck = k - 1 + sz;
-      if (ci < 0 || ci >= nx || cj < 0 || cj >= ny || ck < 0 || ck >= nz) continue;
+      in[q] = ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < nz;
       const int corner = (1 - sx) | ((1 - sy) << 1) | ((1 - sz) << 2);
-      const int64_t cell = ci + nx * (cj + ny * ck);
+      const int64_t cell = in[q] ? ci + nx * (cj + ny * ck) : 0;
 #pragma unroll
       for (int d = 0; d < NC; ++d)
-        acc[d] += Fc[((int64_t)col * 8 * NC + corner * NC + d) * n_cells + cell];
+        fc[q][d] = Fc[((int64_t)col * 8 * NC + corner * NC + d) * n_cells + cell];
     }
+    T acc[NC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d) acc[d] = T(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+#pragma unroll
+      for (int d = 0; d < NC; ++d) {
+        const T a = acc[d] + fc[q][d];
+        acc[d] = in[q] ? a : acc[d];
+      }
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
       const int64_t dof = n * NC + d;
-      Y[dof * ncols + col] = (fixed && fixed[dof]) ? X[dof * ncols + col] : acc[d];
+      Y[dof * ncols + col] = fv[d] ? xv[d] : acc[d];
     }
   }
 }
@@ -901,7 +922,9 @@ static int tet_apply_t(skt_operator* op, int ncols, const T* g, const T* vol, co
   const size_t cbytes = (size_t)n_cells * 8 * op->ncomp * ncols * sizeof(T);
   if (op->tet_grid && op->tet_code && op->tet_f && cbytes <= op->tet_f_bytes &&
       tet_grid_apply_enabled()) {
-    // grid of cells: per-cell corner forces, then the node gather
+    // grid of cells: per-cell corner forces, then the node gather (which
+    // reads the Dirichlet flags unconditionally)
+    SKT_CHECK(op->fixed != nullptr, "internal: tet operator without Dirichlet flags");
     T* F = (T*)op->tet_f;
     const dim3 g1((unsigned)((n_cells + kTgCells - 1) / kTgCells), (unsigned)ncols);
     const dim3 g2(std::min<unsigned>(grid_for(op->n_nodes), 65535u), (unsigned)ncols);
