@@ -282,7 +282,9 @@ __global__ void k_pinit(int64_t n, int k, int kc, int cp, const T* __restrict__ 
 // x += alpha p; r -= alpha q; partial r.z (Jacobi z = dinv r; 0 when dinv
 // is null -- the preconditioned product is then taken after the V-cycle)
 // and r.r
-template <typename T>
+// (WX: also x += alpha p; otherwise the direction update of the same
+// iteration does it, before it overwrites p, which saves a read of p)
+template <typename T, bool WX>
 __global__ void k_update(int64_t n, int k, int kc, int cp, const double* __restrict__ sc,
                          const T* __restrict__ p, const T* __restrict__ q,
                          const T* __restrict__ dinv, T* __restrict__ x, T* __restrict__ r,
@@ -294,7 +296,7 @@ __global__ void k_update(int64_t n, int k, int kc, int cp, const double* __restr
     const T alpha = (T)sc[c * kSc + S_ALPHA];
     for (int64_t i = i0; i < n; i += istep) {
       const int64_t t = ((int64_t)g * n + i) * kc + j;
-      x[t] += alpha * p[t];
+      if constexpr (WX) x[t] += alpha * p[t];
       const T rv = r[t] - alpha * q[t];
       r[t] = rv;
       if (dinv) rz += (double)rv * (double)(dinv[i] * rv);
@@ -306,18 +308,30 @@ __global__ void k_update(int64_t n, int k, int kc, int cp, const double* __restr
 }
 
 // p = z + beta p (z null: Jacobi z = dinv r); (nbx, groups) grid as the
-// reduction kernels, so the row and column come without 64-bit divisions
-template <typename T>
+// reduction kernels, so the row and column come without 64-bit divisions.
+// WX: first x += alpha p with this iteration's alpha (the update k_update<T,
+// false> left out), also for a column that stopped at this iteration's
+// r.r test: its alpha is nonzero exactly when k_update saw it live.
+template <typename T, bool WX>
 __global__ void k_pupdate(int64_t n, int k, int kc, int cp, const double* __restrict__ sc,
                           const T* __restrict__ r, const T* __restrict__ dinv,
-                          const T* __restrict__ z, T* __restrict__ p) {
+                          const T* __restrict__ z, T* __restrict__ p, T* __restrict__ x) {
   SKT_BLK_SETUP
-  if (!lane_on || !col_live(sc, c)) return;
+  if (!lane_on) return;
+  const bool live = col_live(sc, c);
+  const T alpha = WX ? (T)sc[c * kSc + S_ALPHA] : T(0);
+  const bool upd_x = WX && alpha != T(0);
+  if (!live && !upd_x) return;
   const T beta = (T)sc[c * kSc + S_BETA];
   for (int64_t i = i0; i < n; i += istep) {
     const int64_t t = ((int64_t)g * n + i) * kc + j;
+    // loads unpredicated (a lane returned above unless it stores something),
+    // stores predicated: no load waits inside a branch of its own
+    const T pv = p[t];
+    const T xv = WX ? x[t] : T(0);
     const T zv = z ? z[t] : dinv[i] * r[t];
-    p[t] = zv + beta * p[t];
+    if (upd_x) x[t] = xv + alpha * pv;
+    if (live) p[t] = zv + beta * pv;
   }
 }
 
@@ -613,9 +627,17 @@ static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t
       pq_nb = (int)b.nbx;
     }
     hipLaunchKernelGGL(k_alpha, dim3(b.k), dim3(kWave), 0, st, pq_nb, b.k, s->part, s->sc);
-    hipLaunchKernelGGL(k_update<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
-                       b.cp, s->sc, (const T*)v.p, (const T*)v.q,
-                       mg ? (const T*)nullptr : (const T*)v.dinv, v.x, v.r, s->part);
+    // x += alpha p goes with the direction update, except before a reliable
+    // update, which reads x in between
+    const bool x_now = it == 0 && replace_first;
+    if (x_now)
+      hipLaunchKernelGGL((k_update<T, true>), dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k,
+                         b.kc, b.cp, s->sc, (const T*)v.p, (const T*)v.q,
+                         mg ? (const T*)nullptr : (const T*)v.dinv, v.x, v.r, s->part);
+    else
+      hipLaunchKernelGGL((k_update<T, false>), dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n,
+                         b.k, b.kc, b.cp, s->sc, (const T*)v.p, (const T*)v.q,
+                         mg ? (const T*)nullptr : (const T*)v.dinv, v.x, v.r, s->part);
     const double* rz_part = nullptr;
     int rz_nb = 0;
     if (it == 0 && replace_first) {
@@ -648,10 +670,14 @@ static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t
       hipLaunchKernelGGL(k_beta, dim3(b.k), dim3(kWave), 0, st, rz_nb, rz_part, (int)b.nbx,
                          (const double*)s->part, b.k, s->sc);
     }
-    hipLaunchKernelGGL(k_pupdate<T>, dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k, b.kc,
-                       b.cp, s->sc,
-                       (const T*)v.r, (const T*)v.dinv, mg ? (const T*)v.z : (const T*)nullptr,
-                       v.p);
+    if (x_now)
+      hipLaunchKernelGGL((k_pupdate<T, false>), dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n,
+                         b.k, b.kc, b.cp, s->sc, (const T*)v.r, (const T*)v.dinv,
+                         mg ? (const T*)v.z : (const T*)nullptr, v.p, v.x);
+    else
+      hipLaunchKernelGGL((k_pupdate<T, true>), dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n,
+                         b.k, b.kc, b.cp, s->sc, (const T*)v.r, (const T*)v.dinv,
+                         mg ? (const T*)v.z : (const T*)nullptr, v.p, v.x);
   }
   SKT_LAUNCH_CHECK();
   return SKT_OK;
