@@ -59,6 +59,7 @@ struct skt_solver {
   // of stopped columns
   int timing = 0;
   hipEvent_t ev[2 * 32] = {};
+  hipEvent_t poll_ev = nullptr;  // completion of a poll's flag copy
   bool timed = false;  // the last chunk's applies were timed
   double apply_ms[2] = {0, 0};
   int64_t n_apply[2] = {0, 0};
@@ -527,9 +528,16 @@ __global__ void k_replace_final(int nb, int k, const double* __restrict__ part_r
 
 // ------------------------------------------------------------ host side
 
+// The host waits for the flag copy by querying its event in a loop rather
+// than by a blocking synchronisation, whose wake-up left the GPU idle for
+// tens of microseconds at every poll.
 static int poll(skt_solver* s, int k, hipStream_t st) {
   SKT_HIP(hipMemcpyAsync(s->host, s->sc, sizeof(double) * kSc * k, hipMemcpyDeviceToHost, st));
-  SKT_HIP(hipStreamSynchronize(st));
+  SKT_HIP(hipEventRecord(s->poll_ev, st));
+  hipError_t e;
+  while ((e = hipEventQuery(s->poll_ev)) == hipErrorNotReady) {
+  }
+  SKT_HIP(e);
   return SKT_OK;
 }
 
@@ -957,6 +965,7 @@ int skt_solver_create(skt_operator* op, int max_cols, skt_solver** out) {
   SKT_HIP(hipMalloc(&s->part, sizeof(double) * kParts * 4 * max_cols));
   SKT_HIP(hipMalloc(&s->sc, sizeof(double) * kSc * max_cols));
   SKT_HIP(hipHostMalloc(&s->host, sizeof(double) * kSc * max_cols, hipHostMallocDefault));
+  SKT_HIP(hipEventCreateWithFlags(&s->poll_ev, hipEventDisableTiming));
   const char* g = std::getenv("SKT_GRAPH");
   s->graphs = (g && std::atoi(g) == 0) ? 0 : 1;
   *out = s;
@@ -1034,6 +1043,7 @@ int skt_solver_destroy(skt_solver* s) {
   if (s->cap) (void)hipStreamDestroy(s->cap);
   for (int i = 0; i < 2 * 32; ++i)
     if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
+  if (s->poll_ev) (void)hipEventDestroy(s->poll_ev);
   mg_free(&s->mg);
   for (double* p : {s->x64, s->f64, s->r64, s->p64, s->q64, s->z64, s->dinv64}) (void)hipFree(p);
   for (float* p : {s->r32, s->p32, s->q32, s->z32, s->d32, s->dinv32}) (void)hipFree(p);

@@ -551,7 +551,10 @@ class DensityMethod:
             return
         buf, ev, ctx = pending
         self._rec_pending = None
-        ev.synchronize()
+        # queried in a loop: a blocking wait's wake-up costs tens of
+        # microseconds of idle GPU between iterations
+        while not ev.query():
+            pass
         self._process_records(buf.numpy().copy(), ctx)
 
     def _process_records(self, rec: np.ndarray, ctx: dict) -> bool:
