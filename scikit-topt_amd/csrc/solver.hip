@@ -602,6 +602,8 @@ template <typename T>
 static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t count,
                           bool replace_first, const double* E64, hipStream_t st) {
   const bool mg = s->used == SKT_PC_MG;
+  const char* xu = std::getenv("SKT_PCG_X_IN_UPDATE");
+  const bool x_in_update = xu && xu[0] == '1' && xu[1] == 0;
   // area0 (s->part) takes the p.q partials and k_update's r.z / r.r; the
   // fused r.z of the V-cycle goes to area1, which is idle between the
   // initialisation and the reliable updates
@@ -636,8 +638,9 @@ static int pcg_iterations(skt_solver* s, const Blk& b, const Vecs<T>& v, int64_t
     }
     hipLaunchKernelGGL(k_alpha, dim3(b.k), dim3(kWave), 0, st, pq_nb, b.k, s->part, s->sc);
     // x += alpha p goes with the direction update, except before a reliable
-    // update, which reads x in between
-    const bool x_now = it == 0 && replace_first;
+    // update, which reads x in between (SKT_PCG_X_IN_UPDATE=1: always in
+    // k_update)
+    const bool x_now = (it == 0 && replace_first) || x_in_update;
     if (x_now)
       hipLaunchKernelGGL((k_update<T, true>), dim3(b.nbx, b.groups), dim3(kBlock), 0, st, b.n, b.k,
                          b.kc, b.cp, s->sc, (const T*)v.p, (const T*)v.q,

@@ -284,6 +284,30 @@ def test_mg_row_blocked_coarsest_apply_matches_wave_per_row(gpu, precision, case
     np.testing.assert_array_equal(out["rows"][0], out["wave"][0])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precond", ["mg", "jacobi"])
+def test_pcg_solution_update_in_direction_update_matches(gpu, precision, precond, monkeypatch):
+    """x += alpha p runs in the direction update of the same iteration (one
+    read of p less); SKT_PCG_X_IN_UPDATE=1 keeps it in the r update.  With 8
+    load columns stopping at different iterations (a stopping column still
+    needs its last alpha p) and the mixed path's reliable updates, the solves
+    agree bit for bit, iteration counts included."""
+    task = oprob.multiload(24, 12, 12, n_loads=8, seed=5)
+    rng = np.random.default_rng(17)
+    rho = rng.uniform(0.05, 1.0, task.n_elements)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("SKT_PCG_X_IN_UPDATE", flag)
+        fem = _fem(task, "elastic", precision, 1e-9, precond)
+        U = np.zeros((task.ndof, task.n_tasks))
+        c = fem.objectives_multi_load(rho, 3.0, U)
+        assert fem.last_solve["converged"]
+        out[flag] = (U, c, fem.last_solve["iters"])
+    assert out["0"][2] == out["1"][2]
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
+    np.testing.assert_array_equal(out["0"][1], out["1"][1])
+
+
 @pytest.mark.parametrize("precision,rtol,ctol", [("fp64", 1e-12, 1e-9), ("mixed", 1e-10, 1e-8)])
 def test_mg_reused_coarsest_inverse_reaches_direct_solutions(gpu, precision, rtol, ctol,
                                                              monkeypatch):
