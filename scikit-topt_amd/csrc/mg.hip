@@ -1805,6 +1805,10 @@ int mg_alloc(MGData* mg, const skt_operator* op, const MGConfig& cfg, int dtype,
   mg->max_cols = max_cols;
   mg->degree = std::max(cfg.degree, 1);
   mg->ratio = cfg.ratio;
+  if (const char* rv = std::getenv("SKT_MG_RATIO")) {  // smoothing interval [lambda/r, lambda]
+    const double r = std::atof(rv);
+    if (r > 1.0 && r < 1000.0) mg->ratio = r;
+  }
   mg->power_iters = cfg.power_iters;
   mg->lag = lag_enabled();
   return SKT_OK;
