@@ -132,7 +132,7 @@ def test_hex8_120x60x60_8load_50_iterations_mixed_within_bounds(gpu, tmp_path):
     assert len(cm) == len(cf) == 50
     cerr = np.abs(cm - cf) / np.abs(cf)
     derr = np.abs(res["mixed"][1] - res["fp64"][1]).max()
-    print(f"[c2 50 iterations] max compliance rel err {cerr.max():.2e}, "
+    print(f"[hex8 120x60x60, 8 loads, 50 OC iterations] max compliance rel err {cerr.max():.2e}, "
           f"max |drho| {derr:.2e}", flush=True)
     assert cerr.max() <= 1e-5
     assert derr <= 1e-4
@@ -170,7 +170,7 @@ def test_tet4_100x50x50_50_iterations_mixed_within_bounds(gpu, tmp_path):
         res[prec] = _run_loop(make, 50)
     cerr = np.abs(res["mixed"][0] - res["fp64"][0]) / np.abs(res["fp64"][0])
     derr = np.abs(res["mixed"][1] - res["fp64"][1]).max()
-    print(f"[c1 50 iterations] max compliance rel err {cerr.max():.2e}, "
+    print(f"[tet4 100x50x50x6, 50 OC iterations] max compliance rel err {cerr.max():.2e}, "
           f"max |drho| {derr:.2e}", flush=True)
     assert len(res["mixed"][0]) == 50
     assert cerr.max() <= 1e-5
@@ -199,7 +199,7 @@ def test_heat_160_logmoc_100_iterations_mixed_within_bounds(gpu, tmp_path):
         res[prec] = _run_loop(make, 100)
     cerr = np.abs(res["mixed"][0] - res["fp64"][0]) / np.abs(res["fp64"][0])
     derr = np.abs(res["mixed"][1] - res["fp64"][1]).max()
-    print(f"[c4 100 iterations] max compliance rel err {cerr.max():.2e}, "
+    print(f"[heat hex8 160^3, 100 LogMOC iterations] max compliance rel err {cerr.max():.2e}, "
           f"max |drho| {derr:.2e}", flush=True)
     assert len(res["mixed"][0]) == 100
     assert cerr.max() <= 1e-5
